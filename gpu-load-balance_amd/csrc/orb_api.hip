@@ -1,0 +1,679 @@
+// orb_api.hip -- the C-ABI of liborb_hip.so: context lifecycle, the
+// service-shaped entry points and the plain (host-driven) build.
+//
+// The plain build restates the level loop of src/orbit.cpp:74-275 over the
+// service entry points: per level one chunk-table build (ServiceCopyCells),
+// one ServiceCount, then up to 32 bisection passes, each one count launch
+// whose per-cell counts come back to the host for the termination rule, then
+// the split on the host and one partition launch sequence.
+#include <atomic>
+#include <chrono>
+#include <new>
+
+#include "orb_internal.hpp"
+
+namespace orb {
+
+static std::atomic<int> g_last_hip_error{0};
+void set_last_hip_error(hipError_t e) { g_last_hip_error.store((int)e); }
+
+bool level_cells(int32_t d, int32_t levels_mode, int32_t l, const std::vector<uint8_t> &exists,
+                 const std::vector<orb_cell> &heap, std::vector<int32_t> &ids, int32_t *bad_id)
+{
+    ids.clear();
+    int32_t D = n_levels(d);
+    if (l < 1) return true;
+    if (levels_mode == ORB_LEVELS_REFERENCE) {
+        // orbit.cpp:102-110
+        if (l >= D) return true;
+        int64_t a = (int64_t(1) << (l - 1)) - 1;
+        int64_t last = 2 * int64_t(d) - (int64_t(1) << D);
+        int64_t p2 = int64_t(1) << l;
+        int64_t b = (last < p2 ? last : p2) - 2;
+        for (int64_t id = a; id <= b; ++id) {
+            if (!exists[(size_t)id]) {
+                if (bad_id) *bad_id = (int32_t)id;
+                return false;
+            }
+            ids.push_back((int32_t)id);
+        }
+        return true;
+    }
+    if (l > D) return true;
+    int64_t a = (int64_t(1) << (l - 1)) - 1;
+    int64_t b = (int64_t(1) << l) - 2;
+    for (int64_t id = a; id <= b; ++id)
+        if (exists[(size_t)id] && heap[(size_t)id].nLeafCells >= 2) ids.push_back((int32_t)id);
+    return true;
+}
+
+// The reference generator (src/services/init.cu:11-25, 47-53): xorshf96 on
+// 64-bit unsigned long with the seeds of init.cu:11, i-major draws, SoA.
+static inline float xorshf96(uint64_t &x, uint64_t &y, uint64_t &z)
+{
+    uint64_t t;
+    x ^= x << 16;
+    x ^= x >> 5;
+    x ^= x << 1;
+    t = x;
+    x = y;
+    y = z;
+    z = t ^ x ^ y;
+    float f = (float)z;                  // uint64 -> float, round to nearest even
+    float q = f / (float)UINT64_MAX;     // ULONG_MAX converts to 2^64
+    return (float)((double)q - 0.5);     // float - double 0.5 in double, narrowed
+}
+
+static void generate_reference(int64_t n, float *x, float *y, float *z)
+{
+    uint64_t sx = 123456789ull, sy = 362436069ull, sz = 521288629ull;
+    for (int64_t i = 0; i < n; ++i) {
+        x[i] = xorshf96(sx, sy, sz);
+        y[i] = xorshf96(sx, sy, sz);
+        z[i] = xorshf96(sx, sy, sz);
+    }
+}
+
+static int reset_ranges(Ctx &c)
+{
+    ORB_TRY(launch_zero_u32(c.stream, c.ranges.p, 2 * (size_t)c.cap));
+    c.hranges.p[0] = 0;
+    c.hranges.p[1] = (uint32_t)c.n;
+    ORB_HIP(hipMemcpyAsync(c.ranges.p, c.hranges.p, 2 * sizeof(uint32_t), hipMemcpyHostToDevice,
+                           c.stream));
+    ORB_HIP(hipStreamSynchronize(c.stream));
+    c.table_ids.clear();
+    return ORB_OK;
+}
+
+static int clear_status(Ctx &c)
+{
+    return launch_zero_u32(c.stream, reinterpret_cast<uint32_t *>(c.status.p), 2);
+}
+
+static int read_status(Ctx &c, int32_t *bad_id)
+{
+    ORB_HIP(hipMemcpyAsync(c.hstatus.p, c.status.p, 2 * sizeof(int32_t), hipMemcpyDeviceToHost,
+                           c.stream));
+    ORB_HIP(hipStreamSynchronize(c.stream));
+    if (c.hstatus.p[0] != 0) {
+        if (bad_id) *bad_id = c.hstatus.p[1];
+        return ORB_ERR_UB;
+    }
+    return ORB_OK;
+}
+
+// Uploads the cells and (re)builds the chunk table if the cell list changed.
+static int prepare(Ctx &c, const orb_cell *cells, int32_t nCells)
+{
+    if (nCells < 0 || (nCells > 0 && !cells)) return ORB_ERR_ARG;
+    for (int32_t i = 0; i < nCells; ++i)
+        if (cells[i].id < 0 || cells[i].id >= c.cap) return ORB_ERR_ARG;
+    ORB_TRY(upload_level(c, cells, nCells));
+    bool same = (int32_t)c.table_ids.size() == nCells;
+    for (int32_t i = 0; same && i < nCells; ++i) same = c.table_ids[i] == cells[i].id;
+    if (!same) {
+        ORB_TRY(build_chunk_table(c, nCells));
+        c.table_ids.resize((size_t)nCells);
+        for (int32_t i = 0; i < nCells; ++i) c.table_ids[i] = cells[i].id;
+    }
+    return ORB_OK;
+}
+
+static int timed_count(Ctx &c, int32_t nCells, bool skip_found, const float *axis)
+{
+    if (c.timing) ORB_HIP(hipEventRecord(c.ev[0], c.stream));
+    ORB_TRY(launch_count(c, nCells, skip_found, axis));
+    if (c.timing) ORB_HIP(hipEventRecord(c.ev[1], c.stream));
+    return ORB_OK;
+}
+
+static int finish_timed_count(Ctx &c)
+{
+    if (!c.timing) return ORB_OK;
+    float ms = 0.f;
+    ORB_HIP(hipEventElapsedTime(&ms, c.ev[0], c.ev[1]));
+    c.tm.count_ms += ms;
+    c.tm.count_launches += 1;
+    return ORB_OK;
+}
+
+// one count-left launch; out[i] gets lt (skip_found) or le counts
+static int count_service(Ctx &c, const orb_cell *cells, int32_t nCells, uint32_t *out,
+                         bool lt_skip_found, const float *axis)
+{
+    if (!out && nCells > 0) return ORB_ERR_ARG;
+    ORB_TRY(prepare(c, cells, nCells));
+    if (nCells == 0) return 0;
+    ORB_TRY(timed_count(c, nCells, lt_skip_found, axis));
+    ORB_HIP(hipMemcpyAsync(c.hcnt.p, c.cnt.p, 2 * sizeof(uint32_t) * (size_t)nCells,
+                           hipMemcpyDeviceToHost, c.stream));
+    ORB_HIP(hipStreamSynchronize(c.stream));
+    ORB_TRY(finish_timed_count(c));
+    for (int32_t i = 0; i < nCells; ++i) {
+        if (lt_skip_found) {
+            if (!cells[i].foundCut) out[i] = c.hcnt.p[2 * i];
+        } else {
+            out[i] = c.hcnt.p[2 * i + 1];
+        }
+    }
+    return (int)(nCells * sizeof(uint32_t));
+}
+
+static int partition_service(Ctx &c, const orb_cell *cells, int32_t nCells, bool hoare,
+                             int32_t *bad_id)
+{
+    ORB_TRY(prepare(c, cells, nCells));
+    if (nCells == 0) return 0;
+    for (int32_t i = 0; i < nCells; ++i)
+        if (right_child_id(cells[i].id) >= c.cap) return ORB_ERR_ARG;
+    ORB_TRY(clear_status(c));
+    if (c.timing) ORB_HIP(hipEventRecord(c.ev[0], c.stream));
+    if (hoare) ORB_TRY(launch_hoare(c, nCells));
+    else ORB_TRY(launch_stable_le(c, nCells));
+    if (c.timing) ORB_HIP(hipEventRecord(c.ev[1], c.stream));
+    int rc = read_status(c, bad_id);
+    if (c.timing) {
+        float ms = 0.f;
+        ORB_HIP(hipEventElapsedTime(&ms, c.ev[0], c.ev[1]));
+        c.tm.partition_ms += ms;
+    }
+    c.table_ids.clear();  // ranges of the children changed; next level rebuilds
+    if (rc < 0) return rc;
+    return 0;
+}
+
+static int count_sizes(Ctx &c, const orb_cell *cells, int32_t nCells, uint32_t *out)
+{
+    if (nCells > 0 && !out) return ORB_ERR_ARG;
+    ORB_TRY(prepare(c, cells, nCells));
+    if (nCells == 0) return 0;
+    ORB_TRY(launch_cell_sizes(c, nCells, c.cnt.p));
+    ORB_HIP(hipMemcpyAsync(c.hcnt.p, c.cnt.p, sizeof(uint32_t) * (size_t)nCells,
+                           hipMemcpyDeviceToHost, c.stream));
+    ORB_HIP(hipStreamSynchronize(c.stream));
+    std::memcpy(out, c.hcnt.p, sizeof(uint32_t) * (size_t)nCells);
+    return (int)(nCells * sizeof(uint32_t));
+}
+
+// ---------------------------------------------------------------------------
+// plain build
+// ---------------------------------------------------------------------------
+static int start_build(Ctx &c)
+{
+    c.heap.assign((size_t)c.cap, orb_cell{});
+    std::memset(c.heap.data(), 0, sizeof(orb_cell) * c.heap.size());
+    c.exists.assign((size_t)c.cap, 0);
+    c.heap[0] = cell_root(c.d);
+    c.exists[0] = 1;
+    if (c.trace) c.trace_counts.assign((size_t)c.cap * ORB_MAX_PASSES, 0xffffffffu);
+    c.tm = orb_timing{};
+    return reset_ranges(c);
+}
+
+static int build_plain(Ctx &c, int32_t levels_mode, orb_build_info *info)
+{
+    ORB_TRY(start_build(c));
+    const int32_t D = n_levels(c.d);
+    std::vector<int32_t> ids;
+    std::vector<orb_cell> cells;
+    std::vector<uint32_t> counts, left;
+    for (int32_t l = 1; l <= D; ++l) {
+        int32_t bad = -1;
+        if (!level_cells(c.d, levels_mode, l, c.exists, c.heap, ids, &bad)) {
+            info->error_cell = bad;
+            return ORB_ERR_UB;
+        }
+        if (levels_mode == ORB_LEVELS_REFERENCE && l >= D) break;
+        info->nlevels = l;
+        const int32_t nCells = (int32_t)ids.size();
+        if (nCells == 0) continue;
+        cells.resize((size_t)nCells);
+        for (int32_t i = 0; i < nCells; ++i) cells[i] = c.heap[(size_t)ids[i]];
+        counts.assign((size_t)nCells, 0u);
+        left.assign((size_t)nCells, 0u);
+
+        ORB_TRY(prepare(c, cells.data(), nCells));          // ServiceCopyCells
+        ORB_TRY(count_sizes(c, cells.data(), nCells, counts.data()));  // ServiceCount
+
+        // bisection controller (orbit.cpp:146-232)
+        bool foundAll = false;
+        int32_t j = 0;
+        while (!foundAll && j < ORB_MAX_PASSES) {
+            j++;
+            foundAll = true;
+            int64_t bytes = 0;
+            for (int32_t i = 0; i < nCells; ++i)
+                if (c.cmp != ORB_CMP_LT || !cells[i].foundCut) bytes += 4 * (int64_t)counts[i];
+            if (c.cmp == ORB_CMP_LT)
+                ORB_TRY(count_service(c, cells.data(), nCells, left.data(), true, nullptr));
+            else
+                ORB_TRY(count_service(c, cells.data(), nCells, left.data(), false, nullptr));
+            c.tm.count_bytes += bytes;
+            for (int32_t i = 0; i < nCells; ++i) {
+                orb_cell &cl = cells[i];
+                if (cl.foundCut) continue;
+                if (c.trace)
+                    c.trace_counts[(size_t)cl.id * ORB_MAX_PASSES + (size_t)(j - 1)] = left[i];
+                int32_t difference = difference_rule(left[i], counts[i], cl.nLeafCells);
+                if (abs(difference) < 3) {
+                    cl.foundCut = true;
+                } else if (difference > 0) {
+                    cl.cutMarginRight = cell_get_cut(cl);
+                    foundAll = false;
+                } else {
+                    cl.cutMarginLeft = cell_get_cut(cl);
+                    foundAll = false;
+                }
+            }
+        }
+        info->level_passes[l - 1] = j;
+        info->total_passes += j;
+
+        // split (orbit.cpp:235-250); the heap keeps the final margins
+        for (int32_t i = 0; i < nCells; ++i) {
+            c.heap[(size_t)ids[i]] = cells[i];
+            orb_cell lc, rc;
+            std::memset(&lc, 0, sizeof lc);
+            std::memset(&rc, 0, sizeof rc);
+            if (!cell_split(cells[i], lc, rc)) {
+                info->error_cell = cells[i].id;
+                return ORB_ERR_UB;
+            }
+            c.heap[(size_t)lc.id] = lc;
+            c.heap[(size_t)rc.id] = rc;
+            c.exists[(size_t)lc.id] = 1;
+            c.exists[(size_t)rc.id] = 1;
+        }
+
+        int32_t bad_id = -1;
+        int rc = partition_service(c, cells.data(), nCells, c.part == ORB_PART_HOARE, &bad_id);
+        if (rc < 0) {
+            info->error_cell = bad_id;
+            return rc;
+        }
+    }
+    return ORB_OK;
+}
+
+}  // namespace orb
+
+using namespace orb;
+
+static inline Ctx *C(orb_ctx *ctx) { return ctx ? &ctx->c : nullptr; }
+
+extern "C" {
+
+const char *orb_error_string(int code)
+{
+    switch (code) {
+    case ORB_OK: return "ok";
+    case ORB_ERR_ARG: return "invalid argument";
+    case ORB_ERR_HIP: return "HIP runtime error";
+    case ORB_ERR_NOMEM: return "out of memory";
+    case ORB_ERR_UB: return "input on which the reference is undefined";
+    case ORB_ERR_STATE: return "call out of sequence";
+    case ORB_ERR_NODEV: return "no GPU";
+    case ORB_ERR_BUSY: return "not admitted";
+    default: return code >= 0 ? "ok" : "unknown error";
+    }
+}
+
+int orb_last_hip_error(void) { return g_last_hip_error.load(); }
+
+int orb_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int64_t orb_heap_capacity(int32_t d) { return heap_capacity(d); }
+
+int orb_init(orb_ctx **out, int64_t nParticles, int32_t d, bool generate, orb_meta_params params)
+{
+    if (!out) return ORB_ERR_ARG;
+    *out = nullptr;
+    if (nParticles < 0 || nParticles > (int64_t(1) << 31) || d < 1 || d > (1 << 24))
+        return ORB_ERR_ARG;
+    if (orb_device_count() < 1) return ORB_ERR_NODEV;
+    orb_ctx *o = new (std::nothrow) orb_ctx;
+    if (!o) return ORB_ERR_NOMEM;
+    Ctx &c = o->c;
+    auto fail = [&](int rc) {
+        orb_finalize(o);
+        return rc;
+    };
+    if (hipGetDevice(&c.device) != hipSuccess) return fail(ORB_ERR_HIP);
+    hipError_t e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        set_last_hip_error(e);
+        return fail(ORB_ERR_HIP);
+    }
+    if (hipEventCreate(&c.ev[0]) != hipSuccess || hipEventCreate(&c.ev[1]) != hipSuccess ||
+        hipEventCreateWithFlags(&c.ev_upload, hipEventDisableTiming) != hipSuccess)
+        return fail(ORB_ERR_HIP);
+    c.n = nParticles;
+    c.stride = ((nParticles + 63) / 64) * 64;
+    if (c.stride == 0) c.stride = 64;
+    c.d = d;
+    c.params = params;
+    c.cmp = params.GPU_COUNT ? ORB_CMP_LE : ORB_CMP_LT;
+    c.part = params.GPU_PARTITION ? ORB_PART_STABLE_LE : ORB_PART_HOARE;
+    c.cap = heap_capacity(d);
+    int rc;
+    if ((rc = c.pos.alloc(3 * (size_t)c.stride)) < 0) return fail(rc);
+    if ((rc = c.ranges.alloc(2 * (size_t)c.cap)) < 0) return fail(rc);
+    if ((rc = c.lpos.alloc((size_t)c.stride)) < 0) return fail(rc);
+    if ((rc = c.rpos.alloc((size_t)c.stride)) < 0) return fail(rc);
+    if ((rc = c.status.alloc(2)) < 0) return fail(rc);
+    if ((rc = c.hstatus.alloc(2)) < 0) return fail(rc);
+    if ((rc = c.hranges.alloc(2 * (size_t)c.cap)) < 0) return fail(rc);
+    if ((rc = ensure_level_capacity(c, (c.cap + 1) / 2)) < 0) return fail(rc);
+    if (generate && nParticles > 0) {
+        PinBuf<float> h;
+        if ((rc = h.alloc(3 * (size_t)c.stride)) < 0) return fail(rc);
+        generate_reference(nParticles, h.p, h.p + c.stride, h.p + 2 * c.stride);
+        e = hipMemcpyAsync(c.pos.p, h.p, 3 * sizeof(float) * (size_t)c.stride,
+                           hipMemcpyHostToDevice, c.stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+        h.release();
+        if (e != hipSuccess) {
+            set_last_hip_error(e);
+            return fail(ORB_ERR_HIP);
+        }
+    }
+    if ((rc = reset_ranges(c)) < 0) return fail(rc);
+    if ((rc = clear_status(c)) < 0) return fail(rc);
+    *out = o;
+    return ORB_OK;
+}
+
+int orb_finalize(orb_ctx *ctx)
+{
+    Ctx *c = C(ctx);
+    if (!c) return ORB_ERR_ARG;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    batched_release(*c);
+    c->pos.release();
+    c->axisT.release();
+    c->tmp.release();
+    c->saved.release();
+    c->ranges.release();
+    c->dcells.release();
+    c->recs.release();
+    c->cell_nchunks.release();
+    c->cell_first.release();
+    c->chunk_cell.release();
+    c->total_chunks.release();
+    c->cnt.release();
+    c->chunk_cnt.release();
+    c->chunk_pre.release();
+    c->scan_partials.release();
+    c->lpos.release();
+    c->rpos.release();
+    c->splits.release();
+    c->status.release();
+    c->ids_dev.release();
+    c->hcells.release();
+    c->hcnt.release();
+    c->hstatus.release();
+    c->hranges.release();
+    for (auto &ev : c->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (c->ev_upload) (void)hipEventDestroy(c->ev_upload);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete ctx;
+    return ORB_OK;
+}
+
+int orb_set_semantics(orb_ctx *ctx, int32_t cmp, int32_t partition)
+{
+    Ctx *c = C(ctx);
+    if (!c) return ORB_ERR_ARG;
+    if (cmp != -1 && cmp != ORB_CMP_LT && cmp != ORB_CMP_LE) return ORB_ERR_ARG;
+    if (partition != -1 && partition != ORB_PART_HOARE && partition != ORB_PART_STABLE_LE)
+        return ORB_ERR_ARG;
+    if (cmp != -1) c->cmp = cmp;
+    if (partition != -1) c->part = partition;
+    return ORB_OK;
+}
+
+int orb_set_particles(orb_ctx *ctx, const float *x, const float *y, const float *z, int64_t n)
+{
+    Ctx *c = C(ctx);
+    if (!c || n != c->n || (n > 0 && (!x || !y || !z))) return ORB_ERR_ARG;
+    if (n > 0) {
+        ORB_HIP(hipMemcpyAsync(c->x(), x, sizeof(float) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+        ORB_HIP(hipMemcpyAsync(c->y(), y, sizeof(float) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+        ORB_HIP(hipMemcpyAsync(c->z(), z, sizeof(float) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    }
+    return reset_ranges(*c);
+}
+
+int orb_get_particles(orb_ctx *ctx, float *x, float *y, float *z, int64_t n)
+{
+    Ctx *c = C(ctx);
+    if (!c || n != c->n || (n > 0 && (!x || !y || !z))) return ORB_ERR_ARG;
+    if (n > 0) {
+        ORB_HIP(hipMemcpyAsync(x, c->x(), sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+        ORB_HIP(hipMemcpyAsync(y, c->y(), sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+        ORB_HIP(hipMemcpyAsync(z, c->z(), sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    }
+    ORB_HIP(hipStreamSynchronize(c->stream));
+    return ORB_OK;
+}
+
+int orb_reset_ranges(orb_ctx *ctx)
+{
+    Ctx *c = C(ctx);
+    if (!c) return ORB_ERR_ARG;
+    return reset_ranges(*c);
+}
+
+int orb_save_particles(orb_ctx *ctx)
+{
+    Ctx *c = C(ctx);
+    if (!c) return ORB_ERR_ARG;
+    ORB_TRY(c->saved.alloc(3 * (size_t)c->stride));
+    ORB_HIP(hipMemcpyAsync(c->saved.p, c->pos.p, 3 * sizeof(float) * (size_t)c->stride,
+                           hipMemcpyDeviceToDevice, c->stream));
+    ORB_HIP(hipStreamSynchronize(c->stream));
+    return ORB_OK;
+}
+
+int orb_restore_particles(orb_ctx *ctx)
+{
+    Ctx *c = C(ctx);
+    if (!c) return ORB_ERR_ARG;
+    if (!c->saved.p) return ORB_ERR_STATE;
+    ORB_HIP(hipMemcpyAsync(c->pos.p, c->saved.p, 3 * sizeof(float) * (size_t)c->stride,
+                           hipMemcpyDeviceToDevice, c->stream));
+    return reset_ranges(*c);
+}
+
+int orb_copy_particles(orb_ctx *ctx)
+{
+    if (!C(ctx)) return ORB_ERR_ARG;
+    return (int)sizeof(int);
+}
+
+int orb_copy_cells(orb_ctx *ctx, const orb_cell *cells, int32_t nCells)
+{
+    Ctx *c = C(ctx);
+    if (!c) return ORB_ERR_ARG;
+    ORB_TRY(prepare(*c, cells, nCells));
+    return 0;
+}
+
+int orb_count(orb_ctx *ctx, const orb_cell *cells, int32_t nCells, uint32_t *out)
+{
+    Ctx *c = C(ctx);
+    if (!c) return ORB_ERR_ARG;
+    return count_sizes(*c, cells, nCells, out);
+}
+
+int orb_make_axis(orb_ctx *ctx, const orb_cell *cells, int32_t nCells)
+{
+    Ctx *c = C(ctx);
+    if (!c) return ORB_ERR_ARG;
+    ORB_TRY(prepare(*c, cells, nCells));
+    if (nCells > 0) ORB_TRY(launch_make_axis(*c, nCells));
+    ORB_HIP(hipStreamSynchronize(c->stream));
+    return (int)sizeof(int);
+}
+
+int orb_count_left(orb_ctx *ctx, const orb_cell *cells, int32_t nCells, uint32_t *out)
+{
+    Ctx *c = C(ctx);
+    if (!c) return ORB_ERR_ARG;
+    return count_service(*c, cells, nCells, out, true, nullptr);
+}
+
+int orb_count_left_gpu_axis(orb_ctx *ctx, const orb_cell *cells, int32_t nCells, uint32_t *out)
+{
+    Ctx *c = C(ctx);
+    if (!c) return ORB_ERR_ARG;
+    if (!c->axisT.p) return ORB_ERR_STATE;  // orb_make_axis fills particlesT first
+    return count_service(*c, cells, nCells, out, false, c->axisT.p);
+}
+
+int orb_count_left_gpu(orb_ctx *ctx, const orb_cell *cells, int32_t nCells, uint32_t *out)
+{
+    Ctx *c = C(ctx);
+    if (!c) return ORB_ERR_ARG;
+    return count_service(*c, cells, nCells, out, false, nullptr);
+}
+
+int orb_partition(orb_ctx *ctx, const orb_cell *cells, int32_t nCells)
+{
+    Ctx *c = C(ctx);
+    if (!c) return ORB_ERR_ARG;
+    return partition_service(*c, cells, nCells, true, nullptr);
+}
+
+int orb_partition_gpu(orb_ctx *ctx, const orb_cell *cells, int32_t nCells)
+{
+    Ctx *c = C(ctx);
+    if (!c) return ORB_ERR_ARG;
+    return partition_service(*c, cells, nCells, false, nullptr);
+}
+
+int orb_run_service(orb_ctx *ctx, int32_t id, const void *vin, int32_t nIn, void *vout,
+                    int32_t nOut)
+{
+    Ctx *c = C(ctx);
+    if (!c || nIn < 0 || nOut < 0) return ORB_ERR_ARG;
+    const orb_cell *cells = static_cast<const orb_cell *>(vin);
+    int32_t nCells = nIn / (int32_t)sizeof(orb_cell);
+    uint32_t *out = static_cast<uint32_t *>(vout);
+    auto need_out = [&]() { return nOut >= nCells * (int32_t)sizeof(uint32_t) && (nCells == 0 || out); };
+    switch (id) {
+    case ORB_PST_COPYPARTICLES: return orb_copy_particles(ctx);
+    case ORB_PST_COPYCELLS: return orb_copy_cells(ctx, cells, nCells);
+    case ORB_PST_COUNT: return need_out() ? orb_count(ctx, cells, nCells, out) : ORB_ERR_ARG;
+    case ORB_PST_MAKEAXIS: return orb_make_axis(ctx, cells, nCells);
+    case ORB_PST_COUNTLEFT: return need_out() ? orb_count_left(ctx, cells, nCells, out) : ORB_ERR_ARG;
+    case ORB_PST_COUNTLEFTAXISGPU:
+        return need_out() ? orb_count_left_gpu_axis(ctx, cells, nCells, out) : ORB_ERR_ARG;
+    case ORB_PST_COUNTLEFTGPU:
+        return need_out() ? orb_count_left_gpu(ctx, cells, nCells, out) : ORB_ERR_ARG;
+    case ORB_PST_PARTITION: return orb_partition(ctx, cells, nCells);
+    case ORB_PST_PARTITIONGPU: return orb_partition_gpu(ctx, cells, nCells);
+    case ORB_PST_SETADD:
+    case ORB_PST_INIT:
+    case ORB_PST_INITGPU:
+    case ORB_PST_FINALIZE:
+        return 0;  // lifecycle goes through orb_init / orb_finalize
+    default: return ORB_ERR_ARG;
+    }
+}
+
+int orb_build(orb_ctx *ctx, int32_t form, int32_t levels_mode, orb_build_info *info)
+{
+    Ctx *c = C(ctx);
+    if (!c || (form != ORB_FORM_PLAIN && form != ORB_FORM_BATCHED)) return ORB_ERR_ARG;
+    if (levels_mode != ORB_LEVELS_REFERENCE && levels_mode != ORB_LEVELS_FULL) return ORB_ERR_ARG;
+    orb_build_info local;
+    if (!info) info = &local;
+    std::memset(info, 0, sizeof *info);
+    info->error_cell = -1;
+    auto t0 = std::chrono::steady_clock::now();
+    int rc = form == ORB_FORM_PLAIN ? build_plain(*c, levels_mode, info)
+                                    : build_batched(*c, levels_mode, info);
+    if (rc >= 0) {
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) {
+            set_last_hip_error(e);
+            rc = ORB_ERR_HIP;
+        }
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    info->ms_total = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    return rc;
+}
+
+int orb_get_heap(orb_ctx *ctx, orb_cell *heap, int64_t cap)
+{
+    Ctx *c = C(ctx);
+    if (!c || !heap || cap < c->cap) return ORB_ERR_ARG;
+    std::memset(heap, 0, sizeof(orb_cell) * (size_t)cap);
+    if (!c->heap.empty()) std::memcpy(heap, c->heap.data(), sizeof(orb_cell) * c->heap.size());
+    return ORB_OK;
+}
+
+int orb_get_exists(orb_ctx *ctx, uint8_t *exists, int64_t cap)
+{
+    Ctx *c = C(ctx);
+    if (!c || !exists || cap < c->cap) return ORB_ERR_ARG;
+    std::memset(exists, 0, (size_t)cap);
+    if (!c->exists.empty()) std::memcpy(exists, c->exists.data(), c->exists.size());
+    return ORB_OK;
+}
+
+int orb_get_ranges(orb_ctx *ctx, uint32_t *ranges, int64_t cap)
+{
+    Ctx *c = C(ctx);
+    if (!c || !ranges || cap < c->cap) return ORB_ERR_ARG;
+    std::memset(ranges, 0, 2 * sizeof(uint32_t) * (size_t)cap);
+    ORB_HIP(hipMemcpyAsync(c->hranges.p, c->ranges.p, 2 * sizeof(uint32_t) * (size_t)c->cap,
+                           hipMemcpyDeviceToHost, c->stream));
+    ORB_HIP(hipStreamSynchronize(c->stream));
+    std::memcpy(ranges, c->hranges.p, 2 * sizeof(uint32_t) * (size_t)c->cap);
+    return ORB_OK;
+}
+
+int orb_enable_trace(orb_ctx *ctx, bool on)
+{
+    Ctx *c = C(ctx);
+    if (!c) return ORB_ERR_ARG;
+    c->trace = on;
+    return ORB_OK;
+}
+
+int orb_get_pass_counts(orb_ctx *ctx, uint32_t *counts, int64_t cap)
+{
+    Ctx *c = C(ctx);
+    if (!c || !counts || cap < c->cap) return ORB_ERR_ARG;
+    if (c->trace_counts.size() != (size_t)c->cap * ORB_MAX_PASSES) return ORB_ERR_STATE;
+    std::memcpy(counts, c->trace_counts.data(), sizeof(uint32_t) * c->trace_counts.size());
+    return ORB_OK;
+}
+
+int orb_enable_timing(orb_ctx *ctx, bool on)
+{
+    Ctx *c = C(ctx);
+    if (!c) return ORB_ERR_ARG;
+    c->timing = on;
+    return ORB_OK;
+}
+
+int orb_get_timing(orb_ctx *ctx, orb_timing *t)
+{
+    Ctx *c = C(ctx);
+    if (!c || !t) return ORB_ERR_ARG;
+    *t = c->tm;
+    return ORB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,345 @@
+// orb_internal.hpp -- internals shared by the HIP translation units of
+// liborb_hip.so: the context, error plumbing, and the Cell rules compiled for
+// host and device.
+//
+// All translation units are compiled with -ffp-contract=off: the cut midpoint
+// (src/cell.h:74-76) and the termination rule (src/orbit.cpp:204-208) are
+// float32 expressions whose results change if a multiply and a subtract are
+// fused.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "orb.h"
+
+namespace orb {
+
+// ---------------------------------------------------------------------------
+// launch geometry
+// ---------------------------------------------------------------------------
+constexpr int kBlock = 256;                 // threads per block (4 wave64s)
+constexpr int kPerThread = 16;              // keys per thread per chunk
+constexpr int kChunk = kBlock * kPerThread; // 4096 keys = 16 KiB per chunk
+
+// ---------------------------------------------------------------------------
+// error plumbing
+// ---------------------------------------------------------------------------
+void set_last_hip_error(hipError_t e);
+
+#define ORB_HIP(call)                                                   \
+    do {                                                                \
+        hipError_t orb_e_ = (call);                                     \
+        if (orb_e_ != hipSuccess) {                                     \
+            ::orb::set_last_hip_error(orb_e_);                          \
+            return ORB_ERR_HIP;                                         \
+        }                                                               \
+    } while (0)
+
+#define ORB_TRY(expr)                                                   \
+    do {                                                                \
+        int orb_rc_ = (expr);                                           \
+        if (orb_rc_ < 0) return orb_rc_;                                \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Cell rules (src/cell.h), host and device
+// ---------------------------------------------------------------------------
+__host__ __device__ inline float cell_cut(float left, float right)
+{
+    // cell.h:74-76: (cutMarginRight + cutMarginLeft) / 2.0 -> float sum,
+    // exact double halving, narrowed back to float.
+    float s = right + left;
+    return (float)((double)s / 2.0);
+}
+
+__host__ __device__ inline float cell_get_cut(const orb_cell &c)
+{
+    return cell_cut(c.cutMarginLeft, c.cutMarginRight);
+}
+
+__host__ __device__ inline int32_t left_child_id(int32_t id) { return (id + 1) * 2 - 1; }
+__host__ __device__ inline int32_t right_child_id(int32_t id) { return (id + 1) * 2; }
+
+__host__ __device__ inline int32_t difference_rule(uint32_t count_left, uint32_t count,
+                                                   int32_t nLeafCells)
+{
+    // orbit.cpp:204-205.  ratio: double quotient narrowed to float; the count
+    // converts to float; product and difference are separate float roundings;
+    // the conversion to int truncates toward zero.
+    float ratio = (float)(ceil(nLeafCells / 2.0) / (double)nLeafCells);
+#if defined(__HIP_DEVICE_COMPILE__)
+    float prod = __fmul_rn((float)count, ratio);
+    float diff = __fsub_rn((float)count_left, prod);
+#else
+    float prod = (float)count * ratio;
+    float diff = (float)count_left - prod;
+#endif
+    return (int32_t)diff;
+}
+
+__host__ __device__ inline void cell_make(orb_cell &c, int32_t id, int32_t nLeaf,
+                                          const float lower[3], const float upper[3])
+{
+    // cell.h:19-45
+    c.id = id;
+    c.nLeafCells = nLeaf;
+    c.prevCutAxis = -1;
+    c.cutAxis = -1;
+    c.foundCut = false;
+    c.cutMarginLeft = 0.0f;
+    c.cutMarginRight = 0.0f;
+    for (int d = 0; d < 3; ++d) {
+        c.lower[d] = lower[d];
+        c.upper[d] = upper[d];
+    }
+}
+
+__host__ __device__ inline int32_t cell_cut_axis(const orb_cell &c)
+{
+    // cell.h:102-121: longest upper-lower, strict '>', first max, else -1
+    int maxD = -1;
+    float maxSize = 0.0f;
+    for (int d = 0; d < 3; d++) {
+        float size = c.upper[d] - c.lower[d];
+        if (size > maxSize) {
+            maxSize = size;
+            maxD = d;
+        }
+    }
+    return maxD;
+}
+
+// cut() + setCutAxis() + setCutMargin() on both children, in the order of
+// orbit.cpp:235-243.  Returns false where the reference would index with
+// cutAxis == -1 (a child without positive extent).
+__host__ __device__ inline bool cell_split(const orb_cell &p, orb_cell &l, orb_cell &r)
+{
+    if (p.cutAxis < 0 || p.cutAxis > 2) return false;
+    int32_t nLeft = (int32_t)ceil(p.nLeafCells / 2.0);
+    int32_t nRight = p.nLeafCells - nLeft;
+    float cut = cell_get_cut(p);
+    cell_make(l, left_child_id(p.id), nLeft, p.lower, p.upper);
+    l.upper[p.cutAxis] = cut;
+    l.prevCutAxis = p.cutAxis;
+    cell_make(r, right_child_id(p.id), nRight, p.lower, p.upper);
+    r.lower[p.cutAxis] = cut;
+    r.prevCutAxis = p.cutAxis;
+    r.cutAxis = cell_cut_axis(r);
+    l.cutAxis = cell_cut_axis(l);
+    if (r.cutAxis < 0 || l.cutAxis < 0) return false;
+    r.cutMarginLeft = r.lower[r.cutAxis];
+    r.cutMarginRight = r.upper[r.cutAxis];
+    l.cutMarginLeft = l.lower[l.cutAxis];
+    l.cutMarginRight = l.upper[l.cutAxis];
+    return true;
+}
+
+inline orb_cell cell_root(int32_t d)
+{
+    // orbit.cpp:45-46, 74-76
+    const float lo[3] = {-0.5f, -0.5f, -0.5f};
+    const float up[3] = {0.5f, 0.5f, 0.5f};
+    orb_cell r;
+    std::memset(&r, 0, sizeof r);
+    cell_make(r, 0, d, lo, up);
+    r.cutAxis = 0;
+    r.cutMarginLeft = r.lower[0];
+    r.cutMarginRight = r.upper[0];
+    return r;
+}
+
+inline int32_t n_levels(int32_t d)
+{
+    // cell.h:65-67 ceil(log2 d), in integers
+    int32_t D = 0;
+    while ((int64_t(1) << D) < int64_t(d)) ++D;
+    return D;
+}
+
+inline int64_t heap_capacity(int32_t d)
+{
+    if (d < 1) return 0;
+    int64_t cap = (int64_t(1) << (n_levels(d) + 1)) - 1;
+    int64_t ref = 2 * int64_t(d) - 1;
+    return cap > ref ? cap : ref;
+}
+
+// ids of the cells processed on level l (1-based), in heap order.
+// REFERENCE: orbit.cpp:102-111 slices; FULL: every existing cell of depth l-1
+// with nLeafCells >= 2.  Returns false for a slice naming an id never created.
+bool level_cells(int32_t d, int32_t levels_mode, int32_t l, const std::vector<uint8_t> &exists,
+                 const std::vector<orb_cell> &heap, std::vector<int32_t> &ids, int32_t *bad_id);
+
+// ---------------------------------------------------------------------------
+// device buffers
+// ---------------------------------------------------------------------------
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    int alloc(size_t count)
+    {
+        if (count <= n && p) return ORB_OK;
+        release();
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            set_last_hip_error(e);
+            p = nullptr;
+            n = 0;
+            return ORB_ERR_NOMEM;
+        }
+        n = count;
+        return ORB_OK;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+template <class T>
+struct PinBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    int alloc(size_t count)
+    {
+        if (count <= n && p) return ORB_OK;
+        release();
+        if (count == 0) count = 1;
+        hipError_t e = hipHostMalloc((void **)&p, count * sizeof(T), hipHostMallocDefault);
+        if (e != hipSuccess) {
+            set_last_hip_error(e);
+            p = nullptr;
+            n = 0;
+            return ORB_ERR_NOMEM;
+        }
+        n = count;
+        return ORB_OK;
+    }
+    void release()
+    {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+// Per-cell record the kernels read (derived from orb_cell + the range map).
+struct CellRec {
+    uint32_t begin;     // absolute index of the first particle
+    uint32_t end;       // one past the last
+    float cut;          // getCut() of the cell
+    int32_t axis;       // cutAxis
+    int32_t found;      // foundCut
+    int32_t nLeaf;      // nLeafCells
+    float margin_l;     // cutMarginLeft
+    float margin_r;     // cutMarginRight
+    int32_t id;         // heap id
+};
+
+// Result of the Hoare split of one cell.
+struct SplitRec {
+    uint32_t TL;        // keys >= cut
+    uint32_t TR;        // keys <= cut
+    uint32_t K;         // swapped pairs
+    uint32_t i;         // split index (cell-local)
+};
+
+// ---------------------------------------------------------------------------
+// the context (the PST leaf's LocalData, src/services/pst.h:6-41)
+// ---------------------------------------------------------------------------
+struct BatchedState;
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t n = 0;
+    int64_t stride = 0;        // floats between the x, y and z arrays (n rounded up to 64)
+    int32_t d = 1;
+    orb_meta_params params{};
+    int32_t cmp = ORB_CMP_LT;
+    int32_t part = ORB_PART_HOARE;
+    int64_t cap = 0;           // heap capacity (ids)
+
+    // particles: one allocation of 3n floats, SoA
+    DevBuf<float> pos;         // x = pos.p, y = pos.p + n, z = pos.p + 2n
+    DevBuf<float> axisT;       // particlesT (orb_make_axis)
+    DevBuf<float> tmp;         // 3n floats: out-of-place partition target
+    DevBuf<float> saved;       // 3n floats: orb_save_particles snapshot
+    DevBuf<uint32_t> ranges;   // [cap][2] cellToRangeMap
+
+    // level tables (sized for the widest level)
+    int64_t max_level_cells = 0;
+    int64_t max_chunks = 0;
+    DevBuf<orb_cell> dcells;
+    DevBuf<CellRec> recs;
+    DevBuf<uint32_t> cell_nchunks;   // [cells+1]
+    DevBuf<uint32_t> cell_first;     // [cells+1] exclusive scan
+    DevBuf<uint32_t> chunk_cell;     // [max_chunks]
+    DevBuf<uint32_t> total_chunks;   // [1]
+    DevBuf<uint32_t> cnt;            // [cells][2] lt, le
+    DevBuf<uint64_t> chunk_cnt;      // [max_chunks+1] packed (nL | nR<<32)
+    DevBuf<uint64_t> chunk_pre;      // [max_chunks+1]
+    DevBuf<uint64_t> scan_partials;  // scan scratch
+    DevBuf<uint32_t> lpos, rpos;     // [n] stopper positions
+    DevBuf<SplitRec> splits;         // [cells]
+    DevBuf<int32_t> status;          // [2]: error flag, failing heap id
+    DevBuf<uint32_t> ids_dev;        // [cells]
+
+    // host staging
+    PinBuf<orb_cell> hcells;
+    PinBuf<uint32_t> hcnt;
+    PinBuf<int32_t> hstatus;
+    PinBuf<uint32_t> hranges;
+    std::vector<int32_t> table_ids;  // ids of the current chunk table
+    int32_t table_cells = 0;
+
+    // build results
+    std::vector<orb_cell> heap;
+    std::vector<uint8_t> exists;
+    std::vector<uint32_t> trace_counts;
+    bool trace = false;
+    bool timing = false;
+    orb_timing tm{};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipEvent_t ev_upload = nullptr;  // last H2D out of hcells (pinned staging reuse)
+    BatchedState *batched = nullptr;
+
+    float *x() const { return pos.p; }
+    float *y() const { return pos.p + stride; }
+    float *z() const { return pos.p + 2 * stride; }
+};
+
+// kernels / launchers (orb_kernels.hip)
+int launch_zero_u32(hipStream_t s, uint32_t *p, size_t n);
+int launch_zero_u64(hipStream_t s, uint64_t *p, size_t n);
+int scan_exclusive_u32(Ctx &c, const uint32_t *in, uint32_t *out, size_t m);
+int scan_exclusive_u64(Ctx &c, const uint64_t *in, uint64_t *out, size_t m);
+int upload_level(Ctx &c, const orb_cell *cells, int32_t nCells);
+int build_chunk_table(Ctx &c, int32_t nCells);
+int launch_count(Ctx &c, int32_t nCells, bool lt_skip_found, const float *axis_override);
+int launch_hoare(Ctx &c, int32_t nCells);
+int launch_stable_le(Ctx &c, int32_t nCells);
+int launch_make_axis(Ctx &c, int32_t nCells);
+int launch_cell_sizes(Ctx &c, int32_t nCells, uint32_t *dst);
+int launch_generate(Ctx &c);
+int ensure_level_capacity(Ctx &c, int64_t nCells);
+
+// batched build (orb_batched.hip)
+int build_batched(Ctx &c, int32_t levels_mode, orb_build_info *info);
+void batched_release(Ctx &c);
+
+}  // namespace orb
+
+struct orb_ctx {
+    orb::Ctx c;
+};

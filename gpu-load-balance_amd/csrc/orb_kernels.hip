@@ -1,0 +1,760 @@
+// orb_kernels.hip -- HIP kernels for gfx950 behind the plain (service) form:
+// level tables, the segmented count-left pass, the exact Hoare partition as
+// scans, the stable <= partition, the axis gather and a device scan.
+//
+// Layout in HBM: particles are float32 SoA, x | y | z, each array padded to a
+// multiple of 64 floats so every array base is 256-byte aligned.  A level is a
+// list of cells with disjoint [begin, end) ranges; each cell is cut into
+// chunks of kChunk = 4096 keys (16 KiB), a chunk never spans two cells, and
+// one 256-thread block (four wave64s) handles one chunk.  The chunk table is
+// built on the device from the range map (no host block table).
+#include "orb_internal.hpp"
+
+namespace orb {
+
+// ---------------------------------------------------------------------------
+// wave64 / block primitives
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ inline T wave_sum(T v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <class T>
+__device__ inline T wave_inclusive_scan(T v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        T t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
+    }
+    return v;
+}
+
+// Sum over a 256-thread block; every thread gets the result.
+template <class T>
+__device__ inline T block_sum(T v, T *smem4)
+{
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) smem4[w] = v;
+    __syncthreads();
+    T r = smem4[0] + smem4[1] + smem4[2] + smem4[3];
+    __syncthreads();
+    return r;
+}
+
+// Exclusive scan over a 256-thread block; *total gets the block sum.
+template <class T>
+__device__ inline T block_exclusive_scan(T v, T *smem4, T *total)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    T inc = wave_inclusive_scan(v);
+    if (lane == 63) smem4[w] = inc;
+    __syncthreads();
+    T pre = 0;
+    for (int i = 0; i < w; ++i) pre += smem4[i];
+    *total = smem4[0] + smem4[1] + smem4[2] + smem4[3];
+    __syncthreads();
+    return pre + inc - v;
+}
+
+__device__ inline const float *axis_ptr(const float *x, const float *y, const float *z, int a)
+{
+    return a == 0 ? x : (a == 1 ? y : z);
+}
+
+// chunk q -> its cell (binary search in the exclusive chunk offsets)
+__device__ inline uint32_t chunk_to_cell(const uint32_t *first, int32_t nCells, uint32_t q)
+{
+    int32_t lo = 0, hi = nCells - 1;
+    while (lo < hi) {  // last cell with first[cell] <= q
+        int32_t mid = (lo + hi + 1) >> 1;
+        if (first[mid] <= q) lo = mid;
+        else hi = mid - 1;
+    }
+    return (uint32_t)lo;
+}
+
+// ---------------------------------------------------------------------------
+// utilities
+// ---------------------------------------------------------------------------
+__global__ void k_zero_u32(uint32_t *p, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+__global__ void k_zero_u64(uint64_t *p, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0ull;
+}
+
+static inline unsigned grid_for(size_t n, unsigned cap = 65536)
+{
+    size_t g = (n + kBlock - 1) / kBlock;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+int launch_zero_u32(hipStream_t s, uint32_t *p, size_t n)
+{
+    hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(n)), dim3(kBlock), 0, s, p, n);
+    ORB_HIP(hipGetLastError());
+    return ORB_OK;
+}
+
+int launch_zero_u64(hipStream_t s, uint64_t *p, size_t n)
+{
+    hipLaunchKernelGGL(k_zero_u64, dim3(grid_for(n)), dim3(kBlock), 0, s, p, n);
+    ORB_HIP(hipGetLastError());
+    return ORB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan (three kernels; m <= kScanTile^2)
+// ---------------------------------------------------------------------------
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;  // 2048
+
+template <class T>
+__global__ void k_scan_reduce(const T *in, size_t m, T *partials)
+{
+    __shared__ T sm[4];
+    size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+    T s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+        if (base + k < m) s += in[base + k];
+    s = block_sum(s, sm);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+template <class T>
+__global__ void k_scan_partials(T *partials, int np)
+{
+    __shared__ T sm[4];
+    T v[kScanItems];
+    int base = threadIdx.x * kScanItems;
+    T s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = (base + k < np) ? partials[base + k] : T(0);
+        s += v[k];
+    }
+    T total;
+    T pre = block_exclusive_scan(s, sm, &total);
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < np) partials[base + k] = pre;
+        pre += v[k];
+    }
+}
+
+template <class T>
+__global__ void k_scan_apply(const T *in, T *out, size_t m, const T *partials)
+{
+    __shared__ T sm[4];
+    size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+    T v[kScanItems];
+    T s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = (base + k < m) ? in[base + k] : T(0);
+        s += v[k];
+    }
+    T total;
+    T pre = block_exclusive_scan(s, sm, &total) + partials[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < m) out[base + k] = pre;
+        pre += v[k];
+    }
+}
+
+template <class T>
+static int scan_exclusive(Ctx &c, const T *in, T *out, size_t m, DevBuf<uint64_t> &scratch)
+{
+    if (m == 0) return ORB_OK;
+    size_t np = (m + kScanTile - 1) / kScanTile;
+    if (np > (size_t)kScanTile) return ORB_ERR_ARG;
+    size_t words = (np * sizeof(T) + sizeof(uint64_t) - 1) / sizeof(uint64_t);
+    ORB_TRY(scratch.alloc(words));
+    T *partials = reinterpret_cast<T *>(scratch.p);
+    hipLaunchKernelGGL(k_scan_reduce<T>, dim3((unsigned)np), dim3(kBlock), 0, c.stream, in, m, partials);
+    hipLaunchKernelGGL(k_scan_partials<T>, dim3(1), dim3(kBlock), 0, c.stream, partials, (int)np);
+    hipLaunchKernelGGL(k_scan_apply<T>, dim3((unsigned)np), dim3(kBlock), 0, c.stream, in, out, m,
+                       (const T *)partials);
+    ORB_HIP(hipGetLastError());
+    return ORB_OK;
+}
+
+int scan_exclusive_u32(Ctx &c, const uint32_t *in, uint32_t *out, size_t m)
+{
+    return scan_exclusive<uint32_t>(c, in, out, m, c.scan_partials);
+}
+
+int scan_exclusive_u64(Ctx &c, const uint64_t *in, uint64_t *out, size_t m)
+{
+    return scan_exclusive<uint64_t>(c, in, out, m, c.scan_partials);
+}
+
+// ---------------------------------------------------------------------------
+// level records and the chunk table (ServiceCopyCells, copyCells.cu:10-64)
+// ---------------------------------------------------------------------------
+__global__ void k_level_recs(const orb_cell *cells, int32_t nCells, const uint32_t *ranges,
+                             CellRec *recs, uint32_t *nchunks)
+{
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > nCells) return;
+    if (i == nCells) {
+        nchunks[nCells] = 0u;
+        return;
+    }
+    const orb_cell &cl = cells[i];
+    CellRec r;
+    r.begin = ranges[2 * (size_t)cl.id];
+    r.end = ranges[2 * (size_t)cl.id + 1];
+    if (r.end < r.begin) r.end = r.begin;
+    r.cut = cell_get_cut(cl);
+    r.axis = cl.cutAxis;
+    r.found = cl.foundCut ? 1 : 0;
+    r.nLeaf = cl.nLeafCells;
+    r.margin_l = cl.cutMarginLeft;
+    r.margin_r = cl.cutMarginRight;
+    r.id = cl.id;
+    recs[i] = r;
+    nchunks[i] = (r.end - r.begin + (uint32_t)kChunk - 1u) / (uint32_t)kChunk;
+}
+
+__global__ void k_fill_chunks(const uint32_t *first, int32_t nCells, uint32_t *chunk_cell,
+                              uint32_t max_chunks)
+{
+    uint32_t total = first[nCells];
+    if (total > max_chunks) total = max_chunks;
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < total;
+         q += gridDim.x * blockDim.x)
+        chunk_cell[q] = chunk_to_cell(first, nCells, q);
+}
+
+int ensure_level_capacity(Ctx &c, int64_t nCells)
+{
+    if (nCells <= c.max_level_cells && c.recs.p) return ORB_OK;
+    int64_t m = nCells > c.max_level_cells ? nCells : c.max_level_cells;
+    c.max_level_cells = m;
+    c.max_chunks = c.n / kChunk + m + 1;
+    ORB_TRY(c.dcells.alloc(m));
+    ORB_TRY(c.recs.alloc(m));
+    ORB_TRY(c.cell_nchunks.alloc(m + 1));
+    ORB_TRY(c.cell_first.alloc(m + 1));
+    ORB_TRY(c.chunk_cell.alloc(c.max_chunks));
+    ORB_TRY(c.cnt.alloc(2 * m));
+    ORB_TRY(c.chunk_cnt.alloc(c.max_chunks + 1));
+    ORB_TRY(c.chunk_pre.alloc(c.max_chunks + 1));
+    ORB_TRY(c.splits.alloc(m));
+    ORB_TRY(c.ids_dev.alloc(m));
+    ORB_TRY(c.hcells.alloc(m));
+    ORB_TRY(c.hcnt.alloc(2 * m));
+    return ORB_OK;
+}
+
+int upload_level(Ctx &c, const orb_cell *cells, int32_t nCells)
+{
+    ORB_TRY(ensure_level_capacity(c, nCells));
+    // the previous upload may still be reading the pinned staging buffer
+    if (c.ev_upload) ORB_HIP(hipEventSynchronize(c.ev_upload));
+    std::memcpy(c.hcells.p, cells, sizeof(orb_cell) * (size_t)nCells);
+    ORB_HIP(hipMemcpyAsync(c.dcells.p, c.hcells.p, sizeof(orb_cell) * (size_t)nCells,
+                           hipMemcpyHostToDevice, c.stream));
+    if (c.ev_upload) ORB_HIP(hipEventRecord(c.ev_upload, c.stream));
+    unsigned g = (unsigned)((nCells + 1 + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_level_recs, dim3(g), dim3(kBlock), 0, c.stream, c.dcells.p, nCells,
+                       c.ranges.p, c.recs.p, c.cell_nchunks.p);
+    ORB_HIP(hipGetLastError());
+    return ORB_OK;
+}
+
+int build_chunk_table(Ctx &c, int32_t nCells)
+{
+    ORB_TRY(scan_exclusive_u32(c, c.cell_nchunks.p, c.cell_first.p, (size_t)nCells + 1));
+    hipLaunchKernelGGL(k_fill_chunks, dim3(grid_for((size_t)c.max_chunks, 16384)), dim3(kBlock), 0,
+                       c.stream, c.cell_first.p, nCells, c.chunk_cell.p, (uint32_t)c.max_chunks);
+    ORB_HIP(hipGetLastError());
+    return ORB_OK;
+}
+
+// chunk q of the level: its cell and [b, e).  Returns false past the table.
+__device__ inline bool chunk_range(uint32_t q, const uint32_t *first, int32_t nCells,
+                                   const uint32_t *chunk_cell, const CellRec *recs,
+                                   uint32_t &cell, uint32_t &b, uint32_t &e)
+{
+    uint32_t total = first[nCells];
+    if (q >= total) return false;
+    cell = chunk_cell[q];
+    const CellRec &r = recs[cell];
+    b = r.begin + (q - first[cell]) * (uint32_t)kChunk;
+    e = b + (uint32_t)kChunk;
+    if (e > r.end) e = r.end;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// count-left pass (ServiceCountLeft / ServiceCountLeftGPU / ...GPUAxis)
+//
+// One block per chunk streams the chunk's keys once with 16-byte loads (head
+// and tail of a misaligned chunk scalar) and counts x < cut and x <= cut
+// together; a wave64 shuffle reduction and four LDS partials give the block
+// sum, and one atomic per block per cell adds it to the cell's counters.
+// Roofline: HBM, 4 bytes per key counted.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_count(const CellRec *recs, int32_t nCells,
+                                                  const uint32_t *first, const uint32_t *chunk_cell,
+                                                  const float *x, const float *y, const float *z,
+                                                  const float *axisT, int skip_found, uint32_t *cnt)
+{
+    __shared__ uint32_t sm[4];
+    uint32_t cell, b, e;
+    if (!chunk_range(blockIdx.x, first, nCells, chunk_cell, recs, cell, b, e)) return;
+    const CellRec r = recs[cell];
+    if (skip_found && r.found) return;
+    if (r.axis < 0 || r.axis > 2) return;
+    const float *key = axisT ? axisT : axis_ptr(x, y, z, r.axis);
+    const float cut = r.cut;
+
+    uint32_t ab = (b + 3u) & ~3u;
+    if (ab > e) ab = e;
+    uint32_t ae = e & ~3u;
+    if (ae < ab) ae = ab;
+    const int t = threadIdx.x;
+    uint32_t lt = 0, le = 0;
+    if ((uint32_t)t < ab - b) {
+        float v = key[b + t];
+        lt += v < cut;
+        le += v <= cut;
+    }
+    if ((uint32_t)t < e - ae) {
+        float v = key[ae + t];
+        lt += v < cut;
+        le += v <= cut;
+    }
+    const float4 *k4 = reinterpret_cast<const float4 *>(key + ab);
+    const uint32_t nq = (ae - ab) >> 2;  // <= kChunk/4 = 1024
+    float4 v[kPerThread / 4];
+#pragma unroll
+    for (int u = 0; u < kPerThread / 4; ++u) {
+        uint32_t j = (uint32_t)t + (uint32_t)u * kBlock;
+        if (j < nq) v[u] = k4[j];
+        else v[u] = make_float4(INFINITY, INFINITY, INFINITY, INFINITY);
+    }
+#pragma unroll
+    for (int u = 0; u < kPerThread / 4; ++u) {
+        lt += (v[u].x < cut) + (v[u].y < cut) + (v[u].z < cut) + (v[u].w < cut);
+        le += (v[u].x <= cut) + (v[u].y <= cut) + (v[u].z <= cut) + (v[u].w <= cut);
+    }
+    // both counts <= 4096: pack into one word for a single reduction
+    uint32_t packed = block_sum(lt | (le << 16), sm);
+    if (t == 0) {
+        atomicAdd(&cnt[2 * cell], packed & 0xffffu);
+        atomicAdd(&cnt[2 * cell + 1], packed >> 16);
+    }
+}
+
+int launch_count(Ctx &c, int32_t nCells, bool skip_found, const float *axis_override)
+{
+    ORB_TRY(launch_zero_u32(c.stream, c.cnt.p, 2 * (size_t)nCells));
+    hipLaunchKernelGGL(k_count, dim3((unsigned)c.max_chunks), dim3(kBlock), 0, c.stream,
+                       c.recs.p, nCells, c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(),
+                       axis_override, skip_found ? 1 : 0, c.cnt.p);
+    ORB_HIP(hipGetLastError());
+    return ORB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Hoare partition as scans (partition.cpp:28-60).
+//
+// For a cell of n keys and cut c let the L-stoppers be the positions with
+// key >= c, left to right L_0 < L_1 < ..., and the R-stoppers the positions
+// with key <= c, right to left R_0 > R_1 > ....  The serial loop swaps
+// (L_t, R_t) for every t < K = #{t : L_t < R_t} (these transpositions are
+// disjoint), stops with i = min(L_K, R_{K-1}) (i = L_0 when K == 0), and then
+// swaps rows i and n-1.  The children are [0, i) and [i, n).
+//
+//   P1a  per chunk: #L and #R (one key read)
+//   scan per-chunk counts -> each chunk's rank offsets inside its cell
+//   P1b  per chunk: block scan of the flags, write every stopper's position
+//        into lpos / rpos at the cell's base + its rank (one key read)
+//   P2   per cell: K by binary search on lpos[t] < rpos[TR-1-t], i, the
+//        children's ranges, undefined-input check
+//   P3   per pair t < K: swap x, y, z of lpos[t] and rpos[TR-1-t]
+//   P4   per cell: swap rows i and n-1
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_stopper_counts(const CellRec *recs, int32_t nCells,
+                                                           const uint32_t *first,
+                                                           const uint32_t *chunk_cell,
+                                                           const float *x, const float *y,
+                                                           const float *z, int le_only,
+                                                           uint64_t *chunk_cnt, uint32_t max_chunks)
+{
+    __shared__ uint32_t sm[4];
+    uint32_t q = blockIdx.x;
+    uint32_t cell, b, e;
+    if (!chunk_range(q, first, nCells, chunk_cell, recs, cell, b, e)) {
+        if (q <= max_chunks && threadIdx.x == 0) chunk_cnt[q] = 0ull;
+        return;
+    }
+    const CellRec r = recs[cell];
+    const float *key = axis_ptr(x, y, z, r.axis < 0 ? 0 : (r.axis > 2 ? 2 : r.axis));
+    const float cut = r.cut;
+    uint32_t nl = 0, nr = 0;
+    for (uint32_t p = b + threadIdx.x; p < e; p += kBlock) {
+        float v = key[p];
+        if (le_only) {
+            nl += v <= cut;   // keys going left
+            nr += v > cut;    // keys going right
+        } else {
+            nl += v >= cut;   // L-stoppers
+            nr += v <= cut;   // R-stoppers
+        }
+    }
+    uint32_t packed = block_sum(nl | (nr << 16), sm);
+    if (threadIdx.x == 0)
+        chunk_cnt[q] = (uint64_t)(packed & 0xffffu) | ((uint64_t)(packed >> 16) << 32);
+}
+
+// LDS staging of one chunk: element k at index k + k/16 (odd stride per
+// thread row, no bank conflicts when thread t reads its 16 consecutive keys).
+__device__ inline int lds_idx(int k) { return k + (k >> 4); }
+constexpr int kLdsKeys = kChunk + kChunk / 16;
+
+__global__ __launch_bounds__(kBlock) void k_stopper_positions(const CellRec *recs, int32_t nCells,
+                                                              const uint32_t *first,
+                                                              const uint32_t *chunk_cell,
+                                                              const uint64_t *chunk_pre,
+                                                              const float *x, const float *y,
+                                                              const float *z, uint32_t *lpos,
+                                                              uint32_t *rpos)
+{
+    __shared__ float skey[kLdsKeys];
+    __shared__ uint32_t sm[4];
+    uint32_t q = blockIdx.x;
+    uint32_t cell, b, e;
+    if (!chunk_range(q, first, nCells, chunk_cell, recs, cell, b, e)) return;
+    const CellRec r = recs[cell];
+    const float *key = axis_ptr(x, y, z, r.axis);
+    const float cut = r.cut;
+    const int n = (int)(e - b);
+    for (int k = threadIdx.x; k < n; k += kBlock) skey[lds_idx(k)] = key[b + k];
+    __syncthreads();
+
+    const int t = threadIdx.x;
+    const int k0 = t * kPerThread;
+    uint32_t myl = 0, myr = 0;
+#pragma unroll
+    for (int j = 0; j < kPerThread; ++j) {
+        int k = k0 + j;
+        if (k < n) {
+            float v = skey[lds_idx(k)];
+            myl += v >= cut;
+            myr += v <= cut;
+        }
+    }
+    uint32_t total;
+    uint32_t pre = block_exclusive_scan(myl | (myr << 16), sm, &total);
+    uint64_t base = chunk_pre[q] - chunk_pre[first[cell]];
+    uint32_t rl = (uint32_t)(base & 0xffffffffull) + (pre & 0xffffu);
+    uint32_t rr = (uint32_t)(base >> 32) + (pre >> 16);
+    const uint32_t off = b - r.begin;  // cell-local position of the chunk start
+#pragma unroll
+    for (int j = 0; j < kPerThread; ++j) {
+        int k = k0 + j;
+        if (k < n) {
+            float v = skey[lds_idx(k)];
+            if (v >= cut) lpos[r.begin + rl++] = off + (uint32_t)k;
+            if (v <= cut) rpos[r.begin + rr++] = off + (uint32_t)k;
+        }
+    }
+}
+
+__device__ inline void report_error(int32_t *status, int32_t id)
+{
+    if (atomicCAS(&status[0], 0, 1) == 0) status[1] = id;
+}
+
+__global__ void k_hoare_split(const CellRec *recs, int32_t nCells, const uint32_t *first,
+                              const uint64_t *chunk_pre, const uint32_t *lpos,
+                              const uint32_t *rpos, SplitRec *splits, uint32_t *ranges,
+                              int32_t *status, int64_t cap)
+{
+    int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nCells) return;
+    const CellRec r = recs[c];
+    uint64_t tot = chunk_pre[first[c + 1]] - chunk_pre[first[c]];
+    uint32_t TL = (uint32_t)(tot & 0xffffffffull), TR = (uint32_t)(tot >> 32);
+    uint32_t n = r.end - r.begin;
+    SplitRec s{TL, TR, 0u, 0u};
+    int64_t lid = left_child_id(r.id), rid = right_child_id(r.id);
+    if (n == 0 || TL == 0 || r.axis < 0 || r.axis > 2 || rid >= cap) {
+        // empty cell or no key >= cut: the reference's loop leaves the cell
+        report_error(status, r.id);
+        s.TL = 0u;  // k_hoare_final skips the cell
+        splits[c] = s;
+        return;
+    }
+    const uint32_t *L = lpos + r.begin;
+    const uint32_t *R = rpos + r.begin;  // left-to-right order; R_t = R[TR-1-t]
+    uint32_t lo = 0, hi = TL < TR ? TL : TR;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (L[mid] < R[TR - 1 - mid]) lo = mid + 1;
+        else hi = mid;
+    }
+    uint32_t K = lo, i;
+    if (K == 0) {
+        i = L[0];
+    } else {
+        i = R[TR - K];
+        if (K < TL && L[K] < i) i = L[K];
+    }
+    s.K = K;
+    s.i = i;
+    splits[c] = s;
+    ranges[2 * lid] = r.begin;
+    ranges[2 * lid + 1] = r.begin + i;
+    ranges[2 * rid] = r.begin + i;
+    ranges[2 * rid + 1] = r.end;
+}
+
+__global__ __launch_bounds__(kBlock) void k_hoare_swap(const CellRec *recs, int32_t nCells,
+                                                       const uint32_t *first,
+                                                       const uint32_t *chunk_cell,
+                                                       const SplitRec *splits,
+                                                       const uint32_t *lpos, const uint32_t *rpos,
+                                                       float *x, float *y, float *z)
+{
+    uint32_t q = blockIdx.x;
+    uint32_t cell, b, e;
+    if (!chunk_range(q, first, nCells, chunk_cell, recs, cell, b, e)) return;
+    const CellRec r = recs[cell];
+    const SplitRec s = splits[cell];
+    uint32_t t0 = b - r.begin;  // this chunk handles pair indices [t0, t0 + len)
+    uint32_t t1 = e - r.begin;
+    if (t1 > s.K) t1 = s.K;
+    if (t0 >= t1) return;
+    const uint32_t *L = lpos + r.begin;
+    const uint32_t *R = rpos + r.begin;
+    for (uint32_t t = t0 + threadIdx.x; t < t1; t += kBlock) {
+        uint32_t pa = r.begin + L[t];
+        uint32_t pb = r.begin + R[s.TR - 1 - t];
+        float ax = x[pa], ay = y[pa], az = z[pa];
+        float bx = x[pb], by = y[pb], bz = z[pb];
+        x[pa] = bx; y[pa] = by; z[pa] = bz;
+        x[pb] = ax; y[pb] = ay; z[pb] = az;
+    }
+}
+
+__global__ void k_hoare_final(const CellRec *recs, int32_t nCells, const SplitRec *splits,
+                              float *x, float *y, float *z)
+{
+    int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nCells) return;
+    const CellRec r = recs[c];
+    const SplitRec s = splits[c];
+    uint32_t n = r.end - r.begin;
+    if (n == 0 || s.TL == 0) return;  // reported by k_hoare_split
+    uint32_t pa = r.begin + s.i, pb = r.begin + n - 1;
+    if (pa == pb) return;
+    float ax = x[pa], ay = y[pa], az = z[pa];
+    x[pa] = x[pb]; y[pa] = y[pb]; z[pa] = z[pb];
+    x[pb] = ax; y[pb] = ay; z[pb] = az;
+}
+
+int launch_hoare(Ctx &c, int32_t nCells)
+{
+    const unsigned gq = (unsigned)c.max_chunks;
+    hipLaunchKernelGGL(k_stopper_counts, dim3(gq + 1), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
+                       c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(), 0, c.chunk_cnt.p,
+                       (uint32_t)c.max_chunks);
+    ORB_HIP(hipGetLastError());
+    ORB_TRY(scan_exclusive_u64(c, c.chunk_cnt.p, c.chunk_pre.p, (size_t)c.max_chunks + 1));
+    hipLaunchKernelGGL(k_stopper_positions, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
+                       c.cell_first.p, c.chunk_cell.p, c.chunk_pre.p, c.x(), c.y(), c.z(), c.lpos.p,
+                       c.rpos.p);
+    ORB_HIP(hipGetLastError());
+    unsigned gc = (unsigned)((nCells + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_hoare_split, dim3(gc), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
+                       c.cell_first.p, c.chunk_pre.p, c.lpos.p, c.rpos.p, c.splits.p, c.ranges.p,
+                       c.status.p, (int64_t)c.cap);
+    ORB_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_hoare_swap, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
+                       c.cell_first.p, c.chunk_cell.p, c.splits.p, c.lpos.p, c.rpos.p, c.x(), c.y(),
+                       c.z());
+    ORB_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_hoare_final, dim3(gc), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
+                       c.splits.p, c.x(), c.y(), c.z());
+    ORB_HIP(hipGetLastError());
+    return ORB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// stable <= partition (ServicePartitionGPU semantics, deterministic order)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_stable_scatter(const CellRec *recs, int32_t nCells,
+                                                           const uint32_t *first,
+                                                           const uint32_t *chunk_cell,
+                                                           const uint64_t *chunk_pre,
+                                                           const float *x, const float *y,
+                                                           const float *z, float *ox, float *oy,
+                                                           float *oz)
+{
+    __shared__ float skey[kLdsKeys];
+    __shared__ uint32_t sm[4];
+    uint32_t q = blockIdx.x;
+    uint32_t cell, b, e;
+    if (!chunk_range(q, first, nCells, chunk_cell, recs, cell, b, e)) return;
+    const CellRec r = recs[cell];
+    const float *key = axis_ptr(x, y, z, r.axis);
+    const float cut = r.cut;
+    const int n = (int)(e - b);
+    for (int k = threadIdx.x; k < n; k += kBlock) skey[lds_idx(k)] = key[b + k];
+    __syncthreads();
+    const int t = threadIdx.x;
+    const int k0 = t * kPerThread;
+    uint32_t myl = 0, myr = 0;
+#pragma unroll
+    for (int j = 0; j < kPerThread; ++j) {
+        int k = k0 + j;
+        if (k < n) {
+            float v = skey[lds_idx(k)];
+            myl += v <= cut;
+            myr += v > cut;
+        }
+    }
+    uint32_t total;
+    uint32_t pre = block_exclusive_scan(myl | (myr << 16), sm, &total);
+    uint64_t base = chunk_pre[q] - chunk_pre[first[cell]];
+    uint64_t tot = chunk_pre[first[cell + 1]] - chunk_pre[first[cell]];
+    uint32_t nle = (uint32_t)(tot & 0xffffffffull);
+    uint32_t dl = r.begin + (uint32_t)(base & 0xffffffffull) + (pre & 0xffffu);
+    uint32_t dr = r.begin + nle + (uint32_t)(base >> 32) + (pre >> 16);
+#pragma unroll
+    for (int j = 0; j < kPerThread; ++j) {
+        int k = k0 + j;
+        if (k < n) {
+            uint32_t p = b + (uint32_t)k;
+            float v = skey[lds_idx(k)];
+            uint32_t d = v <= cut ? dl++ : dr++;
+            ox[d] = x[p];
+            oy[d] = y[p];
+            oz[d] = z[p];
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy_back(const CellRec *recs, int32_t nCells,
+                                                      const uint32_t *first,
+                                                      const uint32_t *chunk_cell, const float *ox,
+                                                      const float *oy, const float *oz, float *x,
+                                                      float *y, float *z)
+{
+    uint32_t cell, b, e;
+    if (!chunk_range(blockIdx.x, first, nCells, chunk_cell, recs, cell, b, e)) return;
+    for (uint32_t p = b + threadIdx.x; p < e; p += kBlock) {
+        x[p] = ox[p];
+        y[p] = oy[p];
+        z[p] = oz[p];
+    }
+}
+
+__global__ void k_stable_ranges(const CellRec *recs, int32_t nCells, const uint32_t *first,
+                                const uint64_t *chunk_pre, SplitRec *splits, uint32_t *ranges,
+                                int32_t *status, int64_t cap)
+{
+    int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nCells) return;
+    const CellRec r = recs[c];
+    uint64_t tot = chunk_pre[first[c + 1]] - chunk_pre[first[c]];
+    uint32_t nle = (uint32_t)(tot & 0xffffffffull);
+    int64_t lid = left_child_id(r.id), rid = right_child_id(r.id);
+    if (r.axis < 0 || r.axis > 2 || rid >= cap) {
+        report_error(status, r.id);
+        return;
+    }
+    splits[c] = SplitRec{nle, (uint32_t)(tot >> 32), 0u, nle};
+    ranges[2 * lid] = r.begin;
+    ranges[2 * lid + 1] = r.begin + nle;
+    ranges[2 * rid] = r.begin + nle;
+    ranges[2 * rid + 1] = r.end;
+}
+
+int launch_stable_le(Ctx &c, int32_t nCells)
+{
+    ORB_TRY(c.tmp.alloc(3 * (size_t)c.stride));
+    const unsigned gq = (unsigned)c.max_chunks;
+    hipLaunchKernelGGL(k_stopper_counts, dim3(gq + 1), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
+                       c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(), 1, c.chunk_cnt.p,
+                       (uint32_t)c.max_chunks);
+    ORB_HIP(hipGetLastError());
+    ORB_TRY(scan_exclusive_u64(c, c.chunk_cnt.p, c.chunk_pre.p, (size_t)c.max_chunks + 1));
+    float *ox = c.tmp.p, *oy = c.tmp.p + c.stride, *oz = c.tmp.p + 2 * c.stride;
+    hipLaunchKernelGGL(k_stable_scatter, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
+                       c.cell_first.p, c.chunk_cell.p, c.chunk_pre.p, c.x(), c.y(), c.z(), ox, oy,
+                       oz);
+    ORB_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_copy_back, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
+                       c.cell_first.p, c.chunk_cell.p, ox, oy, oz, c.x(), c.y(), c.z());
+    ORB_HIP(hipGetLastError());
+    unsigned gc = (unsigned)((nCells + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_stable_ranges, dim3(gc), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
+                       c.cell_first.p, c.chunk_pre.p, c.splits.p, c.ranges.p, c.status.p,
+                       (int64_t)c.cap);
+    ORB_HIP(hipGetLastError());
+    return ORB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// ServiceMakeAxis gather and ServiceCount sizes
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_make_axis(const CellRec *recs, int32_t nCells,
+                                                      const uint32_t *first,
+                                                      const uint32_t *chunk_cell, const float *x,
+                                                      const float *y, const float *z, float *axisT)
+{
+    uint32_t cell, b, e;
+    if (!chunk_range(blockIdx.x, first, nCells, chunk_cell, recs, cell, b, e)) return;
+    const CellRec r = recs[cell];
+    if (r.axis < 0 || r.axis > 2) return;
+    const float *key = axis_ptr(x, y, z, r.axis);
+    for (uint32_t p = b + threadIdx.x; p < e; p += kBlock) axisT[p] = key[p];
+}
+
+int launch_make_axis(Ctx &c, int32_t nCells)
+{
+    ORB_TRY(c.axisT.alloc((size_t)c.stride));
+    hipLaunchKernelGGL(k_make_axis, dim3((unsigned)c.max_chunks), dim3(kBlock), 0, c.stream,
+                       c.recs.p, nCells, c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(),
+                       c.axisT.p);
+    ORB_HIP(hipGetLastError());
+    return ORB_OK;
+}
+
+__global__ void k_cell_sizes(const CellRec *recs, int32_t nCells, uint32_t *out)
+{
+    int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < nCells) out[c] = recs[c].end - recs[c].begin;
+}
+
+int launch_cell_sizes(Ctx &c, int32_t nCells, uint32_t *dst)
+{
+    unsigned gc = (unsigned)((nCells + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_cell_sizes, dim3(gc), dim3(kBlock), 0, c.stream, c.recs.p, nCells, dst);
+    ORB_HIP(hipGetLastError());
+    return ORB_OK;
+}
+
+}  // namespace orb

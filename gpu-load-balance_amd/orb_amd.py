@@ -1,0 +1,290 @@
+"""ctypes binding of liborb_hip.so (the C-ABI declared in include/orb.h).
+
+This is the Python face of the MI355X ORB build used by the tests and the
+benchmark.  It loads the in-tree HIP library and raises if it is missing or
+no GPU is usable: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liborb_hip.so")
+
+OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_UB, ERR_STATE, ERR_NODEV, ERR_BUSY = 0, -1, -2, -3, -4, -5, -6, -7
+LEVELS_REFERENCE, LEVELS_FULL = 0, 1
+CMP_LT, CMP_LE = 0, 1
+PART_HOARE, PART_STABLE_LE = 0, 1
+FORM_PLAIN, FORM_BATCHED = 0, 1
+MAX_PASSES = 32
+
+# enum pst_service (src/services/pst.h:66-81)
+(PST_SRV_STOP, PST_SETADD, PST_INIT, PST_INITGPU, PST_COPYPARTICLES, PST_COPYCELLS,
+ PST_COUNTLEFTGPU, PST_COUNTLEFTAXISGPU, PST_COUNTLEFT, PST_PARTITION, PST_PARTITIONGPU,
+ PST_COUNT, PST_FINALIZE, PST_MAKEAXIS) = range(14)
+
+
+class Cell(C.Structure):
+    """struct Cell (src/cell.h:9-17), 52 bytes."""
+    _fields_ = [
+        ("id", C.c_int32), ("nLeafCells", C.c_int32), ("prevCutAxis", C.c_int32),
+        ("cutAxis", C.c_int32), ("foundCut", C.c_bool), ("cutMarginLeft", C.c_float),
+        ("cutMarginRight", C.c_float), ("lower", C.c_float * 3), ("upper", C.c_float * 3),
+    ]
+
+
+assert C.sizeof(Cell) == 52
+
+CELL_DTYPE = np.dtype({
+    "names": ["id", "nLeafCells", "prevCutAxis", "cutAxis", "foundCut",
+              "cutMarginLeft", "cutMarginRight", "lower", "upper"],
+    "formats": ["<i4", "<i4", "<i4", "<i4", "u1", "<f4", "<f4", ("<f4", 3), ("<f4", 3)],
+    "offsets": [0, 4, 8, 12, 16, 20, 24, 28, 40],
+    "itemsize": 52,
+})
+
+
+class MetaParams(C.Structure):
+    """struct META_PARAMS (src/constants.h:13-17)."""
+    _fields_ = [("GPU_COUNT", C.c_bool), ("GPU_PARTITION", C.c_bool), ("FAST_MEDIAN", C.c_bool)]
+
+
+class BuildInfo(C.Structure):
+    _fields_ = [("nlevels", C.c_int32), ("total_passes", C.c_int32),
+                ("level_passes", C.c_int32 * 64), ("error_cell", C.c_int32), ("pad", C.c_int32),
+                ("ms_total", C.c_double)]
+
+
+class Timing(C.Structure):
+    _fields_ = [("count_ms", C.c_double), ("partition_ms", C.c_double), ("other_ms", C.c_double),
+                ("count_bytes", C.c_int64), ("count_launches", C.c_int32), ("pad", C.c_int32)]
+
+
+class OrbError(RuntimeError):
+    def __init__(self, code, what=""):
+        self.code = code
+        msg = lib().orb_error_string(code).decode() if _lib else str(code)
+        super().__init__(f"{what}: {msg} ({code})")
+
+
+_lib = None
+
+# exported symbols and their prototypes (restype, argtypes)
+_vp, _i32, _i64, _b = C.c_void_p, C.c_int32, C.c_int64, C.c_bool
+PROTOTYPES = {
+    "orb_init": (C.c_int, [C.POINTER(_vp), _i64, _i32, _b, MetaParams]),
+    "orb_finalize": (C.c_int, [_vp]),
+    "orb_set_semantics": (C.c_int, [_vp, _i32, _i32]),
+    "orb_set_particles": (C.c_int, [_vp, _vp, _vp, _vp, _i64]),
+    "orb_get_particles": (C.c_int, [_vp, _vp, _vp, _vp, _i64]),
+    "orb_reset_ranges": (C.c_int, [_vp]),
+    "orb_save_particles": (C.c_int, [_vp]),
+    "orb_restore_particles": (C.c_int, [_vp]),
+    "orb_copy_particles": (C.c_int, [_vp]),
+    "orb_copy_cells": (C.c_int, [_vp, _vp, _i32]),
+    "orb_count": (C.c_int, [_vp, _vp, _i32, _vp]),
+    "orb_make_axis": (C.c_int, [_vp, _vp, _i32]),
+    "orb_count_left": (C.c_int, [_vp, _vp, _i32, _vp]),
+    "orb_count_left_gpu_axis": (C.c_int, [_vp, _vp, _i32, _vp]),
+    "orb_count_left_gpu": (C.c_int, [_vp, _vp, _i32, _vp]),
+    "orb_partition": (C.c_int, [_vp, _vp, _i32]),
+    "orb_partition_gpu": (C.c_int, [_vp, _vp, _i32]),
+    "orb_run_service": (C.c_int, [_vp, _i32, _vp, _i32, _vp, _i32]),
+    "orb_heap_capacity": (C.c_int64, [_i32]),
+    "orb_build": (C.c_int, [_vp, _i32, _i32, C.POINTER(BuildInfo)]),
+    "orb_get_heap": (C.c_int, [_vp, _vp, _i64]),
+    "orb_get_ranges": (C.c_int, [_vp, _vp, _i64]),
+    "orb_get_exists": (C.c_int, [_vp, _vp, _i64]),
+    "orb_enable_trace": (C.c_int, [_vp, _b]),
+    "orb_get_pass_counts": (C.c_int, [_vp, _vp, _i64]),
+    "orb_get_timing": (C.c_int, [_vp, C.POINTER(Timing)]),
+    "orb_enable_timing": (C.c_int, [_vp, _b]),
+    "orb_error_string": (C.c_char_p, [C.c_int]),
+    "orb_last_hip_error": (C.c_int, []),
+    "orb_device_count": (C.c_int, []),
+}
+
+
+def lib():
+    """Loads liborb_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP library missing: {LIB_PATH} (run make -C gpu-load-balance_amd)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in PROTOTYPES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _check(rc, what):
+    if rc < 0:
+        raise OrbError(rc, what)
+    return rc
+
+
+def cells_array(cells):
+    """Converts a numpy CELL_DTYPE array (or list of Cell) to a contiguous array."""
+    if isinstance(cells, np.ndarray):
+        return np.ascontiguousarray(cells, dtype=CELL_DTYPE)
+    a = np.zeros(len(cells), CELL_DTYPE)
+    for i, c in enumerate(cells):
+        a[i] = (c.id, c.nLeafCells, c.prevCutAxis, c.cutAxis, c.foundCut, c.cutMarginLeft,
+                c.cutMarginRight, tuple(c.lower), tuple(c.upper))
+    return a
+
+
+class Context:
+    """One device-resident particle set (ServiceInit) and its ORB state."""
+
+    def __init__(self, n, d, generate=True, params=(False, False, False)):
+        L = lib()
+        if L.orb_device_count() < 1:
+            raise OrbError(ERR_NODEV, "orb_init")
+        h = C.c_void_p()
+        _check(L.orb_init(C.byref(h), int(n), int(d), bool(generate), MetaParams(*params)), "orb_init")
+        self.h = h
+        self.n = int(n)
+        self.d = int(d)
+        self.cap = int(L.orb_heap_capacity(int(d)))
+
+    def close(self):
+        if self.h:
+            lib().orb_finalize(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- particles / semantics
+    def set_semantics(self, cmp=-1, partition=-1):
+        _check(lib().orb_set_semantics(self.h, cmp, partition), "orb_set_semantics")
+
+    def set_particles(self, x, y, z):
+        x, y, z = (np.ascontiguousarray(a, np.float32) for a in (x, y, z))
+        _check(lib().orb_set_particles(self.h, _p(x), _p(y), _p(z), self.n), "orb_set_particles")
+
+    def get_particles(self):
+        x = np.empty(self.n, np.float32)
+        y = np.empty(self.n, np.float32)
+        z = np.empty(self.n, np.float32)
+        _check(lib().orb_get_particles(self.h, _p(x), _p(y), _p(z), self.n), "orb_get_particles")
+        return x, y, z
+
+    def save_particles(self):
+        _check(lib().orb_save_particles(self.h), "orb_save_particles")
+
+    def restore_particles(self):
+        _check(lib().orb_restore_particles(self.h), "orb_restore_particles")
+
+    def reset_ranges(self):
+        _check(lib().orb_reset_ranges(self.h), "orb_reset_ranges")
+
+    # ---- services (Cell[] in, uint32[] out)
+    def _cells_call(self, fn, cells, with_out, out=None):
+        a = cells_array(cells)
+        if with_out:
+            if out is None:
+                out = np.zeros(len(a), np.uint32)
+            rc = fn(self.h, _p(a), len(a), _p(out))
+            _check(rc, fn.__name__)
+            return out
+        return _check(fn(self.h, _p(a), len(a)), fn.__name__)
+
+    def copy_particles(self):
+        return _check(lib().orb_copy_particles(self.h), "orb_copy_particles")
+
+    def copy_cells(self, cells):
+        return self._cells_call(lib().orb_copy_cells, cells, False)
+
+    def count(self, cells):
+        return self._cells_call(lib().orb_count, cells, True)
+
+    def make_axis(self, cells):
+        return self._cells_call(lib().orb_make_axis, cells, False)
+
+    def count_left(self, cells, out=None):
+        return self._cells_call(lib().orb_count_left, cells, True, out)
+
+    def count_left_gpu_axis(self, cells):
+        return self._cells_call(lib().orb_count_left_gpu_axis, cells, True)
+
+    def count_left_gpu(self, cells):
+        return self._cells_call(lib().orb_count_left_gpu, cells, True)
+
+    def partition(self, cells):
+        return self._cells_call(lib().orb_partition, cells, False)
+
+    def partition_gpu(self, cells):
+        return self._cells_call(lib().orb_partition_gpu, cells, False)
+
+    def run_service(self, sid, cells=None, n_out=0):
+        a = cells_array(cells if cells is not None else [])
+        out = np.zeros(max(n_out, 1), np.uint32)
+        rc = lib().orb_run_service(self.h, sid, _p(a), a.nbytes, _p(out), 4 * n_out)
+        _check(rc, "orb_run_service")
+        return rc, out[:n_out]
+
+    # ---- whole build
+    def build(self, form=FORM_PLAIN, levels_mode=LEVELS_REFERENCE):
+        info = BuildInfo()
+        rc = lib().orb_build(self.h, form, levels_mode, C.byref(info))
+        if rc < 0:
+            err = OrbError(rc, "orb_build")
+            err.error_cell = info.error_cell
+            raise err
+        return {
+            "nlevels": info.nlevels,
+            "total_passes": info.total_passes,
+            "level_passes": list(info.level_passes[:info.nlevels]),
+            "ms_total": info.ms_total,
+        }
+
+    def heap(self):
+        h = np.zeros(self.cap, CELL_DTYPE)
+        _check(lib().orb_get_heap(self.h, _p(h), self.cap), "orb_get_heap")
+        return h
+
+    def ranges(self):
+        r = np.zeros((self.cap, 2), np.uint32)
+        _check(lib().orb_get_ranges(self.h, _p(r), self.cap), "orb_get_ranges")
+        return r
+
+    def exists(self):
+        e = np.zeros(self.cap, np.uint8)
+        _check(lib().orb_get_exists(self.h, _p(e), self.cap), "orb_get_exists")
+        return e
+
+    def enable_trace(self, on=True):
+        _check(lib().orb_enable_trace(self.h, bool(on)), "orb_enable_trace")
+
+    def pass_counts(self):
+        pc = np.zeros((self.cap, MAX_PASSES), np.uint32)
+        _check(lib().orb_get_pass_counts(self.h, _p(pc), self.cap), "orb_get_pass_counts")
+        return pc
+
+    def enable_timing(self, on=True):
+        _check(lib().orb_enable_timing(self.h, bool(on)), "orb_enable_timing")
+
+    def timing(self):
+        t = Timing()
+        _check(lib().orb_get_timing(self.h, C.byref(t)), "orb_get_timing")
+        return {"count_ms": t.count_ms, "partition_ms": t.partition_ms, "other_ms": t.other_ms,
+                "count_bytes": t.count_bytes, "count_launches": t.count_launches}

@@ -1,0 +1,244 @@
+/*
+ * orb.h -- C-ABI of the MI355X-native ORB (orthogonal recursive bisection)
+ * build: liborb_hip.so, HIP kernels for gfx950.
+ *
+ * The entry points mirror the reference's service boundary: each service in
+ * the reference is an mdl::BasicService whose
+ *     int Service(PST pst, void *vin, int nIn, void *vout, int nOut)
+ * takes Cell[nCells] and returns the number of bytes written into vout
+ * (src/services/TraversePST.h:24-45, src/services/pst.h:66-81).  Here a
+ * context replaces the PST leaf's LocalData (src/services/pst.h:6-41): it
+ * owns the device-resident particles, the cell-to-range map and the scratch
+ * buffers.  One context drives one GPU share; there are no ranks.
+ *
+ * Conventions
+ *  - Cell and output buffers are caller-owned host memory.
+ *  - Every entry point returns >= 0 on success (a byte count for services,
+ *    0 otherwise) and a negative ORB_ERR_* code on failure; nothing calls
+ *    exit() (the reference's CUDA_CHECK did, src/constants.h:19-23).
+ *  - One context per host thread; contexts share no state.
+ *  - Particles are float32, axis-major SoA (x[n], y[n], z[n]), as blitz's
+ *    storage ordering (0,1) lays out particles(N,3) in
+ *    src/services/init.cu:32-45.
+ */
+#ifndef ORB_H
+#define ORB_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* struct Cell (src/cell.h:9-17): same fields, order, padding; 52 bytes. */
+typedef struct orb_cell {
+    int32_t id;
+    int32_t nLeafCells;
+    int32_t prevCutAxis;
+    int32_t cutAxis;
+    bool foundCut;
+    float cutMarginLeft;
+    float cutMarginRight;
+    float lower[3];
+    float upper[3];
+} orb_cell;
+
+/* struct META_PARAMS (src/constants.h:13-17).  o=0: all false; o=1:
+ * GPU_COUNT; o=2: GPU_COUNT + GPU_PARTITION (src/orbit.cpp:48-66).       */
+typedef struct orb_meta_params {
+    bool GPU_COUNT;
+    bool GPU_PARTITION;
+    bool FAST_MEDIAN;
+} orb_meta_params;
+
+/* error codes */
+enum {
+    ORB_OK = 0,
+    ORB_ERR_ARG = -1,     /* invalid argument / size                          */
+    ORB_ERR_HIP = -2,     /* a HIP runtime call failed (orb_last_hip_error)   */
+    ORB_ERR_NOMEM = -3,   /* device or host allocation failed                 */
+    ORB_ERR_UB = -4,      /* input on which the reference is undefined: an
+                             empty cell, a cell with no key >= cut (Hoare runs
+                             outside the cell, partition.cpp:38-52), or a child
+                             with no positive extent (cell.h:102-126)          */
+    ORB_ERR_STATE = -5,   /* call out of sequence                             */
+    ORB_ERR_NODEV = -6,   /* no GPU / HIP extension not usable                */
+    ORB_ERR_BUSY = -7,    /* serving: not admitted (memory or slots)          */
+};
+
+/* service ids: enum pst_service (src/services/pst.h:66-81) */
+enum {
+    ORB_PST_SRV_STOP = 0,
+    ORB_PST_SETADD,
+    ORB_PST_INIT,
+    ORB_PST_INITGPU,
+    ORB_PST_COPYPARTICLES,
+    ORB_PST_COPYCELLS,
+    ORB_PST_COUNTLEFTGPU,
+    ORB_PST_COUNTLEFTAXISGPU,
+    ORB_PST_COUNTLEFT,
+    ORB_PST_PARTITION,
+    ORB_PST_PARTITIONGPU,
+    ORB_PST_COUNT,
+    ORB_PST_FINALIZE,
+    ORB_PST_MAKEAXIS,
+};
+
+/* level schedules */
+enum {
+    ORB_LEVELS_REFERENCE = 0, /* l = 1 .. ceil(log2 d)-1 and the reference's
+                                 heap slices (src/orbit.cpp:102-111): d = 2^k
+                                 ends with d/2 particle ranges                 */
+    ORB_LEVELS_FULL = 1,      /* split every cell with nLeafCells >= 2 until
+                                 each has one leaf: d ranges for any d         */
+};
+
+/* comparison used by the count-left passes */
+enum { ORB_CMP_LT = 0,  /* x <  cut: ServiceCountLeft (countLeft.cpp:35)       */
+       ORB_CMP_LE = 1   /* x <= cut: the GPU services (countLeftGPUAxis.cu:160) */ };
+
+/* partition used after each level */
+enum { ORB_PART_HOARE = 0,      /* in-place Hoare, partition.cpp:28-60: exact order */
+       ORB_PART_STABLE_LE = 1   /* keys <= cut first, stable (set semantics of
+                                   partitionGPU.cu:58-231, deterministic order) */ };
+
+/* build forms */
+enum { ORB_FORM_PLAIN = 0,    /* host-driven: one count launch per bisection
+                                 pass per level, counts read back each pass   */
+       ORB_FORM_BATCHED = 1   /* device-driven: all cells of a level bisected
+                                 together, multi-step replay, no host round
+                                 trips inside the build                       */ };
+
+#define ORB_MAX_PASSES 32
+
+typedef struct orb_ctx orb_ctx;
+
+/* ------------------------------------------------------------------ */
+/* lifecycle                                                           */
+/* ------------------------------------------------------------------ */
+
+/* ServiceInit::Service (src/services/init.cu:27-144).  Allocates the
+ * device-resident particle set for nParticles (one GPU = one rank's share)
+ * and the cell-to-range map for d leaves, sets range(0) = [0, n).  With
+ * generate=true fills x, y, z with the reference generator (xorshf96 with
+ * the seeds of init.cu:11, i-major draw order, init.cu:47-53).  The
+ * comparison and partition follow params as in orbit.cpp:48-66.         */
+int orb_init(orb_ctx **ctx, int64_t nParticles, int32_t d, bool generate,
+             orb_meta_params params);
+
+/* ServiceFinalize::Service (src/services/finalize.cu:10-48). */
+int orb_finalize(orb_ctx *ctx);
+
+/* Override the comparison / partition chosen from META_PARAMS
+ * (ORB_CMP_*, ORB_PART_*); -1 keeps the current value.                   */
+int orb_set_semantics(orb_ctx *ctx, int32_t cmp, int32_t partition);
+
+/* The generate=false input path (init.cu:54-59 loads a Tipsy file):
+ * copies host SoA arrays of ctx's particle count to the device and resets
+ * the range map to range(0) = [0, n).                                    */
+int orb_set_particles(orb_ctx *ctx, const float *x, const float *y, const float *z, int64_t n);
+/* Copies the device particles (current order) back to host arrays.       */
+int orb_get_particles(orb_ctx *ctx, float *x, float *y, float *z, int64_t n);
+/* Resets the range map to range(0) = [0, n) (particles keep their order). */
+int orb_reset_ranges(orb_ctx *ctx);
+/* Device-side snapshot of the particle order, and its restore (which also
+ * resets the range map): repeated builds from the same input without a
+ * host round trip.  No counterpart in the reference (its particles are
+ * generated once per run, init.cu:47-53).                                */
+int orb_save_particles(orb_ctx *ctx);
+int orb_restore_particles(orb_ctx *ctx);
+
+/* ------------------------------------------------------------------ */
+/* services: Cell[nCells] in, bytes written (>= 0) or ORB_ERR_* out     */
+/* ------------------------------------------------------------------ */
+
+/* ServiceCopyParticles (copyParticles.cu:10-60): particles already live on
+ * the device; kept so the reference's service sequence runs unchanged.   */
+int orb_copy_particles(orb_ctx *ctx);
+/* ServiceCopyCells (copyCells.cu:10-64): builds the level's chunk table on
+ * the device from the cells' ranges (no host block table).               */
+int orb_copy_cells(orb_ctx *ctx, const orb_cell *cells, int32_t nCells);
+/* ServiceCount (count.cpp:8-19): out[i] = end - begin of cells[i].id.    */
+int orb_count(orb_ctx *ctx, const orb_cell *cells, int32_t nCells, uint32_t *out);
+/* ServiceMakeAxis (makeAxis.cpp:9-33): gathers each cell's cut-axis
+ * column into the device particlesT buffer over [begin, end) exactly (the
+ * reference's inclusive blitz Range copies one element past the cell).   */
+int orb_make_axis(orb_ctx *ctx, const orb_cell *cells, int32_t nCells);
+/* ServiceCountLeft (countLeft.cpp:9-42): out[i] = #{x < cut} over the
+ * cell's cut-axis column, found cells skipped (their out[i] untouched).  */
+int orb_count_left(orb_ctx *ctx, const orb_cell *cells, int32_t nCells, uint32_t *out);
+/* ServiceCountLeftGPUAxis (countLeftGPUAxis.cu:188-259): out[i] =
+ * #{particlesT <= cut} for every cell (particlesT from orb_make_axis).   */
+int orb_count_left_gpu_axis(orb_ctx *ctx, const orb_cell *cells, int32_t nCells, uint32_t *out);
+/* ServiceCountLeftGPU (countLeftGPU.cu:80-162): out[i] = #{x_axis <= cut}
+ * on the SoA arrays with each cell's own axis, every cell.               */
+int orb_count_left_gpu(orb_ctx *ctx, const orb_cell *cells, int32_t nCells, uint32_t *out);
+/* ServicePartition (partition.cpp:18-65): Hoare partition of every cell at
+ * getCut() on cutAxis, moving x, y, z together; writes the children's
+ * ranges [begin, i) and [i, end).  Reproduces the serial loop's exact
+ * permutation (two scans + disjoint transpositions + the final swap).    */
+int orb_partition(orb_ctx *ctx, const orb_cell *cells, int32_t nCells);
+/* ServicePartitionGPU (partitionGPU.cu:282-600): keys <= cut first, then
+ * keys > cut, each in input order; child ranges from the count at the
+ * cut the partition uses.                                                */
+int orb_partition_gpu(orb_ctx *ctx, const orb_cell *cells, int32_t nCells);
+
+/* mdl->RunService(id, nIn, vin, vout) for the services above (the ids of
+ * src/services/pst.h:66-81); vin = Cell[nIn/52] except for COPYPARTICLES
+ * and FINALIZE (ignored).  Returns the service's byte count.             */
+int orb_run_service(orb_ctx *ctx, int32_t service_id, const void *vin, int32_t nIn,
+                    void *vout, int32_t nOut);
+
+/* ------------------------------------------------------------------ */
+/* whole build (the level loop of src/orbit.cpp:74-275)                 */
+/* ------------------------------------------------------------------ */
+
+typedef struct orb_build_info {
+    int32_t nlevels;          /* levels processed                          */
+    int32_t total_passes;     /* bisection passes summed over levels       */
+    int32_t level_passes[64]; /* passes per level (<= 32 each)             */
+    int32_t error_cell;       /* heap id that raised ORB_ERR_UB, else -1   */
+    int32_t pad;
+    double ms_total;          /* host wall time of the build               */
+} orb_build_info;
+
+/* Number of heap slots for d leaves: max(2d-1, 2^(ceil(log2 d)+1)-1).    */
+int64_t orb_heap_capacity(int32_t d);
+/* Runs the whole build on the device-resident particles, starting from
+ * range(0) = [0, n) and the root cell [-0.5, 0.5]^3 with cutAxis 0.       */
+int orb_build(orb_ctx *ctx, int32_t form, int32_t levels_mode, orb_build_info *info);
+/* Results of the last build: the cell heap (by heap id, unused slots
+ * zero), the cell-to-range map ([cap][2]), and which ids exist.          */
+int orb_get_heap(orb_ctx *ctx, orb_cell *heap, int64_t cap);
+int orb_get_ranges(orb_ctx *ctx, uint32_t *ranges, int64_t cap);
+int orb_get_exists(orb_ctx *ctx, uint8_t *exists, int64_t cap);
+/* Trace of per-pass left counts: [cap][32], UINT32_MAX where a cell was
+ * not counted.  Enable before orb_build; costs host memory and copies.   */
+int orb_enable_trace(orb_ctx *ctx, bool on);
+int orb_get_pass_counts(orb_ctx *ctx, uint32_t *counts, int64_t cap);
+
+/* Kernel timing of the last build (device events), milliseconds.         */
+typedef struct orb_timing {
+    double count_ms;       /* bisection count/histogram kernels            */
+    double partition_ms;   /* partition kernels                            */
+    double other_ms;       /* tables, controller, scans                    */
+    int64_t count_bytes;   /* algorithmic bytes read by count kernels      */
+    int32_t count_launches;
+    int32_t pad;
+} orb_timing;
+int orb_get_timing(orb_ctx *ctx, orb_timing *t);
+int orb_enable_timing(orb_ctx *ctx, bool on);
+
+/* ------------------------------------------------------------------ */
+/* errors / device                                                     */
+/* ------------------------------------------------------------------ */
+const char *orb_error_string(int code);
+int orb_last_hip_error(void);
+int orb_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ORB_H */

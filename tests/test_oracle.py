@@ -1,0 +1,437 @@
+"""Pins the CPU oracle (oracle/orb_oracle.c) against sources independent of it.
+
+- generator: the uint64 -> float32 -> double chain against exact rational
+  rounding; the first draws against bounds that follow from the seeds and the
+  64-bit word size; uniform moments.
+- Cell rules: the reference's own src/cell.h (oracle/_ref, and the committed
+  fixture it wrote).
+- termination rule: numpy's IEEE float32 arithmetic.
+- Hoare: the serial loop against the two-scan closed form, invariants, and the
+  thesis worked example.
+- build: an independent numpy re-simulation of each cell's bisection, range
+  tiling and partition invariants, and the level/pass counts of `orbit 20 10 0`.
+"""
+import ctypes as C
+import json
+import math
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ORACLE
+
+F32 = np.float32
+
+
+def bits(v):
+    return int(F32(v).view(np.uint32))
+
+
+def from_bits(b):
+    return np.uint32(b).view(np.float32)
+
+
+# ---------------------------------------------------------------------------
+# generator (src/services/init.cu:11-25)
+# ---------------------------------------------------------------------------
+def rne_float32_of_uint(z):
+    """Exact round-to-nearest-even of a non-negative integer to 24 bits."""
+    if z == 0:
+        return 0
+    e = z.bit_length()
+    if e <= 24:
+        return z
+    sh = e - 24
+    q, r = divmod(z, 1 << sh)
+    half = 1 << (sh - 1)
+    if r > half or (r == half and (q & 1)):
+        q += 1
+    return q << sh
+
+
+def expected_value(z):
+    f = rne_float32_of_uint(z)                       # (float)z
+    q = Fraction(f, 1 << 64)                         # / (float)ULONG_MAX == 2^64, exact
+    d = float(q - Fraction(1, 2))                    # double subtraction, correctly rounded
+    return F32(d)                                    # narrowing to float
+
+
+def test_rng_value_rounding_chain(oracle):
+    L = oracle.lib()
+    rng = np.random.default_rng(1)
+    zs = [0, 1, 2, (1 << 24) - 1, (1 << 24) + 1, (1 << 25) + 3, (1 << 63), (1 << 63) - 1,
+          (1 << 64) - 1, (1 << 64) - (1 << 39), (1 << 64) - (1 << 40) + 1]
+    # halfway cases at several exponents
+    for e in (30, 40, 50, 60, 63):
+        base = 1 << e
+        ulp = 1 << (e - 23)
+        zs += [base + ulp // 2, base + ulp + ulp // 2, base + ulp // 2 + 1, base + ulp // 2 - 1]
+    zs += [int(v) for v in rng.integers(0, 2**63, 3000, dtype=np.uint64)]
+    zs += [int(v) | (1 << 63) for v in rng.integers(0, 2**63, 3000, dtype=np.uint64)]
+    for z in zs:
+        got = F32(L.orc_rng_value(C.c_uint64(z)))
+        assert bits(got) == bits(expected_value(z)), z
+
+
+def test_rng_first_draws_bounds(oracle):
+    # seeds < 2^30; after x ^= x<<16, x ^= x>>5, x ^= x<<1 a word below 2^30
+    # stays below 2^47, so the first three z values are < 2^47 and the draws
+    # lie within 2^-17 of -0.5 (a 32-bit word or another divisor would not).
+    x, y, z = oracle.generate(4)
+    for v in (x[0], y[0], z[0]):
+        assert -0.5 <= float(v) <= -0.5 + 2.0 ** -17
+    # the fourth draw starts from a word below 2^47: z4 < 2^64 and no bound
+    # pins it; the fifth onward are full range.
+    assert float(y[1]) > -0.5 + 2.0 ** -10
+
+
+def test_rng_uniform_moments(oracle):
+    n = 1 << 20
+    x, y, z = oracle.generate(n)
+    for a in (x, y, z):
+        a = a.astype(np.float64)
+        assert a.min() >= -0.5 and a.max() <= 0.5
+        assert abs(a.mean()) < 5 * math.sqrt(1 / 12 / n)
+        assert abs(a.var() - 1 / 12) < 5e-4
+        h, _ = np.histogram(a, bins=64, range=(-0.5, 0.5))
+        assert h.min() > 0.95 * n / 64 and h.max() < 1.05 * n / 64
+    # draws are i-major: x[i], y[i], z[i] are consecutive outputs, so the
+    # three axes are uncorrelated
+    assert abs(np.corrcoef(x, y)[0, 1]) < 0.01
+    assert abs(np.corrcoef(x[1:], x[:-1])[0, 1]) < 0.01
+
+
+# ---------------------------------------------------------------------------
+# Cell rules against the reference's own cell.h
+# ---------------------------------------------------------------------------
+def _cell_from_json(oracle, j):
+    c = oracle.Cell()
+    c.id, c.nLeafCells, c.prevCutAxis, c.cutAxis = j["id"], j["nLeafCells"], j["prevCutAxis"], j["cutAxis"]
+    c.foundCut = j["foundCut"]
+    c.cutMarginLeft = from_bits(j["cutMarginLeft"])
+    c.cutMarginRight = from_bits(j["cutMarginRight"])
+    for k in range(3):
+        c.lower[k] = from_bits(j["lower"][k])
+        c.upper[k] = from_bits(j["upper"][k])
+    return c
+
+
+def _cell_json(c):
+    return {"id": c.id, "nLeafCells": c.nLeafCells, "prevCutAxis": c.prevCutAxis,
+            "cutAxis": c.cutAxis, "foundCut": bool(c.foundCut),
+            "cutMarginLeft": bits(c.cutMarginLeft), "cutMarginRight": bits(c.cutMarginRight),
+            "lower": [bits(v) for v in c.lower], "upper": [bits(v) for v in c.upper]}
+
+
+@pytest.fixture(scope="module")
+def cell_vectors():
+    with open(os.path.join(GOLDEN, "cell_vectors.json")) as f:
+        return json.load(f)
+
+
+def test_get_cut_matches_reference_fixture(oracle, cell_vectors):
+    L = oracle.lib()
+    for v in cell_vectors["get_cut"]:
+        c = oracle.Cell()
+        c.cutMarginLeft = from_bits(v["left"])
+        c.cutMarginRight = from_bits(v["right"])
+        assert bits(L.orc_get_cut(C.byref(c))) == v["cut"], v
+
+
+def test_split_matches_reference_fixture(oracle, cell_vectors):
+    L = oracle.lib()
+    assert len(cell_vectors["split"]) > 200
+    for v in cell_vectors["split"]:
+        p = _cell_from_json(oracle, v["parent"])
+        l, r = oracle.Cell(), oracle.Cell()
+        assert L.orc_split(C.byref(p), C.byref(l), C.byref(r)) == 0
+        assert _cell_json(l) == v["left"]
+        assert _cell_json(r) == v["right"]
+
+
+def test_level_math_matches_reference_fixture(oracle, cell_vectors):
+    L = oracle.lib()
+    for v in cell_vectors["levels"]:
+        d = v["d"]
+        assert L.orc_n_levels(d) == v["n_levels"]
+        assert L.orc_n_cells_on_last_level(d) == v["n_cells_on_last_level"]
+        assert 2 * d - 1 == v["total_cells"]
+        assert L.orc_left_child_id(d) == v["left_child"]
+        assert L.orc_right_child_id(d) == v["right_child"]
+
+
+REF_CELL = os.path.join(ORACLE, "_ref", "libref_cell.so")
+
+
+@pytest.mark.skipif(not os.path.exists(REF_CELL), reason="oracle/_ref not built (no reference tree)")
+def test_split_against_live_reference(oracle):
+    """Fresh random cells through the reference's cell.h and the oracle."""
+    ref = C.CDLL(REF_CELL)
+    fp = C.POINTER(C.c_float)
+    ref.ref_make.argtypes = [C.c_int, C.c_int, fp, fp, C.POINTER(oracle.Cell)]
+    ref.ref_split.argtypes = [C.POINTER(oracle.Cell)] * 3
+    ref.ref_get_cut.argtypes = [C.POINTER(oracle.Cell)]
+    ref.ref_get_cut.restype = C.c_float
+    L = oracle.lib()
+    rng = np.random.default_rng(7)
+    for k in range(3000):
+        lo = rng.uniform(-0.5, 0.4, 3).astype(np.float32)
+        hi = (lo + rng.choice([rng.uniform(1e-7, 0.6), 0.125, 0.25])).astype(np.float32)
+        p = oracle.Cell()
+        ref.ref_make(int(rng.integers(0, 1 << 20)), int(rng.integers(2, 1 << 16)),
+                     lo.ctypes.data_as(fp), hi.ctypes.data_as(fp), C.byref(p))
+        p.cutAxis = int(rng.integers(0, 3))
+        a = p.cutAxis
+        m = np.sort(rng.uniform(lo[a], hi[a], 2).astype(np.float32))
+        p.cutMarginLeft, p.cutMarginRight = m[0], m[1]
+        rl, rr, ol, orr = oracle.Cell(), oracle.Cell(), oracle.Cell(), oracle.Cell()
+        rc_ref = ref.ref_split(C.byref(p), C.byref(rl), C.byref(rr))
+        rc_orc = L.orc_split(C.byref(p), C.byref(ol), C.byref(orr))
+        assert (rc_ref == 0) == (rc_orc == 0)
+        if rc_ref == 0:
+            assert _cell_json(rl) == _cell_json(ol)
+            assert _cell_json(rr) == _cell_json(orr)
+        assert bits(ref.ref_get_cut(C.byref(p))) == bits(L.orc_get_cut(C.byref(p)))
+
+
+# ---------------------------------------------------------------------------
+# termination rule (src/orbit.cpp:204-208) against numpy float32 arithmetic
+# ---------------------------------------------------------------------------
+def numpy_difference(L, Cn, nleaf):
+    ratio = F32(math.ceil(nleaf / 2.0) / nleaf)     # IEEE double quotient, RNE to float
+    prod = F32(F32(Cn) * ratio)                      # float * float
+    diff = F32(F32(L) - prod)                        # float - float
+    return int(diff)                                 # truncation toward zero
+
+
+def test_difference_rule_matches_numpy_float32(oracle):
+    L = oracle.lib()
+    rng = np.random.default_rng(3)
+    leaves = list(range(1, 34)) + [1000, 1023, 1024, 1025, 4097, 1 << 20, 999983]
+    counts = [0, 1, 2, 3, 5, 7, 4095, (1 << 24) - 1, 1 << 24, (1 << 24) + 1, (1 << 24) + 3,
+              (1 << 25) + 5, (1 << 27) + 1, (1 << 31) - 1]
+    counts += [int(v) for v in rng.integers(0, 1 << 31, 200)]
+    for nl in leaves:
+        for cn in counts:
+            ls = {0, cn, cn // 2, cn // 2 + 1, max(cn // 2 - 1, 0), cn // 3, (2 * cn) // 3}
+            ls |= {int(v) for v in rng.integers(0, cn + 1, 6)}
+            for lv in ls:
+                assert L.orc_difference(lv, cn, nl) == numpy_difference(lv, cn, nl), (lv, cn, nl)
+
+
+def test_difference_rule_rounds_above_2p24(oracle):
+    # above 2^24 the float count loses bits: |diff| < 3 accepts a wider window
+    L = oracle.lib()
+    cn = (1 << 27) + 1
+    target = cn / 2
+    accepted = [lv for lv in range(int(target) - 40, int(target) + 40)
+                if abs(L.orc_difference(lv, cn, 2)) < 3]
+    assert len(accepted) > 5  # exact arithmetic would accept exactly 5
+
+
+# ---------------------------------------------------------------------------
+# Hoare (src/services/partition.cpp:28-60)
+# ---------------------------------------------------------------------------
+def test_hoare_serial_equals_closed_form(oracle):
+    rng = np.random.default_rng(11)
+    trials = 0
+    for _ in range(6000):
+        n = int(rng.integers(1, 41))
+        keys = (rng.integers(0, 7, n) / 6.0).astype(np.float32)
+        cut = F32(rng.integers(0, 13) / 12.0)
+        ids = np.arange(n, dtype=np.float32)
+        rc, split, kx, ky, kz = oracle.hoare(keys, ids, np.zeros(n, np.float32), 0, cut)
+        rc2, K, split2, perm = oracle.hoare_closed_form(keys, cut)
+        assert rc == rc2
+        if rc != 0:
+            assert not (keys >= cut).any()
+            continue
+        trials += 1
+        assert split == split2
+        assert np.array_equal(ky.astype(np.int64), perm)
+        assert np.array_equal(kx, keys[perm])
+    assert trials > 4000
+
+
+def test_hoare_invariants(oracle):
+    rng = np.random.default_rng(5)
+    for _ in range(300):
+        n = int(rng.integers(2, 3000))
+        keys = (rng.integers(0, 50, n) / 49.0 - 0.5).astype(np.float32)
+        cut = F32(rng.uniform(-0.5, 0.5))
+        if not (keys >= cut).any():
+            continue
+        y = rng.standard_normal(n).astype(np.float32)
+        z = rng.standard_normal(n).astype(np.float32)
+        rc, i, kx, ky, kz = oracle.hoare(keys, y, z, 0, cut)
+        assert rc == 0
+        assert (kx[:i] <= cut).all() and (kx[i:] >= cut).all()
+        assert (kx[:i] < cut).sum() + (kx[i:] < cut).sum() == (keys < cut).sum()
+        # rows move together: the (x, y, z) multiset is preserved
+        a = np.sort(np.stack([keys, y, z], 1).view("f4,f4,f4").ravel())
+        b = np.sort(np.stack([kx, ky, kz], 1).view("f4,f4,f4").ravel())
+        assert np.array_equal(a, b)
+
+
+def test_hoare_undefined_inputs_are_errors(oracle):
+    keys = np.array([0.1, 0.2, 0.3], np.float32)
+    rc, *_ = oracle.hoare(keys, keys, keys, 0, 0.5)
+    assert rc == oracle.ERR_UB       # every key < cut: the loop leaves the cell
+    rc, *_ = oracle.hoare(keys, keys, keys, 1, 1)
+    assert rc == oracle.ERR_UB       # empty cell
+    rc, i, *_ = oracle.hoare(keys, keys, keys, 0, 0.05)
+    assert rc == 0 and i == 0        # every key > cut: defined, i = begin
+
+
+def test_thesis_worked_example(oracle):
+    """documentation/particles10-12.csv: the thesis's hand-cut example.
+
+    The thesis orders come from the textbook loop without the final
+    swap(i, end-1); undoing that swap on the oracle's output must give them.
+    """
+    with open(os.path.join(GOLDEN, "thesis_example.json")) as f:
+        ex = json.load(f)
+    pts = np.array(ex["points"], np.float64)
+    x = pts[:, 0].astype(np.float32)
+    y = pts[:, 1].astype(np.float32)
+    ids = pts[:, 2].astype(np.float32)
+    c1 = ex["cut1"]
+    rc, i, x1, y1, id1 = oracle.hoare(x, y, ids, c1["axis"], c1["value"])
+    assert rc == 0
+    assert set(id1[:i].astype(int)) == {0, 1, 4, 3} and set(id1[i:].astype(int)) == {2, 5, 6}
+    undone = id1.copy()
+    undone[[i, len(undone) - 1]] = undone[[len(undone) - 1, i]]
+    assert list(undone.astype(int)) == c1["order_after"]
+    # second cut on the left part, from the thesis order
+    order = c1["order_after"]
+    x2, y2, id2 = x[order], y[order], ids[order]
+    c2 = ex["cut2"]
+    b, e = c2["range"]
+    rc, i2, x3, y3, id3 = oracle.hoare(x2, y2, id2, c2["axis"], c2["value"], b, e)
+    assert rc == 0
+    assert set(id3[b:i2].astype(int)) == {0, 3} and set(id3[i2:e].astype(int)) == {4, 1}
+    undone = id3.copy()
+    undone[[i2, e - 1]] = undone[[e - 1, i2]]
+    assert list(undone.astype(int)) == c2["order_after"]
+
+
+def test_stable_le_partition(oracle):
+    rng = np.random.default_rng(2)
+    for _ in range(200):
+        n = int(rng.integers(0, 500))
+        keys = (rng.integers(0, 9, n) / 8.0).astype(np.float32)
+        ids = np.arange(n, dtype=np.float32)
+        cut = F32(rng.integers(0, 9) / 8.0)
+        rc, s, kx, ky, kz = oracle.stable_le(keys, ids, ids, 0, cut)
+        assert rc == 0
+        left = ids[keys <= cut]
+        right = ids[keys > cut]
+        assert s == len(left)
+        assert np.array_equal(ky, np.concatenate([left, right]))
+
+
+# ---------------------------------------------------------------------------
+# the build
+# ---------------------------------------------------------------------------
+def resimulate_cell(keys, lo, hi, n, nleaf, cmp_lt=True, max_passes=32):
+    """Independent numpy bisection of one cell (orbit.cpp:149-231)."""
+    L, R = F32(lo), F32(hi)
+    counts = []
+    found = False
+    ratio = F32(math.ceil(nleaf / 2.0) / nleaf)
+    j = 0
+    while j < max_passes:
+        j += 1
+        cut = F32(F32(L + R) * F32(0.5))
+        c = int(np.count_nonzero(keys < cut if cmp_lt else keys <= cut))
+        counts.append(c)
+        diff = int(F32(F32(c) - F32(F32(n) * ratio)))
+        if abs(diff) < 3:
+            found = True
+            break
+        if diff > 0:
+            R = cut
+        else:
+            L = cut
+    return L, R, found, counts
+
+
+def check_build(oracle, x0, y0, z0, d, levels_mode, cmp=0, partition=0):
+    r = oracle.build(x0, y0, z0, d, levels_mode=levels_mode, cmp=cmp, partition=partition,
+                     pass_counts=True)
+    assert r.rc == 0
+    cols = (r.x, r.y, r.z)
+    heap, ranges, exists = r.heap, r.ranges, r.exists
+    internal = [i for i in range(len(exists)) if exists[i]
+                and 2 * i + 2 < len(exists) and exists[2 * i + 1]]
+    per_level = {}
+    for cid in internal:
+        c = heap[cid]
+        b, e = ranges[cid]
+        lb, le_ = ranges[2 * cid + 1]
+        rb, re_ = ranges[2 * cid + 2]
+        assert lb == b and le_ == rb and re_ == e
+        ax = int(c["cutAxis"])
+        keys = cols[ax][b:e]
+        Lm, Rm, found, counts = resimulate_cell(keys, c["lower"][ax], c["upper"][ax], e - b,
+                                                int(c["nLeafCells"]), cmp == 0)
+        assert bits(Lm) == bits(c["cutMarginLeft"]) and bits(Rm) == bits(c["cutMarginRight"]), cid
+        assert found == bool(c["foundCut"])
+        pc = r.pass_counts[cid]
+        assert list(pc[:len(counts)]) == counts and (pc[len(counts):] == 0xFFFFFFFF).all()
+        cut = F32(F32(Lm + Rm) * F32(0.5))
+        if partition == 0:
+            assert (cols[ax][lb:le_] <= cut).all() and (cols[ax][rb:re_] >= cut).all()
+        else:
+            assert (cols[ax][lb:le_] <= cut).all() and (cols[ax][rb:re_] > cut).all()
+        lvl = int(math.floor(math.log2(cid + 1))) + 1
+        per_level[lvl] = max(per_level.get(lvl, 0), len(counts))
+    for lvl, p in per_level.items():
+        assert r.level_passes[lvl - 1] == p
+    # rows move together
+    a = np.sort(np.stack([x0, y0, z0], 1).view("f4,f4,f4").ravel())
+    b = np.sort(np.stack([r.x, r.y, r.z], 1).view("f4,f4,f4").ravel())
+    assert np.array_equal(a, b)
+    return r
+
+
+@pytest.mark.parametrize("levels_mode", [0, 1])
+@pytest.mark.parametrize("cmp,partition", [(0, 0), (1, 0), (1, 1)])
+def test_build_resimulated_uniform(oracle, levels_mode, cmp, partition):
+    x, y, z = oracle.generate(1 << 14)
+    check_build(oracle, x, y, z, 64, levels_mode, cmp, partition)
+
+
+def test_build_resimulated_tie_heavy(oracle):
+    rng = np.random.default_rng(9)
+    n = 1 << 13
+    x, y, z = ((rng.integers(0, 64, n) / 64.0 - 0.5).astype(np.float32) for _ in range(3))
+    r = check_build(oracle, x, y, z, 32, 1)
+    assert r.total_passes > 0
+
+
+@pytest.mark.parametrize("d", [3, 5, 6, 7, 12, 100])
+def test_build_non_power_of_two_full(oracle, d):
+    x, y, z = oracle.generate(5000)
+    r = check_build(oracle, x, y, z, d, 1)
+    leaves = [i for i in range(len(r.exists)) if r.exists[i] and r.heap[i]["nLeafCells"] == 1]
+    assert len(leaves) == d
+    sizes = sorted(int(r.ranges[i, 1] - r.ranges[i, 0]) for i in leaves)
+    assert sum(sizes) == 5000
+
+
+def test_orbit_20_10_0_schedule(oracle):
+    """`orbit 20 10 0`: 9 levels, 512 ranges (level off-by-one, orbit.cpp:102)."""
+    x, y, z = oracle.generate(1 << 20)
+    r = oracle.build(x, y, z, 1024, levels_mode=0)
+    assert r.rc == 0
+    assert list(r.level_passes) == [18, 17, 16, 15, 14, 14, 14, 13, 12]
+    ids = range(511, 1023)
+    sizes = [int(r.ranges[i, 1] - r.ranges[i, 0]) for i in ids]
+    assert sum(sizes) == 1 << 20 and min(sizes) == 2044 and max(sizes) == 2052
+    rf = oracle.build(x, y, z, 1024, levels_mode=1)
+    assert list(rf.level_passes) == [18, 17, 16, 15, 14, 14, 14, 13, 12, 12]
+    sizes = [int(rf.ranges[i, 1] - rf.ranges[i, 0]) for i in range(1023, 2047)]
+    assert min(sizes) == 1020 and max(sizes) == 1028
