@@ -74,6 +74,54 @@ static void generate_reference(int64_t n, float *x, float *y, float *z)
     }
 }
 
+void timing_reset(Ctx &c)
+{
+    c.tm = orb_timing{};
+    c.tnext = 0;
+    c.tpairs.clear();
+}
+
+int timing_mark(Ctx &c)
+{
+    if (!c.timing) return -1;
+    if (c.tnext == c.tpool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return -1;
+        c.tpool.push_back(e);
+    }
+    int idx = (int)c.tnext++;
+    if (hipEventRecord(c.tpool[(size_t)idx], c.stream) != hipSuccess) return -1;
+    return idx;
+}
+
+int timing_pair(Ctx &c, int begin_mark, int kind)
+{
+    if (begin_mark < 0) return ORB_OK;
+    int end_mark = timing_mark(c);
+    if (end_mark < 0) return ORB_OK;
+    c.tpairs.push_back(Ctx::TPair{begin_mark, end_mark, kind});
+    return ORB_OK;
+}
+
+int timing_collect(Ctx &c)
+{
+    for (const auto &p : c.tpairs) {
+        float ms = 0.f;
+        ORB_HIP(hipEventElapsedTime(&ms, c.tpool[(size_t)p.a], c.tpool[(size_t)p.b]));
+        if (p.kind == kTimeCount) {
+            c.tm.count_ms += ms;
+            c.tm.count_launches += 1;
+        } else if (p.kind == kTimePartition) {
+            c.tm.partition_ms += ms;
+        } else {
+            c.tm.other_ms += ms;
+        }
+    }
+    c.tpairs.clear();
+    c.tnext = 0;
+    return ORB_OK;
+}
+
 static int reset_ranges(Ctx &c)
 {
     ORB_TRY(launch_zero_u32(c.stream, c.ranges.p, 2 * (size_t)c.cap));
@@ -123,8 +171,7 @@ static int prepare(Ctx &c, const orb_cell *cells, int32_t nCells)
 static int timed_count(Ctx &c, int32_t nCells, bool skip_found, const float *axis)
 {
     if (c.timing) ORB_HIP(hipEventRecord(c.ev[0], c.stream));
-    ORB_TRY(launch_count(c, nCells, skip_found, axis));
-    if (c.timing) ORB_HIP(hipEventRecord(c.ev[1], c.stream));
+    ORB_TRY(launch_count(c, nCells, skip_found, axis, c.timing ? c.ev[1] : nullptr));
     return ORB_OK;
 }
 
@@ -362,7 +409,9 @@ int orb_init(orb_ctx **out, int64_t nParticles, int32_t d, bool generate, orb_me
     c.part = params.GPU_PARTITION ? ORB_PART_STABLE_LE : ORB_PART_HOARE;
     c.cap = heap_capacity(d);
     int rc;
-    if ((rc = c.pos.alloc(3 * (size_t)c.stride)) < 0) return fail(rc);
+    // 64 floats of slack: 16-byte loads of a range ending at n may touch 3
+    // floats past the z array
+    if ((rc = c.pos.alloc(3 * (size_t)c.stride + 64)) < 0) return fail(rc);
     if ((rc = c.ranges.alloc(2 * (size_t)c.cap)) < 0) return fail(rc);
     if ((rc = c.lpos.alloc((size_t)c.stride)) < 0) return fail(rc);
     if ((rc = c.rpos.alloc((size_t)c.stride)) < 0) return fail(rc);
@@ -407,6 +456,8 @@ int orb_finalize(orb_ctx *ctx)
     c->chunk_cell.release();
     c->total_chunks.release();
     c->cnt.release();
+    c->stamps.release();
+    c->chunk_ltle.release();
     c->chunk_cnt.release();
     c->chunk_pre.release();
     c->scan_partials.release();
@@ -422,6 +473,7 @@ int orb_finalize(orb_ctx *ctx)
     for (auto &ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (c->ev_upload) (void)hipEventDestroy(c->ev_upload);
+    for (auto &ev : c->tpool) (void)hipEventDestroy(ev);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete ctx;
     return ORB_OK;

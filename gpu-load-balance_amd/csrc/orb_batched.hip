@@ -1,21 +1,713 @@
-// orb_batched.hip -- the batched (device-driven) build.
+// orb_batched.hip -- the batched (device-driven) ORB build.
+//
+// The plain form asks the host after every bisection pass.  Here the whole
+// build runs on the device with no host round trip:
+//
+//  * the cell heap lives in HBM; the level schedule (which heap ids a level
+//    bisects) depends only on d, so the host computes it once;
+//  * every cell of a level is bisected in the same launches.  One streaming
+//    pass over a cell's keys bins each key against the 255 candidate cuts of
+//    the next 8 bisection steps (the midpoints the serial loop could visit,
+//    computed with the same float arithmetic).  The candidates are sorted, so
+//    prefix sums of the 256-bin histogram give count(x < c) (or <= c) at
+//    every candidate, and a replay kernel walks the reference's termination
+//    rule over them: 8 passes of src/orbit.cpp:149-231 per HBM read, bit-exact
+//    by construction.  Four rounds cover the 32-pass cap;
+//  * a cell of at most kSmall keys is handed to a subtree kernel: one block
+//    loads its x, y, z into LDS once, runs every remaining level of the
+//    cell's subtree there (bisection by wave reductions, exact Hoare by
+//    ballot scans) and writes x, y, z back once;
+//  * the partition of the remaining (large) cells uses the same exact Hoare
+//    kernels as the plain form.
 #include "orb_internal.hpp"
 
 namespace orb {
 
+constexpr int kHistK = 8;                      // bisection steps per histogram round
+constexpr int kBins = 1 << kHistK;             // 256 bins (255 candidate cuts)
+constexpr int kRounds = (ORB_MAX_PASSES + kHistK - 1) / kHistK;  // 4
+constexpr int kSmall = 2048;                   // keys per LDS-resident subtree
+constexpr int kSmallLeaves = 128;              // max leaves under a subtree root
+constexpr int kReduceRows = 64;                // chunk histograms per reduce block
+
 struct BatchedState {
+    int32_t d = 0, levels_mode = -1;
+    int64_t cap = 0;
+    int32_t D = 0;
+    std::vector<int32_t> level_n;      // [D+1] cells per level (index l)
+    std::vector<int64_t> level_off;    // [D+1] offset into level_ids
+    DevBuf<int32_t> level_ids;         // all levels' heap ids, concatenated
+    DevBuf<orb_cell> heap;             // [cap]
+    DevBuf<uint8_t> exists;            // [cap]
+    DevBuf<uint8_t> sub;               // [cap] cell handled inside a subtree kernel
+    DevBuf<uint16_t> chunk_hist;       // [max_chunks][256]
+    DevBuf<uint32_t> cell_hist;        // [max_level_cells][256]
+    DevBuf<uint32_t> trace;            // [cap][32]
+    DevBuf<int32_t> level_passes;      // [64]
+    PinBuf<int32_t> hpasses;           // [64]
 };
+
+// ---------------------------------------------------------------------------
+// schedule (host): ids per level, data-independent
+// ---------------------------------------------------------------------------
+static int make_schedule(Ctx &c, BatchedState &b, int32_t levels_mode, bool *complete)
+{
+    b.d = c.d;
+    b.levels_mode = levels_mode;
+    b.cap = c.cap;
+    b.D = n_levels(c.d);
+    // simulate existence and nLeafCells without data: the split rule fixes
+    // them (left = ceil(n/2), right = n - left)
+    std::vector<uint8_t> ex((size_t)c.cap, 0);
+    std::vector<orb_cell> hp((size_t)c.cap);
+    std::memset(hp.data(), 0, sizeof(orb_cell) * hp.size());
+    hp[0].nLeafCells = c.d;
+    ex[0] = 1;
+    std::vector<int32_t> all, ids;
+    b.level_n.assign((size_t)b.D + 2, 0);
+    b.level_off.assign((size_t)b.D + 2, 0);
+    *complete = true;
+    for (int32_t l = 1; l <= b.D; ++l) {
+        int32_t bad = -1;
+        if (!level_cells(c.d, levels_mode, l, ex, hp, ids, &bad)) return ORB_ERR_UB;
+        if (levels_mode == ORB_LEVELS_REFERENCE && l >= b.D) break;
+        b.level_off[l] = (int64_t)all.size();
+        b.level_n[l] = (int32_t)ids.size();
+        // the subtree kernel needs every cell of depth l-1 with >= 2 leaves
+        int64_t a = (int64_t(1) << (l - 1)) - 1, e = (int64_t(1) << l) - 2;
+        int32_t expect = 0;
+        for (int64_t id = a; id <= e; ++id)
+            if (ex[(size_t)id] && hp[(size_t)id].nLeafCells >= 2) expect++;
+        if (expect != (int32_t)ids.size()) *complete = false;
+        for (int32_t id : ids) {
+            all.push_back(id);
+            int32_t nl = hp[(size_t)id].nLeafCells;
+            int32_t left = (nl + 1) / 2;
+            int32_t lid = left_child_id(id), rid = right_child_id(id);
+            if (rid >= c.cap) return ORB_ERR_ARG;
+            hp[(size_t)lid].nLeafCells = left;
+            hp[(size_t)rid].nLeafCells = nl - left;
+            ex[(size_t)lid] = ex[(size_t)rid] = 1;
+        }
+    }
+    ORB_TRY(b.level_ids.alloc(all.size() + 1));
+    if (!all.empty())
+        ORB_HIP(hipMemcpy(b.level_ids.p, all.data(), sizeof(int32_t) * all.size(),
+                          hipMemcpyHostToDevice));
+    return ORB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+__global__ void k_b_init(orb_cell *heap, uint8_t *exists, uint8_t *sub, int64_t cap,
+                         orb_cell root, uint32_t *trace, int64_t trace_n, int32_t *level_passes)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = i; k < cap; k += stride) {
+        orb_cell z;
+        uint32_t *w = reinterpret_cast<uint32_t *>(&z);
+#pragma unroll
+        for (int j = 0; j < (int)(sizeof(orb_cell) / 4); ++j) w[j] = 0u;
+        heap[k] = k == 0 ? root : z;
+        exists[k] = k == 0 ? 1 : 0;
+        sub[k] = 0;
+    }
+    for (int64_t k = i; k < trace_n; k += stride) trace[k] = 0xffffffffu;
+    if (i < 64) level_passes[i] = 0;
+}
+
+// Level records from the device heap; small cells are marked for the
+// subtree kernel, cells inside a finished subtree are skipped.
+__global__ void k_b_recs(const int32_t *ids, int32_t nCells, const orb_cell *heap,
+                         const uint32_t *ranges, const uint8_t *sub, int use_subtree,
+                         CellRec *recs, uint32_t *nchunks)
+{
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > nCells) return;
+    if (i == nCells) {
+        nchunks[nCells] = 0u;
+        return;
+    }
+    const int32_t id = ids[i];
+    const orb_cell cl = heap[id];
+    CellRec r;
+    r.begin = ranges[2 * (size_t)id];
+    r.end = ranges[2 * (size_t)id + 1];
+    if (r.end < r.begin) r.end = r.begin;
+    r.cut = cell_get_cut(cl);
+    r.axis = cl.cutAxis;
+    r.found = 0;
+    r.nLeaf = cl.nLeafCells;
+    r.margin_l = cl.cutMarginLeft;
+    r.margin_r = cl.cutMarginRight;
+    r.id = id;
+    r.passes = 0;
+    const uint32_t n = r.end - r.begin;
+    if (sub[id]) r.flags = 1;
+    else if (use_subtree && n <= (uint32_t)kSmall && cl.nLeafCells <= kSmallLeaves) r.flags = 2;
+    else r.flags = 0;
+    recs[i] = r;
+    nchunks[i] = r.flags ? 0u : (n + (uint32_t)kChunk - 1u) / (uint32_t)kChunk;
+}
+
+// in-order index t in [1, 255] of the depth-8 candidate tree on [lo, hi]
+__device__ inline float candidate(int t, float lo, float hi)
+{
+    int node = kBins / 2, step = kBins / 4;
+    while (node != t) {
+        float m = cell_cut(lo, hi);
+        if (t < node) {
+            hi = m;
+            node -= step;
+        } else {
+            lo = m;
+            node += step;
+        }
+        step >>= 1;
+    }
+    return cell_cut(lo, hi);
+}
+
+// number of candidates <= v (lt) or < v (le) in the sorted cand[1..255]
+__device__ inline int bin_of(const float *cand, float v, int le)
+{
+    int pos = 0;
+#pragma unroll
+    for (int s = kBins / 2; s >= 1; s >>= 1) {
+        float c = cand[pos + s];
+        bool go = le ? (c < v) : (c <= v);
+        pos += go ? s : 0;
+    }
+    return pos;
+}
+
+__global__ __launch_bounds__(kBlock) void k_b_hist(const CellRec *recs, int32_t nCells,
+                                                   const uint32_t *first,
+                                                   const uint32_t *chunk_cell, const float *x,
+                                                   const float *y, const float *z, int le,
+                                                   uint16_t *chunk_hist)
+{
+    __shared__ float cand[kBins];
+    __shared__ uint32_t hist[4][kBins];
+    uint32_t q = blockIdx.x;
+    uint32_t total = first[nCells];
+    if (q >= total) return;
+    const uint32_t cell = chunk_cell[q];
+    const CellRec r = recs[cell];
+    if (r.found || r.flags || r.axis < 0 || r.axis > 2) return;
+    const uint32_t b = r.begin + (q - first[cell]) * (uint32_t)kChunk;
+    uint32_t e = b + (uint32_t)kChunk;
+    if (e > r.end) e = r.end;
+    const int t = threadIdx.x;
+    if (t >= 1) cand[t] = candidate(t, r.margin_l, r.margin_r);
+    hist[0][t] = 0u;
+    hist[1][t] = 0u;
+    hist[2][t] = 0u;
+    hist[3][t] = 0u;
+    __syncthreads();
+    uint32_t *h = hist[t >> 6];
+    const float *key = r.axis == 0 ? x : (r.axis == 1 ? y : z);
+    uint32_t ab = (b + 3u) & ~3u;
+    if (ab > e) ab = e;
+    uint32_t ae = e & ~3u;
+    if (ae < ab) ae = ab;
+    if ((uint32_t)t < ab - b) atomicAdd(&h[bin_of(cand, key[b + t], le)], 1u);
+    if ((uint32_t)t < e - ae) atomicAdd(&h[bin_of(cand, key[ae + t], le)], 1u);
+    const float4 *k4 = reinterpret_cast<const float4 *>(key + ab);
+    const uint32_t nq = (ae - ab) >> 2;
+    float4 v[kPerThread / 4];
+#pragma unroll
+    for (int u = 0; u < kPerThread / 4; ++u) {
+        uint32_t j = (uint32_t)t + (uint32_t)u * kBlock;
+        if (j < nq) v[u] = k4[j];
+    }
+#pragma unroll
+    for (int u = 0; u < kPerThread / 4; ++u) {
+        uint32_t j = (uint32_t)t + (uint32_t)u * kBlock;
+        if (j < nq) {
+            atomicAdd(&h[bin_of(cand, v[u].x, le)], 1u);
+            atomicAdd(&h[bin_of(cand, v[u].y, le)], 1u);
+            atomicAdd(&h[bin_of(cand, v[u].z, le)], 1u);
+            atomicAdd(&h[bin_of(cand, v[u].w, le)], 1u);
+        }
+    }
+    __syncthreads();
+    uint32_t sum = hist[0][t] + hist[1][t] + hist[2][t] + hist[3][t];  // <= 4096
+    chunk_hist[(size_t)q * kBins + t] = (uint16_t)sum;
+}
+
+// Sums chunk histograms into cell histograms: block g takes chunks
+// [g*64, g*64+64), thread t owns bin t, one atomic per (block, cell, bin).
+__global__ __launch_bounds__(kBlock) void k_b_hist_reduce(const CellRec *recs, int32_t nCells,
+                                                          const uint32_t *first,
+                                                          const uint32_t *chunk_cell,
+                                                          const uint16_t *chunk_hist,
+                                                          uint32_t *cell_hist)
+{
+    const uint32_t total = first[nCells];
+    const uint32_t q0 = blockIdx.x * (uint32_t)kReduceRows;
+    if (q0 >= total) return;
+    uint32_t q1 = q0 + kReduceRows;
+    if (q1 > total) q1 = total;
+    const int t = threadIdx.x;
+    uint32_t cur = chunk_cell[q0];
+    bool live = !(recs[cur].found || recs[cur].flags);
+    uint32_t acc = 0;
+    for (uint32_t q = q0; q < q1; ++q) {
+        uint32_t cell = chunk_cell[q];
+        if (cell != cur) {
+            if (live && acc) atomicAdd(&cell_hist[(size_t)cur * kBins + t], acc);
+            cur = cell;
+            live = !(recs[cur].found || recs[cur].flags);
+            acc = 0;
+        }
+        if (live) acc += chunk_hist[(size_t)q * kBins + t];
+    }
+    if (live && acc) atomicAdd(&cell_hist[(size_t)cur * kBins + t], acc);
+}
+
+__device__ inline void record_level_passes(int32_t *level_passes, int32_t l, int32_t passes)
+{
+    // a plain read filters almost every atomic once the level maximum is in
+    if (passes > *((volatile int32_t *)&level_passes[l])) atomicMax(&level_passes[l], passes);
+}
+
+// One wave per cell: prefix sums of the cell histogram in LDS, then lane 0
+// replays up to 8 passes of the bisection controller (orbit.cpp:149-231).
+__global__ __launch_bounds__(kBlock) void k_b_replay(CellRec *recs, int32_t nCells,
+                                                     const uint32_t *cell_hist, orb_cell *heap,
+                                                     uint32_t *trace, int32_t *level_passes,
+                                                     int32_t level)
+{
+    __shared__ uint32_t P[4][kBins + 1];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int32_t c = blockIdx.x * 4 + w;
+    if (c >= nCells) return;  // uniform per wave
+    CellRec r = recs[c];
+    if (r.found || r.flags) return;
+    const uint4 h4 = reinterpret_cast<const uint4 *>(cell_hist + (size_t)c * kBins)[lane];
+    uint32_t s0 = h4.x, s1 = s0 + h4.y, s2 = s1 + h4.z, s3 = s2 + h4.w;
+    uint32_t base = 0;
+    {
+        uint32_t inc = s3;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            uint32_t tt = __shfl_up(inc, off, 64);
+            if (lane >= off) inc += tt;
+        }
+        base = inc - s3;
+    }
+    // P[j] = sum of bins < j
+    P[w][4 * lane + 1] = base + s0;
+    P[w][4 * lane + 2] = base + s1;
+    P[w][4 * lane + 3] = base + s2;
+    P[w][4 * lane + 4] = base + s3;
+    if (lane == 0) P[w][0] = 0u;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    if (lane != 0) return;
+    const uint32_t n = r.end - r.begin;
+    float L = r.margin_l, R = r.margin_r;
+    int node = kBins / 2, step = kBins / 4;
+    int32_t passes = r.passes;
+    bool found = false;
+    for (int s = 0; s < kHistK && passes < ORB_MAX_PASSES; ++s) {
+        const float cut = cell_cut(L, R);
+        const uint32_t cnt = P[w][node];
+        if (trace) trace[(size_t)r.id * ORB_MAX_PASSES + passes] = cnt;
+        passes++;
+        const int32_t diff = difference_rule(cnt, n, r.nLeaf);
+        if (abs(diff) < 3) {
+            found = true;
+            break;
+        }
+        if (diff > 0) {
+            R = cut;
+            node -= step;
+        } else {
+            L = cut;
+            node += step;
+        }
+        step >>= 1;
+    }
+    r.margin_l = L;
+    r.margin_r = R;
+    r.cut = cell_cut(L, R);
+    r.passes = passes;
+    r.found = found ? 1 : 0;
+    recs[c] = r;
+    orb_cell &hc = heap[r.id];
+    hc.cutMarginLeft = L;
+    hc.cutMarginRight = R;
+    hc.foundCut = found;
+    record_level_passes(level_passes, level, passes);
+}
+
+// A cell that hits the pass cap without ending its last round is finished
+// too: the cap rule is part of the replay.  Nothing to do here; the split
+// kernel reads the heap.
+__global__ void k_b_split(const CellRec *recs, int32_t nCells, orb_cell *heap, uint8_t *exists,
+                          int32_t *status, int64_t cap)
+{
+    int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nCells) return;
+    const CellRec r = recs[c];
+    if (r.flags) return;
+    const orb_cell p = heap[r.id];
+    orb_cell lc, rc;
+    uint32_t *wl = reinterpret_cast<uint32_t *>(&lc);
+    uint32_t *wr = reinterpret_cast<uint32_t *>(&rc);
+#pragma unroll
+    for (int j = 0; j < (int)(sizeof(orb_cell) / 4); ++j) wl[j] = wr[j] = 0u;
+    if (right_child_id(r.id) >= cap || !cell_split(p, lc, rc)) {
+        if (atomicCAS(&status[0], 0, 1) == 0) status[1] = r.id;
+        return;
+    }
+    heap[lc.id] = lc;
+    heap[rc.id] = rc;
+    exists[lc.id] = 1;
+    exists[rc.id] = 1;
+}
+
+// ---------------------------------------------------------------------------
+// LDS-resident subtree kernel
+// ---------------------------------------------------------------------------
+struct SubCell {
+    orb_cell cell;
+    uint32_t lb, le;   // local range in the block's LDS arrays
+    int32_t valid;
+    int32_t pad;
+};
+
+__device__ inline float wave_sum_f(float v) { return v; }
+
+__device__ inline uint32_t wave_sum_u32(uint32_t v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Processes every remaining level of the subtree rooted at a small cell.
+// One block per cell of the level (other slots exit).  Waves take the cells
+// of one depth round-robin; a barrier separates depths.
+__global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32_t nCells,
+                                                      orb_cell *heap, uint8_t *exists,
+                                                      uint8_t *sub, uint32_t *ranges, float *x,
+                                                      float *y, float *z, int32_t level,
+                                                      int32_t lmax, int le_cmp, uint32_t *trace,
+                                                      int32_t *level_passes, int32_t *status,
+                                                      int64_t cap)
+{
+    __shared__ float sx[kSmall], sy[kSmall], sz[kSmall];
+    __shared__ uint16_t slpos[kSmall], srpos[kSmall];
+    __shared__ SubCell cur[kSmallLeaves], nxt[kSmallLeaves];
+    __shared__ int32_t s_ncur, s_nvalid, s_err;
+    const int32_t c = blockIdx.x;
+    if (c >= nCells) return;
+    const CellRec rr = recs[c];
+    if (rr.flags != 2) return;
+    const uint32_t begin = rr.begin, n = rr.end - rr.begin;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    for (uint32_t k = (uint32_t)t; k < n; k += kBlock) {
+        sx[k] = x[begin + k];
+        sy[k] = y[begin + k];
+        sz[k] = z[begin + k];
+    }
+    if (t == 0) {
+        cur[0].cell = heap[rr.id];
+        cur[0].lb = 0;
+        cur[0].le = n;
+        cur[0].valid = 1;
+        s_ncur = 1;
+        s_nvalid = 1;
+        s_err = 0;
+    }
+    __syncthreads();
+    for (int32_t l = level; l <= lmax; ++l) {
+        const int32_t ncur = s_ncur;
+        // children of entry k go to 2k and 2k+1: the next depth must fit
+        if (ncur == 0 || s_nvalid == 0 || s_err) break;
+        if (2 * ncur > kSmallLeaves) {
+            // cannot happen while the root has <= kSmallLeaves leaves; refuse
+            // rather than leave the subtree half built
+            if (t == 0 && atomicCAS(&status[0], 0, 1) == 0) status[1] = rr.id;
+            break;
+        }
+        for (int32_t k = w; k < ncur; k += 4) {
+            SubCell sc = cur[k];
+            nxt[2 * k].valid = 0;
+            nxt[2 * k + 1].valid = 0;
+            if (!sc.valid) continue;
+            orb_cell cl = sc.cell;
+            const uint32_t lb = sc.lb, le = sc.le, m = le - lb;
+            const float *key = cl.cutAxis == 0 ? sx : (cl.cutAxis == 1 ? sy : sz);
+            // bisection controller (orbit.cpp:149-231) over the LDS keys
+            float L = cl.cutMarginLeft, R = cl.cutMarginRight;
+            int32_t passes = 0;
+            bool found = false;
+            while (!found && passes < ORB_MAX_PASSES) {
+                const float cut = cell_cut(L, R);
+                uint32_t cnt = 0;
+                for (uint32_t p = lb + lane; p < le; p += 64) {
+                    float v = key[p];
+                    cnt += le_cmp ? (v <= cut) : (v < cut);
+                }
+                cnt = wave_sum_u32(cnt);
+                if (trace && lane == 0) trace[(size_t)cl.id * ORB_MAX_PASSES + passes] = cnt;
+                passes++;
+                const int32_t diff = difference_rule(cnt, m, cl.nLeafCells);
+                if (abs(diff) < 3) found = true;
+                else if (diff > 0) R = cut;
+                else L = cut;
+            }
+            cl.cutMarginLeft = L;
+            cl.cutMarginRight = R;
+            cl.foundCut = found;
+            const float cut = cell_cut(L, R);
+            // Hoare partition as scans, in LDS (partition.cpp:28-60)
+            uint32_t TL = 0, TR = 0;
+            for (uint32_t base = lb; base < le; base += 64) {
+                const uint32_t p = base + lane;
+                const bool in = p < le;
+                const float v = in ? key[p] : 0.0f;
+                const bool isL = in && v >= cut, isR = in && v <= cut;
+                const uint64_t mL = __ballot(isL), mR = __ballot(isR);
+                const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+                if (isL) slpos[lb + TL + __popcll(mL & below)] = (uint16_t)(p - lb);
+                if (isR) srpos[lb + TR + __popcll(mR & below)] = (uint16_t)(p - lb);
+                TL += __popcll(mL);
+                TR += __popcll(mR);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            if (m == 0 || TL == 0) {
+                if (lane == 0) {
+                    s_err = 1;
+                    if (atomicCAS(&status[0], 0, 1) == 0) status[1] = cl.id;
+                }
+                continue;
+            }
+            const uint16_t *Lp = slpos + lb;
+            const uint16_t *Rp = srpos + lb;
+            uint32_t lo = 0, hi = TL < TR ? TL : TR;
+            while (lo < hi) {
+                uint32_t mid = (lo + hi) >> 1;
+                if (Lp[mid] < Rp[TR - 1 - mid]) lo = mid + 1;
+                else hi = mid;
+            }
+            const uint32_t K = lo;
+            uint32_t i;
+            if (K == 0) {
+                i = Lp[0];
+            } else {
+                i = Rp[TR - K];
+                if (K < TL && Lp[K] < i) i = Lp[K];
+            }
+            for (uint32_t tt = lane; tt < K; tt += 64) {
+                const uint32_t a = lb + Lp[tt], b2 = lb + Rp[TR - 1 - tt];
+                float ax = sx[a], ay = sy[a], az = sz[a];
+                sx[a] = sx[b2]; sy[a] = sy[b2]; sz[a] = sz[b2];
+                sx[b2] = ax; sy[b2] = ay; sz[b2] = az;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            if (lane == 0 && lb + i != le - 1) {
+                const uint32_t a = lb + i, b2 = le - 1;
+                float ax = sx[a], ay = sy[a], az = sz[a];
+                sx[a] = sx[b2]; sy[a] = sy[b2]; sz[a] = sz[b2];
+                sx[b2] = ax; sy[b2] = ay; sz[b2] = az;
+            }
+            // split (orbit.cpp:235-250) and bookkeeping
+            orb_cell lc, rc;
+            uint32_t *wl = reinterpret_cast<uint32_t *>(&lc);
+            uint32_t *wr = reinterpret_cast<uint32_t *>(&rc);
+#pragma unroll
+            for (int j = 0; j < (int)(sizeof(orb_cell) / 4); ++j) wl[j] = wr[j] = 0u;
+            const bool ok = right_child_id(cl.id) < cap && cell_split(cl, lc, rc);
+            if (lane == 0) {
+                heap[cl.id] = cl;
+                record_level_passes(level_passes, l, passes);
+                if (!ok) {
+                    s_err = 1;
+                    if (atomicCAS(&status[0], 0, 1) == 0) status[1] = cl.id;
+                } else {
+                    heap[lc.id] = lc;
+                    heap[rc.id] = rc;
+                    exists[lc.id] = 1;
+                    exists[rc.id] = 1;
+                    sub[lc.id] = 1;
+                    sub[rc.id] = 1;
+                    ranges[2 * (size_t)lc.id] = begin + lb;
+                    ranges[2 * (size_t)lc.id + 1] = begin + lb + i;
+                    ranges[2 * (size_t)rc.id] = begin + lb + i;
+                    ranges[2 * (size_t)rc.id + 1] = begin + le;
+                    nxt[2 * k].cell = lc;
+                    nxt[2 * k].lb = lb;
+                    nxt[2 * k].le = lb + i;
+                    nxt[2 * k].valid = lc.nLeafCells >= 2;
+                    nxt[2 * k + 1].cell = rc;
+                    nxt[2 * k + 1].lb = lb + i;
+                    nxt[2 * k + 1].le = le;
+                    nxt[2 * k + 1].valid = rc.nLeafCells >= 2;
+                }
+            }
+        }
+        __syncthreads();
+        const int32_t nn = 2 * ncur;  // <= kSmallLeaves (checked above)
+        for (int32_t k = t; k < nn; k += kBlock) cur[k] = nxt[k];
+        __syncthreads();
+        if (t == 0) {
+            int32_t nv = 0;
+            for (int32_t k = 0; k < nn; ++k) nv += cur[k].valid ? 1 : 0;
+            s_ncur = nn;
+            s_nvalid = nv;
+        }
+        __syncthreads();
+    }
+    for (uint32_t k = (uint32_t)t; k < n; k += kBlock) {
+        x[begin + k] = sx[k];
+        y[begin + k] = sy[k];
+        z[begin + k] = sz[k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// orchestration
+// ---------------------------------------------------------------------------
+static int ensure_state(Ctx &c, int32_t levels_mode, bool *complete)
+{
+    if (!c.batched) c.batched = new (std::nothrow) BatchedState;
+    if (!c.batched) return ORB_ERR_NOMEM;
+    BatchedState &b = *c.batched;
+    ORB_TRY(make_schedule(c, b, levels_mode, complete));
+    ORB_TRY(b.heap.alloc((size_t)c.cap));
+    ORB_TRY(b.exists.alloc((size_t)c.cap));
+    ORB_TRY(b.sub.alloc((size_t)c.cap));
+    ORB_TRY(b.chunk_hist.alloc((size_t)(c.max_chunks + 1) * kBins));
+    ORB_TRY(b.cell_hist.alloc((size_t)c.max_level_cells * kBins));
+    ORB_TRY(b.level_passes.alloc(64));
+    ORB_TRY(b.hpasses.alloc(64));
+    if (c.trace) ORB_TRY(b.trace.alloc((size_t)c.cap * ORB_MAX_PASSES));
+    return ORB_OK;
+}
 
 int build_batched(Ctx &c, int32_t levels_mode, orb_build_info *info)
 {
-    (void)c;
-    (void)levels_mode;
-    (void)info;
-    return ORB_ERR_STATE;
+    bool complete = true;
+    ORB_TRY(ensure_state(c, levels_mode, &complete));
+    BatchedState &b = *c.batched;
+    const bool use_subtree = complete && c.part == ORB_PART_HOARE;
+    const int le = c.cmp == ORB_CMP_LE ? 1 : 0;
+    uint32_t *trace = c.trace ? b.trace.p : nullptr;
+    const int64_t trace_n = c.trace ? c.cap * ORB_MAX_PASSES : 0;
+    timing_reset(c);
+
+    // fresh state: range(0) = [0, n), root cell, clean flags
+    ORB_TRY(launch_zero_u32(c.stream, c.ranges.p, 2 * (size_t)c.cap));
+    c.hranges.p[0] = 0;
+    c.hranges.p[1] = (uint32_t)c.n;
+    ORB_HIP(hipMemcpyAsync(c.ranges.p, c.hranges.p, 2 * sizeof(uint32_t), hipMemcpyHostToDevice,
+                           c.stream));
+    ORB_TRY(launch_zero_u32(c.stream, reinterpret_cast<uint32_t *>(c.status.p), 2));
+    const orb_cell root = cell_root(c.d);
+    hipLaunchKernelGGL(k_b_init, dim3(1024), dim3(kBlock), 0, c.stream, b.heap.p, b.exists.p,
+                       b.sub.p, (int64_t)c.cap, root, trace, trace_n, b.level_passes.p);
+    ORB_HIP(hipGetLastError());
+    c.table_ids.clear();
+
+    const int32_t lmax = levels_mode == ORB_LEVELS_REFERENCE ? b.D - 1 : b.D;
+    for (int32_t l = 1; l <= lmax; ++l) {
+        const int32_t nCells = b.level_n[(size_t)l];
+        info->nlevels = l;
+        if (nCells == 0) continue;
+        const int32_t *ids = b.level_ids.p + b.level_off[(size_t)l];
+        unsigned g1 = (unsigned)((nCells + 1 + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL(k_b_recs, dim3(g1), dim3(kBlock), 0, c.stream, ids, nCells, b.heap.p,
+                           c.ranges.p, b.sub.p, use_subtree ? 1 : 0, c.recs.p, c.cell_nchunks.p);
+        ORB_HIP(hipGetLastError());
+        ORB_TRY(build_chunk_table(c, nCells));
+        if (use_subtree) {
+            hipLaunchKernelGGL(k_b_subtree, dim3((unsigned)nCells), dim3(kBlock), 0, c.stream,
+                               c.recs.p, nCells, b.heap.p, b.exists.p, b.sub.p, c.ranges.p, c.x(),
+                               c.y(), c.z(), l, lmax, le, trace, b.level_passes.p, c.status.p,
+                               (int64_t)c.cap);
+            ORB_HIP(hipGetLastError());
+        }
+        const unsigned gr = (unsigned)((c.max_chunks + kReduceRows - 1) / kReduceRows);
+        const unsigned gw = (unsigned)((nCells + 3) / 4);
+        for (int r = 0; r < kRounds; ++r) {
+            const int tb = timing_mark(c);
+            hipLaunchKernelGGL(k_b_hist, dim3((unsigned)c.max_chunks), dim3(kBlock), 0, c.stream,
+                               c.recs.p, nCells, c.cell_first.p, c.chunk_cell.p, c.x(), c.y(),
+                               c.z(), le, b.chunk_hist.p);
+            ORB_HIP(hipGetLastError());
+            ORB_TRY(timing_pair(c, tb, kTimeCount));
+            ORB_TRY(launch_zero_u32(c.stream, b.cell_hist.p, (size_t)nCells * kBins));
+            hipLaunchKernelGGL(k_b_hist_reduce, dim3(gr), dim3(kBlock), 0, c.stream, c.recs.p,
+                               nCells, c.cell_first.p, c.chunk_cell.p, b.chunk_hist.p,
+                               b.cell_hist.p);
+            hipLaunchKernelGGL(k_b_replay, dim3(gw), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
+                               b.cell_hist.p, b.heap.p, trace, b.level_passes.p, l);
+            ORB_HIP(hipGetLastError());
+        }
+        unsigned gc = (unsigned)((nCells + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL(k_b_split, dim3(gc), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
+                           b.heap.p, b.exists.p, c.status.p, (int64_t)c.cap);
+        ORB_HIP(hipGetLastError());
+        const int tp = timing_mark(c);
+        if (c.part == ORB_PART_HOARE) ORB_TRY(launch_hoare(c, nCells));
+        else ORB_TRY(launch_stable_le(c, nCells));
+        ORB_TRY(timing_pair(c, tp, kTimePartition));
+    }
+    c.table_ids.clear();
+
+    // results back to the host mirror
+    ORB_HIP(hipMemcpyAsync(c.hstatus.p, c.status.p, 2 * sizeof(int32_t), hipMemcpyDeviceToHost,
+                           c.stream));
+    ORB_HIP(hipMemcpyAsync(b.hpasses.p, b.level_passes.p, 64 * sizeof(int32_t),
+                           hipMemcpyDeviceToHost, c.stream));
+    c.heap.resize((size_t)c.cap);
+    c.exists.resize((size_t)c.cap);
+    ORB_HIP(hipMemcpyAsync(c.heap.data(), b.heap.p, sizeof(orb_cell) * (size_t)c.cap,
+                           hipMemcpyDeviceToHost, c.stream));
+    ORB_HIP(hipMemcpyAsync(c.exists.data(), b.exists.p, (size_t)c.cap, hipMemcpyDeviceToHost,
+                           c.stream));
+    if (c.trace) {
+        c.trace_counts.resize((size_t)c.cap * ORB_MAX_PASSES);
+        ORB_HIP(hipMemcpyAsync(c.trace_counts.data(), b.trace.p,
+                               sizeof(uint32_t) * c.trace_counts.size(), hipMemcpyDeviceToHost,
+                               c.stream));
+    }
+    ORB_HIP(hipStreamSynchronize(c.stream));
+    ORB_TRY(timing_collect(c));
+    for (int32_t l = 1; l <= lmax; ++l) {
+        if (b.level_n[(size_t)l] == 0) continue;
+        info->level_passes[l - 1] = b.hpasses.p[l];
+        info->total_passes += b.hpasses.p[l];
+    }
+    if (c.hstatus.p[0] != 0) {
+        info->error_cell = c.hstatus.p[1];
+        return ORB_ERR_UB;
+    }
+    return ORB_OK;
 }
 
 void batched_release(Ctx &c)
 {
+    if (!c.batched) return;
+    BatchedState &b = *c.batched;
+    b.level_ids.release();
+    b.heap.release();
+    b.exists.release();
+    b.sub.release();
+    b.chunk_hist.release();
+    b.cell_hist.release();
+    b.trace.release();
+    b.level_passes.release();
+    b.hpasses.release();
     delete c.batched;
     c.batched = nullptr;
 }

@@ -244,6 +244,14 @@ struct CellRec {
     float margin_l;     // cutMarginLeft
     float margin_r;     // cutMarginRight
     int32_t id;         // heap id
+    int32_t flags;      // batched form: 0 active, 1 inside a finished subtree, 2 small (subtree kernel)
+    int32_t passes;     // batched form: bisection passes so far
+};
+
+// Cut and axis a cell's per-chunk counts were taken at (axis -1: none).
+struct CountStamp {
+    float cut;
+    int32_t axis;
 };
 
 // Result of the Hoare split of one cell.
@@ -287,6 +295,8 @@ struct Ctx {
     DevBuf<uint32_t> chunk_cell;     // [max_chunks]
     DevBuf<uint32_t> total_chunks;   // [1]
     DevBuf<uint32_t> cnt;            // [cells][2] lt, le
+    DevBuf<CountStamp> stamps;       // [cells] cut/axis of the chunk counts
+    DevBuf<uint32_t> chunk_ltle;     // [max_chunks+1] packed lt | le<<16 per chunk
     DevBuf<uint64_t> chunk_cnt;      // [max_chunks+1] packed (nL | nR<<32)
     DevBuf<uint64_t> chunk_pre;      // [max_chunks+1]
     DevBuf<uint64_t> scan_partials;  // scan scratch
@@ -312,12 +322,24 @@ struct Ctx {
     orb_timing tm{};
     hipEvent_t ev[2] = {nullptr, nullptr};
     hipEvent_t ev_upload = nullptr;  // last H2D out of hcells (pinned staging reuse)
+    // timing: events recorded in-stream, read after the build's final sync
+    std::vector<hipEvent_t> tpool;
+    size_t tnext = 0;
+    struct TPair { int a, b, kind; };
+    std::vector<TPair> tpairs;
     BatchedState *batched = nullptr;
 
     float *x() const { return pos.p; }
     float *y() const { return pos.p + stride; }
     float *z() const { return pos.p + 2 * stride; }
 };
+
+// timing helpers (orb_api.hip): marks are -1 when timing is off
+enum { kTimeCount = 0, kTimePartition = 1, kTimeOther = 2 };
+void timing_reset(Ctx &c);
+int timing_mark(Ctx &c);
+int timing_pair(Ctx &c, int begin_mark, int kind);
+int timing_collect(Ctx &c);
 
 // kernels / launchers (orb_kernels.hip)
 int launch_zero_u32(hipStream_t s, uint32_t *p, size_t n);
@@ -326,8 +348,11 @@ int scan_exclusive_u32(Ctx &c, const uint32_t *in, uint32_t *out, size_t m);
 int scan_exclusive_u64(Ctx &c, const uint64_t *in, uint64_t *out, size_t m);
 int upload_level(Ctx &c, const orb_cell *cells, int32_t nCells);
 int build_chunk_table(Ctx &c, int32_t nCells);
-int launch_count(Ctx &c, int32_t nCells, bool lt_skip_found, const float *axis_override);
+int launch_count(Ctx &c, int32_t nCells, bool lt_skip_found, const float *axis_override,
+                 hipEvent_t after_count = nullptr);
 int launch_hoare(Ctx &c, int32_t nCells);
+int launch_stopper_counts(Ctx &c, int32_t nCells, bool le_only);
+int invalidate_stamps(Ctx &c, int32_t nCells);
 int launch_stable_le(Ctx &c, int32_t nCells);
 int launch_make_axis(Ctx &c, int32_t nCells);
 int launch_cell_sizes(Ctx &c, int32_t nCells, uint32_t *dst);

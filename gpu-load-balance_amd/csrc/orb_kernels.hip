@@ -229,6 +229,8 @@ __global__ void k_level_recs(const orb_cell *cells, int32_t nCells, const uint32
     r.margin_l = cl.cutMarginLeft;
     r.margin_r = cl.cutMarginRight;
     r.id = cl.id;
+    r.flags = 0;
+    r.passes = 0;
     recs[i] = r;
     nchunks[i] = (r.end - r.begin + (uint32_t)kChunk - 1u) / (uint32_t)kChunk;
 }
@@ -255,6 +257,8 @@ int ensure_level_capacity(Ctx &c, int64_t nCells)
     ORB_TRY(c.cell_first.alloc(m + 1));
     ORB_TRY(c.chunk_cell.alloc(c.max_chunks));
     ORB_TRY(c.cnt.alloc(2 * m));
+    ORB_TRY(c.stamps.alloc(m));
+    ORB_TRY(c.chunk_ltle.alloc(c.max_chunks + 1));
     ORB_TRY(c.chunk_cnt.alloc(c.max_chunks + 1));
     ORB_TRY(c.chunk_pre.alloc(c.max_chunks + 1));
     ORB_TRY(c.splits.alloc(m));
@@ -280,13 +284,15 @@ int upload_level(Ctx &c, const orb_cell *cells, int32_t nCells)
     return ORB_OK;
 }
 
+int invalidate_stamps(Ctx &c, int32_t nCells);
+
 int build_chunk_table(Ctx &c, int32_t nCells)
 {
     ORB_TRY(scan_exclusive_u32(c, c.cell_nchunks.p, c.cell_first.p, (size_t)nCells + 1));
     hipLaunchKernelGGL(k_fill_chunks, dim3(grid_for((size_t)c.max_chunks, 16384)), dim3(kBlock), 0,
                        c.stream, c.cell_first.p, nCells, c.chunk_cell.p, (uint32_t)c.max_chunks);
     ORB_HIP(hipGetLastError());
-    return ORB_OK;
+    return invalidate_stamps(c, nCells);
 }
 
 // chunk q of the level: its cell and [b, e).  Returns false past the table.
@@ -309,36 +315,26 @@ __device__ inline bool chunk_range(uint32_t q, const uint32_t *first, int32_t nC
 //
 // One block per chunk streams the chunk's keys once with 16-byte loads (head
 // and tail of a misaligned chunk scalar) and counts x < cut and x <= cut
-// together; a wave64 shuffle reduction and four LDS partials give the block
-// sum, and one atomic per block per cell adds it to the cell's counters.
+// together.  A wave64 shuffle reduction and four LDS partials give the block
+// sum, written to the chunk's slot: no atomics (a same-address atomic per
+// block serialises at L2 and costs more than the stream itself when a level
+// has few cells).  k_cell_totals then sums each cell's chunk slots.
 // Roofline: HBM, 4 bytes per key counted.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_count(const CellRec *recs, int32_t nCells,
-                                                  const uint32_t *first, const uint32_t *chunk_cell,
-                                                  const float *x, const float *y, const float *z,
-                                                  const float *axisT, int skip_found, uint32_t *cnt)
+__device__ inline void count_range(const float *key, uint32_t b, uint32_t e, float cut,
+                                   uint32_t &lt, uint32_t &le)
 {
-    __shared__ uint32_t sm[4];
-    uint32_t cell, b, e;
-    if (!chunk_range(blockIdx.x, first, nCells, chunk_cell, recs, cell, b, e)) return;
-    const CellRec r = recs[cell];
-    if (skip_found && r.found) return;
-    if (r.axis < 0 || r.axis > 2) return;
-    const float *key = axisT ? axisT : axis_ptr(x, y, z, r.axis);
-    const float cut = r.cut;
-
     uint32_t ab = (b + 3u) & ~3u;
     if (ab > e) ab = e;
     uint32_t ae = e & ~3u;
     if (ae < ab) ae = ab;
-    const int t = threadIdx.x;
-    uint32_t lt = 0, le = 0;
-    if ((uint32_t)t < ab - b) {
+    const uint32_t t = threadIdx.x;
+    if (t < ab - b) {
         float v = key[b + t];
         lt += v < cut;
         le += v <= cut;
     }
-    if ((uint32_t)t < e - ae) {
+    if (t < e - ae) {
         float v = key[ae + t];
         lt += v < cut;
         le += v <= cut;
@@ -348,7 +344,7 @@ __global__ __launch_bounds__(kBlock) void k_count(const CellRec *recs, int32_t n
     float4 v[kPerThread / 4];
 #pragma unroll
     for (int u = 0; u < kPerThread / 4; ++u) {
-        uint32_t j = (uint32_t)t + (uint32_t)u * kBlock;
+        uint32_t j = t + (uint32_t)u * kBlock;
         if (j < nq) v[u] = k4[j];
         else v[u] = make_float4(INFINITY, INFINITY, INFINITY, INFINITY);
     }
@@ -357,20 +353,114 @@ __global__ __launch_bounds__(kBlock) void k_count(const CellRec *recs, int32_t n
         lt += (v[u].x < cut) + (v[u].y < cut) + (v[u].z < cut) + (v[u].w < cut);
         le += (v[u].x <= cut) + (v[u].y <= cut) + (v[u].z <= cut) + (v[u].w <= cut);
     }
-    // both counts <= 4096: pack into one word for a single reduction
+}
+
+__global__ __launch_bounds__(kBlock) void k_count(const CellRec *recs, int32_t nCells,
+                                                  const uint32_t *first, const uint32_t *chunk_cell,
+                                                  const float *x, const float *y, const float *z,
+                                                  const float *axisT, int skip_found,
+                                                  uint32_t *chunk_ltle)
+{
+    __shared__ uint32_t sm[4];
+    uint32_t cell, b, e;
+    if (!chunk_range(blockIdx.x, first, nCells, chunk_cell, recs, cell, b, e)) return;
+    const CellRec r = recs[cell];
+    if (skip_found && r.found) return;
+    if (r.axis < 0 || r.axis > 2) return;
+    const float *key = axisT ? axisT : axis_ptr(x, y, z, r.axis);
+    uint32_t lt = 0, le = 0;
+    count_range(key, b, e, r.cut, lt, le);
+    // both counts <= 4096: one packed word, one reduction
     uint32_t packed = block_sum(lt | (le << 16), sm);
-    if (t == 0) {
-        atomicAdd(&cnt[2 * cell], packed & 0xffffu);
-        atomicAdd(&cnt[2 * cell + 1], packed >> 16);
+    if (threadIdx.x == 0) chunk_ltle[blockIdx.x] = packed;
+}
+
+// Per-cell sums of the chunk slots; stamps the cut and axis the counts were
+// taken at (a later partition at the same cut reuses the chunk slots).  One
+// block per cell; 16-byte loads with eight in flight per thread keep the
+// root cell's 32K slots to a few load round trips.
+__global__ __launch_bounds__(kBlock) void k_cell_totals(const CellRec *recs, int32_t nCells,
+                                                        const uint32_t *first,
+                                                        const uint32_t *chunk_ltle,
+                                                        int skip_found, int stamp,
+                                                        uint32_t *cnt, CountStamp *stamps)
+{
+    __shared__ uint32_t sm[4];
+    for (int32_t c = blockIdx.x; c < nCells; c += gridDim.x) {
+        const CellRec r = recs[c];
+        if (skip_found && r.found) continue;  // uniform per block
+        const uint32_t q0 = first[c], q1 = first[c + 1];
+        uint32_t lt = 0, le = 0;
+        uint32_t a0 = (q0 + 3u) & ~3u;
+        if (a0 > q1) a0 = q1;
+        uint32_t a1 = q1 & ~3u;
+        if (a1 < a0) a1 = a0;
+        const uint32_t t = threadIdx.x;
+        if (t < a0 - q0) {
+            uint32_t p = chunk_ltle[q0 + t];
+            lt += p & 0xffffu;
+            le += p >> 16;
+        }
+        if (t < q1 - a1) {
+            uint32_t p = chunk_ltle[a1 + t];
+            lt += p & 0xffffu;
+            le += p >> 16;
+        }
+        const uint4 *v4 = reinterpret_cast<const uint4 *>(chunk_ltle + a0);
+        const uint32_t nv = (a1 - a0) >> 2;
+        for (uint32_t base = 0; base < nv; base += 8u * kBlock) {
+            uint4 v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                uint32_t j = base + t + (uint32_t)u * kBlock;
+                v[u] = j < nv ? v4[j] : make_uint4(0u, 0u, 0u, 0u);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                lt += (v[u].x & 0xffffu) + (v[u].y & 0xffffu) + (v[u].z & 0xffffu) + (v[u].w & 0xffffu);
+                le += (v[u].x >> 16) + (v[u].y >> 16) + (v[u].z >> 16) + (v[u].w >> 16);
+            }
+        }
+        lt = block_sum(lt, sm);
+        le = block_sum(le, sm);
+        if (t == 0) {
+            cnt[2 * c] = lt;
+            cnt[2 * c + 1] = le;
+            CountStamp st;
+            st.cut = r.cut;
+            st.axis = stamp ? r.axis : -1;
+            stamps[c] = st;
+        }
     }
 }
 
-int launch_count(Ctx &c, int32_t nCells, bool skip_found, const float *axis_override)
+__global__ void k_invalidate_stamps(CountStamp *stamps, int32_t n)
 {
-    ORB_TRY(launch_zero_u32(c.stream, c.cnt.p, 2 * (size_t)nCells));
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) stamps[i] = CountStamp{0.0f, -1};
+}
+
+int invalidate_stamps(Ctx &c, int32_t nCells)
+{
+    if (nCells <= 0) return ORB_OK;
+    hipLaunchKernelGGL(k_invalidate_stamps, dim3((nCells + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                       c.stream, c.stamps.p, nCells);
+    ORB_HIP(hipGetLastError());
+    return ORB_OK;
+}
+
+int launch_count(Ctx &c, int32_t nCells, bool skip_found, const float *axis_override,
+                 hipEvent_t after_count)
+{
     hipLaunchKernelGGL(k_count, dim3((unsigned)c.max_chunks), dim3(kBlock), 0, c.stream,
                        c.recs.p, nCells, c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(),
-                       axis_override, skip_found ? 1 : 0, c.cnt.p);
+                       axis_override, skip_found ? 1 : 0, c.chunk_ltle.p);
+    ORB_HIP(hipGetLastError());
+    if (after_count) ORB_HIP(hipEventRecord(after_count, c.stream));
+    unsigned g = (unsigned)(nCells < 65536 ? nCells : 65536);
+    hipLaunchKernelGGL(k_cell_totals, dim3(g), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
+                       c.cell_first.p, c.chunk_ltle.p, skip_found ? 1 : 0,
+                       axis_override ? 0 : 1, c.cnt.p, c.stamps.p);
     ORB_HIP(hipGetLastError());
     return ORB_OK;
 }
@@ -385,12 +475,13 @@ int launch_count(Ctx &c, int32_t nCells, bool skip_found, const float *axis_over
 // disjoint), stops with i = min(L_K, R_{K-1}) (i = L_0 when K == 0), and then
 // swaps rows i and n-1.  The children are [0, i) and [i, n).
 //
-//   P1a  per chunk: #L and #R (one key read)
+//   P1a  per chunk: #L and #R; taken from the last count pass when the cell
+//        was counted at this cut (stamp), else one key read
 //   scan per-chunk counts -> each chunk's rank offsets inside its cell
-//   P1b  per chunk: block scan of the flags, write every stopper's position
-//        into lpos / rpos at the cell's base + its rank (one key read)
-//   P2   per cell: K by binary search on lpos[t] < rpos[TR-1-t], i, the
-//        children's ranges, undefined-input check
+//   P1b  per chunk: block scan of the flags, stopper positions staged in LDS
+//        and written contiguously to lpos / rpos at the cell's base + rank
+//   P2   per cell (one wave): K by a 64-way search on lpos[t] < rpos[TR-1-t],
+//        i, the children's ranges, undefined-input check
 //   P3   per pair t < K: swap x, y, z of lpos[t] and rpos[TR-1-t]
 //   P4   per cell: swap rows i and n-1
 // ---------------------------------------------------------------------------
@@ -399,6 +490,8 @@ __global__ __launch_bounds__(kBlock) void k_stopper_counts(const CellRec *recs, 
                                                            const uint32_t *chunk_cell,
                                                            const float *x, const float *y,
                                                            const float *z, int le_only,
+                                                           const CountStamp *stamps,
+                                                           const uint32_t *chunk_ltle,
                                                            uint64_t *chunk_cnt, uint32_t max_chunks)
 {
     __shared__ uint32_t sm[4];
@@ -409,28 +502,59 @@ __global__ __launch_bounds__(kBlock) void k_stopper_counts(const CellRec *recs, 
         return;
     }
     const CellRec r = recs[cell];
-    const float *key = axis_ptr(x, y, z, r.axis < 0 ? 0 : (r.axis > 2 ? 2 : r.axis));
-    const float cut = r.cut;
-    uint32_t nl = 0, nr = 0;
-    for (uint32_t p = b + threadIdx.x; p < e; p += kBlock) {
-        float v = key[p];
-        if (le_only) {
-            nl += v <= cut;   // keys going left
-            nr += v > cut;    // keys going right
-        } else {
-            nl += v >= cut;   // L-stoppers
-            nr += v <= cut;   // R-stoppers
-        }
+    const int axis = r.axis < 0 ? 0 : (r.axis > 2 ? 2 : r.axis);
+    const CountStamp st = stamps[cell];
+    uint32_t lt, le;
+    if (st.axis == r.axis && __float_as_uint(st.cut) == __float_as_uint(r.cut)) {
+        uint32_t p = chunk_ltle[q];  // counted at this cut by the last pass
+        lt = p & 0xffffu;
+        le = p >> 16;
+    } else {
+        lt = 0;
+        le = 0;
+        count_range(axis_ptr(x, y, z, axis), b, e, r.cut, lt, le);
+        uint32_t packed = block_sum(lt | (le << 16), sm);
+        lt = packed & 0xffffu;
+        le = packed >> 16;
     }
-    uint32_t packed = block_sum(nl | (nr << 16), sm);
-    if (threadIdx.x == 0)
-        chunk_cnt[q] = (uint64_t)(packed & 0xffffu) | ((uint64_t)(packed >> 16) << 32);
+    if (threadIdx.x == 0) {
+        uint32_t len = e - b;
+        uint32_t nl = le_only ? le : len - lt;        // <=-partition: keys going left
+        uint32_t nr = le_only ? len - le : le;        // ... keys going right
+        chunk_cnt[q] = (uint64_t)nl | ((uint64_t)nr << 32);
+    }
 }
 
 // LDS staging of one chunk: element k at index k + k/16 (odd stride per
 // thread row, no bank conflicts when thread t reads its 16 consecutive keys).
 __device__ inline int lds_idx(int k) { return k + (k >> 4); }
 constexpr int kLdsKeys = kChunk + kChunk / 16;
+
+// Loads keys [b, e) into LDS with 16-byte global loads over the enclosing
+// aligned range (the particle arrays carry 64 floats of slack past the end,
+// so the last vector stays inside the allocation); only [b, e) is stored.
+__device__ inline int load_chunk_lds(float *skey, const float *key, uint32_t b, uint32_t e)
+{
+    const int n = (int)(e - b);
+    const uint32_t ab = b & ~3u;
+    const uint32_t nv = (e - ab + 3u) >> 2;  // <= kChunk/4 + 1
+    const float4 *k4 = reinterpret_cast<const float4 *>(key + ab);
+    const int shift = (int)(b - ab);
+#pragma unroll
+    for (int u = 0; u < kPerThread / 4 + 1; ++u) {
+        uint32_t j = threadIdx.x + (uint32_t)u * kBlock;
+        if (j < nv) {
+            float4 v = k4[j];
+            int k = (int)(4 * j) - shift;
+            if (k >= 0 && k < n) skey[lds_idx(k)] = v.x;
+            if (k + 1 >= 0 && k + 1 < n) skey[lds_idx(k + 1)] = v.y;
+            if (k + 2 >= 0 && k + 2 < n) skey[lds_idx(k + 2)] = v.z;
+            if (k + 3 >= 0 && k + 3 < n) skey[lds_idx(k + 3)] = v.w;
+        }
+    }
+    __syncthreads();
+    return n;
+}
 
 __global__ __launch_bounds__(kBlock) void k_stopper_positions(const CellRec *recs, int32_t nCells,
                                                               const uint32_t *first,
@@ -441,19 +565,16 @@ __global__ __launch_bounds__(kBlock) void k_stopper_positions(const CellRec *rec
                                                               uint32_t *rpos)
 {
     __shared__ float skey[kLdsKeys];
+    __shared__ uint32_t spos[kChunk];
     __shared__ uint32_t sm[4];
     uint32_t q = blockIdx.x;
     uint32_t cell, b, e;
     if (!chunk_range(q, first, nCells, chunk_cell, recs, cell, b, e)) return;
     const CellRec r = recs[cell];
-    const float *key = axis_ptr(x, y, z, r.axis);
     const float cut = r.cut;
-    const int n = (int)(e - b);
-    for (int k = threadIdx.x; k < n; k += kBlock) skey[lds_idx(k)] = key[b + k];
-    __syncthreads();
+    const int n = load_chunk_lds(skey, axis_ptr(x, y, z, r.axis), b, e);
 
-    const int t = threadIdx.x;
-    const int k0 = t * kPerThread;
+    const int k0 = threadIdx.x * kPerThread;
     uint32_t myl = 0, myr = 0;
 #pragma unroll
     for (int j = 0; j < kPerThread; ++j) {
@@ -465,20 +586,32 @@ __global__ __launch_bounds__(kBlock) void k_stopper_positions(const CellRec *rec
         }
     }
     uint32_t total;
-    uint32_t pre = block_exclusive_scan(myl | (myr << 16), sm, &total);
-    uint64_t base = chunk_pre[q] - chunk_pre[first[cell]];
-    uint32_t rl = (uint32_t)(base & 0xffffffffull) + (pre & 0xffffu);
-    uint32_t rr = (uint32_t)(base >> 32) + (pre >> 16);
+    const uint32_t pre = block_exclusive_scan(myl | (myr << 16), sm, &total);
+    const uint32_t nl = total & 0xffffu, nr = total >> 16;
+    const uint64_t base = chunk_pre[q] - chunk_pre[first[cell]];
     const uint32_t off = b - r.begin;  // cell-local position of the chunk start
+
+    // L-stoppers: local ranks into LDS, then one contiguous run to lpos
+    uint32_t rl = pre & 0xffffu;
 #pragma unroll
     for (int j = 0; j < kPerThread; ++j) {
         int k = k0 + j;
-        if (k < n) {
-            float v = skey[lds_idx(k)];
-            if (v >= cut) lpos[r.begin + rl++] = off + (uint32_t)k;
-            if (v <= cut) rpos[r.begin + rr++] = off + (uint32_t)k;
-        }
+        if (k < n && skey[lds_idx(k)] >= cut) spos[rl++] = off + (uint32_t)k;
     }
+    __syncthreads();
+    uint32_t *dl = lpos + r.begin + (uint32_t)(base & 0xffffffffull);
+    for (uint32_t k = threadIdx.x; k < nl; k += kBlock) dl[k] = spos[k];
+    __syncthreads();
+    // R-stoppers (left-to-right order; R_t is entry TR-1-t)
+    uint32_t rr = pre >> 16;
+#pragma unroll
+    for (int j = 0; j < kPerThread; ++j) {
+        int k = k0 + j;
+        if (k < n && skey[lds_idx(k)] <= cut) spos[rr++] = off + (uint32_t)k;
+    }
+    __syncthreads();
+    uint32_t *dr = rpos + r.begin + (uint32_t)(base >> 32);
+    for (uint32_t k = threadIdx.x; k < nr; k += kBlock) dr[k] = spos[k];
 }
 
 __device__ inline void report_error(int32_t *status, int32_t id)
@@ -486,35 +619,60 @@ __device__ inline void report_error(int32_t *status, int32_t id)
     if (atomicCAS(&status[0], 0, 1) == 0) status[1] = id;
 }
 
-__global__ void k_hoare_split(const CellRec *recs, int32_t nCells, const uint32_t *first,
-                              const uint64_t *chunk_pre, const uint32_t *lpos,
-                              const uint32_t *rpos, SplitRec *splits, uint32_t *ranges,
-                              int32_t *status, int64_t cap)
+// one wave64 per cell
+__global__ __launch_bounds__(kBlock) void k_hoare_split(const CellRec *recs, int32_t nCells,
+                                                        const uint32_t *first,
+                                                        const uint64_t *chunk_pre,
+                                                        const uint32_t *lpos, const uint32_t *rpos,
+                                                        SplitRec *splits, uint32_t *ranges,
+                                                        int32_t *status, int64_t cap)
 {
-    int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nCells) return;
+    const int lane = threadIdx.x & 63;
+    const int32_t c = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    if (c >= nCells) return;  // uniform per wave
     const CellRec r = recs[c];
-    uint64_t tot = chunk_pre[first[c + 1]] - chunk_pre[first[c]];
-    uint32_t TL = (uint32_t)(tot & 0xffffffffull), TR = (uint32_t)(tot >> 32);
-    uint32_t n = r.end - r.begin;
+    if (r.flags) {  // handled elsewhere (batched form's subtree kernel)
+        if (lane == 0) splits[c] = SplitRec{0u, 0u, 0u, 0u};
+        return;
+    }
+    const uint64_t tot = chunk_pre[first[c + 1]] - chunk_pre[first[c]];
+    const uint32_t TL = (uint32_t)(tot & 0xffffffffull), TR = (uint32_t)(tot >> 32);
+    const uint32_t n = r.end - r.begin;
     SplitRec s{TL, TR, 0u, 0u};
-    int64_t lid = left_child_id(r.id), rid = right_child_id(r.id);
+    const int64_t lid = left_child_id(r.id), rid = right_child_id(r.id);
     if (n == 0 || TL == 0 || r.axis < 0 || r.axis > 2 || rid >= cap) {
         // empty cell or no key >= cut: the reference's loop leaves the cell
-        report_error(status, r.id);
-        s.TL = 0u;  // k_hoare_final skips the cell
-        splits[c] = s;
+        if (lane == 0) {
+            report_error(status, r.id);
+            s.TL = 0u;  // k_hoare_final skips the cell
+            splits[c] = s;
+        }
         return;
     }
     const uint32_t *L = lpos + r.begin;
     const uint32_t *R = rpos + r.begin;  // left-to-right order; R_t = R[TR-1-t]
+    // K = #{t : L[t] < R[TR-1-t]}; the predicate is true then false in t
     uint32_t lo = 0, hi = TL < TR ? TL : TR;
-    while (lo < hi) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (L[mid] < R[TR - 1 - mid]) lo = mid + 1;
-        else hi = mid;
+    while (hi - lo > 64u) {
+        const uint32_t span = hi - lo;
+        const uint32_t t = lo + (uint32_t)(((uint64_t)(lane + 1) * span) / 65u);
+        const bool pr = L[t] < R[TR - 1 - t];
+        const uint64_t m = __ballot(pr);
+        if (m == 0ull) {
+            hi = lo + (uint32_t)(((uint64_t)1 * span) / 65u);           // pred(t_0) false
+        } else {
+            const int j = 63 - __clzll((long long)m);                    // last true probe
+            const uint32_t tj = lo + (uint32_t)(((uint64_t)(j + 1) * span) / 65u);
+            const uint32_t tn = j < 63 ? lo + (uint32_t)(((uint64_t)(j + 2) * span) / 65u) : hi;
+            lo = tj + 1;
+            hi = tn;
+        }
     }
-    uint32_t K = lo, i;
+    const uint32_t t = lo + (uint32_t)lane;
+    const bool pr = t < hi && L[t] < R[TR - 1 - t];
+    const uint32_t K = lo + (uint32_t)__popcll(__ballot(pr));
+    if (lane != 0) return;
+    uint32_t i;
     if (K == 0) {
         i = L[0];
     } else {
@@ -574,20 +732,26 @@ __global__ void k_hoare_final(const CellRec *recs, int32_t nCells, const SplitRe
     x[pb] = ax; y[pb] = ay; z[pb] = az;
 }
 
+int launch_stopper_counts(Ctx &c, int32_t nCells, bool le_only)
+{
+    hipLaunchKernelGGL(k_stopper_counts, dim3((unsigned)c.max_chunks + 1), dim3(kBlock), 0,
+                       c.stream, c.recs.p, nCells, c.cell_first.p, c.chunk_cell.p, c.x(), c.y(),
+                       c.z(), le_only ? 1 : 0, c.stamps.p, c.chunk_ltle.p, c.chunk_cnt.p,
+                       (uint32_t)c.max_chunks);
+    ORB_HIP(hipGetLastError());
+    return scan_exclusive_u64(c, c.chunk_cnt.p, c.chunk_pre.p, (size_t)c.max_chunks + 1);
+}
+
 int launch_hoare(Ctx &c, int32_t nCells)
 {
     const unsigned gq = (unsigned)c.max_chunks;
-    hipLaunchKernelGGL(k_stopper_counts, dim3(gq + 1), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
-                       c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(), 0, c.chunk_cnt.p,
-                       (uint32_t)c.max_chunks);
-    ORB_HIP(hipGetLastError());
-    ORB_TRY(scan_exclusive_u64(c, c.chunk_cnt.p, c.chunk_pre.p, (size_t)c.max_chunks + 1));
+    ORB_TRY(launch_stopper_counts(c, nCells, false));
     hipLaunchKernelGGL(k_stopper_positions, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
                        c.cell_first.p, c.chunk_cell.p, c.chunk_pre.p, c.x(), c.y(), c.z(), c.lpos.p,
                        c.rpos.p);
     ORB_HIP(hipGetLastError());
-    unsigned gc = (unsigned)((nCells + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(k_hoare_split, dim3(gc), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
+    unsigned gw = (unsigned)((nCells + 3) / 4);
+    hipLaunchKernelGGL(k_hoare_split, dim3(gw), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
                        c.cell_first.p, c.chunk_pre.p, c.lpos.p, c.rpos.p, c.splits.p, c.ranges.p,
                        c.status.p, (int64_t)c.cap);
     ORB_HIP(hipGetLastError());
@@ -595,6 +759,7 @@ int launch_hoare(Ctx &c, int32_t nCells)
                        c.cell_first.p, c.chunk_cell.p, c.splits.p, c.lpos.p, c.rpos.p, c.x(), c.y(),
                        c.z());
     ORB_HIP(hipGetLastError());
+    unsigned gc = (unsigned)((nCells + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_hoare_final, dim3(gc), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
                        c.splits.p, c.x(), c.y(), c.z());
     ORB_HIP(hipGetLastError());
@@ -618,11 +783,8 @@ __global__ __launch_bounds__(kBlock) void k_stable_scatter(const CellRec *recs, 
     uint32_t cell, b, e;
     if (!chunk_range(q, first, nCells, chunk_cell, recs, cell, b, e)) return;
     const CellRec r = recs[cell];
-    const float *key = axis_ptr(x, y, z, r.axis);
     const float cut = r.cut;
-    const int n = (int)(e - b);
-    for (int k = threadIdx.x; k < n; k += kBlock) skey[lds_idx(k)] = key[b + k];
-    __syncthreads();
+    const int n = load_chunk_lds(skey, axis_ptr(x, y, z, r.axis), b, e);
     const int t = threadIdx.x;
     const int k0 = t * kPerThread;
     uint32_t myl = 0, myr = 0;
@@ -678,6 +840,7 @@ __global__ void k_stable_ranges(const CellRec *recs, int32_t nCells, const uint3
     int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nCells) return;
     const CellRec r = recs[c];
+    if (r.flags) return;
     uint64_t tot = chunk_pre[first[c + 1]] - chunk_pre[first[c]];
     uint32_t nle = (uint32_t)(tot & 0xffffffffull);
     int64_t lid = left_child_id(r.id), rid = right_child_id(r.id);
@@ -696,11 +859,7 @@ int launch_stable_le(Ctx &c, int32_t nCells)
 {
     ORB_TRY(c.tmp.alloc(3 * (size_t)c.stride));
     const unsigned gq = (unsigned)c.max_chunks;
-    hipLaunchKernelGGL(k_stopper_counts, dim3(gq + 1), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
-                       c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(), 1, c.chunk_cnt.p,
-                       (uint32_t)c.max_chunks);
-    ORB_HIP(hipGetLastError());
-    ORB_TRY(scan_exclusive_u64(c, c.chunk_cnt.p, c.chunk_pre.p, (size_t)c.max_chunks + 1));
+    ORB_TRY(launch_stopper_counts(c, nCells, true));
     float *ox = c.tmp.p, *oy = c.tmp.p + c.stride, *oz = c.tmp.p + 2 * c.stride;
     hipLaunchKernelGGL(k_stable_scatter, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
                        c.cell_first.p, c.chunk_cell.p, c.chunk_pre.p, c.x(), c.y(), c.z(), ox, oy,
@@ -735,7 +894,7 @@ __global__ __launch_bounds__(kBlock) void k_make_axis(const CellRec *recs, int32
 
 int launch_make_axis(Ctx &c, int32_t nCells)
 {
-    ORB_TRY(c.axisT.alloc((size_t)c.stride));
+    ORB_TRY(c.axisT.alloc((size_t)c.stride + 64));
     hipLaunchKernelGGL(k_make_axis, dim3((unsigned)c.max_chunks), dim3(kBlock), 0, c.stream,
                        c.recs.p, nCells, c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(),
                        c.axisT.p);

@@ -1,0 +1,74 @@
+"""Parity of the HIP batched form (device-driven build: histogram replay,
+LDS-resident subtrees) with the CPU oracle, bit for bit.  Needs an MI355X."""
+import numpy as np
+import pytest
+from gpu_helpers import assert_same_build, fbits, run_both
+
+import inputs
+
+pytestmark = pytest.mark.gpu
+
+BATCHED = 1
+
+
+@pytest.mark.parametrize("levels_mode", [0, 1])
+def test_batched_orbit_20_10(orb, oracle, levels_mode):
+    x, y, z = oracle.generate(1 << 20)
+    g, r = run_both(orb, oracle, x, y, z, 1024, levels_mode, form=BATCHED)
+    assert_same_build(g, r)
+
+
+@pytest.mark.parametrize("cmp,partition", [(1, 0), (1, 1)])
+def test_batched_gpu_semantics(orb, oracle, cmp, partition):
+    x, y, z = oracle.generate(1 << 18)
+    g, r = run_both(orb, oracle, x, y, z, 256, 1, cmp, partition, form=BATCHED)
+    assert_same_build(g, r)
+
+
+def test_batched_tie_heavy(orb, oracle):
+    x, y, z = inputs.quantized((1 << 18) + 17, levels=512, seed=8)
+    g, r = run_both(orb, oracle, x, y, z, 512, 1, form=BATCHED)
+    assert_same_build(g, r)
+
+
+@pytest.mark.parametrize("n,d", [(5, 2), (1000, 7), (4099, 12), (33333, 100), (70000, 1),
+                                 (3000, 64), (2048, 128), (2049, 128)])
+def test_batched_small_and_odd(orb, oracle, n, d):
+    x, y, z = oracle.generate(n)
+    g, r = run_both(orb, oracle, x, y, z, d, 1, form=BATCHED)
+    assert_same_build(g, r)
+
+
+@pytest.mark.parametrize("n,d,levels_mode", [(1 << 16, 1024, 1), (1 << 16, 1024, 0),
+                                             (1 << 18, 1 << 14, 1), (100000, 777, 1)])
+def test_batched_subtree_levels(orb, oracle, n, d, levels_mode):
+    """Deep trees: the last levels run in the LDS-resident subtree kernel."""
+    x, y, z = oracle.generate(n)
+    g, r = run_both(orb, oracle, x, y, z, d, levels_mode, form=BATCHED)
+    assert_same_build(g, r)
+
+
+def test_batched_clustered_mixed_paths(orb, oracle):
+    """Uneven cells: some reach the subtree size early, others stay large."""
+    x, y, z = inputs.clustered(1 << 18, n_spheres=16, a=0.01, seed=5)
+    g, r = run_both(orb, oracle, x, y, z, 1 << 10, 1, form=BATCHED)
+    assert_same_build(g, r)
+
+
+def test_batched_repeatable_and_equal_to_plain(orb, oracle):
+    n, d = 1 << 19, 512
+    ctx = orb.Context(n, d, generate=True)
+    ctx.save_particles()
+    outs = []
+    for form in (0, 1, 1):
+        ctx.restore_particles()
+        info = ctx.build(form, orb.LEVELS_FULL)
+        outs.append((info["level_passes"], ctx.heap(), ctx.ranges(), ctx.get_particles()))
+    ctx.close()
+    for o in outs[1:]:
+        assert o[0] == outs[0][0]
+        assert np.array_equal(o[2], outs[0][2])
+        for a, b in zip(o[3], outs[0][3]):
+            assert np.array_equal(fbits(a), fbits(b))
+        for name in ("cutMarginLeft", "cutMarginRight"):
+            assert np.array_equal(fbits(o[1][name]), fbits(outs[0][1][name]))
