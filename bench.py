@@ -50,6 +50,22 @@ def run_form(orb, ctx, form, levels, steps, warmup):
     return times, tm_total, info
 
 
+def committed_traffic(kernel="orb::k_count"):
+    """HBM bytes per launch of the count kernel from the committed FETCH_SIZE /
+    WRITE_SIZE summary (rocprofv3 counter passes, scaled by the copy-kernel
+    calibration stated in that file); None if absent."""
+    path = os.path.join(ROOT, "profiles", "orb_build_c2_pmc_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 6 and cells[0] == f"`{kernel}`":
+            read_mb = float(cells[3].replace(",", ""))
+            write_mb = float(cells[5].replace(",", ""))
+            return int((read_mb + write_mb) * 1e6)
+    return None
+
+
 def cpu_baseline(log2n, d, levels):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
@@ -138,7 +154,9 @@ def main():
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4),
-            "traffic": None,
+            "traffic": (committed_traffic() if (args.log2n, args.d, args.levels) == (27, 1024, "full")
+                        else None),
+            "traffic_source": "profiles/orb_build_c2_pmc_traffic.md (FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
             "launches": tm["count_launches"],
             "avg_launch_us": round(tm["count_ms"] * 1e3 / max(tm["count_launches"], 1), 2),
             "bytes_per_launch": int(tm["count_bytes"] / max(tm["count_launches"], 1)),

@@ -19,7 +19,7 @@
 //    ballot scans) and writes x, y, z back once;
 //  * the partition of the remaining (large) cells uses the same exact Hoare
 //    kernels as the plain form.
-#include "orb_internal.hpp"
+#include "orb_device.hpp"
 
 namespace orb {
 
@@ -170,17 +170,23 @@ __device__ inline float candidate(int t, float lo, float hi)
     return cell_cut(lo, hi);
 }
 
-// number of candidates <= v (lt) or < v (le) in the sorted cand[1..255]
-__device__ inline int bin_of(const float *cand, float v, int le)
+// Bin of key v among the sorted candidates cand[1..255] (cand[0] = -inf,
+// cand[256] = +inf): the number of candidates <= v (lt) or < v (le).  Keys
+// outside the margins [lo, hi] never reach here.  The candidates sit close to
+// lo + j*(hi-lo)/256, so an arithmetic estimate is corrected by a short walk
+// of exact comparisons; the walk alone makes the result exact.
+__device__ inline int bin_of(const float *cand, float v, int le, float lo, float scale)
 {
-    int pos = 0;
-#pragma unroll
-    for (int s = kBins / 2; s >= 1; s >>= 1) {
-        float c = cand[pos + s];
-        bool go = le ? (c < v) : (c <= v);
-        pos += go ? s : 0;
+    float ef = fminf(fmaxf((v - lo) * scale, 0.0f), (float)(kBins - 1));
+    int e = (int)ef;
+    if (le) {
+        while (e > 0 && cand[e] >= v) --e;
+        while (e < kBins - 1 && cand[e + 1] < v) ++e;
+    } else {
+        while (e > 0 && cand[e] > v) --e;
+        while (e < kBins - 1 && cand[e + 1] <= v) ++e;
     }
-    return pos;
+    return e;
 }
 
 __global__ __launch_bounds__(kBlock) void k_b_hist(const CellRec *recs, int32_t nCells,
@@ -189,8 +195,9 @@ __global__ __launch_bounds__(kBlock) void k_b_hist(const CellRec *recs, int32_t 
                                                    const float *y, const float *z, int le,
                                                    uint16_t *chunk_hist)
 {
-    __shared__ float cand[kBins];
+    __shared__ float cand[kBins + 1];
     __shared__ uint32_t hist[4][kBins];
+    __shared__ uint32_t sm[4];
     uint32_t q = blockIdx.x;
     uint32_t total = first[nCells];
     if (q >= total) return;
@@ -201,20 +208,33 @@ __global__ __launch_bounds__(kBlock) void k_b_hist(const CellRec *recs, int32_t 
     uint32_t e = b + (uint32_t)kChunk;
     if (e > r.end) e = r.end;
     const int t = threadIdx.x;
-    if (t >= 1) cand[t] = candidate(t, r.margin_l, r.margin_r);
+    const float lo = r.margin_l, hi = r.margin_r;
+    if (t >= 1) cand[t] = candidate(t, lo, hi);
+    else cand[0] = -INFINITY;
+    if (t == 0) cand[kBins] = INFINITY;
     hist[0][t] = 0u;
     hist[1][t] = 0u;
     hist[2][t] = 0u;
     hist[3][t] = 0u;
     __syncthreads();
+    const float width = hi - lo;
+    const float scale = width > 0.0f ? (float)kBins / width : 0.0f;
     uint32_t *h = hist[t >> 6];
     const float *key = r.axis == 0 ? x : (r.axis == 1 ? y : z);
+    // keys below lo fall in bin 0 and keys above hi in bin 255 for both
+    // comparisons: count them in registers, bin the rest
+    uint32_t below = 0, above = 0;
+    auto put = [&](float v) {
+        if (v < lo) below++;
+        else if (v > hi) above++;
+        else atomicAdd(&h[bin_of(cand, v, le, lo, scale)], 1u);
+    };
     uint32_t ab = (b + 3u) & ~3u;
     if (ab > e) ab = e;
     uint32_t ae = e & ~3u;
     if (ae < ab) ae = ab;
-    if ((uint32_t)t < ab - b) atomicAdd(&h[bin_of(cand, key[b + t], le)], 1u);
-    if ((uint32_t)t < e - ae) atomicAdd(&h[bin_of(cand, key[ae + t], le)], 1u);
+    if ((uint32_t)t < ab - b) put(key[b + t]);
+    if ((uint32_t)t < e - ae) put(key[ae + t]);
     const float4 *k4 = reinterpret_cast<const float4 *>(key + ab);
     const uint32_t nq = (ae - ab) >> 2;
     float4 v[kPerThread / 4];
@@ -227,45 +247,67 @@ __global__ __launch_bounds__(kBlock) void k_b_hist(const CellRec *recs, int32_t 
     for (int u = 0; u < kPerThread / 4; ++u) {
         uint32_t j = (uint32_t)t + (uint32_t)u * kBlock;
         if (j < nq) {
-            atomicAdd(&h[bin_of(cand, v[u].x, le)], 1u);
-            atomicAdd(&h[bin_of(cand, v[u].y, le)], 1u);
-            atomicAdd(&h[bin_of(cand, v[u].z, le)], 1u);
-            atomicAdd(&h[bin_of(cand, v[u].w, le)], 1u);
+            put(v[u].x);
+            put(v[u].y);
+            put(v[u].z);
+            put(v[u].w);
         }
     }
-    __syncthreads();
+    const uint32_t ba = block_sum(below | (above << 16), sm);  // each <= 4096
     uint32_t sum = hist[0][t] + hist[1][t] + hist[2][t] + hist[3][t];  // <= 4096
+    if (t == 0) sum += ba & 0xffffu;
+    if (t == kBins - 1) sum += ba >> 16;
     chunk_hist[(size_t)q * kBins + t] = (uint16_t)sum;
 }
 
 // Sums chunk histograms into cell histograms: block g takes chunks
-// [g*64, g*64+64), thread t owns bin t, one atomic per (block, cell, bin).
+// [g*64, g*64+64), thread t owns bin t; the 64 rows are loaded 16 at a time
+// (independent loads), and one atomic per (block, cell, bin) flushes a run.
 __global__ __launch_bounds__(kBlock) void k_b_hist_reduce(const CellRec *recs, int32_t nCells,
                                                           const uint32_t *first,
                                                           const uint32_t *chunk_cell,
                                                           const uint16_t *chunk_hist,
                                                           uint32_t *cell_hist)
 {
+    __shared__ int32_t scell[kReduceRows];  // cell of each row, -1 if not live
     const uint32_t total = first[nCells];
     const uint32_t q0 = blockIdx.x * (uint32_t)kReduceRows;
     if (q0 >= total) return;
-    uint32_t q1 = q0 + kReduceRows;
-    if (q1 > total) q1 = total;
+    const uint32_t nrows = total - q0 < (uint32_t)kReduceRows ? total - q0 : (uint32_t)kReduceRows;
     const int t = threadIdx.x;
-    uint32_t cur = chunk_cell[q0];
-    bool live = !(recs[cur].found || recs[cur].flags);
-    uint32_t acc = 0;
-    for (uint32_t q = q0; q < q1; ++q) {
-        uint32_t cell = chunk_cell[q];
-        if (cell != cur) {
-            if (live && acc) atomicAdd(&cell_hist[(size_t)cur * kBins + t], acc);
-            cur = cell;
-            live = !(recs[cur].found || recs[cur].flags);
-            acc = 0;
+    if (t < kReduceRows) {
+        int32_t cl = -1;
+        if ((uint32_t)t < nrows) {
+            uint32_t cell = chunk_cell[q0 + t];
+            const CellRec &r = recs[cell];
+            if (!(r.found || r.flags)) cl = (int32_t)cell;
         }
-        if (live) acc += chunk_hist[(size_t)q * kBins + t];
+        scell[t] = cl;
     }
-    if (live && acc) atomicAdd(&cell_hist[(size_t)cur * kBins + t], acc);
+    __syncthreads();
+    int32_t cur = -1;
+    uint32_t acc = 0;
+    for (uint32_t r0 = 0; r0 < nrows; r0 += 16) {
+        uint32_t v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            uint32_t row = r0 + u;
+            v[u] = (row < nrows && scell[row] >= 0) ? chunk_hist[(size_t)(q0 + row) * kBins + t] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            uint32_t row = r0 + u;
+            if (row >= nrows) break;
+            const int32_t cl = scell[row];
+            if (cl != cur) {
+                if (cur >= 0 && acc) atomicAdd(&cell_hist[(size_t)cur * kBins + t], acc);
+                cur = cl;
+                acc = 0;
+            }
+            acc += v[u];
+        }
+    }
+    if (cur >= 0 && acc) atomicAdd(&cell_hist[(size_t)cur * kBins + t], acc);
 }
 
 __device__ inline void record_level_passes(int32_t *level_passes, int32_t l, int32_t passes)

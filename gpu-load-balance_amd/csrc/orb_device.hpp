@@ -1,0 +1,64 @@
+// orb_device.hpp -- wave64 and block primitives shared by the kernels.
+#pragma once
+
+#include "orb_internal.hpp"
+
+namespace orb {
+
+// ---------------------------------------------------------------------------
+// wave64 / block primitives
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ inline T wave_sum(T v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <class T>
+__device__ inline T wave_inclusive_scan(T v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        T t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
+    }
+    return v;
+}
+
+// Sum over a 256-thread block; every thread gets the result.
+template <class T>
+__device__ inline T block_sum(T v, T *smem4)
+{
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) smem4[w] = v;
+    __syncthreads();
+    T r = smem4[0] + smem4[1] + smem4[2] + smem4[3];
+    __syncthreads();
+    return r;
+}
+
+// Exclusive scan over a 256-thread block; *total gets the block sum.
+template <class T>
+__device__ inline T block_exclusive_scan(T v, T *smem4, T *total)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    T inc = wave_inclusive_scan(v);
+    if (lane == 63) smem4[w] = inc;
+    __syncthreads();
+    T pre = 0;
+    for (int i = 0; i < w; ++i) pre += smem4[i];
+    *total = smem4[0] + smem4[1] + smem4[2] + smem4[3];
+    __syncthreads();
+    return pre + inc - v;
+}
+
+__device__ inline const float *axis_ptr(const float *x, const float *y, const float *z, int a)
+{
+    return a == 0 ? x : (a == 1 ? y : z);
+}
+
+}  // namespace orb
