@@ -28,11 +28,15 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_form(orb, ctx, form, levels, steps, warmup):
+def run_form(orb, ctx, form, levels, steps, warmup, timed_events):
+    """Times `steps` builds.  With timed_events the kernels are bracketed by HIP
+    events inside the timed builds (the plain form: the count-kernel roofline is
+    measured live); without, the batched form replays its captured graph and
+    one extra, untimed build with events gives the kernel breakdown."""
     for _ in range(warmup):
         ctx.restore_particles()
         ctx.build(form, levels)
-    ctx.enable_timing(True)
+    ctx.enable_timing(timed_events)
     times = []
     tm_total = {"count_ms": 0.0, "partition_ms": 0.0, "other_ms": 0.0, "count_bytes": 0,
                 "count_launches": 0}
@@ -43,9 +47,17 @@ def run_form(orb, ctx, form, levels, steps, warmup):
         info = ctx.build(form, levels)   # synchronizes the stream before returning
         t1 = time.perf_counter()
         times.append((t1 - t0) * 1e3)
+        if timed_events:
+            tm = ctx.timing()
+            for k in tm_total:
+                tm_total[k] += tm[k]
+    if not timed_events:
+        ctx.enable_timing(True)
+        ctx.restore_particles()
+        ctx.build(form, levels)
         tm = ctx.timing()
         for k in tm_total:
-            tm_total[k] += tm[k]
+            tm_total[k] = tm[k] * steps   # per-step figures below divide by steps
     ctx.enable_timing(False)
     return times, tm_total, info
 
@@ -108,7 +120,8 @@ def main():
         form = orb.FORM_PLAIN if name == "plain" else orb.FORM_BATCHED
         log(f"[bench] form={name} warmup={args.warmup} steps={args.steps}")
         try:
-            times, tm, info = run_form(orb, ctx, form, levels, args.steps, args.warmup)
+            times, tm, info = run_form(orb, ctx, form, levels, args.steps, args.warmup,
+                                       timed_events=(name == "plain"))
         except orb.OrbError as e:
             if e.code == orb.ERR_STATE:
                 log(f"[bench] form={name} unavailable")

@@ -712,6 +712,14 @@ int orb_get_pass_counts(orb_ctx *ctx, uint32_t *counts, int64_t cap)
     return ORB_OK;
 }
 
+int orb_enable_graph(orb_ctx *ctx, bool on)
+{
+    Ctx *c = C(ctx);
+    if (!c) return ORB_ERR_ARG;
+    c->use_graph = on;
+    return ORB_OK;
+}
+
 int orb_enable_timing(orb_ctx *ctx, bool on)
 {
     Ctx *c = C(ctx);

@@ -34,6 +34,7 @@ struct BatchedState {
     int32_t d = 0, levels_mode = -1;
     int64_t cap = 0;
     int32_t D = 0;
+    bool complete = true;              // every level lists all its splittable cells
     std::vector<int32_t> level_n;      // [D+1] cells per level (index l)
     std::vector<int64_t> level_off;    // [D+1] offset into level_ids
     DevBuf<int32_t> level_ids;         // all levels' heap ids, concatenated
@@ -45,6 +46,10 @@ struct BatchedState {
     DevBuf<uint32_t> trace;            // [cap][32]
     DevBuf<int32_t> level_passes;      // [64]
     PinBuf<int32_t> hpasses;           // [64]
+    // captured build: valid for one (levels mode, comparison, partition, trace)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int32_t gkey[4] = {-1, -1, -1, -1};
 };
 
 // ---------------------------------------------------------------------------
@@ -100,12 +105,17 @@ static int make_schedule(Ctx &c, BatchedState &b, int32_t levels_mode, bool *com
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
+// Fresh build state: root cell, empty heap and flags, range(0) = [0, n),
+// cleared status and pass counters (no memset or host copy: graph-safe).
 __global__ void k_b_init(orb_cell *heap, uint8_t *exists, uint8_t *sub, int64_t cap,
-                         orb_cell root, uint32_t *trace, int64_t trace_n, int32_t *level_passes)
+                         orb_cell root, uint32_t *trace, int64_t trace_n, int32_t *level_passes,
+                         uint32_t *ranges, uint32_t n, int32_t *status)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t k = i; k < cap; k += stride) {
+        ranges[2 * k] = 0u;
+        ranges[2 * k + 1] = k == 0 ? n : 0u;
         orb_cell z;
         uint32_t *w = reinterpret_cast<uint32_t *>(&z);
 #pragma unroll
@@ -116,6 +126,7 @@ __global__ void k_b_init(orb_cell *heap, uint8_t *exists, uint8_t *sub, int64_t 
     }
     for (int64_t k = i; k < trace_n; k += stride) trace[k] = 0xffffffffu;
     if (i < 64) level_passes[i] = 0;
+    if (i < 2) status[i] = 0;
 }
 
 // Level records from the device heap; small cells are marked for the
@@ -624,7 +635,9 @@ static int ensure_state(Ctx &c, int32_t levels_mode, bool *complete)
     if (!c.batched) c.batched = new (std::nothrow) BatchedState;
     if (!c.batched) return ORB_ERR_NOMEM;
     BatchedState &b = *c.batched;
-    ORB_TRY(make_schedule(c, b, levels_mode, complete));
+    if (b.levels_mode != levels_mode || b.d != c.d) ORB_TRY(make_schedule(c, b, levels_mode, complete));
+    else *complete = b.complete;
+    b.complete = *complete;
     ORB_TRY(b.heap.alloc((size_t)c.cap));
     ORB_TRY(b.exists.alloc((size_t)c.cap));
     ORB_TRY(b.sub.alloc((size_t)c.cap));
@@ -633,37 +646,38 @@ static int ensure_state(Ctx &c, int32_t levels_mode, bool *complete)
     ORB_TRY(b.level_passes.alloc(64));
     ORB_TRY(b.hpasses.alloc(64));
     if (c.trace) ORB_TRY(b.trace.alloc((size_t)c.cap * ORB_MAX_PASSES));
+    // every buffer the launches touch exists before a capture starts
+    ORB_TRY(preallocate_scratch(c));
     return ORB_OK;
 }
 
-int build_batched(Ctx &c, int32_t levels_mode, orb_build_info *info)
+static void drop_graph(BatchedState &b)
 {
-    bool complete = true;
-    ORB_TRY(ensure_state(c, levels_mode, &complete));
+    if (b.exec) (void)hipGraphExecDestroy(b.exec);
+    if (b.graph) (void)hipGraphDestroy(b.graph);
+    b.exec = nullptr;
+    b.graph = nullptr;
+    b.gkey[0] = b.gkey[1] = b.gkey[2] = b.gkey[3] = -1;
+}
+
+// All launches of one batched build, in stream order; no allocation, no
+// host synchronisation, no memset: the sequence can be captured.
+static int enqueue_build(Ctx &c, int32_t levels_mode)
+{
     BatchedState &b = *c.batched;
-    const bool use_subtree = complete && c.part == ORB_PART_HOARE;
+    const bool use_subtree = b.complete && c.part == ORB_PART_HOARE;
     const int le = c.cmp == ORB_CMP_LE ? 1 : 0;
     uint32_t *trace = c.trace ? b.trace.p : nullptr;
     const int64_t trace_n = c.trace ? c.cap * ORB_MAX_PASSES : 0;
-    timing_reset(c);
-
-    // fresh state: range(0) = [0, n), root cell, clean flags
-    ORB_TRY(launch_zero_u32(c.stream, c.ranges.p, 2 * (size_t)c.cap));
-    c.hranges.p[0] = 0;
-    c.hranges.p[1] = (uint32_t)c.n;
-    ORB_HIP(hipMemcpyAsync(c.ranges.p, c.hranges.p, 2 * sizeof(uint32_t), hipMemcpyHostToDevice,
-                           c.stream));
-    ORB_TRY(launch_zero_u32(c.stream, reinterpret_cast<uint32_t *>(c.status.p), 2));
     const orb_cell root = cell_root(c.d);
     hipLaunchKernelGGL(k_b_init, dim3(1024), dim3(kBlock), 0, c.stream, b.heap.p, b.exists.p,
-                       b.sub.p, (int64_t)c.cap, root, trace, trace_n, b.level_passes.p);
+                       b.sub.p, (int64_t)c.cap, root, trace, trace_n, b.level_passes.p, c.ranges.p,
+                       (uint32_t)c.n, c.status.p);
     ORB_HIP(hipGetLastError());
-    c.table_ids.clear();
 
     const int32_t lmax = levels_mode == ORB_LEVELS_REFERENCE ? b.D - 1 : b.D;
     for (int32_t l = 1; l <= lmax; ++l) {
         const int32_t nCells = b.level_n[(size_t)l];
-        info->nlevels = l;
         if (nCells == 0) continue;
         const int32_t *ids = b.level_ids.p + b.level_off[(size_t)l];
         unsigned g1 = (unsigned)((nCells + 1 + kBlock - 1) / kBlock);
@@ -704,6 +718,41 @@ int build_batched(Ctx &c, int32_t levels_mode, orb_build_info *info)
         else ORB_TRY(launch_stable_le(c, nCells));
         ORB_TRY(timing_pair(c, tp, kTimePartition));
     }
+    return ORB_OK;
+}
+
+int build_batched(Ctx &c, int32_t levels_mode, orb_build_info *info)
+{
+    bool complete = true;
+    ORB_TRY(ensure_state(c, levels_mode, &complete));
+    BatchedState &b = *c.batched;
+    timing_reset(c);
+    c.table_ids.clear();
+
+    const int32_t key[4] = {levels_mode, c.cmp, c.part, c.trace ? 1 : 0};
+    const bool graph = c.use_graph && !c.timing;
+    if (graph) {
+        bool same = b.exec != nullptr;
+        for (int k = 0; k < 4 && same; ++k) same = b.gkey[k] == key[k];
+        if (!same) {
+            drop_graph(b);
+            ORB_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+            int rc = enqueue_build(c, levels_mode);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(c.stream, &g);
+            if (rc < 0) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            ORB_HIP(e);
+            b.graph = g;
+            ORB_HIP(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
+            for (int k = 0; k < 4; ++k) b.gkey[k] = key[k];
+        }
+        ORB_HIP(hipGraphLaunch(b.exec, c.stream));
+    } else {
+        ORB_TRY(enqueue_build(c, levels_mode));
+    }
     c.table_ids.clear();
 
     // results back to the host mirror
@@ -725,7 +774,9 @@ int build_batched(Ctx &c, int32_t levels_mode, orb_build_info *info)
     }
     ORB_HIP(hipStreamSynchronize(c.stream));
     ORB_TRY(timing_collect(c));
+    const int32_t lmax = levels_mode == ORB_LEVELS_REFERENCE ? b.D - 1 : b.D;
     for (int32_t l = 1; l <= lmax; ++l) {
+        info->nlevels = l;
         if (b.level_n[(size_t)l] == 0) continue;
         info->level_passes[l - 1] = b.hpasses.p[l];
         info->total_passes += b.hpasses.p[l];
@@ -741,6 +792,7 @@ void batched_release(Ctx &c)
 {
     if (!c.batched) return;
     BatchedState &b = *c.batched;
+    drop_graph(b);
     b.level_ids.release();
     b.heap.release();
     b.exists.release();

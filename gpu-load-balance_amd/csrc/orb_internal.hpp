@@ -319,6 +319,7 @@ struct Ctx {
     std::vector<uint32_t> trace_counts;
     bool trace = false;
     bool timing = false;
+    bool use_graph = true;    // batched form: capture the build once, replay it
     orb_timing tm{};
     hipEvent_t ev[2] = {nullptr, nullptr};
     hipEvent_t ev_upload = nullptr;  // last H2D out of hcells (pinned staging reuse)
@@ -356,7 +357,7 @@ int invalidate_stamps(Ctx &c, int32_t nCells);
 int launch_stable_le(Ctx &c, int32_t nCells);
 int launch_make_axis(Ctx &c, int32_t nCells);
 int launch_cell_sizes(Ctx &c, int32_t nCells, uint32_t *dst);
-int launch_generate(Ctx &c);
+int preallocate_scratch(Ctx &c);
 int ensure_level_capacity(Ctx &c, int64_t nCells);
 
 // batched build (orb_batched.hip)

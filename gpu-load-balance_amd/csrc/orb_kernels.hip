@@ -139,6 +139,16 @@ static int scan_exclusive(Ctx &c, const T *in, T *out, size_t m, DevBuf<uint64_t
     return ORB_OK;
 }
 
+int preallocate_scratch(Ctx &c)
+{
+    // the largest scan: the chunk counts of a level (max_chunks + 1 entries)
+    size_t m = (size_t)c.max_chunks + 1;
+    size_t np = (m + kScanTile - 1) / kScanTile;
+    ORB_TRY(c.scan_partials.alloc(np));
+    if (c.part == ORB_PART_STABLE_LE) ORB_TRY(c.tmp.alloc(3 * (size_t)c.stride));
+    return ORB_OK;
+}
+
 int scan_exclusive_u32(Ctx &c, const uint32_t *in, uint32_t *out, size_t m)
 {
     return scan_exclusive<uint32_t>(c, in, out, m, c.scan_partials);

@@ -100,6 +100,7 @@ PROTOTYPES = {
     "orb_get_pass_counts": (C.c_int, [_vp, _vp, _i64]),
     "orb_get_timing": (C.c_int, [_vp, C.POINTER(Timing)]),
     "orb_enable_timing": (C.c_int, [_vp, _b]),
+    "orb_enable_graph": (C.c_int, [_vp, _b]),
     "orb_error_string": (C.c_char_p, [C.c_int]),
     "orb_last_hip_error": (C.c_int, []),
     "orb_device_count": (C.c_int, []),
@@ -279,6 +280,9 @@ class Context:
         pc = np.zeros((self.cap, MAX_PASSES), np.uint32)
         _check(lib().orb_get_pass_counts(self.h, _p(pc), self.cap), "orb_get_pass_counts")
         return pc
+
+    def enable_graph(self, on=True):
+        _check(lib().orb_enable_graph(self.h, bool(on)), "orb_enable_graph")
 
     def enable_timing(self, on=True):
         _check(lib().orb_enable_timing(self.h, bool(on)), "orb_enable_timing")

@@ -219,6 +219,12 @@ int orb_get_exists(orb_ctx *ctx, uint8_t *exists, int64_t cap);
 int orb_enable_trace(orb_ctx *ctx, bool on);
 int orb_get_pass_counts(orb_ctx *ctx, uint32_t *counts, int64_t cap);
 
+/* Batched form: capture the whole build into a hipGraph on first use and
+ * replay it afterwards (default on; a build with timing enabled runs
+ * eagerly).  The capture is redone when the level mode, comparison,
+ * partition or trace setting changes.                                    */
+int orb_enable_graph(orb_ctx *ctx, bool on);
+
 /* Kernel timing of the last build (device events), milliseconds.         */
 typedef struct orb_timing {
     double count_ms;       /* bisection count/histogram kernels            */

@@ -72,3 +72,35 @@ def test_batched_repeatable_and_equal_to_plain(orb, oracle):
             assert np.array_equal(fbits(a), fbits(b))
         for name in ("cutMarginLeft", "cutMarginRight"):
             assert np.array_equal(fbits(o[1][name]), fbits(outs[0][1][name]))
+
+
+def test_batched_graph_replays_match_eager(orb, oracle):
+    """The captured build replayed several times, with eager work between the
+    replays, gives the eager build's results every time."""
+    n, d = (1 << 18) + 5, 300
+    x, y, z = oracle.generate(n)
+    ctx = orb.Context(n, d, generate=False)
+    ctx.set_particles(x, y, z)
+    ctx.save_particles()
+    ctx.enable_graph(False)
+    ctx.enable_trace(True)
+    ref_info = ctx.build(orb.FORM_BATCHED, orb.LEVELS_FULL)
+    ref = (ctx.heap(), ctx.ranges(), ctx.get_particles(), ctx.pass_counts())
+    ctx.enable_graph(True)
+    for rep in range(4):
+        ctx.restore_particles()
+        info = ctx.build(orb.FORM_BATCHED, orb.LEVELS_FULL)
+        assert info["level_passes"] == ref_info["level_passes"], rep
+        assert np.array_equal(ctx.ranges(), ref[1]), rep
+        for a, b in zip(ctx.get_particles(), ref[2]):
+            assert np.array_equal(fbits(a), fbits(b)), rep
+        for name in ("cutMarginLeft", "cutMarginRight"):
+            assert np.array_equal(fbits(ctx.heap()[name]), fbits(ref[0][name])), rep
+        assert np.array_equal(ctx.pass_counts(), ref[3]), rep
+        # eager work on the same context between replays
+        ctx.restore_particles()
+        ctx.build(orb.FORM_PLAIN, orb.LEVELS_REFERENCE)
+        ctx.restore_particles()
+    ctx.close()
+    r = oracle.build(x, y, z, d, levels_mode=1)
+    assert np.array_equal(ref[1], r.ranges)
