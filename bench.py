@@ -101,7 +101,15 @@ def main():
     ap.add_argument("--forms", default="plain,batched")
     ap.add_argument("--cpu-log2n", type=int, default=26)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--config", choices=["c2", "c3", "c4"], default="c2",
+                    help="c2: uniform 2^27 -> 2^10; c3: clustered 2^26 -> 2^14; "
+                         "c4: uniform 2^28 -> 2^20 with per-level times")
+    ap.add_argument("--per-level", action="store_true")
     args = ap.parse_args()
+    if args.config == "c3":
+        args.log2n, args.d = 26, 1 << 14
+    elif args.config == "c4":
+        args.log2n, args.d, args.per_level = 28, 1 << 20, True
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -111,8 +119,15 @@ def main():
     import orb_amd as orb
     levels = orb.LEVELS_FULL if args.levels == "full" else orb.LEVELS_REFERENCE
     n = 1 << args.log2n
-    log(f"[bench] generating {n} particles")
-    ctx = orb.Context(n, args.d, generate=True)
+    log(f"[bench] generating {n} particles ({args.config})")
+    if args.config == "c3":
+        import inputs
+        x, y, z = inputs.clustered(n, n_spheres=64, a=0.01, background=0.1, seed=2022)
+        ctx = orb.Context(n, args.d, generate=False)
+        ctx.set_particles(x, y, z)
+        del x, y, z
+    else:
+        ctx = orb.Context(n, args.d, generate=True)
     ctx.save_particles()
 
     res = {}
@@ -129,10 +144,34 @@ def main():
             raise
         res[name] = {"ms": sum(times) / len(times), "min_ms": min(times), "tm": tm,
                      "passes": info["total_passes"], "level_passes": info["level_passes"]}
+        if args.per_level:
+            ctx.enable_timing(True)
+            ctx.restore_particles()
+            ctx.build(form, levels)
+            res[name]["level_ms"] = [round(v, 4) for v in ctx.level_timing()]
+            ctx.enable_timing(False)
         log(f"[bench] form={name} mean {res[name]['ms']:.3f} ms min {min(times):.3f} ms "
             f"passes={info['total_passes']} count_ms/step={tm['count_ms'] / args.steps:.3f} "
             f"partition_ms/step={tm['partition_ms'] / args.steps:.3f}")
     ctx.close()
+
+    if args.config != "c2":
+        # parity-test configurations: report each form's build time and levels
+        out = {"config": args.config, "n_particles": n, "leaves": args.d, "levels": args.levels,
+               "forms": {k: {kk: vv for kk, vv in v.items() if kk != "tm"} for k, v in res.items()},
+               "note": "level_ms: device time per level (HIP events, one instrumented build); "
+                       "min-traffic GB/s per level = 28 B x n / level time"}
+        if args.per_level:
+            for k, v in res.items():
+                if "level_ms" in v:
+                    v["level_GBps_min_traffic"] = [round(28.0 * n / (ms * 1e-3) / 1e9, 1)
+                                                   if ms > 0 else None for ms in v["level_ms"]]
+            out["forms"] = {k: {kk: vv for kk, vv in v.items() if kk != "tm"}
+                            for k, v in res.items()}
+        if "plain" in res and "batched" in res:
+            out["batched_speedup"] = round(res["plain"]["ms"] / res["batched"]["ms"], 3)
+        print(json.dumps(out), flush=True)
+        return
 
     best = "batched" if "batched" in res else "plain"
     r = res[best]

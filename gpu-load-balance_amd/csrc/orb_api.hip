@@ -77,6 +77,7 @@ static void generate_reference(int64_t n, float *x, float *y, float *z)
 void timing_reset(Ctx &c)
 {
     c.tm = orb_timing{};
+    c.level_ms.clear();
     c.tnext = 0;
     c.tpairs.clear();
 }
@@ -108,7 +109,11 @@ int timing_collect(Ctx &c)
     for (const auto &p : c.tpairs) {
         float ms = 0.f;
         ORB_HIP(hipEventElapsedTime(&ms, c.tpool[(size_t)p.a], c.tpool[(size_t)p.b]));
-        if (p.kind == kTimeCount) {
+        if (p.kind >= kTimeLevel) {
+            size_t l = (size_t)(p.kind - kTimeLevel);
+            if (c.level_ms.size() <= l) c.level_ms.resize(l + 1, 0.0);
+            c.level_ms[l] += ms;
+        } else if (p.kind == kTimeCount) {
             c.tm.count_ms += ms;
             c.tm.count_launches += 1;
         } else if (p.kind == kTimePartition) {
@@ -248,13 +253,14 @@ static int count_sizes(Ctx &c, const orb_cell *cells, int32_t nCells, uint32_t *
 // ---------------------------------------------------------------------------
 static int start_build(Ctx &c)
 {
+    c.results_on_device = false;
     c.heap.assign((size_t)c.cap, orb_cell{});
     std::memset(c.heap.data(), 0, sizeof(orb_cell) * c.heap.size());
     c.exists.assign((size_t)c.cap, 0);
     c.heap[0] = cell_root(c.d);
     c.exists[0] = 1;
     if (c.trace) c.trace_counts.assign((size_t)c.cap * ORB_MAX_PASSES, 0xffffffffu);
-    c.tm = orb_timing{};
+    timing_reset(c);
     return reset_ranges(c);
 }
 
@@ -275,6 +281,7 @@ static int build_plain(Ctx &c, int32_t levels_mode, orb_build_info *info)
         info->nlevels = l;
         const int32_t nCells = (int32_t)ids.size();
         if (nCells == 0) continue;
+        const int tlevel = timing_mark(c);
         cells.resize((size_t)nCells);
         for (int32_t i = 0; i < nCells; ++i) cells[i] = c.heap[(size_t)ids[i]];
         counts.assign((size_t)nCells, 0u);
@@ -339,8 +346,10 @@ static int build_plain(Ctx &c, int32_t levels_mode, orb_build_info *info)
             info->error_cell = bad_id;
             return rc;
         }
+        ORB_TRY(timing_pair(c, tlevel, kTimeLevel + l));
     }
-    return ORB_OK;
+    ORB_HIP(hipStreamSynchronize(c.stream));
+    return timing_collect(c);
 }
 
 }  // namespace orb
@@ -669,6 +678,7 @@ int orb_get_heap(orb_ctx *ctx, orb_cell *heap, int64_t cap)
 {
     Ctx *c = C(ctx);
     if (!c || !heap || cap < c->cap) return ORB_ERR_ARG;
+    if (c->results_on_device) ORB_TRY(batched_pull_results(*c));
     std::memset(heap, 0, sizeof(orb_cell) * (size_t)cap);
     if (!c->heap.empty()) std::memcpy(heap, c->heap.data(), sizeof(orb_cell) * c->heap.size());
     return ORB_OK;
@@ -678,6 +688,7 @@ int orb_get_exists(orb_ctx *ctx, uint8_t *exists, int64_t cap)
 {
     Ctx *c = C(ctx);
     if (!c || !exists || cap < c->cap) return ORB_ERR_ARG;
+    if (c->results_on_device) ORB_TRY(batched_pull_results(*c));
     std::memset(exists, 0, (size_t)cap);
     if (!c->exists.empty()) std::memcpy(exists, c->exists.data(), c->exists.size());
     return ORB_OK;
@@ -707,9 +718,22 @@ int orb_get_pass_counts(orb_ctx *ctx, uint32_t *counts, int64_t cap)
 {
     Ctx *c = C(ctx);
     if (!c || !counts || cap < c->cap) return ORB_ERR_ARG;
+    if (c->results_on_device) ORB_TRY(batched_pull_results(*c));
     if (c->trace_counts.size() != (size_t)c->cap * ORB_MAX_PASSES) return ORB_ERR_STATE;
     std::memcpy(counts, c->trace_counts.data(), sizeof(uint32_t) * c->trace_counts.size());
     return ORB_OK;
+}
+
+int orb_get_level_timing(orb_ctx *ctx, double *ms, int32_t max_levels)
+{
+    Ctx *c = C(ctx);
+    if (!c || !ms || max_levels < 0) return ORB_ERR_ARG;
+    int32_t n = 0;
+    for (int32_t l = 0; l < max_levels; ++l) {
+        ms[l] = (size_t)l < c->level_ms.size() ? c->level_ms[(size_t)l] : 0.0;
+        if (ms[l] > 0.0) n = l + 1;
+    }
+    return n;
 }
 
 int orb_enable_graph(orb_ctx *ctx, bool on)

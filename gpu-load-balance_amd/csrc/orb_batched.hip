@@ -679,6 +679,7 @@ static int enqueue_build(Ctx &c, int32_t levels_mode)
     for (int32_t l = 1; l <= lmax; ++l) {
         const int32_t nCells = b.level_n[(size_t)l];
         if (nCells == 0) continue;
+        const int tlevel = timing_mark(c);
         const int32_t *ids = b.level_ids.p + b.level_off[(size_t)l];
         unsigned g1 = (unsigned)((nCells + 1 + kBlock - 1) / kBlock);
         hipLaunchKernelGGL(k_b_recs, dim3(g1), dim3(kBlock), 0, c.stream, ids, nCells, b.heap.p,
@@ -717,6 +718,7 @@ static int enqueue_build(Ctx &c, int32_t levels_mode)
         if (c.part == ORB_PART_HOARE) ORB_TRY(launch_hoare(c, nCells));
         else ORB_TRY(launch_stable_le(c, nCells));
         ORB_TRY(timing_pair(c, tp, kTimePartition));
+        ORB_TRY(timing_pair(c, tlevel, kTimeLevel + l));
     }
     return ORB_OK;
 }
@@ -755,24 +757,14 @@ int build_batched(Ctx &c, int32_t levels_mode, orb_build_info *info)
     }
     c.table_ids.clear();
 
-    // results back to the host mirror
+    // status and pass counts come back now; the tree stays in HBM until a
+    // getter asks for it (batched_pull_results)
     ORB_HIP(hipMemcpyAsync(c.hstatus.p, c.status.p, 2 * sizeof(int32_t), hipMemcpyDeviceToHost,
                            c.stream));
     ORB_HIP(hipMemcpyAsync(b.hpasses.p, b.level_passes.p, 64 * sizeof(int32_t),
                            hipMemcpyDeviceToHost, c.stream));
-    c.heap.resize((size_t)c.cap);
-    c.exists.resize((size_t)c.cap);
-    ORB_HIP(hipMemcpyAsync(c.heap.data(), b.heap.p, sizeof(orb_cell) * (size_t)c.cap,
-                           hipMemcpyDeviceToHost, c.stream));
-    ORB_HIP(hipMemcpyAsync(c.exists.data(), b.exists.p, (size_t)c.cap, hipMemcpyDeviceToHost,
-                           c.stream));
-    if (c.trace) {
-        c.trace_counts.resize((size_t)c.cap * ORB_MAX_PASSES);
-        ORB_HIP(hipMemcpyAsync(c.trace_counts.data(), b.trace.p,
-                               sizeof(uint32_t) * c.trace_counts.size(), hipMemcpyDeviceToHost,
-                               c.stream));
-    }
     ORB_HIP(hipStreamSynchronize(c.stream));
+    c.results_on_device = true;
     ORB_TRY(timing_collect(c));
     const int32_t lmax = levels_mode == ORB_LEVELS_REFERENCE ? b.D - 1 : b.D;
     for (int32_t l = 1; l <= lmax; ++l) {
@@ -785,6 +777,27 @@ int build_batched(Ctx &c, int32_t levels_mode, orb_build_info *info)
         info->error_cell = c.hstatus.p[1];
         return ORB_ERR_UB;
     }
+    return ORB_OK;
+}
+
+int batched_pull_results(Ctx &c)
+{
+    if (!c.batched || !c.results_on_device) return ORB_OK;
+    BatchedState &b = *c.batched;
+    c.heap.resize((size_t)c.cap);
+    c.exists.resize((size_t)c.cap);
+    ORB_HIP(hipMemcpyAsync(c.heap.data(), b.heap.p, sizeof(orb_cell) * (size_t)c.cap,
+                           hipMemcpyDeviceToHost, c.stream));
+    ORB_HIP(hipMemcpyAsync(c.exists.data(), b.exists.p, (size_t)c.cap, hipMemcpyDeviceToHost,
+                           c.stream));
+    if (c.trace && b.trace.p) {
+        c.trace_counts.resize((size_t)c.cap * ORB_MAX_PASSES);
+        ORB_HIP(hipMemcpyAsync(c.trace_counts.data(), b.trace.p,
+                               sizeof(uint32_t) * c.trace_counts.size(), hipMemcpyDeviceToHost,
+                               c.stream));
+    }
+    ORB_HIP(hipStreamSynchronize(c.stream));
+    c.results_on_device = false;
     return ORB_OK;
 }
 

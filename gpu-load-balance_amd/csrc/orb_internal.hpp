@@ -320,6 +320,8 @@ struct Ctx {
     bool trace = false;
     bool timing = false;
     bool use_graph = true;    // batched form: capture the build once, replay it
+    bool results_on_device = false;  // batched form: heap/exists/trace not yet mirrored
+    std::vector<double> level_ms;    // per-level time of the last timed build
     orb_timing tm{};
     hipEvent_t ev[2] = {nullptr, nullptr};
     hipEvent_t ev_upload = nullptr;  // last H2D out of hcells (pinned staging reuse)
@@ -336,7 +338,7 @@ struct Ctx {
 };
 
 // timing helpers (orb_api.hip): marks are -1 when timing is off
-enum { kTimeCount = 0, kTimePartition = 1, kTimeOther = 2 };
+enum { kTimeCount = 0, kTimePartition = 1, kTimeOther = 2, kTimeLevel = 100 /* + level */ };
 void timing_reset(Ctx &c);
 int timing_mark(Ctx &c);
 int timing_pair(Ctx &c, int begin_mark, int kind);
@@ -362,6 +364,7 @@ int ensure_level_capacity(Ctx &c, int64_t nCells);
 
 // batched build (orb_batched.hip)
 int build_batched(Ctx &c, int32_t levels_mode, orb_build_info *info);
+int batched_pull_results(Ctx &c);
 void batched_release(Ctx &c);
 
 }  // namespace orb

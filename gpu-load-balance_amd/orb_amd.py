@@ -101,6 +101,7 @@ PROTOTYPES = {
     "orb_get_timing": (C.c_int, [_vp, C.POINTER(Timing)]),
     "orb_enable_timing": (C.c_int, [_vp, _b]),
     "orb_enable_graph": (C.c_int, [_vp, _b]),
+    "orb_get_level_timing": (C.c_int, [_vp, _vp, _i32]),
     "orb_error_string": (C.c_char_p, [C.c_int]),
     "orb_last_hip_error": (C.c_int, []),
     "orb_device_count": (C.c_int, []),
@@ -280,6 +281,11 @@ class Context:
         pc = np.zeros((self.cap, MAX_PASSES), np.uint32)
         _check(lib().orb_get_pass_counts(self.h, _p(pc), self.cap), "orb_get_pass_counts")
         return pc
+
+    def level_timing(self):
+        ms = np.zeros(64, np.float64)
+        k = _check(lib().orb_get_level_timing(self.h, _p(ms), 64), "orb_get_level_timing")
+        return ms[:k].copy()
 
     def enable_graph(self, on=True):
         _check(lib().orb_enable_graph(self.h, bool(on)), "orb_enable_graph")

@@ -210,7 +210,8 @@ int64_t orb_heap_capacity(int32_t d);
  * range(0) = [0, n) and the root cell [-0.5, 0.5]^3 with cutAxis 0.       */
 int orb_build(orb_ctx *ctx, int32_t form, int32_t levels_mode, orb_build_info *info);
 /* Results of the last build: the cell heap (by heap id, unused slots
- * zero), the cell-to-range map ([cap][2]), and which ids exist.          */
+ * zero), the cell-to-range map ([cap][2]), and which ids exist.  After a
+ * batched build the tree stays in HBM; these getters copy it back.       */
 int orb_get_heap(orb_ctx *ctx, orb_cell *heap, int64_t cap);
 int orb_get_ranges(orb_ctx *ctx, uint32_t *ranges, int64_t cap);
 int orb_get_exists(orb_ctx *ctx, uint8_t *exists, int64_t cap);
@@ -235,6 +236,10 @@ typedef struct orb_timing {
     int32_t pad;
 } orb_timing;
 int orb_get_timing(orb_ctx *ctx, orb_timing *t);
+/* Per-level time of the last build run with timing on: ms[l-1] for level l
+ * (device events from the level's first launch to its last).  Returns the
+ * number of levels written.                                              */
+int orb_get_level_timing(orb_ctx *ctx, double *ms, int32_t max_levels);
 int orb_enable_timing(orb_ctx *ctx, bool on);
 
 /* ------------------------------------------------------------------ */
