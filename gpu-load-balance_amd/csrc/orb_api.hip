@@ -728,10 +728,11 @@ int orb_get_level_timing(orb_ctx *ctx, double *ms, int32_t max_levels)
 {
     Ctx *c = C(ctx);
     if (!c || !ms || max_levels < 0) return ORB_ERR_ARG;
+    // level l (1-based) is ms[l-1]
     int32_t n = 0;
-    for (int32_t l = 0; l < max_levels; ++l) {
-        ms[l] = (size_t)l < c->level_ms.size() ? c->level_ms[(size_t)l] : 0.0;
-        if (ms[l] > 0.0) n = l + 1;
+    for (int32_t l = 1; l <= max_levels; ++l) {
+        ms[l - 1] = (size_t)l < c->level_ms.size() ? c->level_ms[(size_t)l] : 0.0;
+        if (ms[l - 1] > 0.0) n = l;
     }
     return n;
 }

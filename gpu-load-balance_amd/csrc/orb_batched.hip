@@ -26,8 +26,12 @@ namespace orb {
 constexpr int kHistK = 8;                      // bisection steps per histogram round
 constexpr int kBins = 1 << kHistK;             // 256 bins (255 candidate cuts)
 constexpr int kRounds = (ORB_MAX_PASSES + kHistK - 1) / kHistK;  // 4
-constexpr int kSmall = 2048;                   // keys per LDS-resident subtree
-constexpr int kSmallLeaves = 128;              // max leaves under a subtree root
+// Subtree kernel limits.  Cells of a level cluster around n / 2^(l-1); the key
+// budget sits 27% above a power of two so a whole level qualifies at once.
+// LDS: 12 B x kSmall (x, y, z) + 4 B x kSmall (positions) + 2 x kSmallLeaves
+// cell records ~ 50 KB, three blocks per CU.
+constexpr int kSmall = 2600;                   // keys per LDS-resident subtree
+constexpr int kSmallLeaves = 64;               // max leaves under a subtree root
 constexpr int kReduceRows = 64;                // chunk histograms per reduce block
 
 struct BatchedState {
@@ -37,7 +41,7 @@ struct BatchedState {
     bool complete = true;              // every level lists all its splittable cells
     std::vector<int32_t> level_n;      // [D+1] cells per level (index l)
     std::vector<int64_t> level_off;    // [D+1] offset into level_ids
-    DevBuf<int32_t> level_ids;         // all levels' heap ids, concatenated
+    DevBuf<LevelEntry> level_ids;      // all levels' cells, concatenated
     DevBuf<orb_cell> heap;             // [cap]
     DevBuf<uint8_t> exists;            // [cap]
     DevBuf<uint8_t> sub;               // [cap] cell handled inside a subtree kernel
@@ -50,34 +54,33 @@ struct BatchedState {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int32_t gkey[4] = {-1, -1, -1, -1};
+    // cell_hist rows are added to by k_b_hist_reduce and consumed and
+    // re-zeroed by k_b_replay in the same round, so the buffer is all zero
+    // between builds; it is cleared after allocation and after a failure
+    bool hist_clean = false;
 };
 
 // ---------------------------------------------------------------------------
 // schedule (host): ids per level, data-independent
 // ---------------------------------------------------------------------------
-static int make_schedule(Ctx &c, BatchedState &b, int32_t levels_mode, bool *complete)
+int schedule_levels(int32_t d, int32_t levels_mode, int64_t cap,
+                    std::vector<std::vector<int32_t>> &levels, bool *complete)
 {
-    b.d = c.d;
-    b.levels_mode = levels_mode;
-    b.cap = c.cap;
-    b.D = n_levels(c.d);
-    // simulate existence and nLeafCells without data: the split rule fixes
-    // them (left = ceil(n/2), right = n - left)
-    std::vector<uint8_t> ex((size_t)c.cap, 0);
-    std::vector<orb_cell> hp((size_t)c.cap);
+    // existence and nLeafCells follow from d alone (left = ceil(n/2),
+    // right = n - left), so the schedule needs no particle data
+    const int32_t D = n_levels(d);
+    std::vector<uint8_t> ex((size_t)cap, 0);
+    std::vector<orb_cell> hp((size_t)cap);
     std::memset(hp.data(), 0, sizeof(orb_cell) * hp.size());
-    hp[0].nLeafCells = c.d;
+    hp[0].nLeafCells = d;
     ex[0] = 1;
-    std::vector<int32_t> all, ids;
-    b.level_n.assign((size_t)b.D + 2, 0);
-    b.level_off.assign((size_t)b.D + 2, 0);
+    std::vector<int32_t> ids;
+    levels.assign((size_t)D + 1, {});
     *complete = true;
-    for (int32_t l = 1; l <= b.D; ++l) {
+    for (int32_t l = 1; l <= D; ++l) {
         int32_t bad = -1;
-        if (!level_cells(c.d, levels_mode, l, ex, hp, ids, &bad)) return ORB_ERR_UB;
-        if (levels_mode == ORB_LEVELS_REFERENCE && l >= b.D) break;
-        b.level_off[l] = (int64_t)all.size();
-        b.level_n[l] = (int32_t)ids.size();
+        if (!level_cells(d, levels_mode, l, ex, hp, ids, &bad)) return ORB_ERR_UB;
+        if (levels_mode == ORB_LEVELS_REFERENCE && l >= D) break;
         // the subtree kernel needs every cell of depth l-1 with >= 2 leaves
         int64_t a = (int64_t(1) << (l - 1)) - 1, e = (int64_t(1) << l) - 2;
         int32_t expect = 0;
@@ -85,19 +88,40 @@ static int make_schedule(Ctx &c, BatchedState &b, int32_t levels_mode, bool *com
             if (ex[(size_t)id] && hp[(size_t)id].nLeafCells >= 2) expect++;
         if (expect != (int32_t)ids.size()) *complete = false;
         for (int32_t id : ids) {
-            all.push_back(id);
             int32_t nl = hp[(size_t)id].nLeafCells;
             int32_t left = (nl + 1) / 2;
             int32_t lid = left_child_id(id), rid = right_child_id(id);
-            if (rid >= c.cap) return ORB_ERR_ARG;
+            if (rid >= cap) return ORB_ERR_ARG;
             hp[(size_t)lid].nLeafCells = left;
             hp[(size_t)rid].nLeafCells = nl - left;
             ex[(size_t)lid] = ex[(size_t)rid] = 1;
         }
+        levels[(size_t)l] = ids;
+    }
+    return ORB_OK;
+}
+
+static int make_schedule(Ctx &c, BatchedState &b, int32_t levels_mode, bool *complete)
+{
+    b.d = c.d;
+    b.levels_mode = levels_mode;
+    b.cap = c.cap;
+    b.D = n_levels(c.d);
+    std::vector<std::vector<int32_t>> levels;
+    ORB_TRY(schedule_levels(c.d, levels_mode, c.cap, levels, complete));
+    const int32_t lmax = levels_mode == ORB_LEVELS_REFERENCE ? b.D - 1 : b.D;
+    std::vector<LevelEntry> all;
+    b.level_n.assign((size_t)b.D + 2, 0);
+    b.level_off.assign((size_t)b.D + 2, 0);
+    for (int32_t l = 1; l < (int32_t)levels.size(); ++l) {
+        b.level_off[(size_t)l] = (int64_t)all.size();
+        b.level_n[(size_t)l] = (int32_t)levels[(size_t)l].size();
+        for (int32_t id : levels[(size_t)l])
+            all.push_back(LevelEntry{id, 0, (int32_t)c.cap, l, 0, lmax, *complete ? 1 : 0, 0});
     }
     ORB_TRY(b.level_ids.alloc(all.size() + 1));
     if (!all.empty())
-        ORB_HIP(hipMemcpy(b.level_ids.p, all.data(), sizeof(int32_t) * all.size(),
+        ORB_HIP(hipMemcpy(b.level_ids.p, all.data(), sizeof(LevelEntry) * all.size(),
                           hipMemcpyHostToDevice));
     return ORB_OK;
 }
@@ -131,7 +155,7 @@ __global__ void k_b_init(orb_cell *heap, uint8_t *exists, uint8_t *sub, int64_t 
 
 // Level records from the device heap; small cells are marked for the
 // subtree kernel, cells inside a finished subtree are skipped.
-__global__ void k_b_recs(const int32_t *ids, int32_t nCells, const orb_cell *heap,
+__global__ void k_b_recs(const LevelEntry *entries, int32_t nCells, const orb_cell *heap,
                          const uint32_t *ranges, const uint8_t *sub, int use_subtree,
                          CellRec *recs, uint32_t *nchunks)
 {
@@ -141,11 +165,13 @@ __global__ void k_b_recs(const int32_t *ids, int32_t nCells, const orb_cell *hea
         nchunks[nCells] = 0u;
         return;
     }
-    const int32_t id = ids[i];
-    const orb_cell cl = heap[id];
+    const LevelEntry en = entries[i];
+    const int32_t id = en.id;
+    const int64_t g = (int64_t)en.hb + id;
+    const orb_cell cl = heap[g];
     CellRec r;
-    r.begin = ranges[2 * (size_t)id];
-    r.end = ranges[2 * (size_t)id + 1];
+    r.begin = ranges[2 * g];
+    r.end = ranges[2 * g + 1];
     if (r.end < r.begin) r.end = r.begin;
     r.cut = cell_get_cut(cl);
     r.axis = cl.cutAxis;
@@ -155,9 +181,15 @@ __global__ void k_b_recs(const int32_t *ids, int32_t nCells, const orb_cell *hea
     r.margin_r = cl.cutMarginRight;
     r.id = id;
     r.passes = 0;
+    r.hb = en.hb;
+    r.capr = en.capr;
+    r.lp = en.lp;
+    r.st = en.st;
+    r.lmax = en.lmax;
     const uint32_t n = r.end - r.begin;
-    if (sub[id]) r.flags = 1;
-    else if (use_subtree && n <= (uint32_t)kSmall && cl.nLeafCells <= kSmallLeaves) r.flags = 2;
+    if (sub[g]) r.flags = 1;
+    else if (use_subtree && en.sub_ok && n <= (uint32_t)kSmall && cl.nLeafCells <= kSmallLeaves)
+        r.flags = 2;
     else r.flags = 0;
     recs[i] = r;
     nchunks[i] = r.flags ? 0u : (n + (uint32_t)kChunk - 1u) / (uint32_t)kChunk;
@@ -200,18 +232,13 @@ __device__ inline int bin_of(const float *cand, float v, int le, float lo, float
     return e;
 }
 
-__global__ __launch_bounds__(kBlock) void k_b_hist(const CellRec *recs, int32_t nCells,
-                                                   const uint32_t *first,
-                                                   const uint32_t *chunk_cell, const float *x,
-                                                   const float *y, const float *z, int le,
-                                                   uint16_t *chunk_hist)
+__device__ inline void k_b_hist_chunk(uint32_t q, const CellRec *recs, const uint32_t *first,
+                                      const uint32_t *chunk_cell, const float *x, const float *y,
+                                      const float *z, int le, uint16_t *chunk_hist)
 {
     __shared__ float cand[kBins + 1];
     __shared__ uint32_t hist[4][kBins];
     __shared__ uint32_t sm[4];
-    uint32_t q = blockIdx.x;
-    uint32_t total = first[nCells];
-    if (q >= total) return;
     const uint32_t cell = chunk_cell[q];
     const CellRec r = recs[cell];
     if (r.found || r.flags || r.axis < 0 || r.axis > 2) return;
@@ -271,6 +298,19 @@ __global__ __launch_bounds__(kBlock) void k_b_hist(const CellRec *recs, int32_t 
     chunk_hist[(size_t)q * kBins + t] = (uint16_t)sum;
 }
 
+__global__ __launch_bounds__(kBlock) void k_b_hist(const CellRec *recs, int32_t nCells,
+                                                   const uint32_t *first,
+                                                   const uint32_t *chunk_cell, const float *x,
+                                                   const float *y, const float *z, int le,
+                                                   uint16_t *chunk_hist)
+{
+    const uint32_t total = first[nCells];
+    for (uint32_t q = blockIdx.x; q < total; q += gridDim.x) {
+        k_b_hist_chunk(q, recs, first, chunk_cell, x, y, z, le, chunk_hist);
+        __syncthreads();  // the next chunk reuses this block's LDS
+    }
+}
+
 // Sums chunk histograms into cell histograms: block g takes chunks
 // [g*64, g*64+64), thread t owns bin t; the 64 rows are loaded 16 at a time
 // (independent loads), and one atomic per (block, cell, bin) flushes a run.
@@ -282,8 +322,8 @@ __global__ __launch_bounds__(kBlock) void k_b_hist_reduce(const CellRec *recs, i
 {
     __shared__ int32_t scell[kReduceRows];  // cell of each row, -1 if not live
     const uint32_t total = first[nCells];
-    const uint32_t q0 = blockIdx.x * (uint32_t)kReduceRows;
-    if (q0 >= total) return;
+    for (uint32_t q0 = blockIdx.x * (uint32_t)kReduceRows; q0 < total;
+         q0 += gridDim.x * (uint32_t)kReduceRows) {
     const uint32_t nrows = total - q0 < (uint32_t)kReduceRows ? total - q0 : (uint32_t)kReduceRows;
     const int t = threadIdx.x;
     if (t < kReduceRows) {
@@ -308,8 +348,7 @@ __global__ __launch_bounds__(kBlock) void k_b_hist_reduce(const CellRec *recs, i
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
             uint32_t row = r0 + u;
-            if (row >= nrows) break;
-            const int32_t cl = scell[row];
+            const int32_t cl = row < nrows ? scell[row] : cur;
             if (cl != cur) {
                 if (cur >= 0 && acc) atomicAdd(&cell_hist[(size_t)cur * kBins + t], acc);
                 cur = cl;
@@ -319,6 +358,8 @@ __global__ __launch_bounds__(kBlock) void k_b_hist_reduce(const CellRec *recs, i
         }
     }
     if (cur >= 0 && acc) atomicAdd(&cell_hist[(size_t)cur * kBins + t], acc);
+    __syncthreads();  // scell is rewritten for the next tile
+    }
 }
 
 __device__ inline void record_level_passes(int32_t *level_passes, int32_t l, int32_t passes)
@@ -330,9 +371,8 @@ __device__ inline void record_level_passes(int32_t *level_passes, int32_t l, int
 // One wave per cell: prefix sums of the cell histogram in LDS, then lane 0
 // replays up to 8 passes of the bisection controller (orbit.cpp:149-231).
 __global__ __launch_bounds__(kBlock) void k_b_replay(CellRec *recs, int32_t nCells,
-                                                     const uint32_t *cell_hist, orb_cell *heap,
-                                                     uint32_t *trace, int32_t *level_passes,
-                                                     int32_t level)
+                                                     uint32_t *cell_hist, orb_cell *heap,
+                                                     uint32_t *trace, int32_t *level_passes)
 {
     __shared__ uint32_t P[4][kBins + 1];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -340,7 +380,9 @@ __global__ __launch_bounds__(kBlock) void k_b_replay(CellRec *recs, int32_t nCel
     if (c >= nCells) return;  // uniform per wave
     CellRec r = recs[c];
     if (r.found || r.flags) return;
-    const uint4 h4 = reinterpret_cast<const uint4 *>(cell_hist + (size_t)c * kBins)[lane];
+    uint4 *row = reinterpret_cast<uint4 *>(cell_hist + (size_t)c * kBins);
+    const uint4 h4 = row[lane];
+    row[lane] = make_uint4(0u, 0u, 0u, 0u);  // the next round starts from zero
     uint32_t s0 = h4.x, s1 = s0 + h4.y, s2 = s1 + h4.z, s3 = s2 + h4.w;
     uint32_t base = 0;
     {
@@ -362,6 +404,7 @@ __global__ __launch_bounds__(kBlock) void k_b_replay(CellRec *recs, int32_t nCel
     __builtin_amdgcn_wave_barrier();
     if (lane != 0) return;
     const uint32_t n = r.end - r.begin;
+    const float target = difference_target(n, r.nLeaf);
     float L = r.margin_l, R = r.margin_r;
     int node = kBins / 2, step = kBins / 4;
     int32_t passes = r.passes;
@@ -369,9 +412,9 @@ __global__ __launch_bounds__(kBlock) void k_b_replay(CellRec *recs, int32_t nCel
     for (int s = 0; s < kHistK && passes < ORB_MAX_PASSES; ++s) {
         const float cut = cell_cut(L, R);
         const uint32_t cnt = P[w][node];
-        if (trace) trace[(size_t)r.id * ORB_MAX_PASSES + passes] = cnt;
+        if (trace) trace[((size_t)r.hb + r.id) * ORB_MAX_PASSES + passes] = cnt;
         passes++;
-        const int32_t diff = difference_rule(cnt, n, r.nLeaf);
+        const int32_t diff = difference_from_target(cnt, target);
         if (abs(diff) < 3) {
             found = true;
             break;
@@ -391,37 +434,38 @@ __global__ __launch_bounds__(kBlock) void k_b_replay(CellRec *recs, int32_t nCel
     r.passes = passes;
     r.found = found ? 1 : 0;
     recs[c] = r;
-    orb_cell &hc = heap[r.id];
+    orb_cell &hc = heap[(int64_t)r.hb + r.id];
     hc.cutMarginLeft = L;
     hc.cutMarginRight = R;
     hc.foundCut = found;
-    record_level_passes(level_passes, level, passes);
+    record_level_passes(level_passes, r.lp, passes);
 }
 
 // A cell that hits the pass cap without ending its last round is finished
 // too: the cap rule is part of the replay.  Nothing to do here; the split
 // kernel reads the heap.
 __global__ void k_b_split(const CellRec *recs, int32_t nCells, orb_cell *heap, uint8_t *exists,
-                          int32_t *status, int64_t cap)
+                          int32_t *status)
 {
     int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nCells) return;
     const CellRec r = recs[c];
     if (r.flags) return;
-    const orb_cell p = heap[r.id];
+    const int64_t hb = r.hb;
+    const orb_cell p = heap[hb + r.id];
     orb_cell lc, rc;
     uint32_t *wl = reinterpret_cast<uint32_t *>(&lc);
     uint32_t *wr = reinterpret_cast<uint32_t *>(&rc);
 #pragma unroll
     for (int j = 0; j < (int)(sizeof(orb_cell) / 4); ++j) wl[j] = wr[j] = 0u;
-    if (right_child_id(r.id) >= cap || !cell_split(p, lc, rc)) {
-        if (atomicCAS(&status[0], 0, 1) == 0) status[1] = r.id;
+    if (right_child_id(r.id) >= r.capr || !cell_split(p, lc, rc)) {
+        if (atomicCAS(&status[r.st], 0, 1) == 0) status[r.st + 1] = r.id;
         return;
     }
-    heap[lc.id] = lc;
-    heap[rc.id] = rc;
-    exists[lc.id] = 1;
-    exists[rc.id] = 1;
+    heap[hb + lc.id] = lc;
+    heap[hb + rc.id] = rc;
+    exists[hb + lc.id] = 1;
+    exists[hb + rc.id] = 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -449,28 +493,29 @@ __device__ inline uint32_t wave_sum_u32(uint32_t v)
 __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32_t nCells,
                                                       orb_cell *heap, uint8_t *exists,
                                                       uint8_t *sub, uint32_t *ranges, float *x,
-                                                      float *y, float *z, int32_t level,
-                                                      int32_t lmax, int le_cmp, uint32_t *trace,
-                                                      int32_t *level_passes, int32_t *status,
-                                                      int64_t cap)
+                                                      float *y, float *z, int le_cmp,
+                                                      uint32_t *trace, int32_t *level_passes,
+                                                      int32_t *status)
 {
     __shared__ float sx[kSmall], sy[kSmall], sz[kSmall];
     __shared__ uint16_t slpos[kSmall], srpos[kSmall];
     __shared__ SubCell cur[kSmallLeaves], nxt[kSmallLeaves];
     __shared__ int32_t s_ncur, s_nvalid, s_err;
-    const int32_t c = blockIdx.x;
-    if (c >= nCells) return;
+    for (int32_t c = blockIdx.x; c < nCells; c += gridDim.x) {
     const CellRec rr = recs[c];
-    if (rr.flags != 2) return;
+    if (rr.flags != 2) continue;  // uniform per block
     const uint32_t begin = rr.begin, n = rr.end - rr.begin;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int64_t hb = rr.hb;
+    const int32_t level = rr.lp & 63, lmax = rr.lmax, lp0 = rr.lp - level;
+    int32_t *st = status + rr.st;
     for (uint32_t k = (uint32_t)t; k < n; k += kBlock) {
         sx[k] = x[begin + k];
         sy[k] = y[begin + k];
         sz[k] = z[begin + k];
     }
     if (t == 0) {
-        cur[0].cell = heap[rr.id];
+        cur[0].cell = heap[hb + rr.id];
         cur[0].lb = 0;
         cur[0].le = n;
         cur[0].valid = 1;
@@ -486,7 +531,7 @@ __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32
         if (2 * ncur > kSmallLeaves) {
             // cannot happen while the root has <= kSmallLeaves leaves; refuse
             // rather than leave the subtree half built
-            if (t == 0 && atomicCAS(&status[0], 0, 1) == 0) status[1] = rr.id;
+            if (t == 0 && atomicCAS(&st[0], 0, 1) == 0) st[1] = rr.id;
             break;
         }
         for (int32_t k = w; k < ncur; k += 4) {
@@ -497,21 +542,25 @@ __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32
             orb_cell cl = sc.cell;
             const uint32_t lb = sc.lb, le = sc.le, m = le - lb;
             const float *key = cl.cutAxis == 0 ? sx : (cl.cutAxis == 1 ? sy : sz);
-            // bisection controller (orbit.cpp:149-231) over the LDS keys
+            // bisection controller (orbit.cpp:149-231) over the LDS keys; a
+            // pass counts with one ballot + popcount per 64 keys (scalar sum,
+            // no cross-lane shuffles)
             float L = cl.cutMarginLeft, R = cl.cutMarginRight;
             int32_t passes = 0;
             bool found = false;
+            const float target = difference_target(m, cl.nLeafCells);
             while (!found && passes < ORB_MAX_PASSES) {
                 const float cut = cell_cut(L, R);
                 uint32_t cnt = 0;
-                for (uint32_t p = lb + lane; p < le; p += 64) {
-                    float v = key[p];
-                    cnt += le_cmp ? (v <= cut) : (v < cut);
+                for (uint32_t base = lb; base < le; base += 64) {
+                    const uint32_t p = base + lane;
+                    const float v = p < le ? key[p] : INFINITY;
+                    const bool below = le_cmp ? (v <= cut) : (v < cut);
+                    cnt += (uint32_t)__popcll(__ballot(below && p < le));
                 }
-                cnt = wave_sum_u32(cnt);
-                if (trace && lane == 0) trace[(size_t)cl.id * ORB_MAX_PASSES + passes] = cnt;
+                if (trace && lane == 0) trace[((size_t)hb + cl.id) * ORB_MAX_PASSES + passes] = cnt;
                 passes++;
-                const int32_t diff = difference_rule(cnt, m, cl.nLeafCells);
+                const int32_t diff = difference_from_target(cnt, target);
                 if (abs(diff) < 3) found = true;
                 else if (diff > 0) R = cut;
                 else L = cut;
@@ -539,17 +588,32 @@ __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32
             if (m == 0 || TL == 0) {
                 if (lane == 0) {
                     s_err = 1;
-                    if (atomicCAS(&status[0], 0, 1) == 0) status[1] = cl.id;
+                    if (atomicCAS(&st[0], 0, 1) == 0) st[1] = cl.id;
                 }
                 continue;
             }
             const uint16_t *Lp = slpos + lb;
             const uint16_t *Rp = srpos + lb;
+            // K = #{t : L[t] < R[TR-1-t]} (true then false in t): 64 probes
+            // per step, the ballot picks the crossing
             uint32_t lo = 0, hi = TL < TR ? TL : TR;
-            while (lo < hi) {
-                uint32_t mid = (lo + hi) >> 1;
-                if (Lp[mid] < Rp[TR - 1 - mid]) lo = mid + 1;
-                else hi = mid;
+            while (hi - lo > 64u) {
+                const uint32_t span = hi - lo;
+                const uint32_t tp = lo + (uint32_t)(((lane + 1) * span) / 65u);
+                const uint64_t mk = __ballot(Lp[tp] < Rp[TR - 1 - tp]);
+                if (mk == 0ull) {
+                    hi = lo + span / 65u;
+                } else {
+                    const int j = 63 - __clzll((long long)mk);
+                    const uint32_t tj = lo + (uint32_t)(((j + 1) * span) / 65u);
+                    const uint32_t tn = j < 63 ? lo + (uint32_t)(((j + 2) * span) / 65u) : hi;
+                    lo = tj + 1;
+                    hi = tn;
+                }
+            }
+            {
+                const uint32_t tp = lo + (uint32_t)lane;
+                lo += (uint32_t)__popcll(__ballot(tp < hi && Lp[tp] < Rp[TR - 1 - tp]));
             }
             const uint32_t K = lo;
             uint32_t i;
@@ -579,24 +643,25 @@ __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32
             uint32_t *wr = reinterpret_cast<uint32_t *>(&rc);
 #pragma unroll
             for (int j = 0; j < (int)(sizeof(orb_cell) / 4); ++j) wl[j] = wr[j] = 0u;
-            const bool ok = right_child_id(cl.id) < cap && cell_split(cl, lc, rc);
+            const bool ok = right_child_id(cl.id) < rr.capr && cell_split(cl, lc, rc);
             if (lane == 0) {
-                heap[cl.id] = cl;
-                record_level_passes(level_passes, l, passes);
+                heap[hb + cl.id] = cl;
+                record_level_passes(level_passes, lp0 + l, passes);
                 if (!ok) {
                     s_err = 1;
-                    if (atomicCAS(&status[0], 0, 1) == 0) status[1] = cl.id;
+                    if (atomicCAS(&st[0], 0, 1) == 0) st[1] = cl.id;
                 } else {
-                    heap[lc.id] = lc;
-                    heap[rc.id] = rc;
-                    exists[lc.id] = 1;
-                    exists[rc.id] = 1;
-                    sub[lc.id] = 1;
-                    sub[rc.id] = 1;
-                    ranges[2 * (size_t)lc.id] = begin + lb;
-                    ranges[2 * (size_t)lc.id + 1] = begin + lb + i;
-                    ranges[2 * (size_t)rc.id] = begin + lb + i;
-                    ranges[2 * (size_t)rc.id + 1] = begin + le;
+                    const int64_t gl = hb + lc.id, gr = hb + rc.id;
+                    heap[gl] = lc;
+                    heap[gr] = rc;
+                    exists[gl] = 1;
+                    exists[gr] = 1;
+                    sub[gl] = 1;
+                    sub[gr] = 1;
+                    ranges[2 * gl] = begin + lb;
+                    ranges[2 * gl + 1] = begin + lb + i;
+                    ranges[2 * gr] = begin + lb + i;
+                    ranges[2 * gr + 1] = begin + le;
                     nxt[2 * k].cell = lc;
                     nxt[2 * k].lb = lb;
                     nxt[2 * k].le = lb + i;
@@ -625,6 +690,8 @@ __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32
         y[begin + k] = sy[k];
         z[begin + k] = sz[k];
     }
+    __syncthreads();  // the next cell reuses the LDS arrays
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -642,6 +709,7 @@ static int ensure_state(Ctx &c, int32_t levels_mode, bool *complete)
     ORB_TRY(b.exists.alloc((size_t)c.cap));
     ORB_TRY(b.sub.alloc((size_t)c.cap));
     ORB_TRY(b.chunk_hist.alloc((size_t)(c.max_chunks + 1) * kBins));
+    if (!b.cell_hist.p) b.hist_clean = false;
     ORB_TRY(b.cell_hist.alloc((size_t)c.max_level_cells * kBins));
     ORB_TRY(b.level_passes.alloc(64));
     ORB_TRY(b.hpasses.alloc(64));
@@ -660,13 +728,55 @@ static void drop_graph(BatchedState &b)
     b.gkey[0] = b.gkey[1] = b.gkey[2] = b.gkey[3] = -1;
 }
 
+int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *heap,
+                  uint8_t *exists, uint8_t *sub, uint16_t *chunk_hist, uint32_t *cell_hist,
+                  uint32_t *trace, int32_t *level_passes, int32_t *status, bool any_subtree)
+{
+    const bool use_subtree = any_subtree && c.part == ORB_PART_HOARE;
+    const int le = c.cmp == ORB_CMP_LE ? 1 : 0;
+    unsigned g1 = (unsigned)((nCells + 1 + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_b_recs, dim3(g1), dim3(kBlock), 0, c.stream, entries, nCells, heap,
+                       c.ranges.p, sub, use_subtree ? 1 : 0, c.recs.p, c.cell_nchunks.p);
+    ORB_HIP(hipGetLastError());
+    ORB_TRY(build_chunk_table(c, nCells));
+    if (use_subtree) {
+        const unsigned gs = (unsigned)(nCells < 4096 ? nCells : 4096);
+        hipLaunchKernelGGL(k_b_subtree, dim3(gs), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
+                           heap, exists, sub, c.ranges.p, c.x(), c.y(), c.z(), le, trace,
+                           level_passes, status);
+        ORB_HIP(hipGetLastError());
+    }
+    const unsigned gq = chunk_grid(c);
+    const unsigned gr = (unsigned)((gq + kReduceRows - 1) / kReduceRows * 4);
+    const unsigned gw = (unsigned)((nCells + 3) / 4);
+    for (int r = 0; r < kRounds; ++r) {
+        const int tb = timing_mark(c);
+        hipLaunchKernelGGL(k_b_hist, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
+                           c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(), le, chunk_hist);
+        ORB_HIP(hipGetLastError());
+        ORB_TRY(timing_pair(c, tb, kTimeCount));
+        hipLaunchKernelGGL(k_b_hist_reduce, dim3(gr), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
+                           c.cell_first.p, c.chunk_cell.p, chunk_hist, cell_hist);
+        hipLaunchKernelGGL(k_b_replay, dim3(gw), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
+                           cell_hist, heap, trace, level_passes);
+        ORB_HIP(hipGetLastError());
+    }
+    unsigned gc = (unsigned)((nCells + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_b_split, dim3(gc), dim3(kBlock), 0, c.stream, c.recs.p, nCells, heap,
+                       exists, status);
+    ORB_HIP(hipGetLastError());
+    const int tp = timing_mark(c);
+    if (c.part == ORB_PART_HOARE) ORB_TRY(launch_hoare(c, nCells, status));
+    else ORB_TRY(launch_stable_le(c, nCells, status));
+    ORB_TRY(timing_pair(c, tp, kTimePartition));
+    return ORB_OK;
+}
+
 // All launches of one batched build, in stream order; no allocation, no
 // host synchronisation, no memset: the sequence can be captured.
 static int enqueue_build(Ctx &c, int32_t levels_mode)
 {
     BatchedState &b = *c.batched;
-    const bool use_subtree = b.complete && c.part == ORB_PART_HOARE;
-    const int le = c.cmp == ORB_CMP_LE ? 1 : 0;
     uint32_t *trace = c.trace ? b.trace.p : nullptr;
     const int64_t trace_n = c.trace ? c.cap * ORB_MAX_PASSES : 0;
     const orb_cell root = cell_root(c.d);
@@ -674,50 +784,14 @@ static int enqueue_build(Ctx &c, int32_t levels_mode)
                        b.sub.p, (int64_t)c.cap, root, trace, trace_n, b.level_passes.p, c.ranges.p,
                        (uint32_t)c.n, c.status.p);
     ORB_HIP(hipGetLastError());
-
     const int32_t lmax = levels_mode == ORB_LEVELS_REFERENCE ? b.D - 1 : b.D;
     for (int32_t l = 1; l <= lmax; ++l) {
         const int32_t nCells = b.level_n[(size_t)l];
         if (nCells == 0) continue;
         const int tlevel = timing_mark(c);
-        const int32_t *ids = b.level_ids.p + b.level_off[(size_t)l];
-        unsigned g1 = (unsigned)((nCells + 1 + kBlock - 1) / kBlock);
-        hipLaunchKernelGGL(k_b_recs, dim3(g1), dim3(kBlock), 0, c.stream, ids, nCells, b.heap.p,
-                           c.ranges.p, b.sub.p, use_subtree ? 1 : 0, c.recs.p, c.cell_nchunks.p);
-        ORB_HIP(hipGetLastError());
-        ORB_TRY(build_chunk_table(c, nCells));
-        if (use_subtree) {
-            hipLaunchKernelGGL(k_b_subtree, dim3((unsigned)nCells), dim3(kBlock), 0, c.stream,
-                               c.recs.p, nCells, b.heap.p, b.exists.p, b.sub.p, c.ranges.p, c.x(),
-                               c.y(), c.z(), l, lmax, le, trace, b.level_passes.p, c.status.p,
-                               (int64_t)c.cap);
-            ORB_HIP(hipGetLastError());
-        }
-        const unsigned gr = (unsigned)((c.max_chunks + kReduceRows - 1) / kReduceRows);
-        const unsigned gw = (unsigned)((nCells + 3) / 4);
-        for (int r = 0; r < kRounds; ++r) {
-            const int tb = timing_mark(c);
-            hipLaunchKernelGGL(k_b_hist, dim3((unsigned)c.max_chunks), dim3(kBlock), 0, c.stream,
-                               c.recs.p, nCells, c.cell_first.p, c.chunk_cell.p, c.x(), c.y(),
-                               c.z(), le, b.chunk_hist.p);
-            ORB_HIP(hipGetLastError());
-            ORB_TRY(timing_pair(c, tb, kTimeCount));
-            ORB_TRY(launch_zero_u32(c.stream, b.cell_hist.p, (size_t)nCells * kBins));
-            hipLaunchKernelGGL(k_b_hist_reduce, dim3(gr), dim3(kBlock), 0, c.stream, c.recs.p,
-                               nCells, c.cell_first.p, c.chunk_cell.p, b.chunk_hist.p,
-                               b.cell_hist.p);
-            hipLaunchKernelGGL(k_b_replay, dim3(gw), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
-                               b.cell_hist.p, b.heap.p, trace, b.level_passes.p, l);
-            ORB_HIP(hipGetLastError());
-        }
-        unsigned gc = (unsigned)((nCells + kBlock - 1) / kBlock);
-        hipLaunchKernelGGL(k_b_split, dim3(gc), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
-                           b.heap.p, b.exists.p, c.status.p, (int64_t)c.cap);
-        ORB_HIP(hipGetLastError());
-        const int tp = timing_mark(c);
-        if (c.part == ORB_PART_HOARE) ORB_TRY(launch_hoare(c, nCells));
-        else ORB_TRY(launch_stable_le(c, nCells));
-        ORB_TRY(timing_pair(c, tp, kTimePartition));
+        ORB_TRY(enqueue_level(c, b.level_ids.p + b.level_off[(size_t)l], nCells, b.heap.p,
+                              b.exists.p, b.sub.p, b.chunk_hist.p, b.cell_hist.p, trace,
+                              b.level_passes.p, c.status.p, b.complete));
         ORB_TRY(timing_pair(c, tlevel, kTimeLevel + l));
     }
     return ORB_OK;
@@ -731,6 +805,10 @@ int build_batched(Ctx &c, int32_t levels_mode, orb_build_info *info)
     timing_reset(c);
     c.table_ids.clear();
 
+    if (!b.hist_clean) {
+        ORB_TRY(launch_zero_u32(c.stream, b.cell_hist.p, (size_t)c.max_level_cells * kBins));
+        b.hist_clean = true;
+    }
     const int32_t key[4] = {levels_mode, c.cmp, c.part, c.trace ? 1 : 0};
     const bool graph = c.use_graph && !c.timing;
     if (graph) {
@@ -775,6 +853,7 @@ int build_batched(Ctx &c, int32_t levels_mode, orb_build_info *info)
     }
     if (c.hstatus.p[0] != 0) {
         info->error_cell = c.hstatus.p[1];
+        b.hist_clean = false;  // an aborted subtree may leave rows behind
         return ORB_ERR_UB;
     }
     return ORB_OK;

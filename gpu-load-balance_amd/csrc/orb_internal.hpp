@@ -83,6 +83,27 @@ __host__ __device__ inline int32_t difference_rule(uint32_t count_left, uint32_t
     return (int32_t)diff;
 }
 
+// The same rule split in two: the product (float)count * ratio depends only on
+// the cell, so a loop over passes computes it once (identical roundings).
+__host__ __device__ inline float difference_target(uint32_t count, int32_t nLeafCells)
+{
+    float ratio = (float)(ceil(nLeafCells / 2.0) / (double)nLeafCells);
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __fmul_rn((float)count, ratio);
+#else
+    return (float)count * ratio;
+#endif
+}
+
+__host__ __device__ inline int32_t difference_from_target(uint32_t count_left, float target)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return (int32_t)__fsub_rn((float)count_left, target);
+#else
+    return (int32_t)((float)count_left - target);
+#endif
+}
+
 __host__ __device__ inline void cell_make(orb_cell &c, int32_t id, int32_t nLeaf,
                                           const float lower[3], const float upper[3])
 {
@@ -246,6 +267,27 @@ struct CellRec {
     int32_t id;         // heap id
     int32_t flags;      // batched form: 0 active, 1 inside a finished subtree, 2 small (subtree kernel)
     int32_t passes;     // batched form: bisection passes so far
+    // where the cell's build keeps its state (several builds can share a
+    // launch): heap/range/flag index = hb + id, the build's heap capacity,
+    // its pass counter for this level, its status pair, its last level
+    int32_t hb;
+    int32_t capr;
+    int32_t lp;
+    int32_t st;
+    int32_t lmax;
+};
+
+// One cell of a level step: the build it belongs to and where that build's
+// state lives (see CellRec).
+struct LevelEntry {
+    int32_t id;       // heap id within its build
+    int32_t hb;       // heap base of the build
+    int32_t capr;     // heap capacity of the build
+    int32_t lp;       // pass-counter index: build slot * 64 + level
+    int32_t st;       // status pair index: 2 * build slot
+    int32_t lmax;     // last level of the build
+    int32_t sub_ok;   // the build's schedule lets a subtree finish in LDS
+    int32_t pad;
 };
 
 // Cut and axis a cell's per-chunk counts were taken at (axis -1: none).
@@ -348,17 +390,25 @@ int timing_collect(Ctx &c);
 int launch_zero_u32(hipStream_t s, uint32_t *p, size_t n);
 int launch_zero_u64(hipStream_t s, uint64_t *p, size_t n);
 int scan_exclusive_u32(Ctx &c, const uint32_t *in, uint32_t *out, size_t m);
-int scan_exclusive_u64(Ctx &c, const uint64_t *in, uint64_t *out, size_t m);
+int scan_exclusive_u64(Ctx &c, const uint64_t *in, uint64_t *out, size_t m,
+                       const uint32_t *lim = nullptr);
+unsigned chunk_grid(const Ctx &c);
 int upload_level(Ctx &c, const orb_cell *cells, int32_t nCells);
 int build_chunk_table(Ctx &c, int32_t nCells);
 int launch_count(Ctx &c, int32_t nCells, bool lt_skip_found, const float *axis_override,
                  hipEvent_t after_count = nullptr);
-int launch_hoare(Ctx &c, int32_t nCells);
+int launch_hoare(Ctx &c, int32_t nCells, int32_t *status = nullptr);
 int launch_stopper_counts(Ctx &c, int32_t nCells, bool le_only);
 int invalidate_stamps(Ctx &c, int32_t nCells);
-int launch_stable_le(Ctx &c, int32_t nCells);
+int launch_stable_le(Ctx &c, int32_t nCells, int32_t *status = nullptr);
+int schedule_levels(int32_t d, int32_t levels_mode, int64_t cap,
+                    std::vector<std::vector<int32_t>> &levels, bool *complete);
 int launch_make_axis(Ctx &c, int32_t nCells);
 int launch_cell_sizes(Ctx &c, int32_t nCells, uint32_t *dst);
+// one level step of the batched form over `entries` (device array)
+int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *heap,
+                  uint8_t *exists, uint8_t *sub, uint16_t *chunk_hist, uint32_t *cell_hist,
+                  uint32_t *trace, int32_t *level_passes, int32_t *status, bool any_subtree);
 int preallocate_scratch(Ctx &c);
 int ensure_level_capacity(Ctx &c, int64_t nCells);
 

@@ -67,15 +67,25 @@ int launch_zero_u64(hipStream_t s, uint64_t *p, size_t n)
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlock * kScanItems;  // 2048
 
+// Entries at or past *lim (when given) count as zero and are not read; the
+// output is written up to and including index *lim (the grand total).
+__device__ inline size_t scan_limit(size_t m, const uint32_t *lim)
+{
+    if (!lim) return m;
+    size_t l = (size_t)(*lim);
+    return l < m ? l : m;
+}
+
 template <class T>
-__global__ void k_scan_reduce(const T *in, size_t m, T *partials)
+__global__ void k_scan_reduce(const T *in, size_t m, T *partials, const uint32_t *lim)
 {
     __shared__ T sm[4];
     size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+    const size_t mr = scan_limit(m, lim);
     T s = 0;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k)
-        if (base + k < m) s += in[base + k];
+        if (base + k < mr) s += in[base + k];
     s = block_sum(s, sm);
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
@@ -102,28 +112,31 @@ __global__ void k_scan_partials(T *partials, int np)
 }
 
 template <class T>
-__global__ void k_scan_apply(const T *in, T *out, size_t m, const T *partials)
+__global__ void k_scan_apply(const T *in, T *out, size_t m, const T *partials, const uint32_t *lim)
 {
     __shared__ T sm[4];
     size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+    const size_t mr = scan_limit(m, lim);
+    const size_t mw = lim ? (mr + 1 < m ? mr + 1 : m) : m;
     T v[kScanItems];
     T s = 0;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        v[k] = (base + k < m) ? in[base + k] : T(0);
+        v[k] = (base + k < mr) ? in[base + k] : T(0);
         s += v[k];
     }
     T total;
     T pre = block_exclusive_scan(s, sm, &total) + partials[blockIdx.x];
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        if (base + k < m) out[base + k] = pre;
+        if (base + k < mw) out[base + k] = pre;
         pre += v[k];
     }
 }
 
 template <class T>
-static int scan_exclusive(Ctx &c, const T *in, T *out, size_t m, DevBuf<uint64_t> &scratch)
+static int scan_exclusive(Ctx &c, const T *in, T *out, size_t m, DevBuf<uint64_t> &scratch,
+                          const uint32_t *lim = nullptr)
 {
     if (m == 0) return ORB_OK;
     size_t np = (m + kScanTile - 1) / kScanTile;
@@ -131,12 +144,21 @@ static int scan_exclusive(Ctx &c, const T *in, T *out, size_t m, DevBuf<uint64_t
     size_t words = (np * sizeof(T) + sizeof(uint64_t) - 1) / sizeof(uint64_t);
     ORB_TRY(scratch.alloc(words));
     T *partials = reinterpret_cast<T *>(scratch.p);
-    hipLaunchKernelGGL(k_scan_reduce<T>, dim3((unsigned)np), dim3(kBlock), 0, c.stream, in, m, partials);
+    hipLaunchKernelGGL(k_scan_reduce<T>, dim3((unsigned)np), dim3(kBlock), 0, c.stream, in, m, partials,
+                       lim);
     hipLaunchKernelGGL(k_scan_partials<T>, dim3(1), dim3(kBlock), 0, c.stream, partials, (int)np);
     hipLaunchKernelGGL(k_scan_apply<T>, dim3((unsigned)np), dim3(kBlock), 0, c.stream, in, out, m,
-                       (const T *)partials);
+                       (const T *)partials, lim);
     ORB_HIP(hipGetLastError());
     return ORB_OK;
+}
+
+unsigned chunk_grid(const Ctx &c)
+{
+    // chunk kernels loop over the level's chunks: a bounded grid keeps a
+    // level with few chunks from paying for a million empty blocks
+    const int64_t cap = 65536;
+    return (unsigned)(c.max_chunks < cap ? (c.max_chunks > 0 ? c.max_chunks : 1) : cap);
 }
 
 int preallocate_scratch(Ctx &c)
@@ -154,16 +176,16 @@ int scan_exclusive_u32(Ctx &c, const uint32_t *in, uint32_t *out, size_t m)
     return scan_exclusive<uint32_t>(c, in, out, m, c.scan_partials);
 }
 
-int scan_exclusive_u64(Ctx &c, const uint64_t *in, uint64_t *out, size_t m)
+int scan_exclusive_u64(Ctx &c, const uint64_t *in, uint64_t *out, size_t m, const uint32_t *lim)
 {
-    return scan_exclusive<uint64_t>(c, in, out, m, c.scan_partials);
+    return scan_exclusive<uint64_t>(c, in, out, m, c.scan_partials, lim);
 }
 
 // ---------------------------------------------------------------------------
 // level records and the chunk table (ServiceCopyCells, copyCells.cu:10-64)
 // ---------------------------------------------------------------------------
 __global__ void k_level_recs(const orb_cell *cells, int32_t nCells, const uint32_t *ranges,
-                             CellRec *recs, uint32_t *nchunks)
+                             CellRec *recs, uint32_t *nchunks, int32_t cap)
 {
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i > nCells) return;
@@ -185,6 +207,11 @@ __global__ void k_level_recs(const orb_cell *cells, int32_t nCells, const uint32
     r.id = cl.id;
     r.flags = 0;
     r.passes = 0;
+    r.hb = 0;
+    r.capr = cap;
+    r.lp = 0;
+    r.st = 0;
+    r.lmax = 0;
     recs[i] = r;
     nchunks[i] = (r.end - r.begin + (uint32_t)kChunk - 1u) / (uint32_t)kChunk;
 }
@@ -233,7 +260,7 @@ int upload_level(Ctx &c, const orb_cell *cells, int32_t nCells)
     if (c.ev_upload) ORB_HIP(hipEventRecord(c.ev_upload, c.stream));
     unsigned g = (unsigned)((nCells + 1 + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_level_recs, dim3(g), dim3(kBlock), 0, c.stream, c.dcells.p, nCells,
-                       c.ranges.p, c.recs.p, c.cell_nchunks.p);
+                       c.ranges.p, c.recs.p, c.cell_nchunks.p, (int32_t)c.cap);
     ORB_HIP(hipGetLastError());
     return ORB_OK;
 }
@@ -309,7 +336,7 @@ __device__ inline void count_range(const float *key, uint32_t b, uint32_t e, flo
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_count(const CellRec *recs, int32_t nCells,
+__device__ inline void k_count_chunk(uint32_t q, const CellRec *recs, int32_t nCells,
                                                   const uint32_t *first, const uint32_t *chunk_cell,
                                                   const float *x, const float *y, const float *z,
                                                   const float *axisT, int skip_found,
@@ -317,7 +344,7 @@ __global__ __launch_bounds__(kBlock) void k_count(const CellRec *recs, int32_t n
 {
     __shared__ uint32_t sm[4];
     uint32_t cell, b, e;
-    if (!chunk_range(blockIdx.x, first, nCells, chunk_cell, recs, cell, b, e)) return;
+    if (!chunk_range(q, first, nCells, chunk_cell, recs, cell, b, e)) return;
     const CellRec r = recs[cell];
     if (skip_found && r.found) return;
     if (r.axis < 0 || r.axis > 2) return;
@@ -326,8 +353,22 @@ __global__ __launch_bounds__(kBlock) void k_count(const CellRec *recs, int32_t n
     count_range(key, b, e, r.cut, lt, le);
     // both counts <= 4096: one packed word, one reduction
     uint32_t packed = block_sum(lt | (le << 16), sm);
-    if (threadIdx.x == 0) chunk_ltle[blockIdx.x] = packed;
+    if (threadIdx.x == 0) chunk_ltle[q] = packed;
 }
+
+__global__ __launch_bounds__(kBlock) void k_count(const CellRec *recs, int32_t nCells,
+                                                  const uint32_t *first, const uint32_t *chunk_cell,
+                                                  const float *x, const float *y, const float *z,
+                                                  const float *axisT, int skip_found,
+                                                  uint32_t *chunk_ltle)
+{
+    const uint32_t total = first[nCells];
+    for (uint32_t q = blockIdx.x; q < total; q += gridDim.x) {
+        k_count_chunk(q, recs, nCells, first, chunk_cell, x, y, z, axisT, skip_found, chunk_ltle);
+        __syncthreads();  // the next chunk reuses this block's LDS
+    }
+}
+
 
 // Per-cell sums of the chunk slots; stamps the cut and axis the counts were
 // taken at (a later partition at the same cut reuses the chunk slots).  One
@@ -406,7 +447,7 @@ int invalidate_stamps(Ctx &c, int32_t nCells)
 int launch_count(Ctx &c, int32_t nCells, bool skip_found, const float *axis_override,
                  hipEvent_t after_count)
 {
-    hipLaunchKernelGGL(k_count, dim3((unsigned)c.max_chunks), dim3(kBlock), 0, c.stream,
+    hipLaunchKernelGGL(k_count, dim3(chunk_grid(c)), dim3(kBlock), 0, c.stream,
                        c.recs.p, nCells, c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(),
                        axis_override, skip_found ? 1 : 0, c.chunk_ltle.p);
     ORB_HIP(hipGetLastError());
@@ -439,7 +480,7 @@ int launch_count(Ctx &c, int32_t nCells, bool skip_found, const float *axis_over
 //   P3   per pair t < K: swap x, y, z of lpos[t] and rpos[TR-1-t]
 //   P4   per cell: swap rows i and n-1
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_stopper_counts(const CellRec *recs, int32_t nCells,
+__device__ inline void k_stopper_counts_chunk(uint32_t q, const CellRec *recs, int32_t nCells,
                                                            const uint32_t *first,
                                                            const uint32_t *chunk_cell,
                                                            const float *x, const float *y,
@@ -449,12 +490,9 @@ __global__ __launch_bounds__(kBlock) void k_stopper_counts(const CellRec *recs, 
                                                            uint64_t *chunk_cnt, uint32_t max_chunks)
 {
     __shared__ uint32_t sm[4];
-    uint32_t q = blockIdx.x;
     uint32_t cell, b, e;
-    if (!chunk_range(q, first, nCells, chunk_cell, recs, cell, b, e)) {
-        if (q <= max_chunks && threadIdx.x == 0) chunk_cnt[q] = 0ull;
-        return;
-    }
+    (void)max_chunks;
+    if (!chunk_range(q, first, nCells, chunk_cell, recs, cell, b, e)) return;
     const CellRec r = recs[cell];
     const int axis = r.axis < 0 ? 0 : (r.axis > 2 ? 2 : r.axis);
     const CountStamp st = stamps[cell];
@@ -478,6 +516,23 @@ __global__ __launch_bounds__(kBlock) void k_stopper_counts(const CellRec *recs, 
         chunk_cnt[q] = (uint64_t)nl | ((uint64_t)nr << 32);
     }
 }
+
+__global__ __launch_bounds__(kBlock) void k_stopper_counts(const CellRec *recs, int32_t nCells,
+                                                           const uint32_t *first,
+                                                           const uint32_t *chunk_cell,
+                                                           const float *x, const float *y,
+                                                           const float *z, int le_only,
+                                                           const CountStamp *stamps,
+                                                           const uint32_t *chunk_ltle,
+                                                           uint64_t *chunk_cnt, uint32_t max_chunks)
+{
+    const uint32_t total = first[nCells];
+    for (uint32_t q = blockIdx.x; q < total; q += gridDim.x) {
+        k_stopper_counts_chunk(q, recs, nCells, first, chunk_cell, x, y, z, le_only, stamps, chunk_ltle, chunk_cnt, max_chunks);
+        __syncthreads();  // the next chunk reuses this block's LDS
+    }
+}
+
 
 // LDS staging of one chunk: element k at index k + k/16 (odd stride per
 // thread row, no bank conflicts when thread t reads its 16 consecutive keys).
@@ -510,7 +565,7 @@ __device__ inline int load_chunk_lds(float *skey, const float *key, uint32_t b, 
     return n;
 }
 
-__global__ __launch_bounds__(kBlock) void k_stopper_positions(const CellRec *recs, int32_t nCells,
+__device__ inline void k_stopper_positions_chunk(uint32_t q, const CellRec *recs, int32_t nCells,
                                                               const uint32_t *first,
                                                               const uint32_t *chunk_cell,
                                                               const uint64_t *chunk_pre,
@@ -521,7 +576,6 @@ __global__ __launch_bounds__(kBlock) void k_stopper_positions(const CellRec *rec
     __shared__ float skey[kLdsKeys];
     __shared__ uint32_t spos[kChunk];
     __shared__ uint32_t sm[4];
-    uint32_t q = blockIdx.x;
     uint32_t cell, b, e;
     if (!chunk_range(q, first, nCells, chunk_cell, recs, cell, b, e)) return;
     const CellRec r = recs[cell];
@@ -568,6 +622,22 @@ __global__ __launch_bounds__(kBlock) void k_stopper_positions(const CellRec *rec
     for (uint32_t k = threadIdx.x; k < nr; k += kBlock) dr[k] = spos[k];
 }
 
+__global__ __launch_bounds__(kBlock) void k_stopper_positions(const CellRec *recs, int32_t nCells,
+                                                              const uint32_t *first,
+                                                              const uint32_t *chunk_cell,
+                                                              const uint64_t *chunk_pre,
+                                                              const float *x, const float *y,
+                                                              const float *z, uint32_t *lpos,
+                                                              uint32_t *rpos)
+{
+    const uint32_t total = first[nCells];
+    for (uint32_t q = blockIdx.x; q < total; q += gridDim.x) {
+        k_stopper_positions_chunk(q, recs, nCells, first, chunk_cell, chunk_pre, x, y, z, lpos, rpos);
+        __syncthreads();  // the next chunk reuses this block's LDS
+    }
+}
+
+
 __device__ inline void report_error(int32_t *status, int32_t id)
 {
     if (atomicCAS(&status[0], 0, 1) == 0) status[1] = id;
@@ -593,11 +663,12 @@ __global__ __launch_bounds__(kBlock) void k_hoare_split(const CellRec *recs, int
     const uint32_t TL = (uint32_t)(tot & 0xffffffffull), TR = (uint32_t)(tot >> 32);
     const uint32_t n = r.end - r.begin;
     SplitRec s{TL, TR, 0u, 0u};
+    (void)cap;
     const int64_t lid = left_child_id(r.id), rid = right_child_id(r.id);
-    if (n == 0 || TL == 0 || r.axis < 0 || r.axis > 2 || rid >= cap) {
+    if (n == 0 || TL == 0 || r.axis < 0 || r.axis > 2 || rid >= r.capr) {
         // empty cell or no key >= cut: the reference's loop leaves the cell
         if (lane == 0) {
-            report_error(status, r.id);
+            report_error(status + r.st, r.id);
             s.TL = 0u;  // k_hoare_final skips the cell
             splits[c] = s;
         }
@@ -636,20 +707,20 @@ __global__ __launch_bounds__(kBlock) void k_hoare_split(const CellRec *recs, int
     s.K = K;
     s.i = i;
     splits[c] = s;
-    ranges[2 * lid] = r.begin;
-    ranges[2 * lid + 1] = r.begin + i;
-    ranges[2 * rid] = r.begin + i;
-    ranges[2 * rid + 1] = r.end;
+    const int64_t gl = r.hb + lid, gr = r.hb + rid;
+    ranges[2 * gl] = r.begin;
+    ranges[2 * gl + 1] = r.begin + i;
+    ranges[2 * gr] = r.begin + i;
+    ranges[2 * gr + 1] = r.end;
 }
 
-__global__ __launch_bounds__(kBlock) void k_hoare_swap(const CellRec *recs, int32_t nCells,
+__device__ inline void k_hoare_swap_chunk(uint32_t q, const CellRec *recs, int32_t nCells,
                                                        const uint32_t *first,
                                                        const uint32_t *chunk_cell,
                                                        const SplitRec *splits,
                                                        const uint32_t *lpos, const uint32_t *rpos,
                                                        float *x, float *y, float *z)
 {
-    uint32_t q = blockIdx.x;
     uint32_t cell, b, e;
     if (!chunk_range(q, first, nCells, chunk_cell, recs, cell, b, e)) return;
     const CellRec r = recs[cell];
@@ -670,6 +741,21 @@ __global__ __launch_bounds__(kBlock) void k_hoare_swap(const CellRec *recs, int3
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_hoare_swap(const CellRec *recs, int32_t nCells,
+                                                       const uint32_t *first,
+                                                       const uint32_t *chunk_cell,
+                                                       const SplitRec *splits,
+                                                       const uint32_t *lpos, const uint32_t *rpos,
+                                                       float *x, float *y, float *z)
+{
+    const uint32_t total = first[nCells];
+    for (uint32_t q = blockIdx.x; q < total; q += gridDim.x) {
+        k_hoare_swap_chunk(q, recs, nCells, first, chunk_cell, splits, lpos, rpos, x, y, z);
+        __syncthreads();  // the next chunk reuses this block's LDS
+    }
+}
+
+
 __global__ void k_hoare_final(const CellRec *recs, int32_t nCells, const SplitRec *splits,
                               float *x, float *y, float *z)
 {
@@ -688,17 +774,19 @@ __global__ void k_hoare_final(const CellRec *recs, int32_t nCells, const SplitRe
 
 int launch_stopper_counts(Ctx &c, int32_t nCells, bool le_only)
 {
-    hipLaunchKernelGGL(k_stopper_counts, dim3((unsigned)c.max_chunks + 1), dim3(kBlock), 0,
+    hipLaunchKernelGGL(k_stopper_counts, dim3(chunk_grid(c)), dim3(kBlock), 0,
                        c.stream, c.recs.p, nCells, c.cell_first.p, c.chunk_cell.p, c.x(), c.y(),
                        c.z(), le_only ? 1 : 0, c.stamps.p, c.chunk_ltle.p, c.chunk_cnt.p,
                        (uint32_t)c.max_chunks);
     ORB_HIP(hipGetLastError());
-    return scan_exclusive_u64(c, c.chunk_cnt.p, c.chunk_pre.p, (size_t)c.max_chunks + 1);
+    return scan_exclusive_u64(c, c.chunk_cnt.p, c.chunk_pre.p, (size_t)c.max_chunks + 1,
+                              c.cell_first.p + nCells);
 }
 
-int launch_hoare(Ctx &c, int32_t nCells)
+int launch_hoare(Ctx &c, int32_t nCells, int32_t *status)
 {
-    const unsigned gq = (unsigned)c.max_chunks;
+    if (!status) status = c.status.p;
+    const unsigned gq = chunk_grid(c);
     ORB_TRY(launch_stopper_counts(c, nCells, false));
     hipLaunchKernelGGL(k_stopper_positions, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
                        c.cell_first.p, c.chunk_cell.p, c.chunk_pre.p, c.x(), c.y(), c.z(), c.lpos.p,
@@ -707,7 +795,7 @@ int launch_hoare(Ctx &c, int32_t nCells)
     unsigned gw = (unsigned)((nCells + 3) / 4);
     hipLaunchKernelGGL(k_hoare_split, dim3(gw), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
                        c.cell_first.p, c.chunk_pre.p, c.lpos.p, c.rpos.p, c.splits.p, c.ranges.p,
-                       c.status.p, (int64_t)c.cap);
+                       status, (int64_t)c.cap);
     ORB_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_hoare_swap, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
                        c.cell_first.p, c.chunk_cell.p, c.splits.p, c.lpos.p, c.rpos.p, c.x(), c.y(),
@@ -723,7 +811,7 @@ int launch_hoare(Ctx &c, int32_t nCells)
 // ---------------------------------------------------------------------------
 // stable <= partition (ServicePartitionGPU semantics, deterministic order)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_stable_scatter(const CellRec *recs, int32_t nCells,
+__device__ inline void k_stable_scatter_chunk(uint32_t q, const CellRec *recs, int32_t nCells,
                                                            const uint32_t *first,
                                                            const uint32_t *chunk_cell,
                                                            const uint64_t *chunk_pre,
@@ -733,7 +821,6 @@ __global__ __launch_bounds__(kBlock) void k_stable_scatter(const CellRec *recs, 
 {
     __shared__ float skey[kLdsKeys];
     __shared__ uint32_t sm[4];
-    uint32_t q = blockIdx.x;
     uint32_t cell, b, e;
     if (!chunk_range(q, first, nCells, chunk_cell, recs, cell, b, e)) return;
     const CellRec r = recs[cell];
@@ -772,20 +859,50 @@ __global__ __launch_bounds__(kBlock) void k_stable_scatter(const CellRec *recs, 
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_copy_back(const CellRec *recs, int32_t nCells,
+__global__ __launch_bounds__(kBlock) void k_stable_scatter(const CellRec *recs, int32_t nCells,
+                                                           const uint32_t *first,
+                                                           const uint32_t *chunk_cell,
+                                                           const uint64_t *chunk_pre,
+                                                           const float *x, const float *y,
+                                                           const float *z, float *ox, float *oy,
+                                                           float *oz)
+{
+    const uint32_t total = first[nCells];
+    for (uint32_t q = blockIdx.x; q < total; q += gridDim.x) {
+        k_stable_scatter_chunk(q, recs, nCells, first, chunk_cell, chunk_pre, x, y, z, ox, oy, oz);
+        __syncthreads();  // the next chunk reuses this block's LDS
+    }
+}
+
+
+__device__ inline void k_copy_back_chunk(uint32_t q, const CellRec *recs, int32_t nCells,
                                                       const uint32_t *first,
                                                       const uint32_t *chunk_cell, const float *ox,
                                                       const float *oy, const float *oz, float *x,
                                                       float *y, float *z)
 {
     uint32_t cell, b, e;
-    if (!chunk_range(blockIdx.x, first, nCells, chunk_cell, recs, cell, b, e)) return;
+    if (!chunk_range(q, first, nCells, chunk_cell, recs, cell, b, e)) return;
     for (uint32_t p = b + threadIdx.x; p < e; p += kBlock) {
         x[p] = ox[p];
         y[p] = oy[p];
         z[p] = oz[p];
     }
 }
+
+__global__ __launch_bounds__(kBlock) void k_copy_back(const CellRec *recs, int32_t nCells,
+                                                      const uint32_t *first,
+                                                      const uint32_t *chunk_cell, const float *ox,
+                                                      const float *oy, const float *oz, float *x,
+                                                      float *y, float *z)
+{
+    const uint32_t total = first[nCells];
+    for (uint32_t q = blockIdx.x; q < total; q += gridDim.x) {
+        k_copy_back_chunk(q, recs, nCells, first, chunk_cell, ox, oy, oz, x, y, z);
+        __syncthreads();  // the next chunk reuses this block's LDS
+    }
+}
+
 
 __global__ void k_stable_ranges(const CellRec *recs, int32_t nCells, const uint32_t *first,
                                 const uint64_t *chunk_pre, SplitRec *splits, uint32_t *ranges,
@@ -797,22 +914,25 @@ __global__ void k_stable_ranges(const CellRec *recs, int32_t nCells, const uint3
     if (r.flags) return;
     uint64_t tot = chunk_pre[first[c + 1]] - chunk_pre[first[c]];
     uint32_t nle = (uint32_t)(tot & 0xffffffffull);
+    (void)cap;
     int64_t lid = left_child_id(r.id), rid = right_child_id(r.id);
-    if (r.axis < 0 || r.axis > 2 || rid >= cap) {
-        report_error(status, r.id);
+    if (r.axis < 0 || r.axis > 2 || rid >= r.capr) {
+        report_error(status + r.st, r.id);
         return;
     }
     splits[c] = SplitRec{nle, (uint32_t)(tot >> 32), 0u, nle};
-    ranges[2 * lid] = r.begin;
-    ranges[2 * lid + 1] = r.begin + nle;
-    ranges[2 * rid] = r.begin + nle;
-    ranges[2 * rid + 1] = r.end;
+    const int64_t gl = r.hb + lid, gr = r.hb + rid;
+    ranges[2 * gl] = r.begin;
+    ranges[2 * gl + 1] = r.begin + nle;
+    ranges[2 * gr] = r.begin + nle;
+    ranges[2 * gr + 1] = r.end;
 }
 
-int launch_stable_le(Ctx &c, int32_t nCells)
+int launch_stable_le(Ctx &c, int32_t nCells, int32_t *status)
 {
+    if (!status) status = c.status.p;
     ORB_TRY(c.tmp.alloc(3 * (size_t)c.stride));
-    const unsigned gq = (unsigned)c.max_chunks;
+    const unsigned gq = chunk_grid(c);
     ORB_TRY(launch_stopper_counts(c, nCells, true));
     float *ox = c.tmp.p, *oy = c.tmp.p + c.stride, *oz = c.tmp.p + 2 * c.stride;
     hipLaunchKernelGGL(k_stable_scatter, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
@@ -824,7 +944,7 @@ int launch_stable_le(Ctx &c, int32_t nCells)
     ORB_HIP(hipGetLastError());
     unsigned gc = (unsigned)((nCells + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_stable_ranges, dim3(gc), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
-                       c.cell_first.p, c.chunk_pre.p, c.splits.p, c.ranges.p, c.status.p,
+                       c.cell_first.p, c.chunk_pre.p, c.splits.p, c.ranges.p, status,
                        (int64_t)c.cap);
     ORB_HIP(hipGetLastError());
     return ORB_OK;
@@ -833,23 +953,36 @@ int launch_stable_le(Ctx &c, int32_t nCells)
 // ---------------------------------------------------------------------------
 // ServiceMakeAxis gather and ServiceCount sizes
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_make_axis(const CellRec *recs, int32_t nCells,
+__device__ inline void k_make_axis_chunk(uint32_t q, const CellRec *recs, int32_t nCells,
                                                       const uint32_t *first,
                                                       const uint32_t *chunk_cell, const float *x,
                                                       const float *y, const float *z, float *axisT)
 {
     uint32_t cell, b, e;
-    if (!chunk_range(blockIdx.x, first, nCells, chunk_cell, recs, cell, b, e)) return;
+    if (!chunk_range(q, first, nCells, chunk_cell, recs, cell, b, e)) return;
     const CellRec r = recs[cell];
     if (r.axis < 0 || r.axis > 2) return;
     const float *key = axis_ptr(x, y, z, r.axis);
     for (uint32_t p = b + threadIdx.x; p < e; p += kBlock) axisT[p] = key[p];
 }
 
+__global__ __launch_bounds__(kBlock) void k_make_axis(const CellRec *recs, int32_t nCells,
+                                                      const uint32_t *first,
+                                                      const uint32_t *chunk_cell, const float *x,
+                                                      const float *y, const float *z, float *axisT)
+{
+    const uint32_t total = first[nCells];
+    for (uint32_t q = blockIdx.x; q < total; q += gridDim.x) {
+        k_make_axis_chunk(q, recs, nCells, first, chunk_cell, x, y, z, axisT);
+        __syncthreads();  // the next chunk reuses this block's LDS
+    }
+}
+
+
 int launch_make_axis(Ctx &c, int32_t nCells)
 {
     ORB_TRY(c.axisT.alloc((size_t)c.stride + 64));
-    hipLaunchKernelGGL(k_make_axis, dim3((unsigned)c.max_chunks), dim3(kBlock), 0, c.stream,
+    hipLaunchKernelGGL(k_make_axis, dim3(chunk_grid(c)), dim3(kBlock), 0, c.stream,
                        c.recs.p, nCells, c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(),
                        c.axisT.p);
     ORB_HIP(hipGetLastError());
