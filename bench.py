@@ -90,6 +90,156 @@ def cpu_baseline(log2n, d, levels):
     return (t1 - t0) * 1e3, r
 
 
+def pct(v, q):
+    v = sorted(v)
+    if not v:
+        return None
+    k = min(len(v) - 1, max(0, int(round(q / 100.0 * (len(v) - 1)))))
+    return v[k]
+
+
+def serve_bench(args):
+    """Serving workload: requests with N log-uniform in [2^16, 2^24] and d
+    log-uniform in [2^4, 2^12], half uniform and half Plummer-clustered,
+    particles resident in HBM.  Run 1 (saturated): every request queued at
+    t=0, builds/s = requests / makespan.  Run 2 (Poisson): arrivals at 70% of
+    the saturated rate, latency = done - arrival (p50/p99).  The comparator is
+    the same requests built one after another with the plain form (measured
+    build + tree copy-back per request), fed the same arrival times."""
+    import numpy as np
+    import inputs
+    import orb_amd as orb
+
+    K = args.requests
+    rng = np.random.default_rng(args.seed)
+    log(f"[serve] generating particle pools (2^24 uniform, 2^24 clustered)")
+    top = 1 << 24
+    pool = [np.stack([rng.uniform(-0.5, 0.5, top).astype(np.float32) for _ in range(3)]),
+            np.stack(inputs.clustered(top, n_spheres=8, a=0.02, background=0.05, seed=args.seed))]
+    reqs = []
+    for k in range(K):
+        n = int(round(2.0 ** rng.uniform(16, 24)))
+        d = int(round(2.0 ** rng.uniform(4, 12)))
+        kind = k % 2
+        off = int(rng.integers(0, top - n + 1))
+        reqs.append((kind, off, n, d))
+    tot_particles = sum(r[2] for r in reqs)
+    log(f"[serve] {K} requests, {tot_particles} particles in all")
+
+    # comparator: serial plain builds, inputs in HBM, tree copied back
+    serial_plain, serial_batched = [], []
+    for k, (kind, off, n, d) in enumerate(reqs):
+        src = pool[kind][:, off:off + n]
+        ctx = orb.Context(n, d, generate=False)
+        ctx.set_particles(src[0], src[1], src[2])
+        ctx.save_particles()
+        for form, dst in ((orb.FORM_PLAIN, serial_plain), (orb.FORM_BATCHED, serial_batched)):
+            ctx.restore_particles()
+            t0 = time.perf_counter()
+            ctx.build(form, orb.LEVELS_FULL)
+            ctx.heap()
+            ctx.ranges()
+            dst.append((time.perf_counter() - t0) * 1e3)
+        ctx.close()
+    log(f"[serve] serial plain {sum(serial_plain):.1f} ms, serial batched "
+        f"{sum(serial_batched):.1f} ms for all requests")
+
+    dev = [None] * K
+
+    def upload():
+        for k, (kind, off, n, d) in enumerate(reqs):
+            if dev[k] is None:
+                dev[k] = [orb.DeviceArray(pool[kind][a, off:off + n]) for a in range(3)]
+            else:
+                for a in range(3):
+                    dev[k][a].upload(pool[kind][a, off:off + n])
+        orb.hip().hipDeviceSynchronize()
+
+    def run_server(arrivals):
+        upload()
+        srv = orb.Server(particle_capacity=args.serve_particles, cell_capacity=1 << 20,
+                         max_active=args.serve_active, levels_mode=orb.LEVELS_FULL)
+        # one untimed request warms the kernels' first launches
+        w = dev[0]
+        wid = srv.submit_device(w[0].address, w[1].address, w[2].address,
+                                reqs[0][2], reqs[0][3])
+        while srv.step():
+            pass
+        srv.fetch(wid, reqs[0][2], reqs[0][3], particles=False)
+        upload()
+        ids = {}
+        lat = [None] * K
+        errors = 0
+        t0 = time.perf_counter()
+        i = done = 0
+        while done < K:
+            now = time.perf_counter() - t0
+            while i < K and arrivals[i] <= now:
+                t = dev[i]
+                ids[srv.submit_device(t[0].address, t[1].address, t[2].address,
+                                      reqs[i][2], reqs[i][3])] = i
+                i += 1
+            left = srv.step()
+            while True:
+                rid = srv.next_done()
+                if rid is None:
+                    break
+                k = ids.pop(rid)
+                res = srv.fetch(rid, reqs[k][2], reqs[k][3], particles=False)
+                errors += res["error"] != 0
+                lat[k] = res["done_ms"] * 1e-3 - (t0 + arrivals[k])
+                done += 1
+            if left == 0 and i < K:
+                time.sleep(max(0.0, arrivals[i] - (time.perf_counter() - t0)))
+        stats = srv.stats()
+        srv.close()
+        makespan = max(arrivals[k] + lat[k] for k in range(K)) - arrivals[0]
+        return [v * 1e3 for v in lat], makespan, stats, errors
+
+    def simulate(service_ms, arrivals):
+        t, lat = 0.0, []
+        for a, s in zip(arrivals, service_ms):
+            t = max(t, a) + s * 1e-3
+            lat.append((t - a) * 1e3)
+        return lat, t - arrivals[0]
+
+    log("[serve] saturated run")
+    zero = [0.0] * K
+    lat_s, span_s, st_s, err_s = run_server(zero)
+    mu = K / span_s
+    rate = args.rate if args.rate > 0 else 0.7 * mu
+    gaps = rng.exponential(1.0 / rate, K)
+    arrivals = list(np.cumsum(gaps) - gaps[0])
+    log(f"[serve] Poisson run at {rate:.1f} requests/s")
+    lat_p, span_p, st_p, err_p = run_server(arrivals)
+    sp_lat, sp_span = simulate(serial_plain, arrivals)
+    sb_lat, sb_span = simulate(serial_batched, arrivals)
+    out = {
+        "config": "c5", "workload": "serving: N log-uniform [2^16,2^24], d log-uniform "
+                                    "[2^4,2^12], 50% uniform / 50% Plummer-clustered, Poisson "
+                                    "arrivals, full trees, particles resident in HBM",
+        "requests": K, "particles": tot_particles,
+        "saturated": {"server_builds_per_s": round(mu, 2),
+                      "serial_plain_builds_per_s": round(K / (sum(serial_plain) * 1e-3), 2),
+                      "serial_batched_builds_per_s": round(K / (sum(serial_batched) * 1e-3), 2),
+                      "server_vs_serial_plain": round(mu * sum(serial_plain) * 1e-3 / K, 3),
+                      "max_batch": st_s["max_batch"], "steps": st_s["steps"], "errors": err_s},
+        "poisson": {"rate_per_s": round(rate, 2),
+                    "server_p50_ms": round(pct(lat_p, 50), 3),
+                    "server_p99_ms": round(pct(lat_p, 99), 3),
+                    "server_builds_per_s": round(K / span_p, 2),
+                    "serial_plain_p50_ms": round(pct(sp_lat, 50), 3),
+                    "serial_plain_p99_ms": round(pct(sp_lat, 99), 3),
+                    "serial_plain_builds_per_s": round(K / sp_span, 2),
+                    "serial_batched_p50_ms": round(pct(sb_lat, 50), 3),
+                    "serial_batched_p99_ms": round(pct(sb_lat, 99), 3),
+                    "max_batch": st_p["max_batch"], "errors": err_p},
+        "note": "serial_* latencies: the measured per-request build times (inputs in HBM, "
+                "tree copied back) queued first-come first-served on the same arrival times",
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,10 +251,15 @@ def main():
     ap.add_argument("--forms", default="plain,batched")
     ap.add_argument("--cpu-log2n", type=int, default=26)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", choices=["c2", "c3", "c4"], default="c2",
+    ap.add_argument("--config", choices=["c2", "c3", "c4", "c5"], default="c2",
                     help="c2: uniform 2^27 -> 2^10; c3: clustered 2^26 -> 2^14; "
-                         "c4: uniform 2^28 -> 2^20 with per-level times")
+                         "c4: uniform 2^28 -> 2^20 with per-level times; c5: serving")
     ap.add_argument("--per-level", action="store_true")
+    ap.add_argument("--requests", type=int, default=160)
+    ap.add_argument("--rate", type=float, default=0.0)
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--serve-particles", type=int, default=1 << 28)
+    ap.add_argument("--serve-active", type=int, default=64)
     args = ap.parse_args()
     if args.config == "c3":
         args.log2n, args.d = 26, 1 << 14
@@ -115,6 +270,10 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                           "(multi-GPU sharding is a design note, not a measured path)"}))
         sys.exit(2)
+
+    if args.config == "c5":
+        serve_bench(args)
+        return
 
     import orb_amd as orb
     levels = orb.LEVELS_FULL if args.levels == "full" else orb.LEVELS_REFERENCE

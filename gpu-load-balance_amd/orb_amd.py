@@ -61,6 +61,26 @@ class Timing(C.Structure):
                 ("count_bytes", C.c_int64), ("count_launches", C.c_int32), ("pad", C.c_int32)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("particle_capacity", C.c_int64), ("cell_capacity", C.c_int64),
+                ("max_active", C.c_int32), ("levels_mode", C.c_int32), ("cmp", C.c_int32),
+                ("partition", C.c_int32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(k, C.c_int64) for k in (
+        "submitted", "admitted", "completed", "steps", "cells_stepped", "max_batch", "pending",
+        "active", "finishing", "particle_capacity", "particles_free", "cell_capacity",
+        "cells_free")]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("n", C.c_int64), ("d", C.c_int32), ("error", C.c_int32),
+                ("error_cell", C.c_int32), ("nlevels", C.c_int32), ("total_passes", C.c_int32),
+                ("level_passes", C.c_int32 * 64), ("pad", C.c_int32), ("submit_ms", C.c_double),
+                ("admit_ms", C.c_double), ("done_ms", C.c_double)]
+
+
 class OrbError(RuntimeError):
     def __init__(self, code, what=""):
         self.code = code
@@ -102,6 +122,17 @@ PROTOTYPES = {
     "orb_enable_timing": (C.c_int, [_vp, _b]),
     "orb_enable_graph": (C.c_int, [_vp, _b]),
     "orb_get_level_timing": (C.c_int, [_vp, _vp, _i32]),
+    "orb_server_create": (C.c_int, [C.POINTER(_vp), C.POINTER(ServerConfig)]),
+    "orb_server_destroy": (C.c_int, [_vp]),
+    "orb_server_submit": (C.c_int, [_vp, _vp, _vp, _vp, _i64, _i32, C.POINTER(C.c_uint64)]),
+    "orb_server_submit_device": (C.c_int, [_vp, _vp, _vp, _vp, _i64, _i32,
+                                           C.POINTER(C.c_uint64)]),
+    "orb_server_step": (C.c_int, [_vp]),
+    "orb_server_poll": (C.c_int, [_vp, C.c_uint64]),
+    "orb_server_next_done": (C.c_int, [_vp, C.POINTER(C.c_uint64)]),
+    "orb_server_fetch": (C.c_int, [_vp, C.c_uint64, _vp, _vp, _vp, _vp, _vp, _vp, _i64,
+                                   C.POINTER(RequestInfo)]),
+    "orb_server_get_stats": (C.c_int, [_vp, C.POINTER(ServerStats)]),
     "orb_error_string": (C.c_char_p, [C.c_int]),
     "orb_last_hip_error": (C.c_int, []),
     "orb_device_count": (C.c_int, []),
@@ -124,7 +155,7 @@ def lib():
 
 
 def _p(a):
-    return a.ctypes.data_as(C.c_void_p)
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
 def _check(rc, what):
@@ -298,3 +329,146 @@ class Context:
         _check(lib().orb_get_timing(self.h, C.byref(t)), "orb_get_timing")
         return {"count_ms": t.count_ms, "partition_ms": t.partition_ms, "other_ms": t.other_ms,
                 "count_bytes": t.count_bytes, "count_launches": t.count_launches}
+
+
+_hip = None
+
+
+def hip():
+    """The HIP runtime liborb_hip.so is linked against (one runtime per
+    process: a second one, such as a framework's bundled copy, would not see
+    the GPU once this one owns it)."""
+    global _hip
+    if _hip is None:
+        lib()
+        h = C.CDLL("libamdhip64.so.7")
+        h.hipMalloc.argtypes = [C.POINTER(_vp), C.c_size_t]
+        h.hipFree.argtypes = [_vp]
+        h.hipMemcpy.argtypes = [_vp, _vp, C.c_size_t, C.c_int]
+        h.hipDeviceSynchronize.argtypes = []
+        for f in (h.hipMalloc, h.hipFree, h.hipMemcpy, h.hipDeviceSynchronize):
+            f.restype = C.c_int
+        _hip = h
+    return _hip
+
+
+class DeviceArray:
+    """A float32 array in device memory (hipMalloc), for device requests."""
+
+    def __init__(self, host):
+        host = np.ascontiguousarray(host, np.float32)
+        self.n = host.size
+        self.ptr = C.c_void_p()
+        if hip().hipMalloc(C.byref(self.ptr), max(self.n, 1) * 4) != 0:
+            raise OrbError(ERR_NOMEM, "hipMalloc")
+        self.upload(host)
+
+    def upload(self, host):
+        host = np.ascontiguousarray(host, np.float32)
+        if hip().hipMemcpy(self.ptr, _p(host), self.n * 4, 1) != 0:
+            raise OrbError(ERR_HIP, "hipMemcpy")
+
+    def numpy(self):
+        out = np.empty(self.n, np.float32)
+        if hip().hipMemcpy(_p(out), self.ptr, self.n * 4, 2) != 0:
+            raise OrbError(ERR_HIP, "hipMemcpy")
+        return out
+
+    @property
+    def address(self):
+        return self.ptr.value
+
+    def free(self):
+        if self.ptr:
+            hip().hipFree(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching ORB server (include/orb_serve.h)."""
+
+    def __init__(self, particle_capacity=0, cell_capacity=0, max_active=16,
+                 levels_mode=LEVELS_FULL, cmp=CMP_LT, partition=PART_HOARE):
+        L = lib()
+        if L.orb_device_count() < 1:
+            raise OrbError(ERR_NODEV, "orb_server_create")
+        cfg = ServerConfig(int(particle_capacity), int(cell_capacity), int(max_active),
+                           int(levels_mode), int(cmp), int(partition))
+        h = C.c_void_p()
+        _check(L.orb_server_create(C.byref(h), C.byref(cfg)), "orb_server_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().orb_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, x, y, z, d):
+        x, y, z = (np.ascontiguousarray(a, np.float32) for a in (x, y, z))
+        rid = C.c_uint64()
+        _check(lib().orb_server_submit(self.h, _p(x), _p(y), _p(z), len(x), int(d),
+                                       C.byref(rid)), "orb_server_submit")
+        return rid.value
+
+    def submit_device(self, xp, yp, zp, n, d):
+        """Particles already in device memory (integer addresses, e.g. a
+        torch tensor's data_ptr()); reordered in place when the build ends."""
+        rid = C.c_uint64()
+        _check(lib().orb_server_submit_device(self.h, C.c_void_p(xp), C.c_void_p(yp),
+                                              C.c_void_p(zp), int(n), int(d), C.byref(rid)),
+               "orb_server_submit_device")
+        return rid.value
+
+    def step(self):
+        return _check(lib().orb_server_step(self.h), "orb_server_step")
+
+    def poll(self, rid):
+        return _check(lib().orb_server_poll(self.h, rid), "orb_server_poll") == 1
+
+    def next_done(self):
+        """Id of a request finished since the last call, or None."""
+        rid = C.c_uint64()
+        rc = _check(lib().orb_server_next_done(self.h, C.byref(rid)), "orb_server_next_done")
+        return rid.value if rc == 1 else None
+
+    def fetch(self, rid, n, d, particles=True):
+        cap = int(lib().orb_heap_capacity(int(d)))
+        x = np.empty(n, np.float32) if particles else None
+        y = np.empty(n, np.float32) if particles else None
+        z = np.empty(n, np.float32) if particles else None
+        ranges = np.zeros((cap, 2), np.uint32)
+        heap = np.zeros(cap, CELL_DTYPE)
+        exists = np.zeros(cap, np.uint8)
+        info = RequestInfo()
+        rc = lib().orb_server_fetch(self.h, rid, _p(x), _p(y), _p(z), _p(ranges), _p(heap),
+                                    _p(exists), cap, C.byref(info))
+        if rc < 0 and rc != ERR_UB:
+            raise OrbError(rc, "orb_server_fetch")
+        return {"x": x, "y": y, "z": z, "ranges": ranges, "heap": heap, "exists": exists,
+                "error": info.error, "nlevels": info.nlevels,
+                "level_passes": list(info.level_passes[:info.nlevels]),
+                "total_passes": info.total_passes, "submit_ms": info.submit_ms,
+                "admit_ms": info.admit_ms, "done_ms": info.done_ms}
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().orb_server_get_stats(self.h, C.byref(st)), "orb_server_get_stats")
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
