@@ -214,27 +214,35 @@ __device__ inline float candidate(int t, float lo, float hi)
 }
 
 // Bin of key v among the sorted candidates cand[1..255] (cand[0] = -inf,
-// cand[256] = +inf): the number of candidates <= v (lt) or < v (le).  Keys
-// outside the margins [lo, hi] never reach here.  The candidates sit close to
-// lo + j*(hi-lo)/256, so an arithmetic estimate is corrected by a short walk
-// of exact comparisons; the walk alone makes the result exact.
-__device__ inline int bin_of(const float *cand, float v, int le, float lo, float scale)
+// cand[256] = +inf): the number of candidates <= v (LE = 0) or < v (LE = 1).
+// Keys outside the margins [lo, hi] never reach here.  The candidates sit
+// close to lo + j*(hi-lo)/256, so the arithmetic estimate is checked against
+// its two neighbours (one pair of LDS reads); only an estimate that misses
+// (rounding at a bin edge, or a window a few ulps wide) walks, and the walk
+// alone makes the result exact.
+template <int LE>
+__device__ inline int bin_of(const float *cand, float v, float lo, float scale)
 {
-    float ef = fminf(fmaxf((v - lo) * scale, 0.0f), (float)(kBins - 1));
+    const float ef = fminf(fmaxf((v - lo) * scale, 0.0f), (float)(kBins - 1));
     int e = (int)ef;
-    if (le) {
-        while (e > 0 && cand[e] >= v) --e;
-        while (e < kBins - 1 && cand[e + 1] < v) ++e;
-    } else {
-        while (e > 0 && cand[e] > v) --e;
-        while (e < kBins - 1 && cand[e + 1] <= v) ++e;
+    const float c0 = cand[e], c1 = cand[e + 1];
+    const bool ok = LE ? (c0 < v && v <= c1) : (c0 <= v && v < c1);
+    if (__builtin_expect(!ok, 0)) {
+        if (LE) {
+            while (e > 0 && cand[e] >= v) --e;
+            while (e < kBins - 1 && cand[e + 1] < v) ++e;
+        } else {
+            while (e > 0 && cand[e] > v) --e;
+            while (e < kBins - 1 && cand[e + 1] <= v) ++e;
+        }
     }
     return e;
 }
 
+template <int LE>
 __device__ inline void k_b_hist_chunk(uint32_t q, const CellRec *recs, const uint32_t *first,
                                       const uint32_t *chunk_cell, const float *x, const float *y,
-                                      const float *z, int le, uint16_t *chunk_hist)
+                                      const float *z, uint16_t *chunk_hist)
 {
     __shared__ float cand[kBins + 1];
     __shared__ uint32_t hist[4][kBins];
@@ -265,7 +273,7 @@ __device__ inline void k_b_hist_chunk(uint32_t q, const CellRec *recs, const uin
     auto put = [&](float v) {
         if (v < lo) below++;
         else if (v > hi) above++;
-        else atomicAdd(&h[bin_of(cand, v, le, lo, scale)], 1u);
+        else atomicAdd(&h[bin_of<LE>(cand, v, lo, scale)], 1u);
     };
     uint32_t ab = (b + 3u) & ~3u;
     if (ab > e) ab = e;
@@ -298,15 +306,16 @@ __device__ inline void k_b_hist_chunk(uint32_t q, const CellRec *recs, const uin
     chunk_hist[(size_t)q * kBins + t] = (uint16_t)sum;
 }
 
+template <int LE>
 __global__ __launch_bounds__(kBlock) void k_b_hist(const CellRec *recs, int32_t nCells,
                                                    const uint32_t *first,
                                                    const uint32_t *chunk_cell, const float *x,
-                                                   const float *y, const float *z, int le,
+                                                   const float *y, const float *z,
                                                    uint16_t *chunk_hist)
 {
     const uint32_t total = first[nCells];
     for (uint32_t q = blockIdx.x; q < total; q += gridDim.x) {
-        k_b_hist_chunk(q, recs, first, chunk_cell, x, y, z, le, chunk_hist);
+        k_b_hist_chunk<LE>(q, recs, first, chunk_cell, x, y, z, chunk_hist);
         __syncthreads();  // the next chunk reuses this block's LDS
     }
 }
@@ -368,8 +377,74 @@ __device__ inline void record_level_passes(int32_t *level_passes, int32_t l, int
     if (passes > *((volatile int32_t *)&level_passes[l])) atomicMax(&level_passes[l], passes);
 }
 
-// One wave per cell: prefix sums of the cell histogram in LDS, then lane 0
-// replays up to 8 passes of the bisection controller (orbit.cpp:149-231).
+// One histogram round of the bisection controller (orbit.cpp:149-231),
+// replayed by a whole wave.  P[j] (j in [1, 255]) is the count below
+// candidate cut j of the round that starts from margins [lo, hi].  Counts
+// grow with j and so does the controller's difference, so the passes it
+// makes are the binary-search path towards b, the first node whose
+// difference is > 0 (go left at a node >= b, right otherwise); the round
+// stops at the first node of that path within +-2 of the target.  b comes
+// from one ballot count, the path from b alone, so the 8 counts are read in
+// parallel instead of one dependent read per pass.
+struct RoundOut {
+    float L, R;
+    int32_t passes;
+    bool found;
+};
+
+__device__ inline RoundOut replay_round(const uint32_t *P, float lo, float hi, float target,
+                                        int32_t passes0, int lane, uint32_t *trace_row)
+{
+    uint32_t nle = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int j = 4 * lane + 1 + q;
+        const bool le0 = j < kBins && difference_from_target(P[j], target) <= 0;
+        nle += (uint32_t)__popcll(__ballot(le0));
+    }
+    const int b = 1 + (int)nle;
+    const int steps = kHistK < ORB_MAX_PASSES - passes0 ? kHistK : ORB_MAX_PASSES - passes0;
+    int nodes[kHistK];
+    {
+        int node = kBins / 2, step = kBins / 4;
+#pragma unroll
+        for (int d = 0; d < kHistK; ++d) {
+            nodes[d] = node;
+            node = node >= b ? node - step : node + step;
+            step >>= 1;
+        }
+    }
+    uint32_t cnt[kHistK];
+#pragma unroll
+    for (int d = 0; d < kHistK; ++d) cnt[d] = P[nodes[d]];
+    int s = steps;
+#pragma unroll
+    for (int d = kHistK - 1; d >= 0; --d)
+        if (d < steps && abs(difference_from_target(cnt[d], target)) < 3) s = d;
+    const bool found = s < steps;
+    const int moves = found ? s : steps;
+    float L = lo, R = hi;
+#pragma unroll
+    for (int d = 0; d < kHistK; ++d) {
+        if (d < moves) {
+            const float cut = cell_cut(L, R);
+            if (nodes[d] >= b) R = cut;
+            else L = cut;
+        }
+    }
+    const int used = found ? s + 1 : steps;
+    if (trace_row && lane < used) {
+        uint32_t v = 0;
+#pragma unroll
+        for (int d = 0; d < kHistK; ++d)
+            if (d == lane) v = cnt[d];
+        trace_row[passes0 + lane] = v;
+    }
+    return RoundOut{L, R, passes0 + used, found};
+}
+
+// One wave per cell: prefix sums of the cell histogram in LDS, then the
+// wave replays up to 8 passes of the bisection controller.
 __global__ __launch_bounds__(kBlock) void k_b_replay(CellRec *recs, int32_t nCells,
                                                      uint32_t *cell_hist, orb_cell *heap,
                                                      uint32_t *trace, int32_t *level_passes)
@@ -402,32 +477,15 @@ __global__ __launch_bounds__(kBlock) void k_b_replay(CellRec *recs, int32_t nCel
     if (lane == 0) P[w][0] = 0u;
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
-    if (lane != 0) return;
     const uint32_t n = r.end - r.begin;
     const float target = difference_target(n, r.nLeaf);
-    float L = r.margin_l, R = r.margin_r;
-    int node = kBins / 2, step = kBins / 4;
-    int32_t passes = r.passes;
-    bool found = false;
-    for (int s = 0; s < kHistK && passes < ORB_MAX_PASSES; ++s) {
-        const float cut = cell_cut(L, R);
-        const uint32_t cnt = P[w][node];
-        if (trace) trace[((size_t)r.hb + r.id) * ORB_MAX_PASSES + passes] = cnt;
-        passes++;
-        const int32_t diff = difference_from_target(cnt, target);
-        if (abs(diff) < 3) {
-            found = true;
-            break;
-        }
-        if (diff > 0) {
-            R = cut;
-            node -= step;
-        } else {
-            L = cut;
-            node += step;
-        }
-        step >>= 1;
-    }
+    const RoundOut ro = replay_round(P[w], r.margin_l, r.margin_r, target, r.passes, lane,
+                                     trace ? trace + ((size_t)r.hb + r.id) * ORB_MAX_PASSES
+                                           : nullptr);
+    if (lane != 0) return;
+    const float L = ro.L, R = ro.R;
+    const int32_t passes = ro.passes;
+    const bool found = ro.found;
     r.margin_l = L;
     r.margin_r = R;
     r.cut = cell_cut(L, R);
@@ -478,18 +536,43 @@ struct SubCell {
     int32_t pad;
 };
 
-__device__ inline float wave_sum_f(float v) { return v; }
-
-__device__ inline uint32_t wave_sum_u32(uint32_t v)
+__device__ inline void lds_wave_sync()
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
 }
 
-// Processes every remaining level of the subtree rooted at a small cell.
-// One block per cell of the level (other slots exit).  Waves take the cells
-// of one depth round-robin; a barrier separates depths.
+// Keys of [pb, pe) in LDS below the cut (lt) or at most the cut (le), summed
+// over the wave: one ballot + popcount per 64 keys, four LDS reads in flight.
+__device__ inline uint32_t wave_count_below(const float *key, uint32_t pb, uint32_t pe, float cut,
+                                            int le_cmp, int lane)
+{
+    uint32_t cnt = 0;
+    for (uint32_t base = pb; base < pe; base += 256u) {
+        float v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t p = base + (uint32_t)(64 * u + lane);
+            v[u] = p < pe ? key[p] : INFINITY;  // never below a finite cut
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const bool below = le_cmp ? (v[u] <= cut) : (v[u] < cut);
+            cnt += (uint32_t)__popcll(__ballot(below));
+        }
+    }
+    return cnt;
+}
+
+// Every remaining level of the subtree under a small cell, in LDS.  Depth by
+// depth, the cells of one depth are taken four at a time, one per wave; at
+// depths with fewer than four cells the waves of the block split each cell's
+// keys between them and add their counts through LDS, one barrier per pass.
+// The bisection controller (orbit.cpp:149-231) runs pass by pass: at these
+// sizes a pass is a few LDS reads per lane.  The Hoare partition
+// (partition.cpp:28-60) is the stopper closed form: stopper lists by ballot
+// scans, the crossing K by a 64-way search, K swaps, the final swap.  Every
+// barrier is reached by all waves the same number of times.
 __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32_t nCells,
                                                       orb_cell *heap, uint8_t *exists,
                                                       uint8_t *sub, uint32_t *ranges, float *x,
@@ -500,22 +583,63 @@ __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32
     __shared__ float sx[kSmall], sy[kSmall], sz[kSmall];
     __shared__ uint16_t slpos[kSmall], srpos[kSmall];
     __shared__ SubCell cur[kSmallLeaves], nxt[kSmallLeaves];
+    __shared__ uint32_t sCnt[2][4];
+    __shared__ int32_t sAct[2][4];
+    __shared__ uint32_t sTL[4], sTR[4], sK[4], sI[4];
     __shared__ int32_t s_ncur, s_nvalid, s_err;
-    for (int32_t c = blockIdx.x; c < nCells; c += gridDim.x) {
-    const CellRec rr = recs[c];
-    if (rr.flags != 2) continue;  // uniform per block
-    const uint32_t begin = rr.begin, n = rr.end - rr.begin;
+    // the block's pass maxima per level of the build slot in s_lp0, flushed
+    // with one atomicMax per level when the slot changes and at the end
+    __shared__ int32_t s_lpmax[64];
+    __shared__ int32_t s_lp0;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    if (t < 64) s_lpmax[t] = 0;
+    if (t == 0) s_lp0 = -1;
+    __syncthreads();
+    CellRec rn;  // the next cell's record, loaded one cell ahead
+    if (blockIdx.x < (uint32_t)nCells) rn = recs[blockIdx.x];
+    for (int32_t c = blockIdx.x; c < nCells; c += gridDim.x) {
+    const CellRec rr = rn;
+    if (c + (int32_t)gridDim.x < nCells) rn = recs[c + gridDim.x];
+    if (rr.flags != 2) continue;  // uniform per block
+    if ((rr.lp - (rr.lp & 63)) != s_lp0) {
+        if (t < 64 && s_lp0 >= 0 && s_lpmax[t] > 0) atomicMax(&level_passes[s_lp0 + t], s_lpmax[t]);
+        __syncthreads();
+        if (t < 64) s_lpmax[t] = 0;
+        if (t == 0) s_lp0 = rr.lp - (rr.lp & 63);
+        __syncthreads();
+    }
+    const uint32_t begin = rr.begin, n = rr.end - rr.begin;
     const int64_t hb = rr.hb;
-    const int32_t level = rr.lp & 63, lmax = rr.lmax, lp0 = rr.lp - level;
+    const int32_t level = rr.lp & 63, lmax = rr.lmax;
     int32_t *st = status + rr.st;
-    for (uint32_t k = (uint32_t)t; k < n; k += kBlock) {
-        sx[k] = x[begin + k];
-        sy[k] = y[begin + k];
-        sz[k] = z[begin + k];
+    orb_cell root;
+    if (t == 0) root = heap[hb + rr.id];
+    {
+        // every load of the cell in flight before the first LDS store: one
+        // memory latency per cell instead of one per 256 keys
+        constexpr int kIt = (kSmall + kBlock - 1) / kBlock;
+        float vx[kIt], vy[kIt], vz[kIt];
+#pragma unroll
+        for (int u = 0; u < kIt; ++u) {
+            const uint32_t k = (uint32_t)(t + u * kBlock);
+            if (k < n) {
+                vx[u] = x[begin + k];
+                vy[u] = y[begin + k];
+                vz[u] = z[begin + k];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kIt; ++u) {
+            const uint32_t k = (uint32_t)(t + u * kBlock);
+            if (k < n) {
+                sx[k] = vx[u];
+                sy[k] = vy[u];
+                sz[k] = vz[u];
+            }
+        }
     }
     if (t == 0) {
-        cur[0].cell = heap[hb + rr.id];
+        cur[0].cell = root;
         cur[0].lb = 0;
         cur[0].le = n;
         cur[0].valid = 1;
@@ -526,7 +650,6 @@ __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32
     __syncthreads();
     for (int32_t l = level; l <= lmax; ++l) {
         const int32_t ncur = s_ncur;
-        // children of entry k go to 2k and 2k+1: the next depth must fit
         if (ncur == 0 || s_nvalid == 0 || s_err) break;
         if (2 * ncur > kSmallLeaves) {
             // cannot happen while the root has <= kSmallLeaves leaves; refuse
@@ -534,123 +657,191 @@ __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32
             if (t == 0 && atomicCAS(&st[0], 0, 1) == 0) st[1] = rr.id;
             break;
         }
-        for (int32_t k = w; k < ncur; k += 4) {
-            SubCell sc = cur[k];
-            nxt[2 * k].valid = 0;
-            nxt[2 * k + 1].valid = 0;
-            if (!sc.valid) continue;
+        const int g = ncur >= 4 ? 1 : 4 / ncur;          // waves per cell
+        const int nslots = ncur >= 4 ? 4 : ncur;
+        const int nbatch = ncur >= 4 ? ncur / 4 : 1;      // ncur is a power of two
+        const int slot = w / g, part = w % g, w0 = slot * g;
+        for (int bt = 0; bt < nbatch; ++bt) {
+            const int32_t k = bt * nslots + slot;
+            const SubCell sc = cur[k];
+            const bool valid = sc.valid != 0;
             orb_cell cl = sc.cell;
             const uint32_t lb = sc.lb, le = sc.le, m = le - lb;
             const float *key = cl.cutAxis == 0 ? sx : (cl.cutAxis == 1 ? sy : sz);
-            // bisection controller (orbit.cpp:149-231) over the LDS keys; a
-            // pass counts with one ballot + popcount per 64 keys (scalar sum,
-            // no cross-lane shuffles)
+            if (part == 0 && lane == 0) {
+                nxt[2 * k].valid = 0;
+                nxt[2 * k + 1].valid = 0;
+            }
+            // ---- counting: the g waves of a cell run the same controller on
+            // the same summed counts
+            const uint32_t seg = ((m + (uint32_t)g - 1u) / (uint32_t)g + 63u) & ~63u;
+            uint32_t pb = lb + (uint32_t)part * seg, pe = pb + seg;
+            if (pb > le) pb = le;
+            if (pe > le) pe = le;
+            const float target = difference_target(m, cl.nLeafCells);
+            uint32_t *trow = trace ? trace + ((size_t)hb + cl.id) * ORB_MAX_PASSES : nullptr;
             float L = cl.cutMarginLeft, R = cl.cutMarginRight;
             int32_t passes = 0;
             bool found = false;
-            const float target = difference_target(m, cl.nLeafCells);
-            while (!found && passes < ORB_MAX_PASSES) {
-                const float cut = cell_cut(L, R);
-                uint32_t cnt = 0;
-                for (uint32_t base = lb; base < le; base += 64) {
-                    const uint32_t p = base + lane;
-                    const float v = p < le ? key[p] : INFINITY;
-                    const bool below = le_cmp ? (v <= cut) : (v < cut);
-                    cnt += (uint32_t)__popcll(__ballot(below && p < le));
+            if (g == 1) {
+                while (valid && !found && passes < ORB_MAX_PASSES) {
+                    const float cut = cell_cut(L, R);
+                    const uint32_t cnt = wave_count_below(key, lb, le, cut, le_cmp, lane);
+                    if (trow && lane == 0) trow[passes] = cnt;
+                    passes++;
+                    const int32_t diff = difference_from_target(cnt, target);
+                    if (abs(diff) < 3) found = true;
+                    else if (diff > 0) R = cut;
+                    else L = cut;
                 }
-                if (trace && lane == 0) trace[((size_t)hb + cl.id) * ORB_MAX_PASSES + passes] = cnt;
-                passes++;
-                const int32_t diff = difference_from_target(cnt, target);
-                if (abs(diff) < 3) found = true;
-                else if (diff > 0) R = cut;
-                else L = cut;
-            }
-            cl.cutMarginLeft = L;
-            cl.cutMarginRight = R;
-            cl.foundCut = found;
-            const float cut = cell_cut(L, R);
-            // Hoare partition as scans, in LDS (partition.cpp:28-60)
-            uint32_t TL = 0, TR = 0;
-            for (uint32_t base = lb; base < le; base += 64) {
-                const uint32_t p = base + lane;
-                const bool in = p < le;
-                const float v = in ? key[p] : 0.0f;
-                const bool isL = in && v >= cut, isR = in && v <= cut;
-                const uint64_t mL = __ballot(isL), mR = __ballot(isR);
-                const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-                if (isL) slpos[lb + TL + __popcll(mL & below)] = (uint16_t)(p - lb);
-                if (isR) srpos[lb + TR + __popcll(mR & below)] = (uint16_t)(p - lb);
-                TL += __popcll(mL);
-                TR += __popcll(mR);
-            }
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-            __builtin_amdgcn_wave_barrier();
-            if (m == 0 || TL == 0) {
-                if (lane == 0) {
-                    s_err = 1;
-                    if (atomicCAS(&st[0], 0, 1) == 0) st[1] = cl.id;
-                }
-                continue;
-            }
-            const uint16_t *Lp = slpos + lb;
-            const uint16_t *Rp = srpos + lb;
-            // K = #{t : L[t] < R[TR-1-t]} (true then false in t): 64 probes
-            // per step, the ballot picks the crossing
-            uint32_t lo = 0, hi = TL < TR ? TL : TR;
-            while (hi - lo > 64u) {
-                const uint32_t span = hi - lo;
-                const uint32_t tp = lo + (uint32_t)(((lane + 1) * span) / 65u);
-                const uint64_t mk = __ballot(Lp[tp] < Rp[TR - 1 - tp]);
-                if (mk == 0ull) {
-                    hi = lo + span / 65u;
-                } else {
-                    const int j = 63 - __clzll((long long)mk);
-                    const uint32_t tj = lo + (uint32_t)(((j + 1) * span) / 65u);
-                    const uint32_t tn = j < 63 ? lo + (uint32_t)(((j + 2) * span) / 65u) : hi;
-                    lo = tj + 1;
-                    hi = tn;
-                }
-            }
-            {
-                const uint32_t tp = lo + (uint32_t)lane;
-                lo += (uint32_t)__popcll(__ballot(tp < hi && Lp[tp] < Rp[TR - 1 - tp]));
-            }
-            const uint32_t K = lo;
-            uint32_t i;
-            if (K == 0) {
-                i = Lp[0];
             } else {
-                i = Rp[TR - K];
-                if (K < TL && Lp[K] < i) i = Lp[K];
+                for (int ps = 0;; ++ps) {
+                    const bool act = valid && !found && passes < ORB_MAX_PASSES;
+                    const float cut = cell_cut(L, R);
+                    const uint32_t part_cnt = act ? wave_count_below(key, pb, pe, cut, le_cmp, lane) : 0u;
+                    if (lane == 0) {
+                        sCnt[ps & 1][w] = part_cnt;
+                        sAct[ps & 1][w] = act ? 1 : 0;
+                    }
+                    __syncthreads();  // double-buffered: one barrier per pass
+                    const bool any = (sAct[ps & 1][0] | sAct[ps & 1][1] | sAct[ps & 1][2] |
+                                      sAct[ps & 1][3]) != 0;
+                    if (!any) break;  // the same words for every wave
+                    if (act) {
+                        uint32_t cnt = 0;
+                        for (int q = 0; q < g; ++q) cnt += sCnt[ps & 1][w0 + q];
+                        if (trow && part == 0 && lane == 0) trow[passes] = cnt;
+                        passes++;
+                        const int32_t diff = difference_from_target(cnt, target);
+                        if (abs(diff) < 3) found = true;
+                        else if (diff > 0) R = cut;
+                        else L = cut;
+                    }
+                }
             }
-            for (uint32_t tt = lane; tt < K; tt += 64) {
-                const uint32_t a = lb + Lp[tt], b2 = lb + Rp[TR - 1 - tt];
-                float ax = sx[a], ay = sy[a], az = sz[a];
-                sx[a] = sx[b2]; sy[a] = sy[b2]; sz[a] = sz[b2];
-                sx[b2] = ax; sy[b2] = ay; sz[b2] = az;
+            __syncthreads();
+            // ---- partition: stopper lists (each wave one contiguous part)
+            const float cut = cell_cut(L, R);
+            uint32_t tl = 0, tr = 0;
+            if (valid) {
+                for (uint32_t base = pb; base < pe; base += 64) {
+                    const uint32_t p = base + (uint32_t)lane;
+                    const bool in = p < pe;
+                    const float v = in ? key[p] : 0.0f;
+                    tl += (uint32_t)__popcll(__ballot(in && v >= cut));
+                    tr += (uint32_t)__popcll(__ballot(in && v <= cut));
+                }
             }
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-            __builtin_amdgcn_wave_barrier();
-            if (lane == 0 && lb + i != le - 1) {
-                const uint32_t a = lb + i, b2 = le - 1;
-                float ax = sx[a], ay = sy[a], az = sz[a];
-                sx[a] = sx[b2]; sy[a] = sy[b2]; sz[a] = sz[b2];
-                sx[b2] = ax; sy[b2] = ay; sz[b2] = az;
-            }
-            // split (orbit.cpp:235-250) and bookkeeping
-            orb_cell lc, rc;
-            uint32_t *wl = reinterpret_cast<uint32_t *>(&lc);
-            uint32_t *wr = reinterpret_cast<uint32_t *>(&rc);
-#pragma unroll
-            for (int j = 0; j < (int)(sizeof(orb_cell) / 4); ++j) wl[j] = wr[j] = 0u;
-            const bool ok = right_child_id(cl.id) < rr.capr && cell_split(cl, lc, rc);
             if (lane == 0) {
+                sTL[w] = tl;
+                sTR[w] = tr;
+            }
+            __syncthreads();
+            uint32_t oL = 0, oR = 0, TL = 0, TR = 0;
+            for (int q = 0; q < g; ++q) {
+                const uint32_t a = sTL[w0 + q], b2 = sTR[w0 + q];
+                if (q < part) {
+                    oL += a;
+                    oR += b2;
+                }
+                TL += a;
+                TR += b2;
+            }
+            uint16_t *const Lp = slpos + lb;
+            uint16_t *const Rp = srpos + lb;
+            if (valid) {
+                const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+                for (uint32_t base = pb; base < pe; base += 64) {
+                    const uint32_t p = base + (uint32_t)lane;
+                    const bool in = p < pe;
+                    const float v = in ? key[p] : 0.0f;
+                    const bool isL = in && v >= cut, isR = in && v <= cut;
+                    const uint64_t mL = __ballot(isL), mR = __ballot(isR);
+                    if (isL) Lp[oL + __popcll(mL & below)] = (uint16_t)(p - lb);
+                    if (isR) Rp[oR + __popcll(mR & below)] = (uint16_t)(p - lb);
+                    oL += (uint32_t)__popcll(mL);
+                    oR += (uint32_t)__popcll(mR);
+                }
+            }
+            __syncthreads();
+            const bool bad = valid && (m == 0 || TL == 0);
+            if (valid && !bad && part == 0) {
+                // K = #{t : L[t] < R[TR-1-t]} (true then false in t)
+                uint32_t lo = 0, hi = TL < TR ? TL : TR;
+                while (hi - lo > 64u) {
+                    const uint32_t span = hi - lo;
+                    const uint32_t tp = lo + (uint32_t)(((lane + 1) * span) / 65u);
+                    const uint64_t mk = __ballot(Lp[tp] < Rp[TR - 1 - tp]);
+                    if (mk == 0ull) {
+                        hi = lo + span / 65u;
+                    } else {
+                        const int j = 63 - __clzll((long long)mk);
+                        const uint32_t tj = lo + (uint32_t)(((j + 1) * span) / 65u);
+                        const uint32_t tn = j < 63 ? lo + (uint32_t)(((j + 2) * span) / 65u) : hi;
+                        lo = tj + 1;
+                        hi = tn;
+                    }
+                }
+                {
+                    const uint32_t tp = lo + (uint32_t)lane;
+                    lo += (uint32_t)__popcll(__ballot(tp < hi && Lp[tp] < Rp[TR - 1 - tp]));
+                }
+                const uint32_t K = lo;
+                uint32_t i;
+                if (K == 0) {
+                    i = Lp[0];
+                } else {
+                    i = Rp[TR - K];
+                    if (K < TL && Lp[K] < i) i = Lp[K];
+                }
+                if (lane == 0) {
+                    sK[slot] = K;
+                    sI[slot] = i;
+                }
+            }
+            __syncthreads();
+            if (valid && !bad) {
+                const uint32_t K = sK[slot];
+                for (uint32_t tt = (uint32_t)(part * 64 + lane); tt < K; tt += (uint32_t)(g * 64)) {
+                    const uint32_t a = lb + Lp[tt], b2 = lb + Rp[TR - 1 - tt];
+                    const float ax = sx[a], ay = sy[a], az = sz[a];
+                    sx[a] = sx[b2];
+                    sy[a] = sy[b2];
+                    sz[a] = sz[b2];
+                    sx[b2] = ax;
+                    sy[b2] = ay;
+                    sz[b2] = az;
+                }
+            }
+            __syncthreads();
+            if (valid && part == 0 && lane == 0) {
+                cl.cutMarginLeft = L;
+                cl.cutMarginRight = R;
+                cl.foundCut = found;
                 heap[hb + cl.id] = cl;
-                record_level_passes(level_passes, lp0 + l, passes);
+                atomicMax(&s_lpmax[l], passes);  // LDS: the slots' waves finish together
+
+                orb_cell lc, rc;
+                uint32_t *wl = reinterpret_cast<uint32_t *>(&lc);
+                uint32_t *wr = reinterpret_cast<uint32_t *>(&rc);
+#pragma unroll
+                for (int j = 0; j < (int)(sizeof(orb_cell) / 4); ++j) wl[j] = wr[j] = 0u;
+                const uint32_t i = sI[slot];
+                const bool ok = !bad && right_child_id(cl.id) < rr.capr && cell_split(cl, lc, rc);
                 if (!ok) {
                     s_err = 1;
                     if (atomicCAS(&st[0], 0, 1) == 0) st[1] = cl.id;
                 } else {
+                    if (lb + i != le - 1) {
+                        const uint32_t a = lb + i, b2 = le - 1;
+                        const float ax = sx[a], ay = sy[a], az = sz[a];
+                        sx[a] = sx[b2];
+                        sy[a] = sy[b2];
+                        sz[a] = sz[b2];
+                        sx[b2] = ax;
+                        sy[b2] = ay;
+                        sz[b2] = az;
+                    }
                     const int64_t gl = hb + lc.id, gr = hb + rc.id;
                     heap[gl] = lc;
                     heap[gr] = rc;
@@ -672,8 +863,8 @@ __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32
                     nxt[2 * k + 1].valid = rc.nLeafCells >= 2;
                 }
             }
+            __syncthreads();  // the next batch reuses the position lists and the slots
         }
-        __syncthreads();
         const int32_t nn = 2 * ncur;  // <= kSmallLeaves (checked above)
         for (int32_t k = t; k < nn; k += kBlock) cur[k] = nxt[k];
         __syncthreads();
@@ -692,6 +883,7 @@ __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32
     }
     __syncthreads();  // the next cell reuses the LDS arrays
     }
+    if (t < 64 && s_lp0 >= 0 && s_lpmax[t] > 0) atomicMax(&level_passes[s_lp0 + t], s_lpmax[t]);
 }
 
 // ---------------------------------------------------------------------------
@@ -751,8 +943,12 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
     const unsigned gw = (unsigned)((nCells + 3) / 4);
     for (int r = 0; r < kRounds; ++r) {
         const int tb = timing_mark(c);
-        hipLaunchKernelGGL(k_b_hist, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
-                           c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(), le, chunk_hist);
+        if (le)
+            hipLaunchKernelGGL(k_b_hist<1>, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
+                               c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(), chunk_hist);
+        else
+            hipLaunchKernelGGL(k_b_hist<0>, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
+                               c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(), chunk_hist);
         ORB_HIP(hipGetLastError());
         ORB_TRY(timing_pair(c, tb, kTimeCount));
         hipLaunchKernelGGL(k_b_hist_reduce, dim3(gr), dim3(kBlock), 0, c.stream, c.recs.p, nCells,

@@ -136,6 +136,19 @@ __host__ __device__ inline int32_t cell_cut_axis(const orb_cell &c)
     return maxD;
 }
 
+// a[k] for k in [0, 2] by selects (no indexed private memory on the GPU)
+__host__ __device__ inline float get3(const float a[3], int k)
+{
+    return k == 0 ? a[0] : (k == 1 ? a[1] : a[2]);
+}
+
+__host__ __device__ inline void set3(float a[3], int k, float v)
+{
+    a[0] = k == 0 ? v : a[0];
+    a[1] = k == 1 ? v : a[1];
+    a[2] = k == 2 ? v : a[2];
+}
+
 // cut() + setCutAxis() + setCutMargin() on both children, in the order of
 // orbit.cpp:235-243.  Returns false where the reference would index with
 // cutAxis == -1 (a child without positive extent).
@@ -146,18 +159,18 @@ __host__ __device__ inline bool cell_split(const orb_cell &p, orb_cell &l, orb_c
     int32_t nRight = p.nLeafCells - nLeft;
     float cut = cell_get_cut(p);
     cell_make(l, left_child_id(p.id), nLeft, p.lower, p.upper);
-    l.upper[p.cutAxis] = cut;
+    set3(l.upper, p.cutAxis, cut);
     l.prevCutAxis = p.cutAxis;
     cell_make(r, right_child_id(p.id), nRight, p.lower, p.upper);
-    r.lower[p.cutAxis] = cut;
+    set3(r.lower, p.cutAxis, cut);
     r.prevCutAxis = p.cutAxis;
     r.cutAxis = cell_cut_axis(r);
     l.cutAxis = cell_cut_axis(l);
     if (r.cutAxis < 0 || l.cutAxis < 0) return false;
-    r.cutMarginLeft = r.lower[r.cutAxis];
-    r.cutMarginRight = r.upper[r.cutAxis];
-    l.cutMarginLeft = l.lower[l.cutAxis];
-    l.cutMarginRight = l.upper[l.cutAxis];
+    r.cutMarginLeft = get3(r.lower, r.cutAxis);
+    r.cutMarginRight = get3(r.upper, r.cutAxis);
+    l.cutMarginLeft = get3(l.lower, l.cutAxis);
+    l.cutMarginRight = get3(l.upper, l.cutAxis);
     return true;
 }
 

@@ -565,61 +565,65 @@ __device__ inline int load_chunk_lds(float *skey, const float *key, uint32_t b, 
     return n;
 }
 
+// Each wave owns a contiguous quarter of the chunk (1024 keys, 16 per lane in
+// registers, one coalesced dword load each).  Pass 1 counts its stoppers with
+// ballots; the four wave counts give each wave its rank offset; pass 2 writes
+// each stopper's position at its rank: lanes of one ballot write consecutive
+// entries, so the stores stay contiguous without LDS staging.
 __device__ inline void k_stopper_positions_chunk(uint32_t q, const CellRec *recs, int32_t nCells,
-                                                              const uint32_t *first,
-                                                              const uint32_t *chunk_cell,
-                                                              const uint64_t *chunk_pre,
-                                                              const float *x, const float *y,
-                                                              const float *z, uint32_t *lpos,
-                                                              uint32_t *rpos)
+                                                 const uint32_t *first,
+                                                 const uint32_t *chunk_cell,
+                                                 const uint64_t *chunk_pre,
+                                                 const float *x, const float *y,
+                                                 const float *z, uint32_t *lpos,
+                                                 uint32_t *rpos)
 {
-    __shared__ float skey[kLdsKeys];
-    __shared__ uint32_t spos[kChunk];
-    __shared__ uint32_t sm[4];
+    constexpr int kSeg = kChunk / (kBlock / 64);  // 1024 keys per wave
+    constexpr int kIt = kSeg / 64;                 // 16 per lane
+    __shared__ uint32_t s_cnt[kBlock / 64];
     uint32_t cell, b, e;
     if (!chunk_range(q, first, nCells, chunk_cell, recs, cell, b, e)) return;
     const CellRec r = recs[cell];
     const float cut = r.cut;
-    const int n = load_chunk_lds(skey, axis_ptr(x, y, z, r.axis), b, e);
-
-    const int k0 = threadIdx.x * kPerThread;
-    uint32_t myl = 0, myr = 0;
+    const float *key = axis_ptr(x, y, z, r.axis);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t sb = b + (uint32_t)(w * kSeg);
+    float v[kIt];
 #pragma unroll
-    for (int j = 0; j < kPerThread; ++j) {
-        int k = k0 + j;
-        if (k < n) {
-            float v = skey[lds_idx(k)];
-            myl += v >= cut;
-            myr += v <= cut;
-        }
+    for (int j = 0; j < kIt; ++j) {
+        const uint32_t p = sb + (uint32_t)(64 * j + lane);
+        v[j] = p < e ? key[p] : 0.0f;
     }
-    uint32_t total;
-    const uint32_t pre = block_exclusive_scan(myl | (myr << 16), sm, &total);
-    const uint32_t nl = total & 0xffffu, nr = total >> 16;
+    uint32_t nl = 0, nr = 0;
+#pragma unroll
+    for (int j = 0; j < kIt; ++j) {
+        const bool in = sb + (uint32_t)(64 * j + lane) < e;
+        nl += (uint32_t)__popcll(__ballot(in && v[j] >= cut));
+        nr += (uint32_t)__popcll(__ballot(in && v[j] <= cut));
+    }
+    if (lane == 0) s_cnt[w] = nl | (nr << 16);  // each <= 1024
+    __syncthreads();
+    uint32_t ol = 0, orr = 0;
+    for (int u = 0; u < w; ++u) {
+        ol += s_cnt[u] & 0xffffu;
+        orr += s_cnt[u] >> 16;
+    }
     const uint64_t base = chunk_pre[q] - chunk_pre[first[cell]];
-    const uint32_t off = b - r.begin;  // cell-local position of the chunk start
-
-    // L-stoppers: local ranks into LDS, then one contiguous run to lpos
-    uint32_t rl = pre & 0xffffu;
+    uint32_t *dl = lpos + r.begin + (uint32_t)(base & 0xffffffffull) + ol;
+    uint32_t *dr = rpos + r.begin + (uint32_t)(base >> 32) + orr;
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t off = sb - r.begin;  // cell-local position of this wave's first key
 #pragma unroll
-    for (int j = 0; j < kPerThread; ++j) {
-        int k = k0 + j;
-        if (k < n && skey[lds_idx(k)] >= cut) spos[rl++] = off + (uint32_t)k;
+    for (int j = 0; j < kIt; ++j) {
+        const uint32_t k = (uint32_t)(64 * j + lane);
+        const bool in = sb + k < e;
+        const bool isL = in && v[j] >= cut, isR = in && v[j] <= cut;
+        const uint64_t mL = __ballot(isL), mR = __ballot(isR);
+        if (isL) dl[__popcll(mL & below)] = off + k;
+        if (isR) dr[__popcll(mR & below)] = off + k;
+        dl += __popcll(mL);
+        dr += __popcll(mR);
     }
-    __syncthreads();
-    uint32_t *dl = lpos + r.begin + (uint32_t)(base & 0xffffffffull);
-    for (uint32_t k = threadIdx.x; k < nl; k += kBlock) dl[k] = spos[k];
-    __syncthreads();
-    // R-stoppers (left-to-right order; R_t is entry TR-1-t)
-    uint32_t rr = pre >> 16;
-#pragma unroll
-    for (int j = 0; j < kPerThread; ++j) {
-        int k = k0 + j;
-        if (k < n && skey[lds_idx(k)] <= cut) spos[rr++] = off + (uint32_t)k;
-    }
-    __syncthreads();
-    uint32_t *dr = rpos + r.begin + (uint32_t)(base >> 32);
-    for (uint32_t k = threadIdx.x; k < nr; k += kBlock) dr[k] = spos[k];
 }
 
 __global__ __launch_bounds__(kBlock) void k_stopper_positions(const CellRec *recs, int32_t nCells,
@@ -714,12 +718,15 @@ __global__ __launch_bounds__(kBlock) void k_hoare_split(const CellRec *recs, int
     ranges[2 * gr + 1] = r.end;
 }
 
+// Pairs in batches of kSwapBatch per thread: all index loads of a batch, then
+// all six-array gathers, then the stores, so each thread keeps several HBM
+// round trips in flight instead of one dependent chain per pair.
+constexpr int kSwapBatch = 8;
+
 __device__ inline void k_hoare_swap_chunk(uint32_t q, const CellRec *recs, int32_t nCells,
-                                                       const uint32_t *first,
-                                                       const uint32_t *chunk_cell,
-                                                       const SplitRec *splits,
-                                                       const uint32_t *lpos, const uint32_t *rpos,
-                                                       float *x, float *y, float *z)
+                                          const uint32_t *first, const uint32_t *chunk_cell,
+                                          const SplitRec *splits, const uint32_t *lpos,
+                                          const uint32_t *rpos, float *x, float *y, float *z)
 {
     uint32_t cell, b, e;
     if (!chunk_range(q, first, nCells, chunk_cell, recs, cell, b, e)) return;
@@ -731,13 +738,38 @@ __device__ inline void k_hoare_swap_chunk(uint32_t q, const CellRec *recs, int32
     if (t0 >= t1) return;
     const uint32_t *L = lpos + r.begin;
     const uint32_t *R = rpos + r.begin;
-    for (uint32_t t = t0 + threadIdx.x; t < t1; t += kBlock) {
-        uint32_t pa = r.begin + L[t];
-        uint32_t pb = r.begin + R[s.TR - 1 - t];
-        float ax = x[pa], ay = y[pa], az = z[pa];
-        float bx = x[pb], by = y[pb], bz = z[pb];
-        x[pa] = bx; y[pa] = by; z[pa] = bz;
-        x[pb] = ax; y[pb] = ay; z[pb] = az;
+    for (uint32_t t = t0 + threadIdx.x; t < t1; t += kBlock * kSwapBatch) {
+        uint32_t pa[kSwapBatch], pb[kSwapBatch];
+#pragma unroll
+        for (int u = 0; u < kSwapBatch; ++u) {
+            const uint32_t tt = t + (uint32_t)(u * kBlock);
+            pa[u] = tt < t1 ? r.begin + L[tt] : 0u;
+            pb[u] = tt < t1 ? r.begin + R[s.TR - 1 - tt] : 0u;
+        }
+        float ax[kSwapBatch], ay[kSwapBatch], az[kSwapBatch];
+        float bx[kSwapBatch], by[kSwapBatch], bz[kSwapBatch];
+#pragma unroll
+        for (int u = 0; u < kSwapBatch; ++u) {
+            if (t + (uint32_t)(u * kBlock) < t1) {
+                ax[u] = x[pa[u]];
+                ay[u] = y[pa[u]];
+                az[u] = z[pa[u]];
+                bx[u] = x[pb[u]];
+                by[u] = y[pb[u]];
+                bz[u] = z[pb[u]];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kSwapBatch; ++u) {
+            if (t + (uint32_t)(u * kBlock) < t1) {
+                x[pa[u]] = bx[u];
+                y[pa[u]] = by[u];
+                z[pa[u]] = bz[u];
+                x[pb[u]] = ax[u];
+                y[pb[u]] = ay[u];
+                z[pb[u]] = az[u];
+            }
+        }
     }
 }
 
