@@ -213,7 +213,7 @@ static int count_service(Ctx &c, const orb_cell *cells, int32_t nCells, uint32_t
 }
 
 static int partition_service(Ctx &c, const orb_cell *cells, int32_t nCells, bool hoare,
-                             int32_t *bad_id)
+                             int32_t *bad_id, bool full_cover = false)
 {
     ORB_TRY(prepare(c, cells, nCells));
     if (nCells == 0) return 0;
@@ -222,7 +222,7 @@ static int partition_service(Ctx &c, const orb_cell *cells, int32_t nCells, bool
     ORB_TRY(clear_status(c));
     if (c.timing) ORB_HIP(hipEventRecord(c.ev[0], c.stream));
     if (hoare) ORB_TRY(launch_hoare(c, nCells));
-    else ORB_TRY(launch_stable_le(c, nCells));
+    else ORB_TRY(launch_stable_le(c, nCells, nullptr, full_cover));
     if (c.timing) ORB_HIP(hipEventRecord(c.ev[1], c.stream));
     int rc = read_status(c, bad_id);
     if (c.timing) {
@@ -253,6 +253,7 @@ static int count_sizes(Ctx &c, const orb_cell *cells, int32_t nCells, uint32_t *
 // ---------------------------------------------------------------------------
 static int start_build(Ctx &c)
 {
+    ORB_TRY(particles_home(c));
     c.results_on_device = false;
     c.heap.assign((size_t)c.cap, orb_cell{});
     std::memset(c.heap.data(), 0, sizeof(orb_cell) * c.heap.size());
@@ -340,14 +341,20 @@ static int build_plain(Ctx &c, int32_t levels_mode, orb_build_info *info)
             c.exists[(size_t)rc.id] = 1;
         }
 
+        // a level whose cells hold every particle lets the stable partition
+        // flip buffers instead of copying back
+        int64_t covered = 0;
+        for (int32_t i = 0; i < nCells; ++i) covered += counts[i];
         int32_t bad_id = -1;
-        int rc = partition_service(c, cells.data(), nCells, c.part == ORB_PART_HOARE, &bad_id);
+        int rc = partition_service(c, cells.data(), nCells, c.part == ORB_PART_HOARE, &bad_id,
+                                   covered == c.n);
         if (rc < 0) {
             info->error_cell = bad_id;
             return rc;
         }
         ORB_TRY(timing_pair(c, tlevel, kTimeLevel + l));
     }
+    ORB_TRY(particles_home(c));
     ORB_HIP(hipStreamSynchronize(c.stream));
     return timing_collect(c);
 }
@@ -457,6 +464,9 @@ int orb_finalize(orb_ctx *ctx)
     c->axisT.release();
     c->tmp.release();
     c->saved.release();
+    c->pid.release();
+    c->pid_tmp.release();
+    c->pid_saved.release();
     c->ranges.release();
     c->dcells.release();
     c->recs.release();
@@ -504,6 +514,8 @@ int orb_set_particles(orb_ctx *ctx, const float *x, const float *y, const float 
 {
     Ctx *c = C(ctx);
     if (!c || n != c->n || (n > 0 && (!x || !y || !z))) return ORB_ERR_ARG;
+    c->in_tmp = false;
+    if (c->track_ids) ORB_TRY(launch_iota(c->stream, c->pid.p, (size_t)c->stride));
     if (n > 0) {
         ORB_HIP(hipMemcpyAsync(c->x(), x, sizeof(float) * (size_t)n, hipMemcpyHostToDevice, c->stream));
         ORB_HIP(hipMemcpyAsync(c->y(), y, sizeof(float) * (size_t)n, hipMemcpyHostToDevice, c->stream));
@@ -536,9 +548,17 @@ int orb_save_particles(orb_ctx *ctx)
 {
     Ctx *c = C(ctx);
     if (!c) return ORB_ERR_ARG;
+    ORB_TRY(particles_home(*c));
     ORB_TRY(c->saved.alloc(3 * (size_t)c->stride));
     ORB_HIP(hipMemcpyAsync(c->saved.p, c->pos.p, 3 * sizeof(float) * (size_t)c->stride,
                            hipMemcpyDeviceToDevice, c->stream));
+    if (c->track_ids) {
+        ORB_TRY(c->pid_saved.alloc((size_t)c->stride));
+        ORB_HIP(hipMemcpyAsync(c->pid_saved.p, c->pid.p, sizeof(uint32_t) * (size_t)c->stride,
+                               hipMemcpyDeviceToDevice, c->stream));
+    } else {
+        c->pid_saved.release();
+    }
     ORB_HIP(hipStreamSynchronize(c->stream));
     return ORB_OK;
 }
@@ -548,9 +568,50 @@ int orb_restore_particles(orb_ctx *ctx)
     Ctx *c = C(ctx);
     if (!c) return ORB_ERR_ARG;
     if (!c->saved.p) return ORB_ERR_STATE;
+    c->in_tmp = false;
     ORB_HIP(hipMemcpyAsync(c->pos.p, c->saved.p, 3 * sizeof(float) * (size_t)c->stride,
                            hipMemcpyDeviceToDevice, c->stream));
+    if (c->track_ids) {
+        // ids saved with the particles, else the snapshot is the input order
+        if (c->pid_saved.p)
+            ORB_HIP(hipMemcpyAsync(c->pid.p, c->pid_saved.p, sizeof(uint32_t) * (size_t)c->stride,
+                                   hipMemcpyDeviceToDevice, c->stream));
+        else
+            ORB_TRY(launch_iota(c->stream, c->pid.p, (size_t)c->stride));
+    }
     return reset_ranges(*c);
+}
+
+int orb_enable_ids(orb_ctx *ctx, bool on)
+{
+    Ctx *c = C(ctx);
+    if (!c) return ORB_ERR_ARG;
+    if (on == c->track_ids) return ORB_OK;
+    ORB_TRY(particles_home(*c));
+    if (on) {
+        ORB_TRY(c->pid.alloc((size_t)c->stride));
+        ORB_TRY(launch_iota(c->stream, c->pid.p, (size_t)c->stride));
+        if (c->part == ORB_PART_STABLE_LE) ORB_TRY(c->pid_tmp.alloc((size_t)c->stride));
+    } else {
+        c->pid.release();
+        c->pid_tmp.release();
+        c->pid_saved.release();
+    }
+    c->track_ids = on;
+    ORB_HIP(hipStreamSynchronize(c->stream));
+    return ORB_OK;
+}
+
+int orb_get_ids(orb_ctx *ctx, uint32_t *ids, int64_t n)
+{
+    Ctx *c = C(ctx);
+    if (!c || n != c->n || (n > 0 && !ids)) return ORB_ERR_ARG;
+    if (!c->track_ids) return ORB_ERR_STATE;
+    if (n > 0)
+        ORB_HIP(hipMemcpyAsync(ids, c->ids(), sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToHost,
+                               c->stream));
+    ORB_HIP(hipStreamSynchronize(c->stream));
+    return ORB_OK;
 }
 
 int orb_copy_particles(orb_ctx *ctx)

@@ -40,6 +40,7 @@ struct BatchedState {
     int32_t D = 0;
     bool complete = true;              // every level lists all its splittable cells
     std::vector<int32_t> level_n;      // [D+1] cells per level (index l)
+    std::vector<uint8_t> level_cover;  // [D+1] the level's cells hold every particle
     std::vector<int64_t> level_off;    // [D+1] offset into level_ids
     DevBuf<LevelEntry> level_ids;      // all levels' cells, concatenated
     DevBuf<orb_cell> heap;             // [cap]
@@ -53,7 +54,7 @@ struct BatchedState {
     // captured build: valid for one (levels mode, comparison, partition, trace)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    int32_t gkey[4] = {-1, -1, -1, -1};
+    int32_t gkey[5] = {-1, -1, -1, -1, -1};
     // cell_hist rows are added to by k_b_hist_reduce and consumed and
     // re-zeroed by k_b_replay in the same round, so the buffer is all zero
     // between builds; it is cleared after allocation and after a failure
@@ -113,9 +114,15 @@ static int make_schedule(Ctx &c, BatchedState &b, int32_t levels_mode, bool *com
     std::vector<LevelEntry> all;
     b.level_n.assign((size_t)b.D + 2, 0);
     b.level_off.assign((size_t)b.D + 2, 0);
+    b.level_cover.assign((size_t)b.D + 2, 0);
     for (int32_t l = 1; l < (int32_t)levels.size(); ++l) {
         b.level_off[(size_t)l] = (int64_t)all.size();
         b.level_n[(size_t)l] = (int32_t)levels[(size_t)l].size();
+        // every particle sits in a cell of depth l-1 while each earlier level
+        // split all cells of its depth
+        const int32_t expect = l == 1 ? 1 : 2 * b.level_n[(size_t)l - 1];
+        b.level_cover[(size_t)l] = (l == 1 || b.level_cover[(size_t)l - 1]) &&
+                                   b.level_n[(size_t)l] == expect;
         for (int32_t id : levels[(size_t)l])
             all.push_back(LevelEntry{id, 0, (int32_t)c.cap, l, 0, lmax, *complete ? 1 : 0, 0});
     }
@@ -140,11 +147,7 @@ __global__ void k_b_init(orb_cell *heap, uint8_t *exists, uint8_t *sub, int64_t 
     for (int64_t k = i; k < cap; k += stride) {
         ranges[2 * k] = 0u;
         ranges[2 * k + 1] = k == 0 ? n : 0u;
-        orb_cell z;
-        uint32_t *w = reinterpret_cast<uint32_t *>(&z);
-#pragma unroll
-        for (int j = 0; j < (int)(sizeof(orb_cell) / 4); ++j) w[j] = 0u;
-        heap[k] = k == 0 ? root : z;
+        store_root_or_zero(&heap[k], root, k == 0);
         exists[k] = k == 0 ? 1 : 0;
         sub[k] = 0;
     }
@@ -270,10 +273,13 @@ __device__ inline void k_b_hist_chunk(uint32_t q, const CellRec *recs, const uin
     // keys below lo fall in bin 0 and keys above hi in bin 255 for both
     // comparisons: count them in registers, bin the rest
     uint32_t below = 0, above = 0;
-    auto put = [&](float v) {
-        if (v < lo) below++;
-        else if (v > hi) above++;
-        else atomicAdd(&h[bin_of<LE>(cand, v, lo, scale)], 1u);
+    // branch-free counters (an if/else chain over counters can make the
+    // compiler pick a counter by address, which puts them in scratch memory)
+    auto put = [&](float v) __attribute__((always_inline)) {
+        const bool bl = v < lo, ab = v > hi;
+        below += bl ? 1u : 0u;
+        above += ab ? 1u : 0u;
+        if (!(bl || ab)) atomicAdd(&h[bin_of<LE>(cand, v, lo, scale)], 1u);
     };
     uint32_t ab = (b + 3u) & ~3u;
     if (ab > e) ab = e;
@@ -573,15 +579,22 @@ __device__ inline uint32_t wave_count_below(const float *key, uint32_t pb, uint3
 // (partition.cpp:28-60) is the stopper closed form: stopper lists by ballot
 // scans, the crossing K by a 64-way search, K swaps, the final swap.  Every
 // barrier is reached by all waves the same number of times.
+// IDS: particle ids move with the particles; a 16-bit local index rides along
+// in LDS and the ids are permuted once at the write-back.
+template <bool IDS>
 __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32_t nCells,
                                                       orb_cell *heap, uint8_t *exists,
                                                       uint8_t *sub, uint32_t *ranges, float *x,
                                                       float *y, float *z, int le_cmp,
                                                       uint32_t *trace, int32_t *level_passes,
-                                                      int32_t *status)
+                                                      int32_t *status, uint32_t *pid)
 {
     __shared__ float sx[kSmall], sy[kSmall], sz[kSmall];
-    __shared__ uint16_t slpos[kSmall], srpos[kSmall];
+    // the two stopper lists (16-bit), reused as the cell's 32-bit ids at the end
+    __shared__ uint32_t spos_raw[kSmall];
+    __shared__ uint16_t sidx[IDS ? kSmall : 1];
+    uint16_t *const slpos = reinterpret_cast<uint16_t *>(spos_raw);
+    uint16_t *const srpos = slpos + kSmall;
     __shared__ SubCell cur[kSmallLeaves], nxt[kSmallLeaves];
     __shared__ uint32_t sCnt[2][4];
     __shared__ int32_t sAct[2][4];
@@ -612,8 +625,10 @@ __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32
     const int64_t hb = rr.hb;
     const int32_t level = rr.lp & 63, lmax = rr.lmax;
     int32_t *st = status + rr.st;
-    orb_cell root;
-    if (t == 0) root = heap[hb + rr.id];
+    // the root cell's 13 words, one per lane of wave 0 (no struct copy)
+    constexpr int kCellWords = (int)(sizeof(orb_cell) / 4);
+    uint32_t root_w = 0u;
+    if (t < kCellWords) root_w = reinterpret_cast<const uint32_t *>(heap + hb + rr.id)[t];
     {
         // every load of the cell in flight before the first LDS store: one
         // memory latency per cell instead of one per 256 keys
@@ -635,11 +650,12 @@ __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32
                 sx[k] = vx[u];
                 sy[k] = vy[u];
                 sz[k] = vz[u];
+                if (IDS) sidx[k] = (uint16_t)k;
             }
         }
     }
+    if (t < kCellWords) reinterpret_cast<uint32_t *>(&cur[0].cell)[t] = root_w;
     if (t == 0) {
-        cur[0].cell = root;
         cur[0].lb = 0;
         cur[0].le = n;
         cur[0].valid = 1;
@@ -811,6 +827,11 @@ __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32
                     sx[b2] = ax;
                     sy[b2] = ay;
                     sz[b2] = az;
+                    if (IDS) {
+                        const uint16_t ia = sidx[a];
+                        sidx[a] = sidx[b2];
+                        sidx[b2] = ia;
+                    }
                 }
             }
             __syncthreads();
@@ -841,6 +862,11 @@ __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32
                         sx[b2] = ax;
                         sy[b2] = ay;
                         sz[b2] = az;
+                        if (IDS) {
+                            const uint16_t ia = sidx[a];
+                            sidx[a] = sidx[b2];
+                            sidx[b2] = ia;
+                        }
                     }
                     const int64_t gl = hb + lc.id, gr = hb + rc.id;
                     heap[gl] = lc;
@@ -881,6 +907,12 @@ __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32
         y[begin + k] = sy[k];
         z[begin + k] = sz[k];
     }
+    if (IDS) {
+        // the stopper lists are free: stage the cell's ids, then permute them
+        for (uint32_t k = (uint32_t)t; k < n; k += kBlock) spos_raw[k] = pid[begin + k];
+        __syncthreads();
+        for (uint32_t k = (uint32_t)t; k < n; k += kBlock) pid[begin + k] = spos_raw[sidx[k]];
+    }
     __syncthreads();  // the next cell reuses the LDS arrays
     }
     if (t < 64 && s_lp0 >= 0 && s_lpmax[t] > 0) atomicMax(&level_passes[s_lp0 + t], s_lpmax[t]);
@@ -917,12 +949,13 @@ static void drop_graph(BatchedState &b)
     if (b.graph) (void)hipGraphDestroy(b.graph);
     b.exec = nullptr;
     b.graph = nullptr;
-    b.gkey[0] = b.gkey[1] = b.gkey[2] = b.gkey[3] = -1;
+    for (auto &k : b.gkey) k = -1;
 }
 
 int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *heap,
                   uint8_t *exists, uint8_t *sub, uint16_t *chunk_hist, uint32_t *cell_hist,
-                  uint32_t *trace, int32_t *level_passes, int32_t *status, bool any_subtree)
+                  uint32_t *trace, int32_t *level_passes, int32_t *status, bool any_subtree,
+                  bool full_cover)
 {
     const bool use_subtree = any_subtree && c.part == ORB_PART_HOARE;
     const int le = c.cmp == ORB_CMP_LE ? 1 : 0;
@@ -933,9 +966,14 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
     ORB_TRY(build_chunk_table(c, nCells));
     if (use_subtree) {
         const unsigned gs = (unsigned)(nCells < 4096 ? nCells : 4096);
-        hipLaunchKernelGGL(k_b_subtree, dim3(gs), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
-                           heap, exists, sub, c.ranges.p, c.x(), c.y(), c.z(), le, trace,
-                           level_passes, status);
+        if (c.track_ids)
+            hipLaunchKernelGGL(k_b_subtree<true>, dim3(gs), dim3(kBlock), 0, c.stream, c.recs.p,
+                               nCells, heap, exists, sub, c.ranges.p, c.x(), c.y(), c.z(), le, trace,
+                               level_passes, status, c.ids());
+        else
+            hipLaunchKernelGGL(k_b_subtree<false>, dim3(gs), dim3(kBlock), 0, c.stream, c.recs.p,
+                               nCells, heap, exists, sub, c.ranges.p, c.x(), c.y(), c.z(), le, trace,
+                               level_passes, status, nullptr);
         ORB_HIP(hipGetLastError());
     }
     const unsigned gq = chunk_grid(c);
@@ -963,7 +1001,7 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
     ORB_HIP(hipGetLastError());
     const int tp = timing_mark(c);
     if (c.part == ORB_PART_HOARE) ORB_TRY(launch_hoare(c, nCells, status));
-    else ORB_TRY(launch_stable_le(c, nCells, status));
+    else ORB_TRY(launch_stable_le(c, nCells, status, full_cover));
     ORB_TRY(timing_pair(c, tp, kTimePartition));
     return ORB_OK;
 }
@@ -987,10 +1025,11 @@ static int enqueue_build(Ctx &c, int32_t levels_mode)
         const int tlevel = timing_mark(c);
         ORB_TRY(enqueue_level(c, b.level_ids.p + b.level_off[(size_t)l], nCells, b.heap.p,
                               b.exists.p, b.sub.p, b.chunk_hist.p, b.cell_hist.p, trace,
-                              b.level_passes.p, c.status.p, b.complete));
+                              b.level_passes.p, c.status.p, b.complete,
+                              b.level_cover[(size_t)l] != 0));
         ORB_TRY(timing_pair(c, tlevel, kTimeLevel + l));
     }
-    return ORB_OK;
+    return particles_home(c);  // in-stream: the capture ends where it began
 }
 
 int build_batched(Ctx &c, int32_t levels_mode, orb_build_info *info)
@@ -1005,11 +1044,12 @@ int build_batched(Ctx &c, int32_t levels_mode, orb_build_info *info)
         ORB_TRY(launch_zero_u32(c.stream, b.cell_hist.p, (size_t)c.max_level_cells * kBins));
         b.hist_clean = true;
     }
-    const int32_t key[4] = {levels_mode, c.cmp, c.part, c.trace ? 1 : 0};
+    ORB_TRY(particles_home(c));
+    const int32_t key[5] = {levels_mode, c.cmp, c.part, c.trace ? 1 : 0, c.track_ids ? 1 : 0};
     const bool graph = c.use_graph && !c.timing;
     if (graph) {
         bool same = b.exec != nullptr;
-        for (int k = 0; k < 4 && same; ++k) same = b.gkey[k] == key[k];
+        for (int k = 0; k < 5 && same; ++k) same = b.gkey[k] == key[k];
         if (!same) {
             drop_graph(b);
             ORB_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
@@ -1023,7 +1063,7 @@ int build_batched(Ctx &c, int32_t levels_mode, orb_build_info *info)
             ORB_HIP(e);
             b.graph = g;
             ORB_HIP(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
-            for (int k = 0; k < 4; ++k) b.gkey[k] = key[k];
+            for (int k = 0; k < 5; ++k) b.gkey[k] = key[k];
         }
         ORB_HIP(hipGraphLaunch(b.exec, c.stream));
     } else {

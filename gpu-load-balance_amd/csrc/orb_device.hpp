@@ -56,6 +56,16 @@ __device__ inline T block_exclusive_scan(T v, T *smem4, T *total)
     return pre + inc - v;
 }
 
+// Stores the root cell at k == 0 and a zero cell elsewhere, word by word
+// (a select between two whole structs would go through scratch memory).
+__device__ inline void store_root_or_zero(orb_cell *dst, const orb_cell &root, bool is_root)
+{
+    uint32_t *w = reinterpret_cast<uint32_t *>(dst);
+    const uint32_t *rw = reinterpret_cast<const uint32_t *>(&root);
+#pragma unroll
+    for (int j = 0; j < (int)(sizeof(orb_cell) / 4); ++j) w[j] = is_root ? rw[j] : 0u;
+}
+
 __device__ inline const float *axis_ptr(const float *x, const float *y, const float *z, int a)
 {
     return a == 0 ? x : (a == 1 ? y : z);

@@ -333,11 +333,15 @@ struct Ctx {
     int32_t part = ORB_PART_HOARE;
     int64_t cap = 0;           // heap capacity (ids)
 
-    // particles: one allocation of 3n floats, SoA
-    DevBuf<float> pos;         // x = pos.p, y = pos.p + n, z = pos.p + 2n
+    // particles: one allocation of 3 x stride floats (+64 floats of slack), SoA
+    DevBuf<float> pos;         // x = pos.p, y = pos.p + stride, z = pos.p + 2 stride
     DevBuf<float> axisT;       // particlesT (orb_make_axis)
-    DevBuf<float> tmp;         // 3n floats: out-of-place partition target
-    DevBuf<float> saved;       // 3n floats: orb_save_particles snapshot
+    DevBuf<float> tmp;         // same shape: the stable partition's other buffer
+    DevBuf<float> saved;       // orb_save_particles snapshot
+    bool in_tmp = false;       // the stable partition left the particles in tmp
+    // optional particle ids: the input index of the particle at each position
+    bool track_ids = false;
+    DevBuf<uint32_t> pid, pid_tmp, pid_saved;
     DevBuf<uint32_t> ranges;   // [cap][2] cellToRangeMap
 
     // level tables (sized for the widest level)
@@ -387,9 +391,14 @@ struct Ctx {
     std::vector<TPair> tpairs;
     BatchedState *batched = nullptr;
 
-    float *x() const { return pos.p; }
-    float *y() const { return pos.p + stride; }
-    float *z() const { return pos.p + 2 * stride; }
+    // the buffer that holds the particles now, and the other one
+    float *cur() const { return in_tmp ? tmp.p : pos.p; }
+    float *other() const { return in_tmp ? pos.p : tmp.p; }
+    float *x() const { return cur(); }
+    float *y() const { return cur() + stride; }
+    float *z() const { return cur() + 2 * stride; }
+    uint32_t *ids() const { return track_ids ? (in_tmp ? pid_tmp.p : pid.p) : nullptr; }
+    uint32_t *other_ids() const { return track_ids ? (in_tmp ? pid.p : pid_tmp.p) : nullptr; }
 };
 
 // timing helpers (orb_api.hip): marks are -1 when timing is off
@@ -413,7 +422,11 @@ int launch_count(Ctx &c, int32_t nCells, bool lt_skip_found, const float *axis_o
 int launch_hoare(Ctx &c, int32_t nCells, int32_t *status = nullptr);
 int launch_stopper_counts(Ctx &c, int32_t nCells, bool le_only);
 int invalidate_stamps(Ctx &c, int32_t nCells);
-int launch_stable_le(Ctx &c, int32_t nCells, int32_t *status = nullptr);
+int launch_stable_le(Ctx &c, int32_t nCells, int32_t *status = nullptr, bool full_cover = false);
+int ensure_stable_buffers(Ctx &c);
+int launch_copy_u32(hipStream_t s, const uint32_t *src, uint32_t *dst, size_t n);
+int launch_iota(hipStream_t s, uint32_t *p, size_t n);
+int particles_home(Ctx &c);
 int schedule_levels(int32_t d, int32_t levels_mode, int64_t cap,
                     std::vector<std::vector<int32_t>> &levels, bool *complete);
 int launch_make_axis(Ctx &c, int32_t nCells);
@@ -421,7 +434,8 @@ int launch_cell_sizes(Ctx &c, int32_t nCells, uint32_t *dst);
 // one level step of the batched form over `entries` (device array)
 int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *heap,
                   uint8_t *exists, uint8_t *sub, uint16_t *chunk_hist, uint32_t *cell_hist,
-                  uint32_t *trace, int32_t *level_passes, int32_t *status, bool any_subtree);
+                  uint32_t *trace, int32_t *level_passes, int32_t *status, bool any_subtree,
+                  bool full_cover);
 int preallocate_scratch(Ctx &c);
 int ensure_level_capacity(Ctx &c, int64_t nCells);
 

@@ -167,7 +167,68 @@ int preallocate_scratch(Ctx &c)
     size_t m = (size_t)c.max_chunks + 1;
     size_t np = (m + kScanTile - 1) / kScanTile;
     ORB_TRY(c.scan_partials.alloc(np));
-    if (c.part == ORB_PART_STABLE_LE) ORB_TRY(c.tmp.alloc(3 * (size_t)c.stride));
+    if (c.part == ORB_PART_STABLE_LE) ORB_TRY(ensure_stable_buffers(c));
+    return ORB_OK;
+}
+
+int ensure_stable_buffers(Ctx &c)
+{
+    // the other buffer has the particle buffer's shape, slack included
+    ORB_TRY(c.tmp.alloc(3 * (size_t)c.stride + 64));
+    if (c.track_ids) ORB_TRY(c.pid_tmp.alloc((size_t)c.stride));
+    return ORB_OK;
+}
+
+__global__ void k_copy_u32(const uint32_t *src, uint32_t *dst, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t n4 = n / 4;
+    const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
+    uint4 *d4 = reinterpret_cast<uint4 *>(dst);
+    for (size_t k = i; k < n4; k += stride) d4[k] = s4[k];
+    for (size_t k = 4 * n4 + i; k < n; k += stride) dst[k] = src[k];
+}
+
+// src and dst 16-byte aligned (allocation bases)
+int launch_copy_u32(hipStream_t s, const uint32_t *src, uint32_t *dst, size_t n)
+{
+    if (n == 0) return ORB_OK;
+    size_t g = (n / 4 + kBlock - 1) / kBlock;
+    if (g > 8192) g = 8192;
+    if (g < 1) g = 1;
+    hipLaunchKernelGGL(k_copy_u32, dim3((unsigned)g), dim3(kBlock), 0, s, src, dst, n);
+    ORB_HIP(hipGetLastError());
+    return ORB_OK;
+}
+
+__global__ void k_iota(uint32_t *p, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t k = i; k < n; k += stride) p[k] = (uint32_t)k;
+}
+
+int launch_iota(hipStream_t s, uint32_t *p, size_t n)
+{
+    if (n == 0) return ORB_OK;
+    size_t g = (n + kBlock - 1) / kBlock;
+    if (g > 8192) g = 8192;
+    hipLaunchKernelGGL(k_iota, dim3((unsigned)g), dim3(kBlock), 0, s, p, n);
+    ORB_HIP(hipGetLastError());
+    return ORB_OK;
+}
+
+// Brings the particles (and ids) back to the primary buffer after a stable
+// build that ended in the other one: one copy per build instead of one per
+// level.
+int particles_home(Ctx &c)
+{
+    if (!c.in_tmp) return ORB_OK;
+    ORB_TRY(launch_copy_u32(c.stream, reinterpret_cast<const uint32_t *>(c.tmp.p),
+                            reinterpret_cast<uint32_t *>(c.pos.p), 3 * (size_t)c.stride));
+    if (c.track_ids) ORB_TRY(launch_copy_u32(c.stream, c.pid_tmp.p, c.pid.p, (size_t)c.stride));
+    c.in_tmp = false;
     return ORB_OK;
 }
 
@@ -534,37 +595,6 @@ __global__ __launch_bounds__(kBlock) void k_stopper_counts(const CellRec *recs, 
 }
 
 
-// LDS staging of one chunk: element k at index k + k/16 (odd stride per
-// thread row, no bank conflicts when thread t reads its 16 consecutive keys).
-__device__ inline int lds_idx(int k) { return k + (k >> 4); }
-constexpr int kLdsKeys = kChunk + kChunk / 16;
-
-// Loads keys [b, e) into LDS with 16-byte global loads over the enclosing
-// aligned range (the particle arrays carry 64 floats of slack past the end,
-// so the last vector stays inside the allocation); only [b, e) is stored.
-__device__ inline int load_chunk_lds(float *skey, const float *key, uint32_t b, uint32_t e)
-{
-    const int n = (int)(e - b);
-    const uint32_t ab = b & ~3u;
-    const uint32_t nv = (e - ab + 3u) >> 2;  // <= kChunk/4 + 1
-    const float4 *k4 = reinterpret_cast<const float4 *>(key + ab);
-    const int shift = (int)(b - ab);
-#pragma unroll
-    for (int u = 0; u < kPerThread / 4 + 1; ++u) {
-        uint32_t j = threadIdx.x + (uint32_t)u * kBlock;
-        if (j < nv) {
-            float4 v = k4[j];
-            int k = (int)(4 * j) - shift;
-            if (k >= 0 && k < n) skey[lds_idx(k)] = v.x;
-            if (k + 1 >= 0 && k + 1 < n) skey[lds_idx(k + 1)] = v.y;
-            if (k + 2 >= 0 && k + 2 < n) skey[lds_idx(k + 2)] = v.z;
-            if (k + 3 >= 0 && k + 3 < n) skey[lds_idx(k + 3)] = v.w;
-        }
-    }
-    __syncthreads();
-    return n;
-}
-
 // Each wave owns a contiguous quarter of the chunk (1024 keys, 16 per lane in
 // registers, one coalesced dword load each).  Pass 1 counts its stoppers with
 // ballots; the four wave counts give each wave its rank offset; pass 2 writes
@@ -726,7 +756,8 @@ constexpr int kSwapBatch = 8;
 __device__ inline void k_hoare_swap_chunk(uint32_t q, const CellRec *recs, int32_t nCells,
                                           const uint32_t *first, const uint32_t *chunk_cell,
                                           const SplitRec *splits, const uint32_t *lpos,
-                                          const uint32_t *rpos, float *x, float *y, float *z)
+                                          const uint32_t *rpos, float *x, float *y, float *z,
+                                          uint32_t *pid)
 {
     uint32_t cell, b, e;
     if (!chunk_range(q, first, nCells, chunk_cell, recs, cell, b, e)) return;
@@ -770,6 +801,23 @@ __device__ inline void k_hoare_swap_chunk(uint32_t q, const CellRec *recs, int32
                 z[pb[u]] = az[u];
             }
         }
+        if (pid) {  // uniform
+            uint32_t ia[kSwapBatch], ib[kSwapBatch];
+#pragma unroll
+            for (int u = 0; u < kSwapBatch; ++u) {
+                if (t + (uint32_t)(u * kBlock) < t1) {
+                    ia[u] = pid[pa[u]];
+                    ib[u] = pid[pb[u]];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kSwapBatch; ++u) {
+                if (t + (uint32_t)(u * kBlock) < t1) {
+                    pid[pa[u]] = ib[u];
+                    pid[pb[u]] = ia[u];
+                }
+            }
+        }
     }
 }
 
@@ -778,18 +826,19 @@ __global__ __launch_bounds__(kBlock) void k_hoare_swap(const CellRec *recs, int3
                                                        const uint32_t *chunk_cell,
                                                        const SplitRec *splits,
                                                        const uint32_t *lpos, const uint32_t *rpos,
-                                                       float *x, float *y, float *z)
+                                                       float *x, float *y, float *z,
+                                                       uint32_t *pid)
 {
     const uint32_t total = first[nCells];
     for (uint32_t q = blockIdx.x; q < total; q += gridDim.x) {
-        k_hoare_swap_chunk(q, recs, nCells, first, chunk_cell, splits, lpos, rpos, x, y, z);
+        k_hoare_swap_chunk(q, recs, nCells, first, chunk_cell, splits, lpos, rpos, x, y, z, pid);
         __syncthreads();  // the next chunk reuses this block's LDS
     }
 }
 
 
 __global__ void k_hoare_final(const CellRec *recs, int32_t nCells, const SplitRec *splits,
-                              float *x, float *y, float *z)
+                              float *x, float *y, float *z, uint32_t *pid)
 {
     int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nCells) return;
@@ -802,6 +851,11 @@ __global__ void k_hoare_final(const CellRec *recs, int32_t nCells, const SplitRe
     float ax = x[pa], ay = y[pa], az = z[pa];
     x[pa] = x[pb]; y[pa] = y[pb]; z[pa] = z[pb];
     x[pb] = ax; y[pb] = ay; z[pb] = az;
+    if (pid) {
+        const uint32_t ia = pid[pa];
+        pid[pa] = pid[pb];
+        pid[pb] = ia;
+    }
 }
 
 int launch_stopper_counts(Ctx &c, int32_t nCells, bool le_only)
@@ -831,11 +885,11 @@ int launch_hoare(Ctx &c, int32_t nCells, int32_t *status)
     ORB_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_hoare_swap, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
                        c.cell_first.p, c.chunk_cell.p, c.splits.p, c.lpos.p, c.rpos.p, c.x(), c.y(),
-                       c.z());
+                       c.z(), c.ids());
     ORB_HIP(hipGetLastError());
     unsigned gc = (unsigned)((nCells + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_hoare_final, dim3(gc), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
-                       c.splits.p, c.x(), c.y(), c.z());
+                       c.splits.p, c.x(), c.y(), c.z(), c.ids());
     ORB_HIP(hipGetLastError());
     return ORB_OK;
 }
@@ -843,51 +897,69 @@ int launch_hoare(Ctx &c, int32_t nCells, int32_t *status)
 // ---------------------------------------------------------------------------
 // stable <= partition (ServicePartitionGPU semantics, deterministic order)
 // ---------------------------------------------------------------------------
+// Keys <= cut first, then the rest, each in input order.  Each wave owns a
+// contiguous quarter of the chunk with 16 keys per lane in registers; pass 1
+// counts its keys <= cut with ballots, the four wave counts give the wave
+// offsets, pass 2 writes each particle at its rank: the lanes of one ballot
+// write consecutive slots (mbcnt), so the x, y, z (and id) stores coalesce.
 __device__ inline void k_stable_scatter_chunk(uint32_t q, const CellRec *recs, int32_t nCells,
-                                                           const uint32_t *first,
-                                                           const uint32_t *chunk_cell,
-                                                           const uint64_t *chunk_pre,
-                                                           const float *x, const float *y,
-                                                           const float *z, float *ox, float *oy,
-                                                           float *oz)
+                                              const uint32_t *first, const uint32_t *chunk_cell,
+                                              const uint64_t *chunk_pre, const float *x,
+                                              const float *y, const float *z, const uint32_t *pid,
+                                              float *ox, float *oy, float *oz, uint32_t *opid)
 {
-    __shared__ float skey[kLdsKeys];
-    __shared__ uint32_t sm[4];
+    constexpr int kSeg = kChunk / (kBlock / 64);
+    constexpr int kIt = kSeg / 64;
+    __shared__ uint32_t s_cnt[kBlock / 64];
     uint32_t cell, b, e;
     if (!chunk_range(q, first, nCells, chunk_cell, recs, cell, b, e)) return;
     const CellRec r = recs[cell];
     const float cut = r.cut;
-    const int n = load_chunk_lds(skey, axis_ptr(x, y, z, r.axis), b, e);
-    const int t = threadIdx.x;
-    const int k0 = t * kPerThread;
-    uint32_t myl = 0, myr = 0;
+    const float *key = axis_ptr(x, y, z, r.axis);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t sb = b + (uint32_t)(w * kSeg);
+    float v[kIt];
 #pragma unroll
-    for (int j = 0; j < kPerThread; ++j) {
-        int k = k0 + j;
-        if (k < n) {
-            float v = skey[lds_idx(k)];
-            myl += v <= cut;
-            myr += v > cut;
-        }
+    for (int j = 0; j < kIt; ++j) {
+        const uint32_t p = sb + (uint32_t)(64 * j + lane);
+        v[j] = p < e ? key[p] : 0.0f;
     }
-    uint32_t total;
-    uint32_t pre = block_exclusive_scan(myl | (myr << 16), sm, &total);
-    uint64_t base = chunk_pre[q] - chunk_pre[first[cell]];
-    uint64_t tot = chunk_pre[first[cell + 1]] - chunk_pre[first[cell]];
-    uint32_t nle = (uint32_t)(tot & 0xffffffffull);
-    uint32_t dl = r.begin + (uint32_t)(base & 0xffffffffull) + (pre & 0xffffu);
-    uint32_t dr = r.begin + nle + (uint32_t)(base >> 32) + (pre >> 16);
+    uint32_t nle = 0, nin = 0;
 #pragma unroll
-    for (int j = 0; j < kPerThread; ++j) {
-        int k = k0 + j;
-        if (k < n) {
-            uint32_t p = b + (uint32_t)k;
-            float v = skey[lds_idx(k)];
-            uint32_t d = v <= cut ? dl++ : dr++;
+    for (int j = 0; j < kIt; ++j) {
+        const bool in = sb + (uint32_t)(64 * j + lane) < e;
+        nle += (uint32_t)__popcll(__ballot(in && v[j] <= cut));
+        nin += (uint32_t)__popcll(__ballot(in));
+    }
+    if (lane == 0) s_cnt[w] = nle | ((nin - nle) << 16);  // each <= 1024
+    __syncthreads();
+    uint32_t ol = 0, orr = 0;
+    for (int u = 0; u < w; ++u) {
+        ol += s_cnt[u] & 0xffffu;
+        orr += s_cnt[u] >> 16;
+    }
+    const uint64_t base = chunk_pre[q] - chunk_pre[first[cell]];
+    const uint64_t tot = chunk_pre[first[cell + 1]] - chunk_pre[first[cell]];
+    const uint32_t cell_nle = (uint32_t)(tot & 0xffffffffull);
+    uint32_t dl = r.begin + (uint32_t)(base & 0xffffffffull) + ol;
+    uint32_t dr = r.begin + cell_nle + (uint32_t)(base >> 32) + orr;
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+    for (int j = 0; j < kIt; ++j) {
+        const uint32_t p = sb + (uint32_t)(64 * j + lane);
+        const bool in = p < e;
+        const bool left = in && v[j] <= cut, right = in && !(v[j] <= cut);
+        const uint64_t mL = __ballot(left), mR = __ballot(right);
+        if (in) {
+            const uint32_t d = left ? dl + (uint32_t)__popcll(mL & below)
+                                    : dr + (uint32_t)__popcll(mR & below);
             ox[d] = x[p];
             oy[d] = y[p];
             oz[d] = z[p];
+            if (opid) opid[d] = pid[p];
         }
+        dl += (uint32_t)__popcll(mL);
+        dr += (uint32_t)__popcll(mR);
     }
 }
 
@@ -896,22 +968,25 @@ __global__ __launch_bounds__(kBlock) void k_stable_scatter(const CellRec *recs, 
                                                            const uint32_t *chunk_cell,
                                                            const uint64_t *chunk_pre,
                                                            const float *x, const float *y,
-                                                           const float *z, float *ox, float *oy,
-                                                           float *oz)
+                                                           const float *z, const uint32_t *pid,
+                                                           float *ox, float *oy, float *oz,
+                                                           uint32_t *opid)
 {
     const uint32_t total = first[nCells];
     for (uint32_t q = blockIdx.x; q < total; q += gridDim.x) {
-        k_stable_scatter_chunk(q, recs, nCells, first, chunk_cell, chunk_pre, x, y, z, ox, oy, oz);
+        k_stable_scatter_chunk(q, recs, nCells, first, chunk_cell, chunk_pre, x, y, z, pid, ox,
+                               oy, oz, opid);
         __syncthreads();  // the next chunk reuses this block's LDS
     }
 }
 
-
+// The level's ranges back from the other buffer (a level that does not hold
+// every particle cannot flip buffers).
 __device__ inline void k_copy_back_chunk(uint32_t q, const CellRec *recs, int32_t nCells,
-                                                      const uint32_t *first,
-                                                      const uint32_t *chunk_cell, const float *ox,
-                                                      const float *oy, const float *oz, float *x,
-                                                      float *y, float *z)
+                                         const uint32_t *first, const uint32_t *chunk_cell,
+                                         const float *ox, const float *oy, const float *oz,
+                                         const uint32_t *opid, float *x, float *y, float *z,
+                                         uint32_t *pid)
 {
     uint32_t cell, b, e;
     if (!chunk_range(q, first, nCells, chunk_cell, recs, cell, b, e)) return;
@@ -919,22 +994,21 @@ __device__ inline void k_copy_back_chunk(uint32_t q, const CellRec *recs, int32_
         x[p] = ox[p];
         y[p] = oy[p];
         z[p] = oz[p];
+        if (pid) pid[p] = opid[p];
     }
 }
 
 __global__ __launch_bounds__(kBlock) void k_copy_back(const CellRec *recs, int32_t nCells,
                                                       const uint32_t *first,
                                                       const uint32_t *chunk_cell, const float *ox,
-                                                      const float *oy, const float *oz, float *x,
-                                                      float *y, float *z)
+                                                      const float *oy, const float *oz,
+                                                      const uint32_t *opid, float *x, float *y,
+                                                      float *z, uint32_t *pid)
 {
     const uint32_t total = first[nCells];
-    for (uint32_t q = blockIdx.x; q < total; q += gridDim.x) {
-        k_copy_back_chunk(q, recs, nCells, first, chunk_cell, ox, oy, oz, x, y, z);
-        __syncthreads();  // the next chunk reuses this block's LDS
-    }
+    for (uint32_t q = blockIdx.x; q < total; q += gridDim.x)
+        k_copy_back_chunk(q, recs, nCells, first, chunk_cell, ox, oy, oz, opid, x, y, z, pid);
 }
-
 
 __global__ void k_stable_ranges(const CellRec *recs, int32_t nCells, const uint32_t *first,
                                 const uint64_t *chunk_pre, SplitRec *splits, uint32_t *ranges,
@@ -960,20 +1034,27 @@ __global__ void k_stable_ranges(const CellRec *recs, int32_t nCells, const uint3
     ranges[2 * gr + 1] = r.end;
 }
 
-int launch_stable_le(Ctx &c, int32_t nCells, int32_t *status)
+int launch_stable_le(Ctx &c, int32_t nCells, int32_t *status, bool full_cover)
 {
     if (!status) status = c.status.p;
-    ORB_TRY(c.tmp.alloc(3 * (size_t)c.stride));
+    ORB_TRY(ensure_stable_buffers(c));
     const unsigned gq = chunk_grid(c);
     ORB_TRY(launch_stopper_counts(c, nCells, true));
-    float *ox = c.tmp.p, *oy = c.tmp.p + c.stride, *oz = c.tmp.p + 2 * c.stride;
+    float *ob = c.other();
+    float *ox = ob, *oy = ob + c.stride, *oz = ob + 2 * c.stride;
     hipLaunchKernelGGL(k_stable_scatter, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
-                       c.cell_first.p, c.chunk_cell.p, c.chunk_pre.p, c.x(), c.y(), c.z(), ox, oy,
-                       oz);
+                       c.cell_first.p, c.chunk_cell.p, c.chunk_pre.p, c.x(), c.y(), c.z(), c.ids(),
+                       ox, oy, oz, c.other_ids());
     ORB_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_copy_back, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
-                       c.cell_first.p, c.chunk_cell.p, ox, oy, oz, c.x(), c.y(), c.z());
-    ORB_HIP(hipGetLastError());
+    if (full_cover) {
+        // every particle was written to the other buffer: it becomes current
+        c.in_tmp = !c.in_tmp;
+    } else {
+        hipLaunchKernelGGL(k_copy_back, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
+                           c.cell_first.p, c.chunk_cell.p, ox, oy, oz, c.other_ids(), c.x(), c.y(),
+                           c.z(), c.ids());
+        ORB_HIP(hipGetLastError());
+    }
     unsigned gc = (unsigned)((nCells + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_stable_ranges, dim3(gc), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
                        c.cell_first.p, c.chunk_pre.p, c.splits.p, c.ranges.p, status,

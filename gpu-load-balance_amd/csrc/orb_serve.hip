@@ -195,11 +195,7 @@ __global__ void k_s_init(orb_cell *heap, uint8_t *exists, uint8_t *sub, uint32_t
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t k = i; k < cap; k += stride) {
-        orb_cell z;
-        uint32_t *w = reinterpret_cast<uint32_t *>(&z);
-#pragma unroll
-        for (int j = 0; j < (int)(sizeof(orb_cell) / 4); ++j) w[j] = 0u;
-        heap[hb + k] = k == 0 ? root : z;
+        store_root_or_zero(&heap[hb + k], root, k == 0);
         exists[hb + k] = k == 0 ? 1 : 0;
         sub[hb + k] = 0;
         ranges[2 * (hb + k)] = k == 0 ? pb : 0u;
@@ -442,7 +438,7 @@ static int step(Server &S)
             S.flip ^= 1;
             ORB_TRY(enqueue_level(c, S.entries.p, (int32_t)m, S.heap.p, S.exists.p, S.sub.p,
                                   S.chunk_hist.p, S.cell_hist.p, nullptr, S.level_passes.p,
-                                  S.status.p, any_sub));
+                                  S.status.p, any_sub, false));
             S.stats.steps++;
             S.stats.cells_stepped += m;
             if ((int64_t)stepped.size() > S.stats.max_batch) S.stats.max_batch = (int64_t)stepped.size();

@@ -116,6 +116,11 @@ PROTOTYPES = {
     "orb_get_heap": (C.c_int, [_vp, _vp, _i64]),
     "orb_get_ranges": (C.c_int, [_vp, _vp, _i64]),
     "orb_get_exists": (C.c_int, [_vp, _vp, _i64]),
+    "orb_enable_ids": (C.c_int, [_vp, _b]),
+    "orb_get_ids": (C.c_int, [_vp, _vp, _i64]),
+    "orb_get_leaves": (C.c_int, [_vp, _vp, _vp, _i64, C.POINTER(C.c_int64)]),
+    "orb_get_membership": (C.c_int, [_vp, _vp, _i64, _b]),
+    "orb_get_leaf_bounds": (C.c_int, [_vp, _vp, _vp, _i64]),
     "orb_enable_trace": (C.c_int, [_vp, _b]),
     "orb_get_pass_counts": (C.c_int, [_vp, _vp, _i64]),
     "orb_get_timing": (C.c_int, [_vp, C.POINTER(Timing)]),
@@ -304,6 +309,41 @@ class Context:
         e = np.zeros(self.cap, np.uint8)
         _check(lib().orb_get_exists(self.h, _p(e), self.cap), "orb_get_exists")
         return e
+
+    # ---- tree export
+    def enable_ids(self, on=True):
+        _check(lib().orb_enable_ids(self.h, bool(on)), "orb_enable_ids")
+
+    def ids(self):
+        out = np.zeros(self.n, np.uint32)
+        _check(lib().orb_get_ids(self.h, _p(out), self.n), "orb_get_ids")
+        return out
+
+    def leaves(self):
+        """(heap ids, [count][2] ranges) of the last build's leaves."""
+        cnt = C.c_int64()
+        _check(lib().orb_get_leaves(self.h, None, None, 0, C.byref(cnt)), "orb_get_leaves")
+        ids = np.zeros(cnt.value, np.int32)
+        rg = np.zeros((cnt.value, 2), np.uint32)
+        if cnt.value:
+            _check(lib().orb_get_leaves(self.h, _p(ids), _p(rg), cnt.value, C.byref(cnt)),
+                   "orb_get_leaves")
+        return ids, rg
+
+    def membership(self, by_input_order=False):
+        out = np.zeros(self.n, np.int32)
+        _check(lib().orb_get_membership(self.h, _p(out), self.n, bool(by_input_order)),
+               "orb_get_membership")
+        return out
+
+    def leaf_bounds(self):
+        ids, _ = self.leaves()
+        lo = np.zeros((len(ids), 3), np.float32)
+        hi = np.zeros((len(ids), 3), np.float32)
+        if len(ids):
+            _check(lib().orb_get_leaf_bounds(self.h, _p(lo), _p(hi), len(ids)),
+                   "orb_get_leaf_bounds")
+        return lo, hi
 
     def enable_trace(self, on=True):
         _check(lib().orb_enable_trace(self.h, bool(on)), "orb_enable_trace")

@@ -220,6 +220,32 @@ int orb_get_exists(orb_ctx *ctx, uint8_t *exists, int64_t cap);
 int orb_enable_trace(orb_ctx *ctx, bool on);
 int orb_get_pass_counts(orb_ctx *ctx, uint32_t *counts, int64_t cap);
 
+/* ------------------------------------------------------------------ */
+/* tree export.  No counterpart in the reference: it never outputs the   */
+/* tree (src/orbit.cpp:284-286).  A leaf is an existing cell whose        */
+/* children do not exist.                                               */
+/* ------------------------------------------------------------------ */
+
+/* Particle ids: when on, an id array (the input index of each particle)
+ * moves with the particles through every partition; it restarts as the
+ * identity at orb_set_particles and when turned on, and is saved and
+ * restored with the particles.  Off by default (it adds 4 B per moved
+ * particle).                                                            */
+int orb_enable_ids(orb_ctx *ctx, bool on);
+int orb_get_ids(orb_ctx *ctx, uint32_t *ids, int64_t n);
+/* Leaves of the last build in heap-id order: their ids and [begin, end)
+ * ranges ([cap][2]).  *count gets the number of leaves; ORB_ERR_ARG if
+ * cap is smaller (ids/ranges may be NULL to ask for the count).          */
+int orb_get_leaves(orb_ctx *ctx, int32_t *ids, uint32_t *ranges, int64_t cap, int64_t *count);
+/* Leaf heap id of every particle (-1 outside every leaf), indexed by its
+ * current position, or by its input index when by_input_order (needs
+ * orb_enable_ids before the build).                                     */
+int orb_get_membership(orb_ctx *ctx, int32_t *leaf_of, int64_t n, bool by_input_order);
+/* Tight bounding box of each leaf's particles ([count][3] each, leaves in
+ * orb_get_leaves order): segmented min/max on the device.  An empty leaf
+ * gets lower = +inf, upper = -inf.                                      */
+int orb_get_leaf_bounds(orb_ctx *ctx, float *lower, float *upper, int64_t cap);
+
 /* Batched form: capture the whole build into a hipGraph on first use and
  * replay it afterwards (default on; a build with timing enabled runs
  * eagerly).  The capture is redone when the level mode, comparison,

@@ -204,17 +204,35 @@ int32_t orc_difference(uint32_t count_left, uint32_t count, int32_t nLeafCells)
 /* partitions                                                               */
 /* ------------------------------------------------------------------------ */
 
-static void swap_rows(float *x, float *y, float *z, int64_t a, int64_t b)
+static void swap_rows(float *x, float *y, float *z, uint32_t *ids, int64_t a, int64_t b)
 {
-    /* partition.cpp:10-16 */
+    /* partition.cpp:10-16; ids (optional) move with the row */
     float t;
     t = x[a]; x[a] = x[b]; x[b] = t;
     t = y[a]; y[a] = y[b]; y[b] = t;
     t = z[a]; z[a] = z[b]; z[b] = t;
+    if (ids) { uint32_t u = ids[a]; ids[a] = ids[b]; ids[b] = u; }
 }
+
+static int hoare_impl(float *x, float *y, float *z, uint32_t *ids, int64_t begin, int64_t end,
+                      int axis, float cut, int64_t *split);
+static int stable_impl(float *x, float *y, float *z, uint32_t *ids, int64_t begin, int64_t end,
+                       int axis, float cut, int64_t *split);
 
 int orc_hoare(float *x, float *y, float *z, int64_t begin, int64_t end, int axis,
               float cut, int64_t *split)
+{
+    return hoare_impl(x, y, z, NULL, begin, end, axis, cut, split);
+}
+
+int orc_stable_le(float *x, float *y, float *z, int64_t begin, int64_t end, int axis,
+                  float cut, int64_t *split)
+{
+    return stable_impl(x, y, z, NULL, begin, end, axis, cut, split);
+}
+
+static int hoare_impl(float *x, float *y, float *z, uint32_t *ids, int64_t begin, int64_t end,
+                      int axis, float cut, int64_t *split)
 {
     /* partition.cpp:28-60.  The reference's bounds tests come after the key
      * read (`p(i) < cut && i <= end`, `p(j) > cut && j >= begin`); they are
@@ -232,9 +250,9 @@ int orc_hoare(float *x, float *y, float *z, int64_t begin, int64_t end, int axis
         do { i++; } while (i < end && k[i] < cut);
         do { j--; } while (j >= begin && k[j] > cut);
         if (i >= j) break;
-        swap_rows(x, y, z, i, j);
+        swap_rows(x, y, z, ids, i, j);
     }
-    swap_rows(x, y, z, i, end - 1);
+    swap_rows(x, y, z, ids, i, end - 1);
     *split = i;
     return ORC_OK;
 }
@@ -279,8 +297,8 @@ int orc_hoare_closed_form(const float *key, int64_t n, float cut, int64_t *K_out
     return ORC_OK;
 }
 
-int orc_stable_le(float *x, float *y, float *z, int64_t begin, int64_t end, int axis,
-                  float cut, int64_t *split)
+static int stable_impl(float *x, float *y, float *z, uint32_t *ids, int64_t begin, int64_t end,
+                       int axis, float cut, int64_t *split)
 {
     /* Keys <= cut first, then keys > cut, each in input order.  The reference
      * o=2 kernel (partitionGPU.cu:58-231) produces these two sets in a
@@ -289,6 +307,7 @@ int orc_stable_le(float *x, float *y, float *z, int64_t begin, int64_t end, int 
     int64_t n = end - begin;
     float *k = axis == 0 ? x : axis == 1 ? y : z;
     float *tmp = (float *)malloc(sizeof(float) * 3 * (size_t)(n > 0 ? n : 1));
+    uint32_t *tid = ids ? (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1)) : NULL;
     int64_t nl = 0;
     for (int64_t p = begin; p < end; ++p)
         if (k[p] <= cut) nl++;
@@ -298,11 +317,14 @@ int orc_stable_le(float *x, float *y, float *z, int64_t begin, int64_t end, int 
         tmp[w] = x[p];
         tmp[n + w] = y[p];
         tmp[2 * n + w] = z[p];
+        if (tid) tid[w] = ids[p];
     }
     memcpy(x + begin, tmp, sizeof(float) * (size_t)n);
     memcpy(y + begin, tmp + n, sizeof(float) * (size_t)n);
     memcpy(z + begin, tmp + 2 * n, sizeof(float) * (size_t)n);
+    if (tid) memcpy(ids + begin, tid, sizeof(uint32_t) * (size_t)n);
     free(tmp);
+    free(tid);
     *split = begin + nl;
     return ORC_OK;
 }
@@ -377,6 +399,8 @@ int orc_build(int64_t n, int32_t d, const orc_opts *opts, float *x, float *y, fl
     tr->total_passes = 0;
     tr->error_cell = -1;
 
+    if (tr->ids)
+        for (int64_t p = 0; p < n; ++p) tr->ids[p] = (uint32_t)p;
     tr->heap[0] = orc_root(d);
     tr->exists[0] = 1;
     tr->ranges[0] = 0;                   /* init.cu:63-66 */
@@ -457,9 +481,9 @@ int orc_build(int64_t n, int32_t d, const orc_opts *opts, float *x, float *y, fl
             int64_t split = 0;
             int r;
             if (opts->partition == ORC_PART_HOARE)
-                r = orc_hoare(x, y, z, b, e, c->cutAxis, orc_get_cut(c), &split);
+                r = hoare_impl(x, y, z, tr->ids, b, e, c->cutAxis, orc_get_cut(c), &split);
             else
-                r = orc_stable_le(x, y, z, b, e, c->cutAxis, orc_get_cut(c), &split);
+                r = stable_impl(x, y, z, tr->ids, b, e, c->cutAxis, orc_get_cut(c), &split);
             if (r != ORC_OK) { tr->error_cell = c->id; rc = r; goto done; }
             int32_t lid = orc_left_child_id(c->id), rid = orc_right_child_id(c->id);
             tr->ranges[2 * lid] = (uint32_t)b;

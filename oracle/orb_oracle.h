@@ -107,6 +107,8 @@ typedef struct orc_trace {
     int32_t total_passes;
     int32_t error_cell;        /* heap id that raised ORC_ERR_UB, else -1     */
     int32_t pad;
+    uint32_t *ids;             /* optional [n]: set to 0..n-1, then moved with
+                                  the particles by every partition            */
 } orc_trace;
 
 /* Capacity of the heap for the given d and level mode (ids are 0-based).   */

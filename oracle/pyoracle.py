@@ -66,7 +66,7 @@ class Trace(C.Structure):
         ("heap", C.c_void_p), ("ranges", C.c_void_p), ("exists", C.c_void_p),
         ("pass_counts", C.c_void_p), ("level_passes", C.c_void_p),
         ("heap_cap", C.c_int64), ("nlevels", C.c_int32), ("total_passes", C.c_int32),
-        ("error_cell", C.c_int32), ("pad", C.c_int32),
+        ("error_cell", C.c_int32), ("pad", C.c_int32), ("ids", C.c_void_p),
     ]
 
 
@@ -137,7 +137,7 @@ class BuildResult:
 
 
 def build(x, y, z, d, levels_mode=LEVELS_REFERENCE, cmp=CMP_LT, partition=PART_HOARE,
-          max_passes=MAX_PASSES, pass_counts=False):
+          max_passes=MAX_PASSES, pass_counts=False, ids=False):
     """Runs the oracle build in place on copies of x, y, z."""
     L = lib()
     n = len(x)
@@ -150,9 +150,11 @@ def build(x, y, z, d, levels_mode=LEVELS_REFERENCE, cmp=CMP_LT, partition=PART_H
     exists = np.zeros(cap, np.uint8)
     pc = np.zeros((cap, MAX_PASSES), np.uint32) if pass_counts else None
     lp = np.zeros(64, np.int32)
+    pid = np.zeros(max(n, 1), np.uint32) if ids else None
     tr = Trace(heap=heap.ctypes.data, ranges=ranges.ctypes.data, exists=exists.ctypes.data,
                pass_counts=(pc.ctypes.data if pc is not None else None),
-               level_passes=lp.ctypes.data, heap_cap=cap)
+               level_passes=lp.ctypes.data, heap_cap=cap,
+               ids=(pid.ctypes.data if pid is not None else None))
     opts = Opts(levels_mode, cmp, partition, max_passes)
     rc = L.orc_build(n, d, C.byref(opts), ptr(x), ptr(y), ptr(z), C.byref(tr))
     r = BuildResult()
@@ -160,6 +162,7 @@ def build(x, y, z, d, levels_mode=LEVELS_REFERENCE, cmp=CMP_LT, partition=PART_H
     r.x, r.y, r.z = x, y, z
     r.heap, r.ranges, r.exists = heap, ranges, exists
     r.pass_counts = pc
+    r.ids = pid[:n] if pid is not None else None
     r.level_passes = lp[:max(tr.nlevels, 0)].copy()
     r.nlevels = tr.nlevels
     r.total_passes = tr.total_passes

@@ -435,3 +435,20 @@ def test_orbit_20_10_0_schedule(oracle):
     assert list(rf.level_passes) == [18, 17, 16, 15, 14, 14, 14, 13, 12, 12]
     sizes = [int(rf.ranges[i, 1] - rf.ranges[i, 0]) for i in range(1023, 2047)]
     assert min(sizes) == 1020 and max(sizes) == 1028
+
+
+@pytest.mark.parametrize("cmp,partition", [(0, 0), (1, 1)])
+def test_oracle_ids_follow_rows(oracle, cmp, partition):
+    """ids moved by the oracle's partitions: a permutation under which every
+    coordinate of the input lands where the build put it (distinct inputs,
+    so this fixes the ids)."""
+    rng = np.random.default_rng(17)
+    n = 30011
+    x, y, z = (rng.permutation(n).astype(np.float32) / n - np.float32(0.5) for _ in range(3))
+    r = oracle.build(x, y, z, 37, levels_mode=1, cmp=cmp, partition=partition, ids=True)
+    assert r.rc == 0
+    assert np.array_equal(np.sort(r.ids), np.arange(n, dtype=np.uint32))
+    for a, b in zip((x, y, z), (r.x, r.y, r.z)):
+        assert np.array_equal(a[r.ids], b)
+    plain = oracle.build(x, y, z, 37, levels_mode=1, cmp=cmp, partition=partition)
+    assert np.array_equal(plain.x, r.x) and np.array_equal(plain.ranges, r.ranges)
