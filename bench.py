@@ -6,11 +6,16 @@ seeds of init.cu:11) bisected to 2^10 leaves, full tree (10 levels).  A step
 is one whole build from the same initial particle order, inputs resident in
 HBM.  Both build forms run: the plain host-driven form (one count launch per
 bisection pass per level, counts read back each pass) and the batched form
-(device-driven).  `value` is the faster form's build time.
+(device-driven, launch shapes autotuned before the timed builds).  `value`
+is the faster form's build time.
 
 Prints one JSON line.  `roofline` is the count-pass kernel measured with HIP
 events on its own stream over the timed builds; `cpu_baseline` is the CPU
-oracle (a port of the reference's o=0 path, 1 thread) on a bounded sample.
+oracle (a port of the reference's o=0 path) on a bounded sample, on all the
+process's host threads and on one.
+
+Other workloads (--config): c3 clustered 2^26 -> 2^14, c4 uniform 2^28 ->
+2^20 with per-level times, c5 serving (continuous batching of requests).
 """
 import argparse
 import json
@@ -78,13 +83,23 @@ def committed_traffic(kernel="orb::k_count"):
     return None
 
 
-def cpu_baseline(log2n, d, levels):
+def host_threads():
+    """Threads for the all-cores CPU baseline: the process's CPU share,
+    bounded by OMP_NUM_THREADS when set."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def cpu_baseline(log2n, d, levels, threads=1):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     n = 1 << log2n
     x, y, z = pyoracle.generate(n)
     t0 = time.perf_counter()
-    r = pyoracle.build(x, y, z, d, levels_mode=levels)
+    r = pyoracle.build(x, y, z, d, levels_mode=levels, threads=threads)
     t1 = time.perf_counter()
     assert r.rc == 0
     return (t1 - t0) * 1e3, r
@@ -255,6 +270,8 @@ def main():
                     help="c2: uniform 2^27 -> 2^10; c3: clustered 2^26 -> 2^14; "
                          "c4: uniform 2^28 -> 2^20 with per-level times; c5: serving")
     ap.add_argument("--per-level", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batched form with the default launch shapes")
     ap.add_argument("--requests", type=int, default=160)
     ap.add_argument("--rate", type=float, default=0.0)
     ap.add_argument("--seed", type=int, default=7)
@@ -293,6 +310,13 @@ def main():
     for name in args.forms.split(","):
         form = orb.FORM_PLAIN if name == "plain" else orb.FORM_BATCHED
         log(f"[bench] form={name} warmup={args.warmup} steps={args.steps}")
+        tuning = None
+        if name == "batched" and not args.no_autotune:
+            ctx.restore_particles()
+            tuning = ctx.autotune(levels, repeats=2)
+            log(f"[bench] autotune: default {tuning['default_ms']:.3f} ms -> tuned "
+                f"{tuning['tuned_ms']:.3f} ms (device time per build), subtree_max_keys="
+                f"{tuning['subtree_max_keys']}")
         try:
             times, tm, info = run_form(orb, ctx, form, levels, args.steps, args.warmup,
                                        timed_events=(name == "plain"))
@@ -303,6 +327,8 @@ def main():
             raise
         res[name] = {"ms": sum(times) / len(times), "min_ms": min(times), "tm": tm,
                      "passes": info["total_passes"], "level_passes": info["level_passes"]}
+        if tuning is not None:
+            res[name]["tuning"] = tuning
         if args.per_level:
             ctx.enable_timing(True)
             ctx.restore_particles()
@@ -358,6 +384,7 @@ def main():
         "batched_speedup": (round(plain["ms"] / res["batched"]["ms"], 3)
                             if plain and "batched" in res else None),
         "passes": r["passes"],
+        "autotune": res.get("batched", {}).get("tuning"),
         "roofline": {
             "bound": "hbm",
             "kernel": "k_count (plain count-left pass)",
@@ -374,15 +401,23 @@ def main():
         },
     }
     if not args.no_cpu:
-        log(f"[bench] cpu baseline: oracle build of 2^{args.cpu_log2n} particles")
-        ms, _ = cpu_baseline(args.cpu_log2n, args.d, levels)
         scale = n / (1 << args.cpu_log2n)
+        nt = host_threads()
+        log(f"[bench] cpu baseline: oracle build of 2^{args.cpu_log2n} particles, {nt} threads")
+        ms_mt, _ = cpu_baseline(args.cpu_log2n, args.d, levels, threads=nt)
+        log(f"[bench] cpu baseline: the same on 1 thread")
+        ms_1t, _ = cpu_baseline(args.cpu_log2n, args.d, levels, threads=1)
+        note = (f"scaled x{scale:g} to 2^{args.log2n} (linear in N at fixed d; passes grow "
+                f"~log N, so this understates the CPU time)")
         out["cpu_baseline"] = {
-            "value": round(ms * scale, 1), "unit": "ms", "cores": 1, "kind": "port",
-            "sample": f"oracle (o=0 port) full build of 2^{args.cpu_log2n} particles -> "
-                      f"{args.d} leaves took {ms:.1f} ms on 1 host thread; scaled x{scale:g} "
-                      f"to 2^{args.log2n} (linear in N at fixed d; passes grow ~log N, so "
-                      f"this understates the CPU time)",
+            "value": round(ms_mt * scale, 1), "unit": "ms", "cores": nt, "kind": "port",
+            "sample": f"oracle (o=0 port, OpenMP over cells and over a large cell's keys; "
+                      f"same result as 1 thread) full build of 2^{args.cpu_log2n} particles -> "
+                      f"{args.d} leaves took {ms_mt:.1f} ms on {nt} host threads; {note}",
+        }
+        out["cpu_baseline_1_thread"] = {
+            "value": round(ms_1t * scale, 1), "unit": "ms", "cores": 1, "kind": "port",
+            "sample": f"the same build on 1 thread took {ms_1t:.1f} ms; {note}",
         }
     print(json.dumps(out), flush=True)
 

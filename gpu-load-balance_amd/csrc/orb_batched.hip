@@ -160,7 +160,7 @@ __global__ void k_b_init(orb_cell *heap, uint8_t *exists, uint8_t *sub, int64_t 
 // subtree kernel, cells inside a finished subtree are skipped.
 __global__ void k_b_recs(const LevelEntry *entries, int32_t nCells, const orb_cell *heap,
                          const uint32_t *ranges, const uint8_t *sub, int use_subtree,
-                         CellRec *recs, uint32_t *nchunks)
+                         uint32_t sub_max, CellRec *recs, uint32_t *nchunks)
 {
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i > nCells) return;
@@ -191,7 +191,7 @@ __global__ void k_b_recs(const LevelEntry *entries, int32_t nCells, const orb_ce
     r.lmax = en.lmax;
     const uint32_t n = r.end - r.begin;
     if (sub[g]) r.flags = 1;
-    else if (use_subtree && en.sub_ok && n <= (uint32_t)kSmall && cl.nLeafCells <= kSmallLeaves)
+    else if (use_subtree && en.sub_ok && n <= sub_max && cl.nLeafCells <= kSmallLeaves)
         r.flags = 2;
     else r.flags = 0;
     recs[i] = r;
@@ -943,6 +943,11 @@ static int ensure_state(Ctx &c, int32_t levels_mode, bool *complete)
     return ORB_OK;
 }
 
+int32_t subtree_max_keys(const Ctx &c)
+{
+    return c.sub_max_keys < 0 ? kSmall : (c.sub_max_keys > kSmall ? kSmall : c.sub_max_keys);
+}
+
 static void drop_graph(BatchedState &b)
 {
     if (b.exec) (void)hipGraphExecDestroy(b.exec);
@@ -955,17 +960,28 @@ static void drop_graph(BatchedState &b)
 int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *heap,
                   uint8_t *exists, uint8_t *sub, uint16_t *chunk_hist, uint32_t *cell_hist,
                   uint32_t *trace, int32_t *level_passes, int32_t *status, bool any_subtree,
-                  bool full_cover)
+                  bool full_cover, int32_t level)
 {
-    const bool use_subtree = any_subtree && c.part == ORB_PART_HOARE;
+    const int32_t sub_max = subtree_max_keys(c);
+    const bool use_subtree = any_subtree && c.part == ORB_PART_HOARE && sub_max > 0;
     const int le = c.cmp == ORB_CMP_LE ? 1 : 0;
+    c.cur_level = level;  // chunk_grid() and the subtree grid read the level's shapes
+    struct LevelScope {
+        Ctx &c;
+        ~LevelScope() { c.cur_level = -1; }
+    } scope{c};
     unsigned g1 = (unsigned)((nCells + 1 + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_b_recs, dim3(g1), dim3(kBlock), 0, c.stream, entries, nCells, heap,
-                       c.ranges.p, sub, use_subtree ? 1 : 0, c.recs.p, c.cell_nchunks.p);
+                       c.ranges.p, sub, use_subtree ? 1 : 0, (uint32_t)sub_max, c.recs.p,
+                       c.cell_nchunks.p);
     ORB_HIP(hipGetLastError());
     ORB_TRY(build_chunk_table(c, nCells));
     if (use_subtree) {
-        const unsigned gs = (unsigned)(nCells < 4096 ? nCells : 4096);
+        int32_t sb = 4096;
+        if (level >= 0 && level < (int32_t)c.tune_sub_blocks.size() &&
+            c.tune_sub_blocks[(size_t)level] > 0)
+            sb = c.tune_sub_blocks[(size_t)level];
+        const unsigned gs = (unsigned)(nCells < sb ? nCells : sb);
         if (c.track_ids)
             hipLaunchKernelGGL(k_b_subtree<true>, dim3(gs), dim3(kBlock), 0, c.stream, c.recs.p,
                                nCells, heap, exists, sub, c.ranges.p, c.x(), c.y(), c.z(), le, trace,
@@ -1026,7 +1042,7 @@ static int enqueue_build(Ctx &c, int32_t levels_mode)
         ORB_TRY(enqueue_level(c, b.level_ids.p + b.level_off[(size_t)l], nCells, b.heap.p,
                               b.exists.p, b.sub.p, b.chunk_hist.p, b.cell_hist.p, trace,
                               b.level_passes.p, c.status.p, b.complete,
-                              b.level_cover[(size_t)l] != 0));
+                              b.level_cover[(size_t)l] != 0, l));
         ORB_TRY(timing_pair(c, tlevel, kTimeLevel + l));
     }
     return particles_home(c);  // in-stream: the capture ends where it began
@@ -1114,6 +1130,11 @@ int batched_pull_results(Ctx &c)
     ORB_HIP(hipStreamSynchronize(c.stream));
     c.results_on_device = false;
     return ORB_OK;
+}
+
+void batched_drop_graph(Ctx &c)
+{
+    if (c.batched) drop_graph(*c.batched);
 }
 
 void batched_release(Ctx &c)

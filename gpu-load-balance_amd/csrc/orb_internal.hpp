@@ -390,6 +390,11 @@ struct Ctx {
     struct TPair { int a, b, kind; };
     std::vector<TPair> tpairs;
     BatchedState *batched = nullptr;
+    // batched-form launch shapes (chosen by orb_autotune; 0 / -1 = default)
+    int32_t cur_level = -1;              // level being enqueued, -1 outside a build
+    int32_t sub_max_keys = -1;           // cells of at most this many keys go to the subtree
+    std::vector<int32_t> tune_chunk_blocks;  // [level] grid cap of the chunk kernels
+    std::vector<int32_t> tune_sub_blocks;    // [level] grid of the subtree kernel
 
     // the buffer that holds the particles now, and the other one
     float *cur() const { return in_tmp ? tmp.p : pos.p; }
@@ -435,12 +440,14 @@ int launch_cell_sizes(Ctx &c, int32_t nCells, uint32_t *dst);
 int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *heap,
                   uint8_t *exists, uint8_t *sub, uint16_t *chunk_hist, uint32_t *cell_hist,
                   uint32_t *trace, int32_t *level_passes, int32_t *status, bool any_subtree,
-                  bool full_cover);
+                  bool full_cover, int32_t level);
 int preallocate_scratch(Ctx &c);
 int ensure_level_capacity(Ctx &c, int64_t nCells);
 
 // batched build (orb_batched.hip)
 int build_batched(Ctx &c, int32_t levels_mode, orb_build_info *info);
+void batched_drop_graph(Ctx &c);
+int32_t subtree_max_keys(const Ctx &c);
 int batched_pull_results(Ctx &c);
 void batched_release(Ctx &c);
 

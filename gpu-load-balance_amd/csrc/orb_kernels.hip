@@ -157,7 +157,10 @@ unsigned chunk_grid(const Ctx &c)
 {
     // chunk kernels loop over the level's chunks: a bounded grid keeps a
     // level with few chunks from paying for a million empty blocks
-    const int64_t cap = 65536;
+    int64_t cap = 65536;
+    if (c.cur_level >= 0 && c.cur_level < (int32_t)c.tune_chunk_blocks.size() &&
+        c.tune_chunk_blocks[(size_t)c.cur_level] > 0)
+        cap = c.tune_chunk_blocks[(size_t)c.cur_level];
     return (unsigned)(c.max_chunks < cap ? (c.max_chunks > 0 ? c.max_chunks : 1) : cap);
 }
 

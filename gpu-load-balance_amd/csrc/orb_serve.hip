@@ -438,7 +438,7 @@ static int step(Server &S)
             S.flip ^= 1;
             ORB_TRY(enqueue_level(c, S.entries.p, (int32_t)m, S.heap.p, S.exists.p, S.sub.p,
                                   S.chunk_hist.p, S.cell_hist.p, nullptr, S.level_passes.p,
-                                  S.status.p, any_sub, false));
+                                  S.status.p, any_sub, false, -1));
             S.stats.steps++;
             S.stats.cells_stepped += m;
             if ((int64_t)stepped.size() > S.stats.max_batch) S.stats.max_batch = (int64_t)stepped.size();

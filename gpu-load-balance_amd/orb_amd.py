@@ -81,6 +81,12 @@ class RequestInfo(C.Structure):
                 ("admit_ms", C.c_double), ("done_ms", C.c_double)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("subtree_max_keys", C.c_int32), ("nlevels", C.c_int32),
+                ("chunk_blocks", C.c_int32 * 64), ("subtree_blocks", C.c_int32 * 64),
+                ("default_ms", C.c_double), ("tuned_ms", C.c_double)]
+
+
 class OrbError(RuntimeError):
     def __init__(self, code, what=""):
         self.code = code
@@ -116,6 +122,9 @@ PROTOTYPES = {
     "orb_get_heap": (C.c_int, [_vp, _vp, _i64]),
     "orb_get_ranges": (C.c_int, [_vp, _vp, _i64]),
     "orb_get_exists": (C.c_int, [_vp, _vp, _i64]),
+    "orb_autotune": (C.c_int, [_vp, _i32, _i32, C.POINTER(Tuning)]),
+    "orb_set_tuning": (C.c_int, [_vp, C.POINTER(Tuning)]),
+    "orb_get_tuning": (C.c_int, [_vp, C.POINTER(Tuning)]),
     "orb_enable_ids": (C.c_int, [_vp, _b]),
     "orb_get_ids": (C.c_int, [_vp, _vp, _i64]),
     "orb_get_leaves": (C.c_int, [_vp, _vp, _vp, _i64, C.POINTER(C.c_int64)]),
@@ -309,6 +318,39 @@ class Context:
         e = np.zeros(self.cap, np.uint8)
         _check(lib().orb_get_exists(self.h, _p(e), self.cap), "orb_get_exists")
         return e
+
+    # ---- autotuner (batched form)
+    @staticmethod
+    def _tuning_dict(t):
+        n = t.nlevels
+        return {"subtree_max_keys": t.subtree_max_keys, "nlevels": n,
+                "chunk_blocks": list(t.chunk_blocks[:n]),
+                "subtree_blocks": list(t.subtree_blocks[:n]),
+                "default_ms": t.default_ms, "tuned_ms": t.tuned_ms}
+
+    def autotune(self, levels_mode=LEVELS_FULL, repeats=2):
+        t = Tuning()
+        _check(lib().orb_autotune(self.h, int(levels_mode), int(repeats), C.byref(t)),
+               "orb_autotune")
+        return self._tuning_dict(t)
+
+    def tuning(self):
+        t = Tuning()
+        _check(lib().orb_get_tuning(self.h, C.byref(t)), "orb_get_tuning")
+        return self._tuning_dict(t)
+
+    def set_tuning(self, d=None):
+        if d is None:
+            _check(lib().orb_set_tuning(self.h, None), "orb_set_tuning")
+            return
+        t = Tuning()
+        t.subtree_max_keys = int(d.get("subtree_max_keys", -1))
+        t.nlevels = int(d.get("nlevels", 0))
+        for i, v in enumerate(d.get("chunk_blocks", [])[:64]):
+            t.chunk_blocks[i] = int(v)
+        for i, v in enumerate(d.get("subtree_blocks", [])[:64]):
+            t.subtree_blocks[i] = int(v)
+        _check(lib().orb_set_tuning(self.h, C.byref(t)), "orb_set_tuning")
 
     # ---- tree export
     def enable_ids(self, on=True):

@@ -252,6 +252,29 @@ int orb_get_leaf_bounds(orb_ctx *ctx, float *lower, float *upper, int64_t cap);
  * partition or trace setting changes.                                    */
 int orb_enable_graph(orb_ctx *ctx, bool on);
 
+/* Autotuner for the batched form (no counterpart in the reference).  It
+ * picks launch shapes that leave every result unchanged (each variant is
+ * exact): which cells go to the LDS subtree kernel (cells of at most
+ * subtree_max_keys keys and 64 leaves; 0 = none), and per level the grid
+ * cap of the chunk kernels (chunks per block = chunks / blocks) and the
+ * subtree kernel's grid (cells per block).  orb_autotune times trial
+ * builds of the current particles (device events per level, the best of
+ * `repeats` runs each), keeps the fastest choice per level, restores the
+ * particles, and applies the result; the graph is recaptured on the next
+ * build.  Levels are independent for the per-level shapes because every
+ * variant leaves the same particles for the next level.                 */
+typedef struct orb_tuning {
+    int32_t subtree_max_keys;     /* < 0: default (2600)                   */
+    int32_t nlevels;
+    int32_t chunk_blocks[64];     /* [level-1], 0 = default                 */
+    int32_t subtree_blocks[64];   /* [level-1], 0 = default (4096)          */
+    double default_ms;            /* device ms of a build with the defaults */
+    double tuned_ms;              /* device ms of a build with the choices  */
+} orb_tuning;
+int orb_autotune(orb_ctx *ctx, int32_t levels_mode, int32_t repeats, orb_tuning *out);
+int orb_set_tuning(orb_ctx *ctx, const orb_tuning *t);  /* NULL: defaults */
+int orb_get_tuning(orb_ctx *ctx, orb_tuning *t);
+
 /* Kernel timing of the last build (device events), milliseconds.         */
 typedef struct orb_timing {
     double count_ms;       /* bisection count/histogram kernels            */

@@ -408,6 +408,8 @@ int orc_build(int64_t n, int32_t d, const orc_opts *opts, float *x, float *y, fl
 
     int32_t *ids = (int32_t *)malloc(sizeof(int32_t) * (size_t)cap);
     uint32_t *counts = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)cap);
+    uint32_t *left = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)cap);
+    const int nt = opts->threads > 1 ? opts->threads : 1;
     int rc = ORC_OK;
     int32_t D = orc_n_levels(d);
 
@@ -428,24 +430,37 @@ int orc_build(int64_t n, int32_t d, const orc_opts *opts, float *x, float *y, fl
             counts[i] = tr->ranges[2 * id + 1] - tr->ranges[2 * id];
         }
 
-        /* bisection controller (orbit.cpp:146-232) */
+        /* bisection controller (orbit.cpp:146-232).  The counts of a pass
+         * come first (threads over cells, or over the keys of a cell when a
+         * level has fewer cells than threads), then the serial update.     */
         int foundAll = 0;
         int32_t j = 0;
         while (!foundAll && j < max_passes) {
             j++;
             foundAll = 1;
+            const int by_cells = nt > 1 && nCells >= nt;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt) if (by_cells)
             for (int64_t i = 0; i < nCells; ++i) {
-                orc_cell *c = &tr->heap[ids[i]];
+                const orc_cell *c = &tr->heap[ids[i]];
                 if (c->foundCut) continue;  /* countLeft.cpp:19-21 */
                 uint32_t b = tr->ranges[2 * c->id], e = tr->ranges[2 * c->id + 1];
                 const float *col = c->cutAxis == 0 ? x : c->cutAxis == 1 ? y : z;
                 float cut = orc_get_cut(c);
                 uint32_t nLeft = 0;
+                const int by_keys = nt > 1 && !by_cells && (e - b) > 65536u;
                 if (opts->cmp == ORC_CMP_LT) {
+#pragma omp parallel for reduction(+ : nLeft) num_threads(nt) if (by_keys)
                     for (uint32_t p = b; p < e; ++p) nLeft += col[p] < cut;   /* countLeft.cpp:33-36 */
                 } else {
+#pragma omp parallel for reduction(+ : nLeft) num_threads(nt) if (by_keys)
                     for (uint32_t p = b; p < e; ++p) nLeft += col[p] <= cut;  /* countLeftGPUAxis.cu:159-162 */
                 }
+                left[i] = nLeft;
+            }
+            for (int64_t i = 0; i < nCells; ++i) {
+                orc_cell *c = &tr->heap[ids[i]];
+                if (c->foundCut) continue;
+                const uint32_t nLeft = left[i];
                 if (tr->pass_counts) tr->pass_counts[(int64_t)c->id * ORC_MAX_PASSES + (j - 1)] = nLeft;
                 int32_t difference = orc_difference(nLeft, counts[i], c->nLeafCells);
                 if (abs(difference) < 3) {
@@ -474,7 +489,9 @@ int orc_build(int64_t n, int32_t d, const orc_opts *opts, float *x, float *y, fl
             tr->exists[right.id] = 1;
         }
 
-        /* partition (partition.cpp:18-60) */
+        /* partition (partition.cpp:18-60); cells are disjoint ranges     */
+        int bad = -1;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt) if (nt > 1 && nCells > 1)
         for (int64_t i = 0; i < nCells; ++i) {
             orc_cell *c = &tr->heap[ids[i]];
             int64_t b = tr->ranges[2 * c->id], e = tr->ranges[2 * c->id + 1];
@@ -484,16 +501,24 @@ int orc_build(int64_t n, int32_t d, const orc_opts *opts, float *x, float *y, fl
                 r = hoare_impl(x, y, z, tr->ids, b, e, c->cutAxis, orc_get_cut(c), &split);
             else
                 r = stable_impl(x, y, z, tr->ids, b, e, c->cutAxis, orc_get_cut(c), &split);
-            if (r != ORC_OK) { tr->error_cell = c->id; rc = r; goto done; }
+            if (r != ORC_OK) {
+#pragma omp critical
+                {
+                    if (bad < 0 || i < bad) { bad = (int)i; rc = r; }
+                }
+                continue;
+            }
             int32_t lid = orc_left_child_id(c->id), rid = orc_right_child_id(c->id);
             tr->ranges[2 * lid] = (uint32_t)b;
             tr->ranges[2 * lid + 1] = (uint32_t)split;
             tr->ranges[2 * rid] = (uint32_t)split;
             tr->ranges[2 * rid + 1] = (uint32_t)e;
         }
+        if (bad >= 0) { tr->error_cell = tr->heap[ids[bad]].id; goto done; }
     }
 done:
     free(ids);
     free(counts);
+    free(left);
     return rc;
 }

@@ -94,6 +94,10 @@ typedef struct orc_opts {
     int32_t cmp;          /* ORC_CMP_*                                      */
     int32_t partition;    /* ORC_PART_*                                     */
     int32_t max_passes;   /* 32 in the reference                            */
+    int32_t threads;      /* <= 1: serial; else OpenMP over cells and over
+                             the keys of a large cell.  Counts are integer
+                             sums and cells are independent, so the result
+                             is the same for every thread count.          */
 } orc_opts;
 
 typedef struct orc_trace {

@@ -58,7 +58,7 @@ CELL_DTYPE = np.dtype({
 
 class Opts(C.Structure):
     _fields_ = [("levels_mode", C.c_int32), ("cmp", C.c_int32),
-                ("partition", C.c_int32), ("max_passes", C.c_int32)]
+                ("partition", C.c_int32), ("max_passes", C.c_int32), ("threads", C.c_int32)]
 
 
 class Trace(C.Structure):
@@ -137,7 +137,7 @@ class BuildResult:
 
 
 def build(x, y, z, d, levels_mode=LEVELS_REFERENCE, cmp=CMP_LT, partition=PART_HOARE,
-          max_passes=MAX_PASSES, pass_counts=False, ids=False):
+          max_passes=MAX_PASSES, pass_counts=False, ids=False, threads=1):
     """Runs the oracle build in place on copies of x, y, z."""
     L = lib()
     n = len(x)
@@ -155,7 +155,7 @@ def build(x, y, z, d, levels_mode=LEVELS_REFERENCE, cmp=CMP_LT, partition=PART_H
                pass_counts=(pc.ctypes.data if pc is not None else None),
                level_passes=lp.ctypes.data, heap_cap=cap,
                ids=(pid.ctypes.data if pid is not None else None))
-    opts = Opts(levels_mode, cmp, partition, max_passes)
+    opts = Opts(levels_mode, cmp, partition, max_passes, threads)
     rc = L.orc_build(n, d, C.byref(opts), ptr(x), ptr(y), ptr(z), C.byref(tr))
     r = BuildResult()
     r.rc = rc

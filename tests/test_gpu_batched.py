@@ -104,3 +104,56 @@ def test_batched_graph_replays_match_eager(orb, oracle):
     ctx.close()
     r = oracle.build(x, y, z, d, levels_mode=1)
     assert np.array_equal(ref[1], r.ranges)
+
+
+def test_autotune_keeps_results_exact(orb, oracle):
+    """Every shape the autotuner chooses between is exact: the tuned build
+    equals the oracle bit for bit, and the tuner leaves the particles as it
+    found them."""
+    n, d = 1 << 18, 1 << 12   # deep enough for the subtree kernel
+    x, y, z = oracle.generate(n)
+    r = oracle.build(x, y, z, d, levels_mode=1, pass_counts=True)
+    ctx = orb.Context(n, d, generate=False)
+    ctx.set_particles(x, y, z)
+    ctx.enable_trace(True)
+    t = ctx.autotune(orb.LEVELS_FULL, repeats=1)
+    assert t["subtree_max_keys"] in (-1, 1800, 1100, 0)
+    assert set(t["chunk_blocks"]) <= {0, 16384, 4096, 1024}
+    assert set(t["subtree_blocks"]) <= {0, 1024, 2048, 8192}
+    assert t["tuned_ms"] <= t["default_ms"]
+    gx, _, _ = ctx.get_particles()
+    assert np.array_equal(fbits(gx), fbits(x))           # input order restored
+    assert tuple(ctx.ranges()[0]) == (0, n)
+    for _ in range(2):                                    # eager, then graph replay
+        ctx.set_particles(x, y, z)
+        info = ctx.build(orb.FORM_BATCHED, orb.LEVELS_FULL)
+        g = {"info": info, "heap": ctx.heap(), "ranges": ctx.ranges(), "exists": ctx.exists(),
+             "xyz": ctx.get_particles(), "pc": ctx.pass_counts()}
+        assert_same_build(g, r)
+    ctx.close()
+
+
+@pytest.mark.parametrize("shape", [
+    {"subtree_max_keys": 0, "nlevels": 12, "chunk_blocks": [1024] * 12,
+     "subtree_blocks": [1024] * 12},
+    {"subtree_max_keys": 1100, "nlevels": 12, "chunk_blocks": [4096] * 12,
+     "subtree_blocks": [8192] * 12},
+])
+def test_explicit_shapes_are_exact(orb, oracle, shape):
+    n, d = (1 << 18) + 5, 3000
+    x, y, z = oracle.generate(n)
+    r = oracle.build(x, y, z, d, levels_mode=1)
+    ctx = orb.Context(n, d, generate=False)
+    ctx.set_tuning(shape)
+    got = ctx.tuning()
+    assert got["subtree_max_keys"] == shape["subtree_max_keys"]
+    assert got["chunk_blocks"] == shape["chunk_blocks"]
+    for _ in range(2):
+        ctx.set_particles(x, y, z)
+        info = ctx.build(orb.FORM_BATCHED, orb.LEVELS_FULL)
+        g = {"info": info, "heap": ctx.heap(), "ranges": ctx.ranges(), "exists": ctx.exists(),
+             "xyz": ctx.get_particles(), "pc": None}
+        assert_same_build(g, r, trace=False)
+    ctx.set_tuning(None)
+    assert ctx.tuning()["subtree_max_keys"] == -1
+    ctx.close()

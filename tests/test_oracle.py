@@ -452,3 +452,21 @@ def test_oracle_ids_follow_rows(oracle, cmp, partition):
         assert np.array_equal(a[r.ids], b)
     plain = oracle.build(x, y, z, 37, levels_mode=1, cmp=cmp, partition=partition)
     assert np.array_equal(plain.x, r.x) and np.array_equal(plain.ranges, r.ranges)
+
+
+@pytest.mark.parametrize("partition", [0, 1])
+def test_oracle_threads_same_result(oracle, partition):
+    """The multi-threaded oracle (the all-cores CPU baseline) is the serial
+    one: integer counts and disjoint cells do not depend on the split."""
+    rng = np.random.default_rng(23)
+    n = (1 << 18) + 3
+    x, y, z = ((rng.integers(0, 1 << 10, n) / (1 << 10) - 0.5).astype(np.float32)
+               for _ in range(3))
+    a = oracle.build(x, y, z, 700, levels_mode=1, cmp=partition, partition=partition,
+                     pass_counts=True, threads=1)
+    b = oracle.build(x, y, z, 700, levels_mode=1, cmp=partition, partition=partition,
+                     pass_counts=True, threads=4)
+    assert a.rc == b.rc == 0
+    for f in ("x", "y", "z", "ranges", "exists", "pass_counts", "level_passes"):
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
+    assert a.heap.tobytes() == b.heap.tobytes()
