@@ -87,6 +87,12 @@ class Tuning(C.Structure):
                 ("default_ms", C.c_double), ("tuned_ms", C.c_double)]
 
 
+class TipsyHeader(C.Structure):
+    _fields_ = [("time", C.c_double), ("nbodies", C.c_int32), ("ndim", C.c_int32),
+                ("nsph", C.c_int32), ("ndark", C.c_int32), ("nstar", C.c_int32),
+                ("big_endian", C.c_int32)]
+
+
 class OrbError(RuntimeError):
     def __init__(self, code, what=""):
         self.code = code
@@ -147,6 +153,8 @@ PROTOTYPES = {
     "orb_server_fetch": (C.c_int, [_vp, C.c_uint64, _vp, _vp, _vp, _vp, _vp, _vp, _i64,
                                    C.POINTER(RequestInfo)]),
     "orb_server_get_stats": (C.c_int, [_vp, C.POINTER(ServerStats)]),
+    "orb_read_tipsy": (C.c_int, [C.c_char_p, C.POINTER(TipsyHeader), _vp, _vp, _vp, _i64]),
+    "orb_init_tipsy": (C.c_int, [C.POINTER(_vp), C.c_char_p, _i32, MetaParams]),
     "orb_error_string": (C.c_char_p, [C.c_int]),
     "orb_last_hip_error": (C.c_int, []),
     "orb_device_count": (C.c_int, []),
@@ -202,6 +210,22 @@ class Context:
         self.n = int(n)
         self.d = int(d)
         self.cap = int(L.orb_heap_capacity(int(d)))
+
+    @classmethod
+    def from_tipsy(cls, path, d, params=(False, False, False)):
+        """The particles of a Tipsy snapshot (ServiceInit's generate=false)."""
+        L = lib()
+        if L.orb_device_count() < 1:
+            raise OrbError(ERR_NODEV, "orb_init_tipsy")
+        h = C.c_void_p()
+        _check(L.orb_init_tipsy(C.byref(h), str(path).encode(), int(d), MetaParams(*params)),
+               "orb_init_tipsy")
+        self = cls.__new__(cls)
+        self.h = h
+        self.n = int(read_tipsy(path)[0]["nbodies"])
+        self.d = int(d)
+        self.cap = int(L.orb_heap_capacity(int(d)))
+        return self
 
     def close(self):
         if self.h:
@@ -554,3 +578,17 @@ class Server:
         st = ServerStats()
         _check(lib().orb_server_get_stats(self.h, C.byref(st)), "orb_server_get_stats")
         return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+
+def read_tipsy(path):
+    """(header dict, x, y, z) of a Tipsy snapshot (host code, no GPU)."""
+    h = TipsyHeader()
+    p = str(path).encode()
+    _check(lib().orb_read_tipsy(p, C.byref(h), None, None, None, 0), "orb_read_tipsy")
+    n = h.nbodies
+    x = np.empty(n, np.float32)
+    y = np.empty(n, np.float32)
+    z = np.empty(n, np.float32)
+    _check(lib().orb_read_tipsy(p, C.byref(h), _p(x), _p(y), _p(z), n), "orb_read_tipsy")
+    hd = {k: getattr(h, k) for k, _ in TipsyHeader._fields_}
+    return hd, x, y, z

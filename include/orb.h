@@ -292,6 +292,24 @@ int orb_get_level_timing(orb_ctx *ctx, double *ms, int32_t max_levels);
 int orb_enable_timing(orb_ctx *ctx, bool on);
 
 /* ------------------------------------------------------------------ */
+/* Tipsy snapshots: the generate=false input of ServiceInit             */
+/* (src/services/init.cu:54-59, whose TipsyIO source is not in the       */
+/* reference).  Positions of all particles in file order (gas, dark,     */
+/* star).  The standard big-endian (XDR) layout and the native           */
+/* little-endian one are told apart by the header.  Host code only.      */
+/* ------------------------------------------------------------------ */
+typedef struct orb_tipsy_header {
+    double time;
+    int32_t nbodies, ndim, nsph, ndark, nstar;
+    int32_t big_endian;           /* 1: standard XDR file                   */
+} orb_tipsy_header;
+/* Reads the header; with x, y, z non-NULL also the positions (cap >= nbodies). */
+int orb_read_tipsy(const char *path, orb_tipsy_header *hdr, float *x, float *y, float *z,
+                   int64_t cap);
+/* orb_init(generate=false) with the snapshot's positions as the particles. */
+int orb_init_tipsy(orb_ctx **ctx, const char *path, int32_t d, orb_meta_params params);
+
+/* ------------------------------------------------------------------ */
 /* errors / device                                                     */
 /* ------------------------------------------------------------------ */
 const char *orb_error_string(int code);

@@ -30,3 +30,11 @@ def orb():
     if orb_amd.lib().orb_device_count() < 1:
         pytest.fail("liborb_hip.so loaded but no GPU is visible")
     return orb_amd
+
+
+@pytest.fixture(scope="session")
+def orb_host():
+    """The HIP library for its host-only entry points (no GPU needed)."""
+    import orb_amd
+    orb_amd.lib()
+    return orb_amd

@@ -134,3 +134,21 @@ def test_tree_export_errors(orb, oracle):
     assert e.value.code == orb.ERR_STATE
     assert len(ctx.leaves()[0]) == 4
     ctx.close()
+
+
+def test_tipsy_snapshot_build(orb, oracle, tmp_path):
+    """A clustered snapshot read from a Tipsy file builds like the same
+    positions handed over directly."""
+    import inputs
+    x, y, z = inputs.clustered(150000, n_spheres=6, a=0.02, seed=9)
+    P = np.stack([x, y, z], 1)
+    path = tmp_path / "c.std"
+    inputs.write_tipsy(path, gas=P[:20000], dark=P[20000:140000], star=P[140000:])
+    ctx = orb.Context.from_tipsy(path, 500)
+    assert ctx.n == 150000
+    info = ctx.build(orb.FORM_BATCHED, orb.LEVELS_FULL)
+    r = oracle.build(x, y, z, 500, levels_mode=1)
+    g = {"info": info, "heap": ctx.heap(), "ranges": ctx.ranges(), "exists": ctx.exists(),
+         "xyz": ctx.get_particles(), "pc": None}
+    assert_same_build(g, r, trace=False)
+    ctx.close()
