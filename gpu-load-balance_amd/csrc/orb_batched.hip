@@ -23,8 +23,6 @@
 
 namespace orb {
 
-constexpr int kHistK = 8;                      // bisection steps per histogram round
-constexpr int kBins = 1 << kHistK;             // 256 bins (255 candidate cuts)
 constexpr int kRounds = (ORB_MAX_PASSES + kHistK - 1) / kHistK;  // 4
 // Subtree kernel limits.  Cells of a level cluster around n / 2^(l-1); the key
 // budget sits 27% above a power of two so a whole level qualifies at once.
@@ -224,7 +222,7 @@ __device__ inline float candidate(int t, float lo, float hi)
 // (rounding at a bin edge, or a window a few ulps wide) walks, and the walk
 // alone makes the result exact.
 template <int LE>
-__device__ inline int bin_of(const float *cand, float v, float lo, float scale)
+__device__ inline int bin_of(const float *cand, float v, float lo, float scale, int &tie)
 {
     const float ef = fminf(fmaxf((v - lo) * scale, 0.0f), (float)(kBins - 1));
     int e = (int)ef;
@@ -239,16 +237,19 @@ __device__ inline int bin_of(const float *cand, float v, float lo, float scale)
             while (e < kBins - 1 && cand[e + 1] <= v) ++e;
         }
     }
+    // the candidate v equals, if any: cand[e] (<) or cand[e + 1] (<=)
+    tie = LE ? (cand[e + 1] == v ? e + 1 : 0) : (cand[e] == v ? e : 0);
     return e;
 }
 
 template <int LE>
 __device__ inline void k_b_hist_chunk(uint32_t q, const CellRec *recs, const uint32_t *first,
                                       const uint32_t *chunk_cell, const float *x, const float *y,
-                                      const float *z, uint16_t *chunk_hist)
+                                      const float *z, uint16_t *chunk_hist, uint16_t *chunk_tie)
 {
     __shared__ float cand[kBins + 1];
     __shared__ uint32_t hist[4][kBins];
+    __shared__ uint32_t ties[kBins];  // keys equal to candidate j (rare)
     __shared__ uint32_t sm[4];
     const uint32_t cell = chunk_cell[q];
     const CellRec r = recs[cell];
@@ -265,6 +266,7 @@ __device__ inline void k_b_hist_chunk(uint32_t q, const CellRec *recs, const uin
     hist[1][t] = 0u;
     hist[2][t] = 0u;
     hist[3][t] = 0u;
+    ties[t] = 0u;
     __syncthreads();
     const float width = hi - lo;
     const float scale = width > 0.0f ? (float)kBins / width : 0.0f;
@@ -279,7 +281,11 @@ __device__ inline void k_b_hist_chunk(uint32_t q, const CellRec *recs, const uin
         const bool bl = v < lo, ab = v > hi;
         below += bl ? 1u : 0u;
         above += ab ? 1u : 0u;
-        if (!(bl || ab)) atomicAdd(&h[bin_of<LE>(cand, v, lo, scale)], 1u);
+        if (!(bl || ab)) {
+            int tj;
+            atomicAdd(&h[bin_of<LE>(cand, v, lo, scale, tj)], 1u);
+            if (tj) atomicAdd(&ties[tj], 1u);
+        }
     };
     uint32_t ab = (b + 3u) & ~3u;
     if (ab > e) ab = e;
@@ -310,6 +316,7 @@ __device__ inline void k_b_hist_chunk(uint32_t q, const CellRec *recs, const uin
     if (t == 0) sum += ba & 0xffffu;
     if (t == kBins - 1) sum += ba >> 16;
     chunk_hist[(size_t)q * kBins + t] = (uint16_t)sum;
+    chunk_tie[(size_t)q * kBins + t] = (uint16_t)ties[t];
 }
 
 template <int LE>
@@ -317,11 +324,11 @@ __global__ __launch_bounds__(kBlock) void k_b_hist(const CellRec *recs, int32_t 
                                                    const uint32_t *first,
                                                    const uint32_t *chunk_cell, const float *x,
                                                    const float *y, const float *z,
-                                                   uint16_t *chunk_hist)
+                                                   uint16_t *chunk_hist, uint16_t *chunk_tie)
 {
     const uint32_t total = first[nCells];
     for (uint32_t q = blockIdx.x; q < total; q += gridDim.x) {
-        k_b_hist_chunk<LE>(q, recs, first, chunk_cell, x, y, z, chunk_hist);
+        k_b_hist_chunk<LE>(q, recs, first, chunk_cell, x, y, z, chunk_hist, chunk_tie);
         __syncthreads();  // the next chunk reuses this block's LDS
     }
 }
@@ -396,6 +403,7 @@ struct RoundOut {
     float L, R;
     int32_t passes;
     bool found;
+    int32_t node;  // the candidate the round stopped at (when found)
 };
 
 __device__ inline RoundOut replay_round(const uint32_t *P, float lo, float hi, float target,
@@ -446,7 +454,11 @@ __device__ inline RoundOut replay_round(const uint32_t *P, float lo, float hi, f
             if (d == lane) v = cnt[d];
         trace_row[passes0 + lane] = v;
     }
-    return RoundOut{L, R, passes0 + used, found};
+    int32_t stop = 0;
+#pragma unroll
+    for (int d = 0; d < kHistK; ++d)
+        if (d == s) stop = nodes[d];
+    return RoundOut{L, R, passes0 + used, found, found ? stop : 0};
 }
 
 // One wave per cell: prefix sums of the cell histogram in LDS, then the
@@ -492,11 +504,23 @@ __global__ __launch_bounds__(kBlock) void k_b_replay(CellRec *recs, int32_t nCel
     const float L = ro.L, R = ro.R;
     const int32_t passes = ro.passes;
     const bool found = ro.found;
+    // The stopper counts can come from this round's histograms when the cut
+    // is a candidate distinct from its neighbours (equal candidates -- a
+    // window a few ulps wide -- would split the keys equal to the cut over
+    // several tie bins): encode 1 + node, else 1 (found, keys are read).
+    int32_t found_enc = 0;
+    if (found) {
+        const int j = ro.node;
+        const float cj = candidate(j, r.margin_l, r.margin_r);
+        const float cp = j > 1 ? candidate(j - 1, r.margin_l, r.margin_r) : -INFINITY;
+        const float cn = j < kBins - 1 ? candidate(j + 1, r.margin_l, r.margin_r) : INFINITY;
+        found_enc = (cj == cp || cj == cn) ? 1 : 1 + j;
+    }
     r.margin_l = L;
     r.margin_r = R;
     r.cut = cell_cut(L, R);
     r.passes = passes;
-    r.found = found ? 1 : 0;
+    r.found = found_enc;  // the stopper counts read this round's histograms
     recs[c] = r;
     orb_cell &hc = heap[(int64_t)r.hb + r.id];
     hc.cutMarginLeft = L;
@@ -932,7 +956,7 @@ static int ensure_state(Ctx &c, int32_t levels_mode, bool *complete)
     ORB_TRY(b.heap.alloc((size_t)c.cap));
     ORB_TRY(b.exists.alloc((size_t)c.cap));
     ORB_TRY(b.sub.alloc((size_t)c.cap));
-    ORB_TRY(b.chunk_hist.alloc((size_t)(c.max_chunks + 1) * kBins));
+    ORB_TRY(b.chunk_hist.alloc(2 * (size_t)(c.max_chunks + 1) * kBins));  // bins | ties
     if (!b.cell_hist.p) b.hist_clean = false;
     ORB_TRY(b.cell_hist.alloc((size_t)c.max_level_cells * kBins));
     ORB_TRY(b.level_passes.alloc(64));
@@ -975,7 +999,8 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
                        c.ranges.p, sub, use_subtree ? 1 : 0, (uint32_t)sub_max, c.recs.p,
                        c.cell_nchunks.p);
     ORB_HIP(hipGetLastError());
-    ORB_TRY(build_chunk_table(c, nCells));
+    ORB_TRY(build_chunk_table(c, nCells, false));
+    uint16_t *chunk_tie = chunk_hist + (size_t)(c.max_chunks + 1) * kBins;
     if (use_subtree) {
         int32_t sb = 4096;
         if (level >= 0 && level < (int32_t)c.tune_sub_blocks.size() &&
@@ -999,10 +1024,12 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
         const int tb = timing_mark(c);
         if (le)
             hipLaunchKernelGGL(k_b_hist<1>, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
-                               c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(), chunk_hist);
+                               c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(), chunk_hist,
+                               chunk_tie);
         else
             hipLaunchKernelGGL(k_b_hist<0>, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
-                               c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(), chunk_hist);
+                               c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(), chunk_hist,
+                               chunk_tie);
         ORB_HIP(hipGetLastError());
         ORB_TRY(timing_pair(c, tb, kTimeCount));
         hipLaunchKernelGGL(k_b_hist_reduce, dim3(gr), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
@@ -1016,8 +1043,12 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
                        exists, status);
     ORB_HIP(hipGetLastError());
     const int tp = timing_mark(c);
-    if (c.part == ORB_PART_HOARE) ORB_TRY(launch_hoare(c, nCells, status));
-    else ORB_TRY(launch_stable_le(c, nCells, status, full_cover));
+    HistCounts hc;
+    hc.hist = chunk_hist;
+    hc.tie = chunk_tie;
+    hc.le = le;
+    if (c.part == ORB_PART_HOARE) ORB_TRY(launch_hoare(c, nCells, status, &hc));
+    else ORB_TRY(launch_stable_le(c, nCells, status, full_cover, &hc));
     ORB_TRY(timing_pair(c, tp, kTimePartition));
     return ORB_OK;
 }

@@ -26,6 +26,9 @@ namespace orb {
 constexpr int kBlock = 256;                 // threads per block (4 wave64s)
 constexpr int kPerThread = 16;              // keys per thread per chunk
 constexpr int kChunk = kBlock * kPerThread; // 4096 keys = 16 KiB per chunk
+constexpr int kHistK = 8;                   // bisection steps per histogram round
+constexpr int kBins = 1 << kHistK;          // 256 bins (255 candidate cuts)
+static_assert(kBins == kBlock, "one thread per histogram bin");
 
 // ---------------------------------------------------------------------------
 // error plumbing
@@ -273,7 +276,8 @@ struct CellRec {
     uint32_t end;       // one past the last
     float cut;          // getCut() of the cell
     int32_t axis;       // cutAxis
-    int32_t found;      // foundCut
+    int32_t found;      // foundCut; batched form: 1 + the candidate index of the cut in
+                        // the histogram round that found it (0: not found)
     int32_t nLeaf;      // nLeafCells
     float margin_l;     // cutMarginLeft
     float margin_r;     // cutMarginRight
@@ -421,13 +425,22 @@ int scan_exclusive_u64(Ctx &c, const uint64_t *in, uint64_t *out, size_t m,
                        const uint32_t *lim = nullptr);
 unsigned chunk_grid(const Ctx &c);
 int upload_level(Ctx &c, const orb_cell *cells, int32_t nCells);
-int build_chunk_table(Ctx &c, int32_t nCells);
+int build_chunk_table(Ctx &c, int32_t nCells, bool with_stamps = true);
 int launch_count(Ctx &c, int32_t nCells, bool lt_skip_found, const float *axis_override,
                  hipEvent_t after_count = nullptr);
-int launch_hoare(Ctx &c, int32_t nCells, int32_t *status = nullptr);
-int launch_stopper_counts(Ctx &c, int32_t nCells, bool le_only);
+// Per-chunk histograms of the round that found each cell's cut (batched
+// form): the stopper counts come from them instead of a key read.
+struct HistCounts {
+    const uint16_t *hist = nullptr;   // [chunks][256] bin counts
+    const uint16_t *tie = nullptr;    // [chunks][256] keys equal to candidate j
+    int le = 0;                       // bins were built with <= (else <)
+};
+int launch_hoare(Ctx &c, int32_t nCells, int32_t *status = nullptr,
+                 const HistCounts *hc = nullptr);
+int launch_stopper_counts(Ctx &c, int32_t nCells, bool le_only, const HistCounts *hc = nullptr);
 int invalidate_stamps(Ctx &c, int32_t nCells);
-int launch_stable_le(Ctx &c, int32_t nCells, int32_t *status = nullptr, bool full_cover = false);
+int launch_stable_le(Ctx &c, int32_t nCells, int32_t *status = nullptr, bool full_cover = false,
+                     const HistCounts *hc = nullptr);
 int ensure_stable_buffers(Ctx &c);
 int launch_copy_u32(hipStream_t s, const uint32_t *src, uint32_t *dst, size_t n);
 int launch_iota(hipStream_t s, uint32_t *p, size_t n);

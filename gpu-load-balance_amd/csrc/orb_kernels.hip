@@ -331,13 +331,16 @@ int upload_level(Ctx &c, const orb_cell *cells, int32_t nCells)
 
 int invalidate_stamps(Ctx &c, int32_t nCells);
 
-int build_chunk_table(Ctx &c, int32_t nCells)
+// with_stamps: the plain form reuses a count pass's chunk counts for the
+// partition when the stamp matches; the batched form takes them from its
+// histograms and needs no stamps.
+int build_chunk_table(Ctx &c, int32_t nCells, bool with_stamps)
 {
     ORB_TRY(scan_exclusive_u32(c, c.cell_nchunks.p, c.cell_first.p, (size_t)nCells + 1));
     hipLaunchKernelGGL(k_fill_chunks, dim3(grid_for((size_t)c.max_chunks, 16384)), dim3(kBlock), 0,
                        c.stream, c.cell_first.p, nCells, c.chunk_cell.p, (uint32_t)c.max_chunks);
     ORB_HIP(hipGetLastError());
-    return invalidate_stamps(c, nCells);
+    return with_stamps ? invalidate_stamps(c, nCells) : ORB_OK;
 }
 
 // chunk q of the level: its cell and [b, e).  Returns false past the table.
@@ -861,22 +864,85 @@ __global__ void k_hoare_final(const CellRec *recs, int32_t nCells, const SplitRe
     }
 }
 
-int launch_stopper_counts(Ctx &c, int32_t nCells, bool le_only)
+// Stopper counts of a chunk from the histogram round that found its cell's
+// cut at candidate j: keys below the cut = the bins below j (< or <=
+// histogram), and the keys equal to candidate j turn one count into the
+// other.  A cell that hit the pass cap was cut at a midpoint no round
+// counted: its chunks read their keys.
+__device__ inline void k_b_stopper_counts_chunk(uint32_t q, const CellRec *recs, int32_t nCells,
+                                                const uint32_t *first, const uint32_t *chunk_cell,
+                                                const float *x, const float *y, const float *z,
+                                                int le_only, const uint16_t *hist,
+                                                const uint16_t *tie, int hist_le,
+                                                uint64_t *chunk_cnt)
 {
-    hipLaunchKernelGGL(k_stopper_counts, dim3(chunk_grid(c)), dim3(kBlock), 0,
-                       c.stream, c.recs.p, nCells, c.cell_first.p, c.chunk_cell.p, c.x(), c.y(),
-                       c.z(), le_only ? 1 : 0, c.stamps.p, c.chunk_ltle.p, c.chunk_cnt.p,
-                       (uint32_t)c.max_chunks);
+    __shared__ uint32_t sm[4];
+    uint32_t cell, b, e;
+    if (!chunk_range(q, first, nCells, chunk_cell, recs, cell, b, e)) return;
+    const CellRec r = recs[cell];
+    uint32_t lt = 0, le = 0;
+    if (r.found >= 2) {
+        const int j = r.found - 1;
+        const uint32_t t = threadIdx.x;  // kBlock == kBins
+        const uint32_t h = (int)t < j ? (uint32_t)hist[(size_t)q * kBins + t] : 0u;
+        const uint32_t below = block_sum(h, sm);
+        const uint32_t eq = tie[(size_t)q * kBins + (uint32_t)j];
+        if (hist_le) {
+            le = below;
+            lt = below - eq;
+        } else {
+            lt = below;
+            le = below + eq;
+        }
+    } else {
+        const int axis = r.axis < 0 ? 0 : (r.axis > 2 ? 2 : r.axis);
+        count_range(axis_ptr(x, y, z, axis), b, e, r.cut, lt, le);
+        const uint32_t packed = block_sum(lt | (le << 16), sm);
+        lt = packed & 0xffffu;
+        le = packed >> 16;
+    }
+    if (threadIdx.x == 0) {
+        const uint32_t len = e - b;
+        const uint32_t nl = le_only ? le : len - lt;
+        const uint32_t nr = le_only ? len - le : le;
+        chunk_cnt[q] = (uint64_t)nl | ((uint64_t)nr << 32);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_b_stopper_counts(
+    const CellRec *recs, int32_t nCells, const uint32_t *first, const uint32_t *chunk_cell,
+    const float *x, const float *y, const float *z, int le_only, const uint16_t *hist,
+    const uint16_t *tie, int hist_le, uint64_t *chunk_cnt)
+{
+    const uint32_t total = first[nCells];
+    for (uint32_t q = blockIdx.x; q < total; q += gridDim.x) {
+        k_b_stopper_counts_chunk(q, recs, nCells, first, chunk_cell, x, y, z, le_only, hist, tie,
+                                 hist_le, chunk_cnt);
+        __syncthreads();  // the next chunk reuses this block's LDS
+    }
+}
+
+int launch_stopper_counts(Ctx &c, int32_t nCells, bool le_only, const HistCounts *hc)
+{
+    if (hc && hc->hist)
+        hipLaunchKernelGGL(k_b_stopper_counts, dim3(chunk_grid(c)), dim3(kBlock), 0, c.stream,
+                           c.recs.p, nCells, c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(),
+                           le_only ? 1 : 0, hc->hist, hc->tie, hc->le, c.chunk_cnt.p);
+    else
+        hipLaunchKernelGGL(k_stopper_counts, dim3(chunk_grid(c)), dim3(kBlock), 0,
+                           c.stream, c.recs.p, nCells, c.cell_first.p, c.chunk_cell.p, c.x(), c.y(),
+                           c.z(), le_only ? 1 : 0, c.stamps.p, c.chunk_ltle.p, c.chunk_cnt.p,
+                           (uint32_t)c.max_chunks);
     ORB_HIP(hipGetLastError());
     return scan_exclusive_u64(c, c.chunk_cnt.p, c.chunk_pre.p, (size_t)c.max_chunks + 1,
                               c.cell_first.p + nCells);
 }
 
-int launch_hoare(Ctx &c, int32_t nCells, int32_t *status)
+int launch_hoare(Ctx &c, int32_t nCells, int32_t *status, const HistCounts *hc)
 {
     if (!status) status = c.status.p;
     const unsigned gq = chunk_grid(c);
-    ORB_TRY(launch_stopper_counts(c, nCells, false));
+    ORB_TRY(launch_stopper_counts(c, nCells, false, hc));
     hipLaunchKernelGGL(k_stopper_positions, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
                        c.cell_first.p, c.chunk_cell.p, c.chunk_pre.p, c.x(), c.y(), c.z(), c.lpos.p,
                        c.rpos.p);
@@ -1037,12 +1103,13 @@ __global__ void k_stable_ranges(const CellRec *recs, int32_t nCells, const uint3
     ranges[2 * gr + 1] = r.end;
 }
 
-int launch_stable_le(Ctx &c, int32_t nCells, int32_t *status, bool full_cover)
+int launch_stable_le(Ctx &c, int32_t nCells, int32_t *status, bool full_cover,
+                     const HistCounts *hc)
 {
     if (!status) status = c.status.p;
     ORB_TRY(ensure_stable_buffers(c));
     const unsigned gq = chunk_grid(c);
-    ORB_TRY(launch_stopper_counts(c, nCells, true));
+    ORB_TRY(launch_stopper_counts(c, nCells, true, hc));
     float *ob = c.other();
     float *ox = ob, *oy = ob + c.stride, *oz = ob + 2 * c.stride;
     hipLaunchKernelGGL(k_stable_scatter, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,

@@ -226,7 +226,7 @@ static int server_alloc(Server &S)
     ORB_TRY(S.heap.alloc((size_t)H));
     ORB_TRY(S.exists.alloc((size_t)H));
     ORB_TRY(S.sub.alloc((size_t)H));
-    ORB_TRY(S.chunk_hist.alloc((size_t)(c.max_chunks + 1) * 256));
+    ORB_TRY(S.chunk_hist.alloc(2 * (size_t)(c.max_chunks + 1) * kBins));  // bins | ties
     ORB_TRY(S.cell_hist.alloc((size_t)c.max_level_cells * 256));
     ORB_TRY(S.level_passes.alloc((size_t)slots * 64));
     ORB_TRY(S.status.alloc((size_t)slots * 2));
