@@ -483,6 +483,7 @@ int orb_finalize(orb_ctx *ctx)
     c->lpos.release();
     c->rpos.release();
     c->splits.release();
+    c->chunk_comp.release();
     c->status.release();
     c->ids_dev.release();
     c->hcells.release();

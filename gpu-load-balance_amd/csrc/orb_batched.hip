@@ -242,10 +242,49 @@ __device__ inline int bin_of(const float *cand, float v, float lo, float scale, 
     return e;
 }
 
-template <int LE>
+// Histogram of one chunk over the cell's 255 candidate cuts.
+//   MODE 0: the chunk's keys.
+//   MODE 1: the same; each wave also packs the keys of its share that lie in
+//           the window [lo, hi] into its own quarter of the chunk's slots of
+//           ckeys (ballot ranks, no atomics) and records how many
+//           (comp[8q + w]); comp[8q + 4] gets the chunk's keys below lo.
+//           Later windows lie inside this one, so later rounds need only
+//           these keys.
+//   MODE 2: the packed keys; the keys below the packing window join bin 0.
+//           A chunk with no packed key in the current window skips the
+//           candidate set-up and writes bins 0 and 255 only.
+// The keys of chunk [b, e) that wave w of a histogram block handles: 16-byte
+// groups g = 64w + i + 256u (i < 64, u < 4) of the aligned body, plus the
+// misaligned head and tail for the last wave.  Their number, and an offset
+// such that the four waves' shares tile [b, e) exactly: each wave packs its
+// keys into its own share of the chunk's slots.
+__device__ inline void wave_share(uint32_t b, uint32_t e, int w, uint32_t &off, uint32_t &cnt)
+{
+    uint32_t ab = (b + 3u) & ~3u;
+    if (ab > e) ab = e;
+    uint32_t ae = e & ~3u;
+    if (ae < ab) ae = ab;
+    const uint32_t nq = (ae - ab) >> 2;
+    auto keys_of = [&](int wv) {
+        uint32_t k = 0;
+        for (int u = 0; u < kPerThread / 4; ++u) {
+            const uint32_t g0 = (uint32_t)(64 * wv + 256 * u);
+            k += g0 >= nq ? 0u : (nq - g0 < 64u ? nq - g0 : 64u);
+        }
+        k *= 4u;
+        if (wv == kBlock / 64 - 1) k += (ab - b) + (e - ae);
+        return k;
+    };
+    off = 0;
+    for (int v = 0; v < w; ++v) off += keys_of(v);
+    cnt = keys_of(w);
+}
+
+template <int LE, int MODE>
 __device__ inline void k_b_hist_chunk(uint32_t q, const CellRec *recs, const uint32_t *first,
                                       const uint32_t *chunk_cell, const float *x, const float *y,
-                                      const float *z, uint16_t *chunk_hist, uint16_t *chunk_tie)
+                                      const float *z, uint16_t *chunk_hist, uint16_t *chunk_tie,
+                                      float *ckeys, uint32_t *comp)
 {
     __shared__ float cand[kBins + 1];
     __shared__ uint32_t hist[4][kBins];
@@ -257,8 +296,36 @@ __device__ inline void k_b_hist_chunk(uint32_t q, const CellRec *recs, const uin
     const uint32_t b = r.begin + (q - first[cell]) * (uint32_t)kChunk;
     uint32_t e = b + (uint32_t)kChunk;
     if (e > r.end) e = r.end;
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const float lo = r.margin_l, hi = r.margin_r;
+    uint32_t below = 0, above = 0;
+
+    uint32_t share_off = 0, share_cnt = 0;
+    if (MODE != 0) wave_share(b, e, w, share_off, share_cnt);
+    if (MODE == 2) {
+        const uint32_t nc = comp[8 * (size_t)q + w];  // <= share_cnt
+        const float *seg = ckeys + b + share_off;
+        int any = 0;
+        for (uint32_t k = (uint32_t)lane; k < nc; k += 64) {
+            const float v = seg[k];
+            any |= (!(v < lo) && !(v > hi)) ? 1 : 0;
+        }
+        if (!__syncthreads_or(any)) {
+            for (uint32_t k = (uint32_t)lane; k < nc; k += 64) {
+                const float v = seg[k];
+                below += v < lo ? 1u : 0u;
+                above += v > hi ? 1u : 0u;
+            }
+            const uint32_t ba = block_sum(below | (above << 16), sm);
+            uint32_t sum = 0;
+            if (t == 0) sum = (ba & 0xffffu) + comp[8 * (size_t)q + 4];
+            if (t == kBins - 1) sum = ba >> 16;
+            chunk_hist[(size_t)q * kBins + t] = (uint16_t)sum;
+            chunk_tie[(size_t)q * kBins + t] = 0;
+            return;
+        }
+    }
+
     if (t >= 1) cand[t] = candidate(t, lo, hi);
     else cand[0] = -INFINITY;
     if (t == 0) cand[kBins] = INFINITY;
@@ -270,65 +337,92 @@ __device__ inline void k_b_hist_chunk(uint32_t q, const CellRec *recs, const uin
     __syncthreads();
     const float width = hi - lo;
     const float scale = width > 0.0f ? (float)kBins / width : 0.0f;
-    uint32_t *h = hist[t >> 6];
-    const float *key = r.axis == 0 ? x : (r.axis == 1 ? y : z);
+    uint32_t *h = hist[w];
     // keys below lo fall in bin 0 and keys above hi in bin 255 for both
-    // comparisons: count them in registers, bin the rest
-    uint32_t below = 0, above = 0;
+    // comparisons: counted in registers, the rest binned
+    const uint64_t lanes_below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    float *pack = ckeys + b + share_off;  // share_cnt slots, one per key of this wave
+    uint32_t packed = 0;  // wave-uniform
     // branch-free counters (an if/else chain over counters can make the
-    // compiler pick a counter by address, which puts them in scratch memory)
-    auto put = [&](float v) __attribute__((always_inline)) {
-        const bool bl = v < lo, ab = v > hi;
+    // compiler pick a counter by address, which puts them in scratch memory);
+    // every lane of the wave calls put() together (ballot), `ok` masks it
+    auto put = [&](float v, bool ok) __attribute__((always_inline)) {
+        const bool bl = ok && v < lo, abv = ok && v > hi;
+        const bool in = ok && !(v < lo) && !(v > hi);
         below += bl ? 1u : 0u;
-        above += ab ? 1u : 0u;
-        if (!(bl || ab)) {
+        above += abv ? 1u : 0u;
+        if (MODE == 1) {
+            const uint64_t m = __ballot(in);
+            if (m) {  // wave-uniform: most groups have no key in the window
+                if (in) pack[packed + (uint32_t)__popcll(m & lanes_below)] = v;
+                packed += (uint32_t)__popcll(m);
+            }
+        }
+        if (in) {
             int tj;
             atomicAdd(&h[bin_of<LE>(cand, v, lo, scale, tj)], 1u);
             if (tj) atomicAdd(&ties[tj], 1u);
         }
     };
-    uint32_t ab = (b + 3u) & ~3u;
-    if (ab > e) ab = e;
-    uint32_t ae = e & ~3u;
-    if (ae < ab) ae = ab;
-    if ((uint32_t)t < ab - b) put(key[b + t]);
-    if ((uint32_t)t < e - ae) put(key[ae + t]);
-    const float4 *k4 = reinterpret_cast<const float4 *>(key + ab);
-    const uint32_t nq = (ae - ab) >> 2;
-    float4 v[kPerThread / 4];
+    if (MODE == 2) {
+        const uint32_t nc = comp[8 * (size_t)q + w];
+        const float *seg = ckeys + b + share_off;
+        for (uint32_t k = (uint32_t)lane; k < nc; k += 64) put(seg[k], true);
+    } else {
+        const float *key = r.axis == 0 ? x : (r.axis == 1 ? y : z);
+        uint32_t ab = (b + 3u) & ~3u;
+        if (ab > e) ab = e;
+        uint32_t ae = e & ~3u;
+        if (ae < ab) ae = ab;
+        const float4 *k4 = reinterpret_cast<const float4 *>(key + ab);
+        const uint32_t nq = (ae - ab) >> 2;
+        float4 v[kPerThread / 4];
 #pragma unroll
-    for (int u = 0; u < kPerThread / 4; ++u) {
-        uint32_t j = (uint32_t)t + (uint32_t)u * kBlock;
-        if (j < nq) v[u] = k4[j];
-    }
-#pragma unroll
-    for (int u = 0; u < kPerThread / 4; ++u) {
-        uint32_t j = (uint32_t)t + (uint32_t)u * kBlock;
-        if (j < nq) {
-            put(v[u].x);
-            put(v[u].y);
-            put(v[u].z);
-            put(v[u].w);
+        for (int u = 0; u < kPerThread / 4; ++u) {
+            uint32_t j = (uint32_t)t + (uint32_t)u * kBlock;
+            if (j < nq) v[u] = k4[j];
+            else v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
+        // the misaligned head (<= 3 keys) and tail (<= 3 keys) go to the last
+        // wave, whose share of 16-byte groups is short by at least as many
+        const int ht = t - (kBlock - 64);
+        const bool has_h = ht >= 0 && (uint32_t)ht < ab - b;
+        const bool has_t = ht >= 3 && (uint32_t)(ht - 3) < e - ae;
+        const float vh = has_h ? key[b + ht] : 0.0f;
+        const float vt = has_t ? key[ae + (ht - 3)] : 0.0f;
+#pragma unroll
+        for (int u = 0; u < kPerThread / 4; ++u) {
+            const bool ok = (uint32_t)t + (uint32_t)u * kBlock < nq;
+            put(v[u].x, ok);
+            put(v[u].y, ok);
+            put(v[u].z, ok);
+            put(v[u].w, ok);
+        }
+        put(vh, has_h);
+        put(vt, has_t);
+        if (MODE == 1 && lane == 0) comp[8 * (size_t)q + w] = packed;
     }
     const uint32_t ba = block_sum(below | (above << 16), sm);  // each <= 4096
+    if (MODE == 1 && t == 0) comp[8 * (size_t)q + 4] = ba & 0xffffu;
     uint32_t sum = hist[0][t] + hist[1][t] + hist[2][t] + hist[3][t];  // <= 4096
-    if (t == 0) sum += ba & 0xffffu;
+    if (t == 0) sum += (ba & 0xffffu) + (MODE == 2 ? comp[8 * (size_t)q + 4] : 0u);
     if (t == kBins - 1) sum += ba >> 16;
     chunk_hist[(size_t)q * kBins + t] = (uint16_t)sum;
     chunk_tie[(size_t)q * kBins + t] = (uint16_t)ties[t];
 }
 
-template <int LE>
+template <int LE, int MODE>
 __global__ __launch_bounds__(kBlock) void k_b_hist(const CellRec *recs, int32_t nCells,
                                                    const uint32_t *first,
                                                    const uint32_t *chunk_cell, const float *x,
                                                    const float *y, const float *z,
-                                                   uint16_t *chunk_hist, uint16_t *chunk_tie)
+                                                   uint16_t *chunk_hist, uint16_t *chunk_tie,
+                                                   float *ckeys, uint32_t *comp)
 {
     const uint32_t total = first[nCells];
     for (uint32_t q = blockIdx.x; q < total; q += gridDim.x) {
-        k_b_hist_chunk<LE>(q, recs, first, chunk_cell, x, y, z, chunk_hist, chunk_tie);
+        k_b_hist_chunk<LE, MODE>(q, recs, first, chunk_cell, x, y, z, chunk_hist, chunk_tie,
+                                 ckeys, comp);
         __syncthreads();  // the next chunk reuses this block's LDS
     }
 }
@@ -465,7 +559,8 @@ __device__ inline RoundOut replay_round(const uint32_t *P, float lo, float hi, f
 // wave replays up to 8 passes of the bisection controller.
 __global__ __launch_bounds__(kBlock) void k_b_replay(CellRec *recs, int32_t nCells,
                                                      uint32_t *cell_hist, orb_cell *heap,
-                                                     uint32_t *trace, int32_t *level_passes)
+                                                     uint32_t *trace, int32_t *level_passes,
+                                                     uint8_t *exists, int32_t *status)
 {
     __shared__ uint32_t P[4][kBins + 1];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -527,25 +622,16 @@ __global__ __launch_bounds__(kBlock) void k_b_replay(CellRec *recs, int32_t nCel
     hc.cutMarginRight = R;
     hc.foundCut = found;
     record_level_passes(level_passes, r.lp, passes);
-}
-
-// A cell that hits the pass cap without ending its last round is finished
-// too: the cap rule is part of the replay.  Nothing to do here; the split
-// kernel reads the heap.
-__global__ void k_b_split(const CellRec *recs, int32_t nCells, orb_cell *heap, uint8_t *exists,
-                          int32_t *status)
-{
-    int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nCells) return;
-    const CellRec r = recs[c];
-    if (r.flags) return;
-    const int64_t hb = r.hb;
-    const orb_cell p = heap[hb + r.id];
+    if (!found && passes < ORB_MAX_PASSES) return;
+    // the cell is finished (cut found, or the pass cap with the last
+    // midpoint): split it now (orbit.cpp:235-250)
+    const orb_cell p = hc;
     orb_cell lc, rc;
     uint32_t *wl = reinterpret_cast<uint32_t *>(&lc);
     uint32_t *wr = reinterpret_cast<uint32_t *>(&rc);
 #pragma unroll
     for (int j = 0; j < (int)(sizeof(orb_cell) / 4); ++j) wl[j] = wr[j] = 0u;
+    const int64_t hb = r.hb;
     if (right_child_id(r.id) >= r.capr || !cell_split(p, lc, rc)) {
         if (atomicCAS(&status[r.st], 0, 1) == 0) status[r.st + 1] = r.id;
         return;
@@ -555,6 +641,7 @@ __global__ void k_b_split(const CellRec *recs, int32_t nCells, orb_cell *heap, u
     exists[hb + lc.id] = 1;
     exists[hb + rc.id] = 1;
 }
+
 
 // ---------------------------------------------------------------------------
 // LDS-resident subtree kernel
@@ -1022,26 +1109,31 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
     const unsigned gw = (unsigned)((nCells + 3) / 4);
     for (int r = 0; r < kRounds; ++r) {
         const int tb = timing_mark(c);
-        if (le)
-            hipLaunchKernelGGL(k_b_hist<1>, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
-                               c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(), chunk_hist,
-                               chunk_tie);
-        else
-            hipLaunchKernelGGL(k_b_hist<0>, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
-                               c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(), chunk_hist,
-                               chunk_tie);
+        // rounds: all keys, all keys + packing the window, then the packed keys
+        float *ckeys = reinterpret_cast<float *>(c.lpos.p);  // free until the partition
+        const int mode = r == 0 ? 0 : (r == 1 ? 1 : 2);
+#define ORB_HIST(L, M)                                                                           \
+    hipLaunchKernelGGL((k_b_hist<L, M>), dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,  \
+                       c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(), chunk_hist,          \
+                       chunk_tie, ckeys, c.chunk_comp.p)
+        if (le) {
+            if (mode == 0) ORB_HIST(1, 0);
+            else if (mode == 1) ORB_HIST(1, 1);
+            else ORB_HIST(1, 2);
+        } else {
+            if (mode == 0) ORB_HIST(0, 0);
+            else if (mode == 1) ORB_HIST(0, 1);
+            else ORB_HIST(0, 2);
+        }
+#undef ORB_HIST
         ORB_HIP(hipGetLastError());
         ORB_TRY(timing_pair(c, tb, kTimeCount));
         hipLaunchKernelGGL(k_b_hist_reduce, dim3(gr), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
                            c.cell_first.p, c.chunk_cell.p, chunk_hist, cell_hist);
         hipLaunchKernelGGL(k_b_replay, dim3(gw), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
-                           cell_hist, heap, trace, level_passes);
+                           cell_hist, heap, trace, level_passes, exists, status);
         ORB_HIP(hipGetLastError());
     }
-    unsigned gc = (unsigned)((nCells + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(k_b_split, dim3(gc), dim3(kBlock), 0, c.stream, c.recs.p, nCells, heap,
-                       exists, status);
-    ORB_HIP(hipGetLastError());
     const int tp = timing_mark(c);
     HistCounts hc;
     hc.hist = chunk_hist;

@@ -363,7 +363,9 @@ struct Ctx {
     DevBuf<uint64_t> chunk_cnt;      // [max_chunks+1] packed (nL | nR<<32)
     DevBuf<uint64_t> chunk_pre;      // [max_chunks+1]
     DevBuf<uint64_t> scan_partials;  // scan scratch
-    DevBuf<uint32_t> lpos, rpos;     // [n] stopper positions
+    DevBuf<uint32_t> lpos, rpos;     // [n] stopper positions (lpos also holds the
+                                     // batched form's packed keys between rounds)
+    DevBuf<uint32_t> chunk_comp;     // [max_chunks+1][8] packed keys per wave, keys below
     DevBuf<SplitRec> splits;         // [cells]
     DevBuf<int32_t> status;          // [2]: error flag, failing heap id
     DevBuf<uint32_t> ids_dev;        // [cells]

@@ -126,7 +126,7 @@ __global__ void k_scan_apply(const T *in, T *out, size_t m, const T *partials, c
         s += v[k];
     }
     T total;
-    T pre = block_exclusive_scan(s, sm, &total) + partials[blockIdx.x];
+    T pre = block_exclusive_scan(s, sm, &total) + (partials ? partials[blockIdx.x] : T(0));
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         if (base + k < mw) out[base + k] = pre;
@@ -144,6 +144,13 @@ static int scan_exclusive(Ctx &c, const T *in, T *out, size_t m, DevBuf<uint64_t
     size_t words = (np * sizeof(T) + sizeof(uint64_t) - 1) / sizeof(uint64_t);
     ORB_TRY(scratch.alloc(words));
     T *partials = reinterpret_cast<T *>(scratch.p);
+    if (np == 1) {
+        // one tile: the block scan alone (offset 0)
+        hipLaunchKernelGGL(k_scan_apply<T>, dim3(1), dim3(kBlock), 0, c.stream, in, out, m,
+                           (const T *)nullptr, lim);
+        ORB_HIP(hipGetLastError());
+        return ORB_OK;
+    }
     hipLaunchKernelGGL(k_scan_reduce<T>, dim3((unsigned)np), dim3(kBlock), 0, c.stream, in, m, partials,
                        lim);
     hipLaunchKernelGGL(k_scan_partials<T>, dim3(1), dim3(kBlock), 0, c.stream, partials, (int)np);
@@ -307,6 +314,7 @@ int ensure_level_capacity(Ctx &c, int64_t nCells)
     ORB_TRY(c.chunk_cnt.alloc(c.max_chunks + 1));
     ORB_TRY(c.chunk_pre.alloc(c.max_chunks + 1));
     ORB_TRY(c.splits.alloc(m));
+    ORB_TRY(c.chunk_comp.alloc(8 * ((size_t)c.max_chunks + 1)));
     ORB_TRY(c.ids_dev.alloc(m));
     ORB_TRY(c.hcells.alloc(m));
     ORB_TRY(c.hcnt.alloc(2 * m));

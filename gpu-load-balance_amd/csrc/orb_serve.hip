@@ -558,6 +558,7 @@ int orb_server_destroy(orb_server *o)
     c.chunk_pre.release();
     c.scan_partials.release();
     c.splits.release();
+    c.chunk_comp.release();
     c.ids_dev.release();
     c.hcells.release();
     c.hcnt.release();
