@@ -93,6 +93,16 @@ def host_threads():
     return max(1, n)
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
 def cpu_baseline(log2n, d, levels, threads=1):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
@@ -413,7 +423,8 @@ def main():
             "value": round(ms_mt * scale, 1), "unit": "ms", "cores": nt, "kind": "port",
             "sample": f"oracle (o=0 port, OpenMP over cells and over a large cell's keys; "
                       f"same result as 1 thread) full build of 2^{args.cpu_log2n} particles -> "
-                      f"{args.d} leaves took {ms_mt:.1f} ms on {nt} host threads; {note}",
+                      f"{args.d} leaves took {ms_mt:.1f} ms on {nt} host threads "
+                      f"({cpu_model()}, {os.cpu_count()} logical CPUs visible); {note}",
         }
         out["cpu_baseline_1_thread"] = {
             "value": round(ms_1t * scale, 1), "unit": "ms", "cores": 1, "kind": "port",

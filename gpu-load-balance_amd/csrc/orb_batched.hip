@@ -250,9 +250,7 @@ __device__ inline int bin_of(const float *cand, float v, float lo, float scale, 
 //           (comp[8q + w]); comp[8q + 4] gets the chunk's keys below lo.
 //           Later windows lie inside this one, so later rounds need only
 //           these keys.
-//   MODE 2: the packed keys; the keys below the packing window join bin 0.
-//           A chunk with no packed key in the current window skips the
-//           candidate set-up and writes bins 0 and 255 only.
+// Later rounds read the packed keys (k_b_hist_packed).
 // The keys of chunk [b, e) that wave w of a histogram block handles: 16-byte
 // groups g = 64w + i + 256u (i < 64, u < 4) of the aligned body, plus the
 // misaligned head and tail for the last wave.  Their number, and an offset
@@ -301,30 +299,7 @@ __device__ inline void k_b_hist_chunk(uint32_t q, const CellRec *recs, const uin
     uint32_t below = 0, above = 0;
 
     uint32_t share_off = 0, share_cnt = 0;
-    if (MODE != 0) wave_share(b, e, w, share_off, share_cnt);
-    if (MODE == 2) {
-        const uint32_t nc = comp[8 * (size_t)q + w];  // <= share_cnt
-        const float *seg = ckeys + b + share_off;
-        int any = 0;
-        for (uint32_t k = (uint32_t)lane; k < nc; k += 64) {
-            const float v = seg[k];
-            any |= (!(v < lo) && !(v > hi)) ? 1 : 0;
-        }
-        if (!__syncthreads_or(any)) {
-            for (uint32_t k = (uint32_t)lane; k < nc; k += 64) {
-                const float v = seg[k];
-                below += v < lo ? 1u : 0u;
-                above += v > hi ? 1u : 0u;
-            }
-            const uint32_t ba = block_sum(below | (above << 16), sm);
-            uint32_t sum = 0;
-            if (t == 0) sum = (ba & 0xffffu) + comp[8 * (size_t)q + 4];
-            if (t == kBins - 1) sum = ba >> 16;
-            chunk_hist[(size_t)q * kBins + t] = (uint16_t)sum;
-            chunk_tie[(size_t)q * kBins + t] = 0;
-            return;
-        }
-    }
+    if (MODE == 1) wave_share(b, e, w, share_off, share_cnt);
 
     if (t >= 1) cand[t] = candidate(t, lo, hi);
     else cand[0] = -INFINITY;
@@ -364,11 +339,7 @@ __device__ inline void k_b_hist_chunk(uint32_t q, const CellRec *recs, const uin
             if (tj) atomicAdd(&ties[tj], 1u);
         }
     };
-    if (MODE == 2) {
-        const uint32_t nc = comp[8 * (size_t)q + w];
-        const float *seg = ckeys + b + share_off;
-        for (uint32_t k = (uint32_t)lane; k < nc; k += 64) put(seg[k], true);
-    } else {
+    {
         const float *key = r.axis == 0 ? x : (r.axis == 1 ? y : z);
         uint32_t ab = (b + 3u) & ~3u;
         if (ab > e) ab = e;
@@ -405,7 +376,7 @@ __device__ inline void k_b_hist_chunk(uint32_t q, const CellRec *recs, const uin
     const uint32_t ba = block_sum(below | (above << 16), sm);  // each <= 4096
     if (MODE == 1 && t == 0) comp[8 * (size_t)q + 4] = ba & 0xffffu;
     uint32_t sum = hist[0][t] + hist[1][t] + hist[2][t] + hist[3][t];  // <= 4096
-    if (t == 0) sum += (ba & 0xffffu) + (MODE == 2 ? comp[8 * (size_t)q + 4] : 0u);
+    if (t == 0) sum += ba & 0xffffu;
     if (t == kBins - 1) sum += ba >> 16;
     chunk_hist[(size_t)q * kBins + t] = (uint16_t)sum;
     chunk_tie[(size_t)q * kBins + t] = (uint16_t)ties[t];
@@ -424,6 +395,104 @@ __global__ __launch_bounds__(kBlock) void k_b_hist(const CellRec *recs, int32_t 
         k_b_hist_chunk<LE, MODE>(q, recs, first, chunk_cell, x, y, z, chunk_hist, chunk_tie,
                                  ckeys, comp);
         __syncthreads();  // the next chunk reuses this block's LDS
+    }
+}
+
+// Rounds after the packing round: one wave per chunk (the chunk's packed
+// keys are few, so a block per chunk would spend its time on the chain of
+// dependent loads that find the chunk; four independent waves hide it).
+// A chunk with no packed key in the current window writes bins 0 and 255
+// only; otherwise the wave builds its own candidates and LDS histogram.
+template <int LE>
+__global__ __launch_bounds__(kBlock) void k_b_hist_packed(const CellRec *recs, int32_t nCells,
+                                                          const uint32_t *first,
+                                                          const uint32_t *chunk_cell,
+                                                          uint16_t *chunk_hist,
+                                                          uint16_t *chunk_tie, const float *ckeys,
+                                                          const uint32_t *comp)
+{
+    constexpr int kW = kBlock / 64;
+    __shared__ float cand[kW][kBins + 1];
+    __shared__ uint32_t hist[kW][kBins];
+    __shared__ uint32_t ties[kW][kBins];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t total = first[nCells];
+    for (uint32_t q = blockIdx.x * kW + (uint32_t)w; q < total; q += gridDim.x * kW) {
+        const uint32_t cell = chunk_cell[q];
+        const CellRec r = recs[cell];
+        if (r.found || r.flags || r.axis < 0 || r.axis > 2) continue;  // uniform per wave
+        const uint32_t b = r.begin + (q - first[cell]) * (uint32_t)kChunk;
+        uint32_t e = b + (uint32_t)kChunk;
+        if (e > r.end) e = r.end;
+        const float lo = r.margin_l, hi = r.margin_r;
+        uint32_t below = 0, above = 0;
+        bool any = false;
+        for (int sh = 0; sh < kW; ++sh) {
+            uint32_t off, cnt;
+            wave_share(b, e, sh, off, cnt);
+            const uint32_t nc = comp[8 * (size_t)q + sh];
+            for (uint32_t k = (uint32_t)lane; k < nc; k += 64) {
+                const float v = ckeys[b + off + k];
+                below += v < lo ? 1u : 0u;
+                above += v > hi ? 1u : 0u;
+                any = any || (!(v < lo) && !(v > hi));
+            }
+        }
+        below = wave_sum(below);
+        above = wave_sum(above);
+        below += comp[8 * (size_t)q + 4];  // keys below the packing window
+        uint16_t *hrow = chunk_hist + (size_t)q * kBins;
+        uint16_t *trow = chunk_tie + (size_t)q * kBins;
+        if (__ballot(any) == 0ull) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int bin = 4 * lane + i;
+                hrow[bin] = (uint16_t)(bin == 0 ? below : (bin == kBins - 1 ? above : 0u));
+                trow[bin] = 0;
+            }
+            continue;
+        }
+        float *cw = cand[w];
+        uint32_t *hw = hist[w];
+        uint32_t *tw = ties[w];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int j = 4 * lane + i;
+            cw[j] = j == 0 ? -INFINITY : candidate(j, lo, hi);
+            hw[j] = 0u;
+            tw[j] = 0u;
+        }
+        if (lane == 0) cw[kBins] = INFINITY;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        const float width = hi - lo;
+        const float scale = width > 0.0f ? (float)kBins / width : 0.0f;
+        for (int sh = 0; sh < kW; ++sh) {
+            uint32_t off, cnt;
+            wave_share(b, e, sh, off, cnt);
+            const uint32_t nc = comp[8 * (size_t)q + sh];
+            for (uint32_t k = (uint32_t)lane; k < nc; k += 64) {
+                const float v = ckeys[b + off + k];
+                if (!(v < lo) && !(v > hi)) {
+                    int tj;
+                    atomicAdd(&hw[bin_of<LE>(cw, v, lo, scale, tj)], 1u);
+                    if (tj) atomicAdd(&tw[tj], 1u);
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int bin = 4 * lane + i;
+            uint32_t v = hw[bin];
+            if (bin == 0) v += below;
+            if (bin == kBins - 1) v += above;
+            hrow[bin] = (uint16_t)v;
+            trow[bin] = (uint16_t)tw[bin];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();  // the rows are rewritten for the next chunk
     }
 }
 
@@ -1116,14 +1185,21 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
     hipLaunchKernelGGL((k_b_hist<L, M>), dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,  \
                        c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(), chunk_hist,          \
                        chunk_tie, ckeys, c.chunk_comp.p)
+        const unsigned gp = (gq + 3) / 4;  // a wave per chunk
         if (le) {
             if (mode == 0) ORB_HIST(1, 0);
             else if (mode == 1) ORB_HIST(1, 1);
-            else ORB_HIST(1, 2);
+            else
+                hipLaunchKernelGGL(k_b_hist_packed<1>, dim3(gp), dim3(kBlock), 0, c.stream, c.recs.p,
+                                   nCells, c.cell_first.p, c.chunk_cell.p, chunk_hist, chunk_tie,
+                                   ckeys, c.chunk_comp.p);
         } else {
             if (mode == 0) ORB_HIST(0, 0);
             else if (mode == 1) ORB_HIST(0, 1);
-            else ORB_HIST(0, 2);
+            else
+                hipLaunchKernelGGL(k_b_hist_packed<0>, dim3(gp), dim3(kBlock), 0, c.stream, c.recs.p,
+                                   nCells, c.cell_first.p, c.chunk_cell.p, chunk_hist, chunk_tie,
+                                   ckeys, c.chunk_comp.p);
         }
 #undef ORB_HIST
         ORB_HIP(hipGetLastError());

@@ -875,65 +875,68 @@ __global__ void k_hoare_final(const CellRec *recs, int32_t nCells, const SplitRe
 // Stopper counts of a chunk from the histogram round that found its cell's
 // cut at candidate j: keys below the cut = the bins below j (< or <=
 // histogram), and the keys equal to candidate j turn one count into the
-// other.  A cell that hit the pass cap was cut at a midpoint no round
-// counted: its chunks read their keys.
-__device__ inline void k_b_stopper_counts_chunk(uint32_t q, const CellRec *recs, int32_t nCells,
-                                                const uint32_t *first, const uint32_t *chunk_cell,
-                                                const float *x, const float *y, const float *z,
-                                                int le_only, const uint16_t *hist,
-                                                const uint16_t *tie, int hist_le,
-                                                uint64_t *chunk_cnt)
-{
-    __shared__ uint32_t sm[4];
-    uint32_t cell, b, e;
-    if (!chunk_range(q, first, nCells, chunk_cell, recs, cell, b, e)) return;
-    const CellRec r = recs[cell];
-    uint32_t lt = 0, le = 0;
-    if (r.found >= 2) {
-        const int j = r.found - 1;
-        const uint32_t t = threadIdx.x;  // kBlock == kBins
-        const uint32_t h = (int)t < j ? (uint32_t)hist[(size_t)q * kBins + t] : 0u;
-        const uint32_t below = block_sum(h, sm);
-        const uint32_t eq = tie[(size_t)q * kBins + (uint32_t)j];
-        if (hist_le) {
-            le = below;
-            lt = below - eq;
-        } else {
-            lt = below;
-            le = below + eq;
-        }
-    } else {
-        const int axis = r.axis < 0 ? 0 : (r.axis > 2 ? 2 : r.axis);
-        count_range(axis_ptr(x, y, z, axis), b, e, r.cut, lt, le);
-        const uint32_t packed = block_sum(lt | (le << 16), sm);
-        lt = packed & 0xffffu;
-        le = packed >> 16;
-    }
-    if (threadIdx.x == 0) {
-        const uint32_t len = e - b;
-        const uint32_t nl = le_only ? le : len - lt;
-        const uint32_t nr = le_only ? len - le : le;
-        chunk_cnt[q] = (uint64_t)nl | ((uint64_t)nr << 32);
-    }
-}
-
+// other.  One wave per chunk: the row is 512 bytes, so the work is the chain
+// of loads that find it, and four independent waves per block hide it.  A
+// cell that hit the pass cap was cut at a midpoint no round counted: its
+// chunks read their keys (the whole wave).
 __global__ __launch_bounds__(kBlock) void k_b_stopper_counts(
     const CellRec *recs, int32_t nCells, const uint32_t *first, const uint32_t *chunk_cell,
     const float *x, const float *y, const float *z, int le_only, const uint16_t *hist,
     const uint16_t *tie, int hist_le, uint64_t *chunk_cnt)
 {
+    constexpr int kW = kBlock / 64;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t total = first[nCells];
-    for (uint32_t q = blockIdx.x; q < total; q += gridDim.x) {
-        k_b_stopper_counts_chunk(q, recs, nCells, first, chunk_cell, x, y, z, le_only, hist, tie,
-                                 hist_le, chunk_cnt);
-        __syncthreads();  // the next chunk reuses this block's LDS
+    for (uint32_t q = blockIdx.x * kW + (uint32_t)w; q < total; q += gridDim.x * kW) {
+        const uint32_t cell = chunk_cell[q];
+        const CellRec r = recs[cell];
+        const uint32_t b = r.begin + (q - first[cell]) * (uint32_t)kChunk;
+        uint32_t e = b + (uint32_t)kChunk;
+        if (e > r.end) e = r.end;
+        uint32_t lt = 0, le = 0;
+        if (r.found >= 2) {
+            const int j = r.found - 1;
+            const uint16_t *row = hist + (size_t)q * kBins;
+            uint32_t s = 0;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int bin = 4 * lane + i;
+                if (bin < j) s += row[bin];
+            }
+            const uint32_t below = wave_sum(s);
+            const uint32_t eq = tie[(size_t)q * kBins + (uint32_t)j];
+            if (hist_le) {
+                le = below;
+                lt = below - eq;
+            } else {
+                lt = below;
+                le = below + eq;
+            }
+        } else {
+            const int axis = r.axis < 0 ? 0 : (r.axis > 2 ? 2 : r.axis);
+            const float *key = axis_ptr(x, y, z, axis);
+            const float cut = r.cut;
+            for (uint32_t p = b + (uint32_t)lane; p < e; p += 64) {
+                const float v = key[p];
+                lt += v < cut ? 1u : 0u;
+                le += v <= cut ? 1u : 0u;
+            }
+            lt = wave_sum(lt);
+            le = wave_sum(le);
+        }
+        if (lane == 0) {
+            const uint32_t len = e - b;
+            const uint32_t nl = le_only ? le : len - lt;
+            const uint32_t nr = le_only ? len - le : le;
+            chunk_cnt[q] = (uint64_t)nl | ((uint64_t)nr << 32);
+        }
     }
 }
 
 int launch_stopper_counts(Ctx &c, int32_t nCells, bool le_only, const HistCounts *hc)
 {
     if (hc && hc->hist)
-        hipLaunchKernelGGL(k_b_stopper_counts, dim3(chunk_grid(c)), dim3(kBlock), 0, c.stream,
+        hipLaunchKernelGGL(k_b_stopper_counts, dim3((chunk_grid(c) + 3) / 4), dim3(kBlock), 0, c.stream,
                            c.recs.p, nCells, c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(),
                            le_only ? 1 : 0, hc->hist, hc->tie, hc->le, c.chunk_cnt.p);
     else
