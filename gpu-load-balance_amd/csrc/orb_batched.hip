@@ -251,33 +251,6 @@ __device__ inline int bin_of(const float *cand, float v, float lo, float scale, 
 //           Later windows lie inside this one, so later rounds need only
 //           these keys.
 // Later rounds read the packed keys (k_b_hist_packed).
-// The keys of chunk [b, e) that wave w of a histogram block handles: 16-byte
-// groups g = 64w + i + 256u (i < 64, u < 4) of the aligned body, plus the
-// misaligned head and tail for the last wave.  Their number, and an offset
-// such that the four waves' shares tile [b, e) exactly: each wave packs its
-// keys into its own share of the chunk's slots.
-__device__ inline void wave_share(uint32_t b, uint32_t e, int w, uint32_t &off, uint32_t &cnt)
-{
-    uint32_t ab = (b + 3u) & ~3u;
-    if (ab > e) ab = e;
-    uint32_t ae = e & ~3u;
-    if (ae < ab) ae = ab;
-    const uint32_t nq = (ae - ab) >> 2;
-    auto keys_of = [&](int wv) {
-        uint32_t k = 0;
-        for (int u = 0; u < kPerThread / 4; ++u) {
-            const uint32_t g0 = (uint32_t)(64 * wv + 256 * u);
-            k += g0 >= nq ? 0u : (nq - g0 < 64u ? nq - g0 : 64u);
-        }
-        k *= 4u;
-        if (wv == kBlock / 64 - 1) k += (ab - b) + (e - ae);
-        return k;
-    };
-    off = 0;
-    for (int v = 0; v < w; ++v) off += keys_of(v);
-    cnt = keys_of(w);
-}
-
 template <int LE, int MODE>
 __device__ inline void k_b_hist_chunk(uint32_t q, const CellRec *recs, const uint32_t *first,
                                       const uint32_t *chunk_cell, const float *x, const float *y,
@@ -1215,6 +1188,8 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
     hc.hist = chunk_hist;
     hc.tie = chunk_tie;
     hc.le = le;
+    hc.pack = reinterpret_cast<const float *>(c.lpos.p);
+    hc.comp = c.chunk_comp.p;
     if (c.part == ORB_PART_HOARE) ORB_TRY(launch_hoare(c, nCells, status, &hc));
     else ORB_TRY(launch_stable_le(c, nCells, status, full_cover, &hc));
     ORB_TRY(timing_pair(c, tp, kTimePartition));

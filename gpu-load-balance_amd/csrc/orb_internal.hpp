@@ -436,6 +436,8 @@ struct HistCounts {
     const uint16_t *hist = nullptr;   // [chunks][256] bin counts
     const uint16_t *tie = nullptr;    // [chunks][256] keys equal to candidate j
     int le = 0;                       // bins were built with <= (else <)
+    const float *pack = nullptr;      // keys packed by the second round, per chunk share
+    const uint32_t *comp = nullptr;   // [chunks][8] packed counts per share, keys below
 };
 int launch_hoare(Ctx &c, int32_t nCells, int32_t *status = nullptr,
                  const HistCounts *hc = nullptr);

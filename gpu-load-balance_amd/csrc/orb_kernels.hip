@@ -882,7 +882,8 @@ __global__ void k_hoare_final(const CellRec *recs, int32_t nCells, const SplitRe
 __global__ __launch_bounds__(kBlock) void k_b_stopper_counts(
     const CellRec *recs, int32_t nCells, const uint32_t *first, const uint32_t *chunk_cell,
     const float *x, const float *y, const float *z, int le_only, const uint16_t *hist,
-    const uint16_t *tie, int hist_le, uint64_t *chunk_cnt)
+    const uint16_t *tie, int hist_le, const float *pack, const uint32_t *comp,
+    uint64_t *chunk_cnt)
 {
     constexpr int kW = kBlock / 64;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -912,6 +913,24 @@ __global__ __launch_bounds__(kBlock) void k_b_stopper_counts(
                 lt = below;
                 le = below + eq;
             }
+        } else if (r.passes > kHistK) {
+            // the cell went through the packing round: its keys outside the
+            // packing window lie below or above every later cut, so the
+            // packed keys and the count below decide
+            const float cut = r.cut;
+            for (int sh = 0; sh < kW; ++sh) {
+                uint32_t off, cnt;
+                wave_share(b, e, sh, off, cnt);
+                const uint32_t nc = comp[8 * (size_t)q + sh];
+                for (uint32_t k = (uint32_t)lane; k < nc; k += 64) {
+                    const float v = pack[b + off + k];
+                    lt += v < cut ? 1u : 0u;
+                    le += v <= cut ? 1u : 0u;
+                }
+            }
+            const uint32_t bel = comp[8 * (size_t)q + 4];
+            lt = wave_sum(lt) + bel;
+            le = wave_sum(le) + bel;
         } else {
             const int axis = r.axis < 0 ? 0 : (r.axis > 2 ? 2 : r.axis);
             const float *key = axis_ptr(x, y, z, axis);
@@ -938,7 +957,8 @@ int launch_stopper_counts(Ctx &c, int32_t nCells, bool le_only, const HistCounts
     if (hc && hc->hist)
         hipLaunchKernelGGL(k_b_stopper_counts, dim3((chunk_grid(c) + 3) / 4), dim3(kBlock), 0, c.stream,
                            c.recs.p, nCells, c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(),
-                           le_only ? 1 : 0, hc->hist, hc->tie, hc->le, c.chunk_cnt.p);
+                           le_only ? 1 : 0, hc->hist, hc->tie, hc->le, hc->pack, hc->comp,
+                           c.chunk_cnt.p);
     else
         hipLaunchKernelGGL(k_stopper_counts, dim3(chunk_grid(c)), dim3(kBlock), 0,
                            c.stream, c.recs.p, nCells, c.cell_first.p, c.chunk_cell.p, c.x(), c.y(),

@@ -157,3 +157,16 @@ def test_explicit_shapes_are_exact(orb, oracle, shape):
     ctx.set_tuning(None)
     assert ctx.tuning()["subtree_max_keys"] == -1
     ctx.close()
+
+
+@pytest.mark.parametrize("grid_bits", [13, 16])
+def test_batched_capped_and_found_cells(orb, oracle, grid_bits):
+    """Keys on a 2^-13 / 2^-16 grid: some cells end at the 32-pass cap (cut at
+    a midpoint no round counted), others are found, some at a candidate equal
+    to its neighbour; the partition's counts must be exact in every case."""
+    rng = np.random.default_rng(12)
+    n = (1 << 20) + 3
+    x, y, z = ((rng.integers(0, 1 << grid_bits, n) / (1 << grid_bits) - 0.5).astype(np.float32)
+               for _ in range(3))
+    g, r = run_both(orb, oracle, x, y, z, 2048, 1, form=BATCHED)
+    assert_same_build(g, r)
