@@ -271,6 +271,29 @@ __device__ inline void k_b_hist_chunk(uint32_t q, const CellRec *recs, const uin
     const float lo = r.margin_l, hi = r.margin_r;
     uint32_t below = 0, above = 0;
 
+    // the keys first: their memory latency overlaps the candidate set-up
+    const float *key = r.axis == 0 ? x : (r.axis == 1 ? y : z);
+    uint32_t ab = (b + 3u) & ~3u;
+    if (ab > e) ab = e;
+    uint32_t ae = e & ~3u;
+    if (ae < ab) ae = ab;
+    const float4 *k4 = reinterpret_cast<const float4 *>(key + ab);
+    const uint32_t nq = (ae - ab) >> 2;
+    float4 v[kPerThread / 4];
+#pragma unroll
+    for (int u = 0; u < kPerThread / 4; ++u) {
+        uint32_t j = (uint32_t)t + (uint32_t)u * kBlock;
+        if (j < nq) v[u] = k4[j];
+        else v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    // the misaligned head (<= 3 keys) and tail (<= 3 keys) go to the last
+    // wave, whose share of 16-byte groups is short by at least as many
+    const int ht = t - (kBlock - 64);
+    const bool has_h = ht >= 0 && (uint32_t)ht < ab - b;
+    const bool has_t = ht >= 3 && (uint32_t)(ht - 3) < e - ae;
+    const float vh = has_h ? key[b + ht] : 0.0f;
+    const float vt = has_t ? key[ae + (ht - 3)] : 0.0f;
+
     uint32_t share_off = 0, share_cnt = 0;
     if (MODE == 1) wave_share(b, e, w, share_off, share_cnt);
 
@@ -313,27 +336,6 @@ __device__ inline void k_b_hist_chunk(uint32_t q, const CellRec *recs, const uin
         }
     };
     {
-        const float *key = r.axis == 0 ? x : (r.axis == 1 ? y : z);
-        uint32_t ab = (b + 3u) & ~3u;
-        if (ab > e) ab = e;
-        uint32_t ae = e & ~3u;
-        if (ae < ab) ae = ab;
-        const float4 *k4 = reinterpret_cast<const float4 *>(key + ab);
-        const uint32_t nq = (ae - ab) >> 2;
-        float4 v[kPerThread / 4];
-#pragma unroll
-        for (int u = 0; u < kPerThread / 4; ++u) {
-            uint32_t j = (uint32_t)t + (uint32_t)u * kBlock;
-            if (j < nq) v[u] = k4[j];
-            else v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-        // the misaligned head (<= 3 keys) and tail (<= 3 keys) go to the last
-        // wave, whose share of 16-byte groups is short by at least as many
-        const int ht = t - (kBlock - 64);
-        const bool has_h = ht >= 0 && (uint32_t)ht < ab - b;
-        const bool has_t = ht >= 3 && (uint32_t)(ht - 3) < e - ae;
-        const float vh = has_h ? key[b + ht] : 0.0f;
-        const float vt = has_t ? key[ae + (ht - 3)] : 0.0f;
 #pragma unroll
         for (int u = 0; u < kPerThread / 4; ++u) {
             const bool ok = (uint32_t)t + (uint32_t)u * kBlock < nq;
