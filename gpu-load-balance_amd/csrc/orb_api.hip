@@ -429,8 +429,7 @@ int orb_init(orb_ctx **out, int64_t nParticles, int32_t d, bool generate, orb_me
     // floats past the z array
     if ((rc = c.pos.alloc(3 * (size_t)c.stride + 64)) < 0) return fail(rc);
     if ((rc = c.ranges.alloc(2 * (size_t)c.cap)) < 0) return fail(rc);
-    if ((rc = c.lpos.alloc((size_t)c.stride)) < 0) return fail(rc);
-    if ((rc = c.rpos.alloc((size_t)c.stride)) < 0) return fail(rc);
+    if ((rc = c.packed.alloc((size_t)c.stride)) < 0) return fail(rc);
     if ((rc = c.status.alloc(2)) < 0) return fail(rc);
     if ((rc = c.hstatus.alloc(2)) < 0) return fail(rc);
     if ((rc = c.hranges.alloc(2 * (size_t)c.cap)) < 0) return fail(rc);
@@ -480,8 +479,9 @@ int orb_finalize(orb_ctx *ctx)
     c->chunk_cnt.release();
     c->chunk_pre.release();
     c->scan_partials.release();
-    c->lpos.release();
-    c->rpos.release();
+    c->packed.release();
+    c->masks.release();
+    c->chunk_aux.release();
     c->splits.release();
     c->chunk_comp.release();
     c->status.release();

@@ -1154,7 +1154,7 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
     for (int r = 0; r < kRounds; ++r) {
         const int tb = timing_mark(c);
         // rounds: all keys, all keys + packing the window, then the packed keys
-        float *ckeys = reinterpret_cast<float *>(c.lpos.p);  // free until the partition
+        float *ckeys = c.packed.p;
         const int mode = r == 0 ? 0 : (r == 1 ? 1 : 2);
 #define ORB_HIST(L, M)                                                                           \
     hipLaunchKernelGGL((k_b_hist<L, M>), dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,  \
@@ -1190,9 +1190,9 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
     hc.hist = chunk_hist;
     hc.tie = chunk_tie;
     hc.le = le;
-    hc.pack = reinterpret_cast<const float *>(c.lpos.p);
+    hc.pack = c.packed.p;
     hc.comp = c.chunk_comp.p;
-    if (c.part == ORB_PART_HOARE) ORB_TRY(launch_hoare(c, nCells, status, &hc));
+    if (c.part == ORB_PART_HOARE) ORB_TRY(launch_hoare(c, nCells, status));
     else ORB_TRY(launch_stable_le(c, nCells, status, full_cover, &hc));
     ORB_TRY(timing_pair(c, tp, kTimePartition));
     return ORB_OK;

@@ -28,6 +28,7 @@ constexpr int kPerThread = 16;              // keys per thread per chunk
 constexpr int kChunk = kBlock * kPerThread; // 4096 keys = 16 KiB per chunk
 constexpr int kHistK = 8;                   // bisection steps per histogram round
 constexpr int kBins = 1 << kHistK;          // 256 bins (255 candidate cuts)
+constexpr int kMaskWords = kChunk / 64;     // 64-bit stopper-mask words per chunk and side
 static_assert(kBins == kBlock, "one thread per histogram bin");
 
 // ---------------------------------------------------------------------------
@@ -363,8 +364,10 @@ struct Ctx {
     DevBuf<uint64_t> chunk_cnt;      // [max_chunks+1] packed (nL | nR<<32)
     DevBuf<uint64_t> chunk_pre;      // [max_chunks+1]
     DevBuf<uint64_t> scan_partials;  // scan scratch
-    DevBuf<uint32_t> lpos, rpos;     // [n] stopper positions (lpos also holds the
-                                     // batched form's packed keys between rounds)
+    DevBuf<float> packed;            // [n] batched form: window keys packed by the second
+                                     // histogram round, per chunk share
+    DevBuf<uint64_t> masks;          // [max_chunks][2][64] Hoare stopper masks per chunk
+    DevBuf<uint32_t> chunk_aux;      // [max_chunks+1] Hoare swap: each unit's partner chunk
     DevBuf<uint32_t> chunk_comp;     // [max_chunks+1][8] packed keys per wave, keys below
     DevBuf<SplitRec> splits;         // [cells]
     DevBuf<int32_t> status;          // [2]: error flag, failing heap id
@@ -439,8 +442,7 @@ struct HistCounts {
     const float *pack = nullptr;      // keys packed by the second round, per chunk share
     const uint32_t *comp = nullptr;   // [chunks][8] packed counts per share, keys below
 };
-int launch_hoare(Ctx &c, int32_t nCells, int32_t *status = nullptr,
-                 const HistCounts *hc = nullptr);
+int launch_hoare(Ctx &c, int32_t nCells, int32_t *status = nullptr);
 int launch_stopper_counts(Ctx &c, int32_t nCells, bool le_only, const HistCounts *hc = nullptr);
 int invalidate_stamps(Ctx &c, int32_t nCells);
 int launch_stable_le(Ctx &c, int32_t nCells, int32_t *status = nullptr, bool full_cover = false,

@@ -313,6 +313,8 @@ int ensure_level_capacity(Ctx &c, int64_t nCells)
     ORB_TRY(c.chunk_ltle.alloc(c.max_chunks + 1));
     ORB_TRY(c.chunk_cnt.alloc(c.max_chunks + 1));
     ORB_TRY(c.chunk_pre.alloc(c.max_chunks + 1));
+    ORB_TRY(c.masks.alloc(2 * (size_t)kMaskWords * (size_t)c.max_chunks));
+    ORB_TRY(c.chunk_aux.alloc(c.max_chunks + 1));
     ORB_TRY(c.splits.alloc(m));
     ORB_TRY(c.chunk_comp.alloc(8 * ((size_t)c.max_chunks + 1)));
     ORB_TRY(c.ids_dev.alloc(m));
@@ -535,26 +537,9 @@ int launch_count(Ctx &c, int32_t nCells, bool skip_found, const float *axis_over
     return ORB_OK;
 }
 
-// ---------------------------------------------------------------------------
-// Hoare partition as scans (partition.cpp:28-60).
-//
-// For a cell of n keys and cut c let the L-stoppers be the positions with
-// key >= c, left to right L_0 < L_1 < ..., and the R-stoppers the positions
-// with key <= c, right to left R_0 > R_1 > ....  The serial loop swaps
-// (L_t, R_t) for every t < K = #{t : L_t < R_t} (these transpositions are
-// disjoint), stops with i = min(L_K, R_{K-1}) (i = L_0 when K == 0), and then
-// swaps rows i and n-1.  The children are [0, i) and [i, n).
-//
-//   P1a  per chunk: #L and #R; taken from the last count pass when the cell
-//        was counted at this cut (stamp), else one key read
-//   scan per-chunk counts -> each chunk's rank offsets inside its cell
-//   P1b  per chunk: block scan of the flags, stopper positions staged in LDS
-//        and written contiguously to lpos / rpos at the cell's base + rank
-//   P2   per cell (one wave): K by a 64-way search on lpos[t] < rpos[TR-1-t],
-//        i, the children's ranges, undefined-input check
-//   P3   per pair t < K: swap x, y, z of lpos[t] and rpos[TR-1-t]
-//   P4   per cell: swap rows i and n-1
-// ---------------------------------------------------------------------------
+// Per-chunk counts of the keys going left / right (stable partition):
+// taken from the last count pass when the cell was counted at this cut
+// (stamp), else one key read.
 __device__ inline void k_stopper_counts_chunk(uint32_t q, const CellRec *recs, int32_t nCells,
                                                            const uint32_t *first,
                                                            const uint32_t *chunk_cell,
@@ -609,95 +594,204 @@ __global__ __launch_bounds__(kBlock) void k_stopper_counts(const CellRec *recs, 
 }
 
 
-// Each wave owns a contiguous quarter of the chunk (1024 keys, 16 per lane in
-// registers, one coalesced dword load each).  Pass 1 counts its stoppers with
-// ballots; the four wave counts give each wave its rank offset; pass 2 writes
-// each stopper's position at its rank: lanes of one ballot write consecutive
-// entries, so the stores stay contiguous without LDS staging.
-__device__ inline void k_stopper_positions_chunk(uint32_t q, const CellRec *recs, int32_t nCells,
-                                                 const uint32_t *first,
-                                                 const uint32_t *chunk_cell,
-                                                 const uint64_t *chunk_pre,
-                                                 const float *x, const float *y,
-                                                 const float *z, uint32_t *lpos,
-                                                 uint32_t *rpos)
-{
-    constexpr int kSeg = kChunk / (kBlock / 64);  // 1024 keys per wave
-    constexpr int kIt = kSeg / 64;                 // 16 per lane
-    __shared__ uint32_t s_cnt[kBlock / 64];
-    uint32_t cell, b, e;
-    if (!chunk_range(q, first, nCells, chunk_cell, recs, cell, b, e)) return;
-    const CellRec r = recs[cell];
-    const float cut = r.cut;
-    const float *key = axis_ptr(x, y, z, r.axis);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const uint32_t sb = b + (uint32_t)(w * kSeg);
-    float v[kIt];
-#pragma unroll
-    for (int j = 0; j < kIt; ++j) {
-        const uint32_t p = sb + (uint32_t)(64 * j + lane);
-        v[j] = p < e ? key[p] : 0.0f;
-    }
-    uint32_t nl = 0, nr = 0;
-#pragma unroll
-    for (int j = 0; j < kIt; ++j) {
-        const bool in = sb + (uint32_t)(64 * j + lane) < e;
-        nl += (uint32_t)__popcll(__ballot(in && v[j] >= cut));
-        nr += (uint32_t)__popcll(__ballot(in && v[j] <= cut));
-    }
-    if (lane == 0) s_cnt[w] = nl | (nr << 16);  // each <= 1024
-    __syncthreads();
-    uint32_t ol = 0, orr = 0;
-    for (int u = 0; u < w; ++u) {
-        ol += s_cnt[u] & 0xffffu;
-        orr += s_cnt[u] >> 16;
-    }
-    const uint64_t base = chunk_pre[q] - chunk_pre[first[cell]];
-    uint32_t *dl = lpos + r.begin + (uint32_t)(base & 0xffffffffull) + ol;
-    uint32_t *dr = rpos + r.begin + (uint32_t)(base >> 32) + orr;
-    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    const uint32_t off = sb - r.begin;  // cell-local position of this wave's first key
-#pragma unroll
-    for (int j = 0; j < kIt; ++j) {
-        const uint32_t k = (uint32_t)(64 * j + lane);
-        const bool in = sb + k < e;
-        const bool isL = in && v[j] >= cut, isR = in && v[j] <= cut;
-        const uint64_t mL = __ballot(isL), mR = __ballot(isR);
-        if (isL) dl[__popcll(mL & below)] = off + k;
-        if (isR) dr[__popcll(mR & below)] = off + k;
-        dl += __popcll(mL);
-        dr += __popcll(mR);
-    }
-}
+// ---------------------------------------------------------------------------
+// Hoare partition over stopper masks (partition.cpp:28-60).
+//
+// For a cell of n keys and cut c the L-stoppers are the positions whose key
+// is not < c (where `while (key[i] < cut) i++` stops), left to right
+// L_0 < L_1 < ..., and the R-stoppers those whose key is not > c, right to
+// left R_0 > R_1 > ....  The serial loop swaps (L_t, R_t) for every
+// t < K = #{t : L_t < R_t} (disjoint transpositions), stops with
+// i = min(L_K, R_{K-1}) (i = L_0 when K == 0) and then swaps rows i and
+// n-1.  The children are [0, i) and [i, n).
+//
+//   masks  per chunk: the two stopper bitmasks (bit j of word k = chunk
+//          position 64k + j, one ballot per word) and their popcounts
+//   scan   per-chunk counts -> each chunk's index offsets inside its cell
+//   split  per cell (one wave): K from the chunk offsets and one chunk's
+//          masks, then i
+//   swap   per chunk of the cell's sparser side: the positions of its pairs
+//          selected from the masks into LDS, then x, y, z swapped
+//   final  per cell: swap rows i and n-1
+//
+// The masks are 2 bits per key, so the positions never go through HBM.
+// ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(kBlock) void k_stopper_positions(const CellRec *recs, int32_t nCells,
-                                                              const uint32_t *first,
-                                                              const uint32_t *chunk_cell,
-                                                              const uint64_t *chunk_pre,
-                                                              const float *x, const float *y,
-                                                              const float *z, uint32_t *lpos,
-                                                              uint32_t *rpos)
+__global__ __launch_bounds__(kBlock) void k_hoare_masks(const CellRec *recs, int32_t nCells,
+                                                        const uint32_t *first,
+                                                        const uint32_t *chunk_cell,
+                                                        const float *x, const float *y,
+                                                        const float *z, uint64_t *masks,
+                                                        uint64_t *chunk_cnt)
 {
+    constexpr int kW = kBlock / 64;
+    constexpr int kPerWave = kMaskWords / kW;  // 16 words per wave
+    __shared__ uint32_t s_cnt[2][kW];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t total = first[nCells];
     for (uint32_t q = blockIdx.x; q < total; q += gridDim.x) {
-        k_stopper_positions_chunk(q, recs, nCells, first, chunk_cell, chunk_pre, x, y, z, lpos, rpos);
-        __syncthreads();  // the next chunk reuses this block's LDS
+        uint32_t cell, b, e;
+        chunk_range(q, first, nCells, chunk_cell, recs, cell, b, e);
+        const CellRec r = recs[cell];
+        const float *key = axis_ptr(x, y, z, r.axis);
+        const float cut = r.cut;
+        const uint32_t sb = b + (uint32_t)(w * kPerWave * 64);
+        float v[kPerWave];
+#pragma unroll
+        for (int j = 0; j < kPerWave; ++j) {
+            const uint32_t p = sb + (uint32_t)(64 * j + lane);
+            v[j] = p < e ? key[p] : 0.0f;
+        }
+        uint64_t mineL = 0ull, mineR = 0ull;
+        uint32_t nl = 0, nr = 0;
+#pragma unroll
+        for (int j = 0; j < kPerWave; ++j) {
+            const bool in = sb + (uint32_t)(64 * j + lane) < e;
+            const uint64_t mL = __ballot(in && !(v[j] < cut));
+            const uint64_t mR = __ballot(in && !(v[j] > cut));
+            nl += (uint32_t)__popcll(mL);
+            nr += (uint32_t)__popcll(mR);
+            mineL = lane == j ? mL : mineL;
+            mineR = lane == j ? mR : mineR;
+        }
+        uint64_t *dst = masks + (size_t)q * (2 * kMaskWords) + w * kPerWave;
+        if (lane < kPerWave) {
+            dst[lane] = mineL;
+            dst[kMaskWords + lane] = mineR;
+        }
+        if (lane == 0) {
+            s_cnt[0][w] = nl;
+            s_cnt[1][w] = nr;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t L = 0, R = 0;
+            for (int u = 0; u < kW; ++u) {
+                L += s_cnt[0][u];
+                R += s_cnt[1][u];
+            }
+            chunk_cnt[q] = (uint64_t)L | ((uint64_t)R << 32);
+        }
+        __syncthreads();  // the next chunk reuses s_cnt
     }
 }
-
 
 __device__ inline void report_error(int32_t *status, int32_t id)
 {
     if (atomicCAS(&status[0], 0, 1) == 0) status[1] = id;
 }
 
-// one wave64 per cell
+__device__ inline uint32_t lo32(uint64_t v) { return (uint32_t)v; }
+__device__ inline uint32_t hi32(uint64_t v) { return (uint32_t)(v >> 32); }
+
+// Last k in [lo, hi) with pred(k), given pred(lo) and pred true then false
+// over [lo, hi): a 64-way search, one ballot per step.  Every lane calls
+// pred on every step (with lo where its probe is out of range), so pred
+// may shuffle.
+template <class Pred>
+__device__ inline uint32_t wave_last_true(uint32_t lo, uint32_t hi, Pred pred)
+{
+    const int lane = threadIdx.x & 63;
+    while (hi - lo > 1u) {
+        const uint32_t span = hi - lo;
+        const bool dense = span <= 65u;
+        auto probe = [&](uint32_t j) {
+            return dense ? lo + 1u + j : lo + (uint32_t)(((uint64_t)(j + 1u) * span) / 65u);
+        };
+        const uint32_t k = probe((uint32_t)lane);
+        const bool valid = k < hi;
+        const bool pr = pred(valid ? k : lo) && valid;
+        const uint64_t m = __ballot(pr);
+        if (m == 0ull) {
+            hi = probe(0u);
+        } else {
+            const uint32_t j = 63u - (uint32_t)__clzll((long long)m);
+            const uint32_t kn = j < 63u ? probe(j + 1u) : hi;
+            lo = probe(j);
+            hi = kn < hi ? kn : hi;
+        }
+    }
+    return lo;
+}
+
+// A chunk's mask of one side, word `lane` in each lane, with the exclusive
+// prefix of the words' popcounts.
+__device__ inline void load_mask(const uint64_t *masks, uint32_t q, int side, uint64_t &word,
+                                 uint32_t &ex)
+{
+    const int lane = threadIdx.x & 63;
+    word = masks[(size_t)q * (2 * kMaskWords) + (size_t)side * kMaskWords + (uint32_t)lane];
+    const uint32_t c = (uint32_t)__popcll(word);
+    ex = wave_inclusive_scan(c) - c;
+}
+
+// Chunk position (0..4095) of the set bit of rank r of a mask held as by
+// load_mask.  Each lane asks for its own r (below the mask's popcount); all
+// lanes call (shuffles).
+__device__ inline uint32_t wave_select(uint64_t word, uint32_t ex, uint32_t r)
+{
+    int k = 0;  // the last word whose prefix is <= r holds the bit
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) {
+        const uint32_t e = (uint32_t)__shfl((int)ex, k + s, 64);
+        if (e <= r) k += s;
+    }
+    const uint32_t wl = (uint32_t)__shfl((int)(uint32_t)word, k, 64);
+    const uint32_t wh = (uint32_t)__shfl((int)(uint32_t)(word >> 32), k, 64);
+    uint32_t rr = r - (uint32_t)__shfl((int)ex, k, 64);
+    uint64_t wd = ((uint64_t)wh << 32) | wl;
+    uint32_t bit = 0;
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) {
+        const uint32_t c = (uint32_t)__popcll(wd & ((1ull << s) - 1ull));
+        if (rr >= c) {
+            rr -= c;
+            bit += (uint32_t)s;
+            wd >>= s;
+        }
+    }
+    return 64u * (uint32_t)k + bit;
+}
+
+// The chunk offsets of one cell: side 0 counts L-stoppers, side 1
+// R-stoppers, both left to right; P(k) for k in [0, nch] (P(nch) = total).
+struct CellOffsets {
+    const uint64_t *pre;  // chunk_pre + first chunk of the cell
+    uint64_t base;
+    __device__ uint32_t at(int side, uint32_t k) const
+    {
+        const uint64_t p = pre[k] - base;
+        return side ? hi32(p) : lo32(p);
+    }
+    // the chunk in [k0, nch) holding index a (P(k0) <= a < P(nch))
+    __device__ uint32_t chunk_of(int side, uint32_t a, uint32_t k0, uint32_t nch) const
+    {
+        return wave_last_true(k0, nch, [&](uint32_t k) { return at(side, k) <= a; });
+    }
+};
+
+// Cell-local position of index a of one side (a below the side's total).
+__device__ inline uint32_t locate(const CellOffsets &co, const uint64_t *masks, uint32_t f,
+                                  uint32_t nch, int side, uint32_t a)
+{
+    const uint32_t k = co.chunk_of(side, a, 0u, nch);
+    uint64_t wd;
+    uint32_t ex;
+    load_mask(masks, f + k, side, wd, ex);
+    return k * (uint32_t)kChunk + wave_select(wd, ex, a - co.at(side, k));
+}
+
+// One wave64 per cell.  With A(k) the L-stoppers before chunk k and B(k)
+// the R-stoppers at or after it, let k* be the last chunk with
+// A(k*) <= B(k*) (A grows, B shrinks).  Every t below
+// max(A(k*), B(k*+1)) is swapped and none from min(B(k*), A(k*+1)) on, and
+// for t in between both L_t and R_t lie in chunk k*: its masks decide K.
 __global__ __launch_bounds__(kBlock) void k_hoare_split(const CellRec *recs, int32_t nCells,
                                                         const uint32_t *first,
                                                         const uint64_t *chunk_pre,
-                                                        const uint32_t *lpos, const uint32_t *rpos,
+                                                        const uint64_t *masks,
                                                         SplitRec *splits, uint32_t *ranges,
-                                                        int32_t *status, int64_t cap)
+                                                        int32_t *status)
 {
     const int lane = threadIdx.x & 63;
     const int32_t c = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
@@ -707,51 +801,51 @@ __global__ __launch_bounds__(kBlock) void k_hoare_split(const CellRec *recs, int
         if (lane == 0) splits[c] = SplitRec{0u, 0u, 0u, 0u};
         return;
     }
-    const uint64_t tot = chunk_pre[first[c + 1]] - chunk_pre[first[c]];
-    const uint32_t TL = (uint32_t)(tot & 0xffffffffull), TR = (uint32_t)(tot >> 32);
+    const uint32_t f = first[c], nch = first[c + 1] - f;
+    const CellOffsets co{chunk_pre + f, chunk_pre[f]};
+    const uint32_t TL = co.at(0, nch), TR = co.at(1, nch);
     const uint32_t n = r.end - r.begin;
     SplitRec s{TL, TR, 0u, 0u};
-    (void)cap;
     const int64_t lid = left_child_id(r.id), rid = right_child_id(r.id);
     if (n == 0 || TL == 0 || r.axis < 0 || r.axis > 2 || rid >= r.capr) {
         // empty cell or no key >= cut: the reference's loop leaves the cell
         if (lane == 0) {
             report_error(status + r.st, r.id);
-            s.TL = 0u;  // k_hoare_final skips the cell
+            s.TL = 0u;  // the swap kernels skip the cell
             splits[c] = s;
         }
         return;
     }
-    const uint32_t *L = lpos + r.begin;
-    const uint32_t *R = rpos + r.begin;  // left-to-right order; R_t = R[TR-1-t]
-    // K = #{t : L[t] < R[TR-1-t]}; the predicate is true then false in t
-    uint32_t lo = 0, hi = TL < TR ? TL : TR;
-    while (hi - lo > 64u) {
-        const uint32_t span = hi - lo;
-        const uint32_t t = lo + (uint32_t)(((uint64_t)(lane + 1) * span) / 65u);
-        const bool pr = L[t] < R[TR - 1 - t];
-        const uint64_t m = __ballot(pr);
-        if (m == 0ull) {
-            hi = lo + (uint32_t)(((uint64_t)1 * span) / 65u);           // pred(t_0) false
-        } else {
-            const int j = 63 - __clzll((long long)m);                    // last true probe
-            const uint32_t tj = lo + (uint32_t)(((uint64_t)(j + 1) * span) / 65u);
-            const uint32_t tn = j < 63 ? lo + (uint32_t)(((uint64_t)(j + 2) * span) / 65u) : hi;
-            lo = tj + 1;
-            hi = tn;
+    // A(nch) = TL > 0 = B(nch): k* < nch
+    const uint32_t ks = wave_last_true(0u, nch, [&](uint32_t k) {
+        return co.at(0, k) + co.at(1, k) <= TR;  // A(k) <= TR - R-stoppers before k
+    });
+    const uint32_t A0 = co.at(0, ks), A1 = co.at(0, ks + 1);
+    const uint32_t B0 = TR - co.at(1, ks), B1 = TR - co.at(1, ks + 1);
+    const uint32_t lo_t = A0 > B1 ? A0 : B1, hi_t = B0 < A1 ? B0 : A1;
+    uint64_t wL, wR;
+    uint32_t exL, exR;
+    load_mask(masks, f + ks, 0, wL, exL);
+    load_mask(masks, f + ks, 1, wR, exR);
+    // u in [1, m] stands for t = lo_t + u - 1; u = 0 is true by definition
+    const uint32_t m = hi_t - lo_t;
+    const uint32_t K = lo_t + wave_last_true(0u, m + 1u, [&](uint32_t u) {
+        const uint32_t t = lo_t + (u == 0u ? 0u : u - 1u);
+        const uint32_t pl = wave_select(wL, exL, t - A0);
+        const uint32_t pr = wave_select(wR, exR, B0 - 1u - t);
+        return u == 0u || pl < pr;
+    });
+    uint32_t i;
+    if (K == 0u) {
+        i = locate(co, masks, f, nch, 0, 0u);
+    } else {
+        i = locate(co, masks, f, nch, 1, TR - K);  // R_{K-1}
+        if (K < TL) {
+            const uint32_t lk = locate(co, masks, f, nch, 0, K);
+            i = lk < i ? lk : i;
         }
     }
-    const uint32_t t = lo + (uint32_t)lane;
-    const bool pr = t < hi && L[t] < R[TR - 1 - t];
-    const uint32_t K = lo + (uint32_t)__popcll(__ballot(pr));
     if (lane != 0) return;
-    uint32_t i;
-    if (K == 0) {
-        i = L[0];
-    } else {
-        i = R[TR - K];
-        if (K < TL && L[K] < i) i = L[K];
-    }
     s.K = K;
     s.i = i;
     splits[c] = s;
@@ -762,94 +856,222 @@ __global__ __launch_bounds__(kBlock) void k_hoare_split(const CellRec *recs, int
     ranges[2 * gr + 1] = r.end;
 }
 
-// Pairs in batches of kSwapBatch per thread: all index loads of a batch, then
-// all six-array gathers, then the stores, so each thread keeps several HBM
-// round trips in flight instead of one dependent chain per pair.
+// A swap block owns the pairs whose stopper on the cell's sparser side (the
+// larger of [0, i) and [i, n), which holds as many swapped stoppers as the
+// other) lies in its chunk: at most 4096 pairs, whose partners on the
+// denser side span about two chunks.
+struct SwapUnit {
+    int owner;        // side whose stoppers lie in the chunk (0: L, 1: R)
+    uint32_t t0, t1;  // pairs [t0, t1)
+};
+
+__device__ inline SwapUnit swap_unit(const CellOffsets &co, const SplitRec &s, uint32_t n,
+                                     uint32_t kc)
+{
+    SwapUnit u;
+    u.owner = 2ull * s.i >= (uint64_t)n ? 0 : 1;
+    if (u.owner == 0) {
+        u.t0 = co.at(0, kc);
+        u.t1 = co.at(0, kc + 1);
+    } else {
+        u.t0 = s.TR - co.at(1, kc + 1);
+        u.t1 = s.TR - co.at(1, kc);
+    }
+    if (u.t1 > s.K) u.t1 = s.K;
+    return u;
+}
+
+// The indices of one side that pairs [t0, t1) use: L-stopper t, R-stopper
+// TR - 1 - t (R-stoppers indexed left to right).
+__device__ inline void side_range(int side, const SwapUnit &u, uint32_t TR, uint32_t &a0,
+                                  uint32_t &a1)
+{
+    a0 = side ? TR - u.t1 : u.t0;
+    a1 = side ? TR - u.t0 : u.t1;
+}
+
+// One thread per chunk: the cell-local chunk that holds the first partner
+// index of the chunk's swap unit (binary search over the cell's chunk
+// offsets, which sit in L2).
+__global__ __launch_bounds__(kBlock) void k_hoare_partners(const CellRec *recs, int32_t nCells,
+                                                           const uint32_t *first,
+                                                           const uint32_t *chunk_cell,
+                                                           const uint64_t *chunk_pre,
+                                                           const SplitRec *splits,
+                                                           uint32_t *partner)
+{
+    const uint32_t total = first[nCells];
+    for (uint32_t q = blockIdx.x * kBlock + threadIdx.x; q < total; q += gridDim.x * kBlock) {
+        const uint32_t cell = chunk_cell[q];
+        const SplitRec s = splits[cell];
+        uint32_t kp = 0;
+        if (s.K != 0u) {
+            const CellRec r = recs[cell];
+            const uint32_t f = first[cell], nch = first[cell + 1] - f;
+            const CellOffsets co{chunk_pre + f, chunk_pre[f]};
+            const SwapUnit u = swap_unit(co, s, r.end - r.begin, q - f);
+            if (u.t0 < u.t1) {
+                const int side = 1 - u.owner;
+                uint32_t a0, a1;
+                side_range(side, u, s.TR, a0, a1);
+                uint32_t lo = 0, hi = nch;  // last k with P(k) <= a0
+                while (hi - lo > 1u) {
+                    const uint32_t mid = lo + (hi - lo) / 2u;
+                    if (co.at(side, mid) <= a0) lo = mid;
+                    else hi = mid;
+                }
+                kp = lo;
+            }
+        }
+        partner[q] = kp;
+    }
+}
+
+// Swaps m pairs: positions sl[j] and sr[j] (cell-local) for j < m.  Off is
+// the type of the byte offsets from the cell's first particle: with 32-bit
+// offsets every address is one register over a uniform base.
 constexpr int kSwapBatch = 8;
 
-__device__ inline void k_hoare_swap_chunk(uint32_t q, const CellRec *recs, int32_t nCells,
-                                          const uint32_t *first, const uint32_t *chunk_cell,
-                                          const SplitRec *splits, const uint32_t *lpos,
-                                          const uint32_t *rpos, float *x, float *y, float *z,
-                                          uint32_t *pid)
+template <class Off>
+__device__ inline void swap_pairs(const uint32_t *sl, const uint32_t *sr, uint32_t m,
+                                  uint32_t begin, float *x, float *y, float *z, uint32_t *pid)
 {
-    uint32_t cell, b, e;
-    if (!chunk_range(q, first, nCells, chunk_cell, recs, cell, b, e)) return;
-    const CellRec r = recs[cell];
-    const SplitRec s = splits[cell];
-    uint32_t t0 = b - r.begin;  // this chunk handles pair indices [t0, t0 + len)
-    uint32_t t1 = e - r.begin;
-    if (t1 > s.K) t1 = s.K;
-    if (t0 >= t1) return;
-    const uint32_t *L = lpos + r.begin;
-    const uint32_t *R = rpos + r.begin;
-    for (uint32_t t = t0 + threadIdx.x; t < t1; t += kBlock * kSwapBatch) {
-        uint32_t pa[kSwapBatch], pb[kSwapBatch];
+    char *const xb = reinterpret_cast<char *>(x + begin);
+    char *const yb = reinterpret_cast<char *>(y + begin);
+    char *const zb = reinterpret_cast<char *>(z + begin);
+    auto at = [](char *base, Off o) -> float & { return *reinterpret_cast<float *>(base + o); };
+    for (uint32_t j0 = 0; j0 < m; j0 += (uint32_t)(kBlock * kSwapBatch)) {
+        const uint32_t j = j0 + threadIdx.x;
+        Off pa[kSwapBatch], pb[kSwapBatch];
 #pragma unroll
         for (int u = 0; u < kSwapBatch; ++u) {
-            const uint32_t tt = t + (uint32_t)(u * kBlock);
-            pa[u] = tt < t1 ? r.begin + L[tt] : 0u;
-            pb[u] = tt < t1 ? r.begin + R[s.TR - 1 - tt] : 0u;
+            const uint32_t jj = j + (uint32_t)(u * kBlock);
+            pa[u] = jj < m ? (Off)sl[jj] * (Off)4 : (Off)0;
+            pb[u] = jj < m ? (Off)sr[jj] * (Off)4 : (Off)0;
         }
-        float ax[kSwapBatch], ay[kSwapBatch], az[kSwapBatch];
-        float bx[kSwapBatch], by[kSwapBatch], bz[kSwapBatch];
+        // defined on every path: otherwise the arrays are carried from batch
+        // to batch in registers
+        float ax[kSwapBatch] = {}, ay[kSwapBatch] = {}, az[kSwapBatch] = {};
+        float bx[kSwapBatch] = {}, by[kSwapBatch] = {}, bz[kSwapBatch] = {};
 #pragma unroll
         for (int u = 0; u < kSwapBatch; ++u) {
-            if (t + (uint32_t)(u * kBlock) < t1) {
-                ax[u] = x[pa[u]];
-                ay[u] = y[pa[u]];
-                az[u] = z[pa[u]];
-                bx[u] = x[pb[u]];
-                by[u] = y[pb[u]];
-                bz[u] = z[pb[u]];
+            if (j + (uint32_t)(u * kBlock) < m) {
+                ax[u] = at(xb, pa[u]);
+                ay[u] = at(yb, pa[u]);
+                az[u] = at(zb, pa[u]);
+                bx[u] = at(xb, pb[u]);
+                by[u] = at(yb, pb[u]);
+                bz[u] = at(zb, pb[u]);
             }
         }
 #pragma unroll
         for (int u = 0; u < kSwapBatch; ++u) {
-            if (t + (uint32_t)(u * kBlock) < t1) {
-                x[pa[u]] = bx[u];
-                y[pa[u]] = by[u];
-                z[pa[u]] = bz[u];
-                x[pb[u]] = ax[u];
-                y[pb[u]] = ay[u];
-                z[pb[u]] = az[u];
+            if (j + (uint32_t)(u * kBlock) < m) {
+                at(xb, pa[u]) = bx[u];
+                at(yb, pa[u]) = by[u];
+                at(zb, pa[u]) = bz[u];
+                at(xb, pb[u]) = ax[u];
+                at(yb, pb[u]) = ay[u];
+                at(zb, pb[u]) = az[u];
             }
         }
         if (pid) {  // uniform
-            uint32_t ia[kSwapBatch], ib[kSwapBatch];
+            char *const ib_ = reinterpret_cast<char *>(pid + begin);
+            auto id_at = [&](Off o) -> uint32_t & { return *reinterpret_cast<uint32_t *>(ib_ + o); };
+            uint32_t ia[kSwapBatch] = {}, ib[kSwapBatch] = {};
 #pragma unroll
             for (int u = 0; u < kSwapBatch; ++u) {
-                if (t + (uint32_t)(u * kBlock) < t1) {
-                    ia[u] = pid[pa[u]];
-                    ib[u] = pid[pb[u]];
+                if (j + (uint32_t)(u * kBlock) < m) {
+                    ia[u] = id_at(pa[u]);
+                    ib[u] = id_at(pb[u]);
                 }
             }
 #pragma unroll
             for (int u = 0; u < kSwapBatch; ++u) {
-                if (t + (uint32_t)(u * kBlock) < t1) {
-                    pid[pa[u]] = ib[u];
-                    pid[pb[u]] = ia[u];
+                if (j + (uint32_t)(u * kBlock) < m) {
+                    id_at(pa[u]) = ib[u];
+                    id_at(pb[u]) = ia[u];
                 }
             }
         }
     }
 }
 
+// Positions of a unit's pairs: each thread takes 16 bits of the owner
+// chunk's mask (first round only) and of the partner chunk's, one packed
+// block scan ranks both, and every stopper whose index the unit uses goes
+// to its slot (L index t -> t - t0, R index j -> TR - 1 - j - t0).  Later
+// rounds continue on the partner side only, chunk by chunk.
 __global__ __launch_bounds__(kBlock) void k_hoare_swap(const CellRec *recs, int32_t nCells,
                                                        const uint32_t *first,
                                                        const uint32_t *chunk_cell,
+                                                       const uint64_t *chunk_pre,
+                                                       const uint64_t *masks,
                                                        const SplitRec *splits,
-                                                       const uint32_t *lpos, const uint32_t *rpos,
+                                                       const uint32_t *partner,
                                                        float *x, float *y, float *z,
                                                        uint32_t *pid)
 {
+    __shared__ uint32_t spos[2][kChunk];  // cell-local positions of L_t, R_t at t - t0
+    __shared__ uint32_t sm[kBlock / 64];
     const uint32_t total = first[nCells];
+    const int j = threadIdx.x;
+    const int wdi = j >> 2, sh = (j & 3) * 16;  // this thread's 16 bits of a chunk mask
     for (uint32_t q = blockIdx.x; q < total; q += gridDim.x) {
-        k_hoare_swap_chunk(q, recs, nCells, first, chunk_cell, splits, lpos, rpos, x, y, z, pid);
-        __syncthreads();  // the next chunk reuses this block's LDS
+        const uint32_t cell = chunk_cell[q];
+        const SplitRec s = splits[cell];
+        if (s.K == 0u) continue;  // nothing to swap, or reported by k_hoare_split
+        const CellRec r = recs[cell];
+        const uint32_t n = r.end - r.begin;
+        const uint32_t f = first[cell], nch = first[cell + 1] - f, kc = q - f;
+        const CellOffsets co{chunk_pre + f, chunk_pre[f]};
+        const SwapUnit u = swap_unit(co, s, n, kc);
+        if (u.t0 >= u.t1) continue;  // uniform
+        const int so = u.owner, sp = 1 - so;
+        uint32_t ao0, ao1, ap0, ap1;
+        side_range(so, u, s.TR, ao0, ao1);
+        side_range(sp, u, s.TR, ap0, ap1);
+        auto put = [&](int side, uint32_t k, uint32_t idx, uint32_t bits, uint32_t a0,
+                       uint32_t a1) {
+            if (idx >= a1 || idx + (uint32_t)__popc(bits) <= a0) return;
+            const uint32_t pos0 = k * (uint32_t)kChunk + 16u * (uint32_t)j;
+            uint32_t *dst = spos[side];
+            while (bits) {
+                const uint32_t b = (uint32_t)__builtin_ctz(bits);
+                bits &= bits - 1u;
+                if (idx >= a0 && idx < a1) dst[side ? s.TR - 1u - idx - u.t0 : idx - u.t0] = pos0 + b;
+                ++idx;
+            }
+        };
+        uint32_t kp = partner[q];
+        bool own = true;
+        while (true) {  // uniform
+            const uint32_t bo =
+                own ? (uint32_t)(masks[(size_t)(f + kc) * (2 * kMaskWords) +
+                                       (size_t)so * kMaskWords + (uint32_t)wdi] >> sh) & 0xffffu
+                    : 0u;
+            const uint32_t bp = (uint32_t)(masks[(size_t)(f + kp) * (2 * kMaskWords) +
+                                                 (size_t)sp * kMaskWords + (uint32_t)wdi] >> sh) &
+                                0xffffu;
+            uint32_t tot;
+            const uint32_t ex = block_exclusive_scan((uint32_t)__popc(bo) | ((uint32_t)__popc(bp) << 16),
+                                                     sm, &tot);
+            if (own) put(so, kc, co.at(so, kc) + (ex & 0xffffu), bo, ao0, ao1);
+            put(sp, kp, co.at(sp, kp) + (ex >> 16), bp, ap0, ap1);
+            own = false;
+            const uint32_t next = co.at(sp, kp + 1u);
+            if (next >= ap1) break;
+            // next < ap1 <= P(nch): chunk kp + 1 exists; skip empty chunks by search
+            kp = co.at(sp, kp + 2u) > next ? kp + 1u : co.chunk_of(sp, next, kp + 1u, nch);
+        }
+        __syncthreads();
+        if (n <= (1u << 30))  // uniform: byte offsets inside the cell fit 32 bits
+            swap_pairs<uint32_t>(spos[0], spos[1], u.t1 - u.t0, r.begin, x, y, z, pid);
+        else
+            swap_pairs<uint64_t>(spos[0], spos[1], u.t1 - u.t0, r.begin, x, y, z, pid);
+        __syncthreads();  // the next unit overwrites spos
     }
 }
-
 
 __global__ void k_hoare_final(const CellRec *recs, int32_t nCells, const SplitRec *splits,
                               float *x, float *y, float *z, uint32_t *pid)
@@ -969,23 +1191,27 @@ int launch_stopper_counts(Ctx &c, int32_t nCells, bool le_only, const HistCounts
                               c.cell_first.p + nCells);
 }
 
-int launch_hoare(Ctx &c, int32_t nCells, int32_t *status, const HistCounts *hc)
+int launch_hoare(Ctx &c, int32_t nCells, int32_t *status)
 {
     if (!status) status = c.status.p;
     const unsigned gq = chunk_grid(c);
-    ORB_TRY(launch_stopper_counts(c, nCells, false, hc));
-    hipLaunchKernelGGL(k_stopper_positions, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
-                       c.cell_first.p, c.chunk_cell.p, c.chunk_pre.p, c.x(), c.y(), c.z(), c.lpos.p,
-                       c.rpos.p);
+    hipLaunchKernelGGL(k_hoare_masks, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
+                       c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(), c.masks.p,
+                       c.chunk_cnt.p);
     ORB_HIP(hipGetLastError());
+    ORB_TRY(scan_exclusive_u64(c, c.chunk_cnt.p, c.chunk_pre.p, (size_t)c.max_chunks + 1,
+                               c.cell_first.p + nCells));
     unsigned gw = (unsigned)((nCells + 3) / 4);
     hipLaunchKernelGGL(k_hoare_split, dim3(gw), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
-                       c.cell_first.p, c.chunk_pre.p, c.lpos.p, c.rpos.p, c.splits.p, c.ranges.p,
-                       status, (int64_t)c.cap);
+                       c.cell_first.p, c.chunk_pre.p, c.masks.p, c.splits.p, c.ranges.p, status);
+    ORB_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_hoare_partners, dim3(grid_for((size_t)c.max_chunks, 4096)), dim3(kBlock),
+                       0, c.stream, c.recs.p, nCells, c.cell_first.p, c.chunk_cell.p,
+                       c.chunk_pre.p, c.splits.p, c.chunk_aux.p);
     ORB_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_hoare_swap, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
-                       c.cell_first.p, c.chunk_cell.p, c.splits.p, c.lpos.p, c.rpos.p, c.x(), c.y(),
-                       c.z(), c.ids());
+                       c.cell_first.p, c.chunk_cell.p, c.chunk_pre.p, c.masks.p, c.splits.p,
+                       c.chunk_aux.p, c.x(), c.y(), c.z(), c.ids());
     ORB_HIP(hipGetLastError());
     unsigned gc = (unsigned)((nCells + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_hoare_final, dim3(gc), dim3(kBlock), 0, c.stream, c.recs.p, nCells,

@@ -216,8 +216,7 @@ static int server_alloc(Server &S)
     c.d = 1;
     ORB_TRY(c.pos.alloc(3 * (size_t)c.stride + 64));
     ORB_TRY(c.ranges.alloc(2 * (size_t)H));
-    ORB_TRY(c.lpos.alloc((size_t)c.stride));
-    ORB_TRY(c.rpos.alloc((size_t)c.stride));
+    ORB_TRY(c.packed.alloc((size_t)c.stride));
     ORB_TRY(c.status.alloc(2));
     ORB_TRY(c.hstatus.alloc(2));
     ORB_TRY(c.hranges.alloc(2));
@@ -541,8 +540,9 @@ int orb_server_destroy(orb_server *o)
     // engine buffers
     c.pos.release();
     c.ranges.release();
-    c.lpos.release();
-    c.rpos.release();
+    c.packed.release();
+    c.masks.release();
+    c.chunk_aux.release();
     c.status.release();
     c.hstatus.release();
     c.hranges.release();
