@@ -927,19 +927,65 @@ __global__ __launch_bounds__(kBlock) void k_hoare_partners(const CellRec *recs, 
     }
 }
 
-// Swaps m pairs: positions sl[j] and sr[j] (cell-local) for j < m.  Off is
-// the type of the byte offsets from the cell's first particle: with 32-bit
-// offsets every address is one register over a uniform base.
+// Swaps m pairs: positions sl[j] and sr[j] (cell-local) for j < m, with
+// kSwapBatch pairs per thread in flight (all loads, then all stores).
 constexpr int kSwapBatch = 8;
 
-template <class Off>
+// Particle arrays seen from a cell's first particle.  Narrow: raw buffer
+// accesses with 32-bit byte offsets over one descriptor per array (one
+// address register per access; the cell holds fewer than 2^30 particles).
+// Wide: plain 64-bit addressing.
+template <bool kNarrow>
+struct CellArrays;
+
+template <>
+struct CellArrays<true> {
+    __amdgpu_buffer_rsrc_t a[4];
+    __device__ CellArrays(float *x, float *y, float *z, uint32_t *pid, uint32_t begin)
+    {
+        // raw buffer, 32-bit data format, no range limit below 4 GiB
+        a[0] = __builtin_amdgcn_make_buffer_rsrc(x + begin, 0, -1, 0x00020000);
+        a[1] = __builtin_amdgcn_make_buffer_rsrc(y + begin, 0, -1, 0x00020000);
+        a[2] = __builtin_amdgcn_make_buffer_rsrc(z + begin, 0, -1, 0x00020000);
+        a[3] = __builtin_amdgcn_make_buffer_rsrc(pid ? pid + begin : nullptr, 0, -1, 0x00020000);
+    }
+    using Off = uint32_t;
+    __device__ uint32_t load(int arr, Off o) const
+    {
+        return __builtin_amdgcn_raw_buffer_load_b32(a[arr], (int)o, 0, 0);
+    }
+    __device__ void store(int arr, Off o, uint32_t v) const
+    {
+        __builtin_amdgcn_raw_buffer_store_b32(v, a[arr], (int)o, 0, 0);
+    }
+};
+
+template <>
+struct CellArrays<false> {
+    char *a[4];
+    __device__ CellArrays(float *x, float *y, float *z, uint32_t *pid, uint32_t begin)
+    {
+        a[0] = reinterpret_cast<char *>(x + begin);
+        a[1] = reinterpret_cast<char *>(y + begin);
+        a[2] = reinterpret_cast<char *>(z + begin);
+        a[3] = reinterpret_cast<char *>(pid ? pid + begin : nullptr);
+    }
+    using Off = uint64_t;
+    __device__ uint32_t load(int arr, Off o) const
+    {
+        return *reinterpret_cast<const uint32_t *>(a[arr] + o);
+    }
+    __device__ void store(int arr, Off o, uint32_t v) const
+    {
+        *reinterpret_cast<uint32_t *>(a[arr] + o) = v;
+    }
+};
+
+template <bool kNarrow>
 __device__ inline void swap_pairs(const uint32_t *sl, const uint32_t *sr, uint32_t m,
-                                  uint32_t begin, float *x, float *y, float *z, uint32_t *pid)
+                                  const CellArrays<kNarrow> &ca, bool ids)
 {
-    char *const xb = reinterpret_cast<char *>(x + begin);
-    char *const yb = reinterpret_cast<char *>(y + begin);
-    char *const zb = reinterpret_cast<char *>(z + begin);
-    auto at = [](char *base, Off o) -> float & { return *reinterpret_cast<float *>(base + o); };
+    using Off = typename CellArrays<kNarrow>::Off;
     for (uint32_t j0 = 0; j0 < m; j0 += (uint32_t)(kBlock * kSwapBatch)) {
         const uint32_t j = j0 + threadIdx.x;
         Off pa[kSwapBatch], pb[kSwapBatch];
@@ -949,48 +995,29 @@ __device__ inline void swap_pairs(const uint32_t *sl, const uint32_t *sr, uint32
             pa[u] = jj < m ? (Off)sl[jj] * (Off)4 : (Off)0;
             pb[u] = jj < m ? (Off)sr[jj] * (Off)4 : (Off)0;
         }
+        const int na = ids ? 4 : 3;  // uniform
         // defined on every path: otherwise the arrays are carried from batch
         // to batch in registers
-        float ax[kSwapBatch] = {}, ay[kSwapBatch] = {}, az[kSwapBatch] = {};
-        float bx[kSwapBatch] = {}, by[kSwapBatch] = {}, bz[kSwapBatch] = {};
+        uint32_t va[4][kSwapBatch] = {}, vb[4][kSwapBatch] = {};
 #pragma unroll
-        for (int u = 0; u < kSwapBatch; ++u) {
-            if (j + (uint32_t)(u * kBlock) < m) {
-                ax[u] = at(xb, pa[u]);
-                ay[u] = at(yb, pa[u]);
-                az[u] = at(zb, pa[u]);
-                bx[u] = at(xb, pb[u]);
-                by[u] = at(yb, pb[u]);
-                bz[u] = at(zb, pb[u]);
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < kSwapBatch; ++u) {
-            if (j + (uint32_t)(u * kBlock) < m) {
-                at(xb, pa[u]) = bx[u];
-                at(yb, pa[u]) = by[u];
-                at(zb, pa[u]) = bz[u];
-                at(xb, pb[u]) = ax[u];
-                at(yb, pb[u]) = ay[u];
-                at(zb, pb[u]) = az[u];
-            }
-        }
-        if (pid) {  // uniform
-            char *const ib_ = reinterpret_cast<char *>(pid + begin);
-            auto id_at = [&](Off o) -> uint32_t & { return *reinterpret_cast<uint32_t *>(ib_ + o); };
-            uint32_t ia[kSwapBatch] = {}, ib[kSwapBatch] = {};
+        for (int arr = 0; arr < 4; ++arr) {
+            if (arr >= na) break;
 #pragma unroll
             for (int u = 0; u < kSwapBatch; ++u) {
                 if (j + (uint32_t)(u * kBlock) < m) {
-                    ia[u] = id_at(pa[u]);
-                    ib[u] = id_at(pb[u]);
+                    va[arr][u] = ca.load(arr, pa[u]);
+                    vb[arr][u] = ca.load(arr, pb[u]);
                 }
             }
+        }
+#pragma unroll
+        for (int arr = 0; arr < 4; ++arr) {
+            if (arr >= na) break;
 #pragma unroll
             for (int u = 0; u < kSwapBatch; ++u) {
                 if (j + (uint32_t)(u * kBlock) < m) {
-                    id_at(pa[u]) = ib[u];
-                    id_at(pb[u]) = ia[u];
+                    ca.store(arr, pa[u], vb[arr][u]);
+                    ca.store(arr, pb[u], va[arr][u]);
                 }
             }
         }
@@ -1002,6 +1029,7 @@ __device__ inline void swap_pairs(const uint32_t *sl, const uint32_t *sr, uint32
 // block scan ranks both, and every stopper whose index the unit uses goes
 // to its slot (L index t -> t - t0, R index j -> TR - 1 - j - t0).  Later
 // rounds continue on the partner side only, chunk by chunk.
+template <bool kNarrow>
 __global__ __launch_bounds__(kBlock) void k_hoare_swap(const CellRec *recs, int32_t nCells,
                                                        const uint32_t *first,
                                                        const uint32_t *chunk_cell,
@@ -1065,10 +1093,8 @@ __global__ __launch_bounds__(kBlock) void k_hoare_swap(const CellRec *recs, int3
             kp = co.at(sp, kp + 2u) > next ? kp + 1u : co.chunk_of(sp, next, kp + 1u, nch);
         }
         __syncthreads();
-        if (n <= (1u << 30))  // uniform: byte offsets inside the cell fit 32 bits
-            swap_pairs<uint32_t>(spos[0], spos[1], u.t1 - u.t0, r.begin, x, y, z, pid);
-        else
-            swap_pairs<uint64_t>(spos[0], spos[1], u.t1 - u.t0, r.begin, x, y, z, pid);
+        const CellArrays<kNarrow> ca(x, y, z, pid, r.begin);
+        swap_pairs<kNarrow>(spos[0], spos[1], u.t1 - u.t0, ca, pid != nullptr);
         __syncthreads();  // the next unit overwrites spos
     }
 }
@@ -1209,7 +1235,9 @@ int launch_hoare(Ctx &c, int32_t nCells, int32_t *status)
                        0, c.stream, c.recs.p, nCells, c.cell_first.p, c.chunk_cell.p,
                        c.chunk_pre.p, c.splits.p, c.chunk_aux.p);
     ORB_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_hoare_swap, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
+    // byte offsets inside any cell fit 32 bits below 2^30 particles
+    auto swap = c.stride < (int64_t(1) << 30) ? k_hoare_swap<true> : k_hoare_swap<false>;
+    hipLaunchKernelGGL(swap, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
                        c.cell_first.p, c.chunk_cell.p, c.chunk_pre.p, c.masks.p, c.splits.p,
                        c.chunk_aux.p, c.x(), c.y(), c.z(), c.ids());
     ORB_HIP(hipGetLastError());
