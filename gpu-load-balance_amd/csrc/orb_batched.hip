@@ -242,20 +242,13 @@ __device__ inline int bin_of(const float *cand, float v, float lo, float scale, 
     return e;
 }
 
-// Histogram of one chunk over the cell's 255 candidate cuts.
-//   MODE 0: the chunk's keys.
-//   MODE 1: the same; each wave also packs the keys of its share that lie in
-//           the window [lo, hi] into its own quarter of the chunk's slots of
-//           ckeys (ballot ranks, no atomics) and records how many
-//           (comp[8q + w]); comp[8q + 4] gets the chunk's keys below lo.
-//           Later windows lie inside this one, so later rounds need only
-//           these keys.
-// Later rounds read the packed keys (k_b_hist_packed).
-template <int LE, int MODE>
+// First histogram round: one block per chunk bins every key of the chunk
+// over the cell's 255 candidate cuts (keys outside the margins are counted
+// in registers) and writes the chunk's 256-bin row and its tie row.
+template <int LE>
 __device__ inline void k_b_hist_chunk(uint32_t q, const CellRec *recs, const uint32_t *first,
                                       const uint32_t *chunk_cell, const float *x, const float *y,
-                                      const float *z, uint16_t *chunk_hist, uint16_t *chunk_tie,
-                                      float *ckeys, uint32_t *comp)
+                                      const float *z, uint16_t *chunk_hist, uint16_t *chunk_tie)
 {
     __shared__ float cand[kBins + 1];
     __shared__ uint32_t hist[4][kBins];
@@ -267,7 +260,7 @@ __device__ inline void k_b_hist_chunk(uint32_t q, const CellRec *recs, const uin
     const uint32_t b = r.begin + (q - first[cell]) * (uint32_t)kChunk;
     uint32_t e = b + (uint32_t)kChunk;
     if (e > r.end) e = r.end;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int t = threadIdx.x, w = t >> 6;
     const float lo = r.margin_l, hi = r.margin_r;
     uint32_t below = 0, above = 0;
 
@@ -294,9 +287,6 @@ __device__ inline void k_b_hist_chunk(uint32_t q, const CellRec *recs, const uin
     const float vh = has_h ? key[b + ht] : 0.0f;
     const float vt = has_t ? key[ae + (ht - 3)] : 0.0f;
 
-    uint32_t share_off = 0, share_cnt = 0;
-    if (MODE == 1) wave_share(b, e, w, share_off, share_cnt);
-
     if (t >= 1) cand[t] = candidate(t, lo, hi);
     else cand[0] = -INFINITY;
     if (t == 0) cand[kBins] = INFINITY;
@@ -310,46 +300,31 @@ __device__ inline void k_b_hist_chunk(uint32_t q, const CellRec *recs, const uin
     const float scale = width > 0.0f ? (float)kBins / width : 0.0f;
     uint32_t *h = hist[w];
     // keys below lo fall in bin 0 and keys above hi in bin 255 for both
-    // comparisons: counted in registers, the rest binned
-    const uint64_t lanes_below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    float *pack = ckeys + b + share_off;  // share_cnt slots, one per key of this wave
-    uint32_t packed = 0;  // wave-uniform
-    // branch-free counters (an if/else chain over counters can make the
-    // compiler pick a counter by address, which puts them in scratch memory);
-    // every lane of the wave calls put() together (ballot), `ok` masks it
+    // comparisons: counted in registers, the rest binned.  Branch-free
+    // counters (an if/else chain over counters can make the compiler pick a
+    // counter by address, which puts them in scratch memory)
     auto put = [&](float v, bool ok) __attribute__((always_inline)) {
         const bool bl = ok && v < lo, abv = ok && v > hi;
         const bool in = ok && !(v < lo) && !(v > hi);
         below += bl ? 1u : 0u;
         above += abv ? 1u : 0u;
-        if (MODE == 1) {
-            const uint64_t m = __ballot(in);
-            if (m) {  // wave-uniform: most groups have no key in the window
-                if (in) pack[packed + (uint32_t)__popcll(m & lanes_below)] = v;
-                packed += (uint32_t)__popcll(m);
-            }
-        }
         if (in) {
             int tj;
             atomicAdd(&h[bin_of<LE>(cand, v, lo, scale, tj)], 1u);
             if (tj) atomicAdd(&ties[tj], 1u);
         }
     };
-    {
 #pragma unroll
-        for (int u = 0; u < kPerThread / 4; ++u) {
-            const bool ok = (uint32_t)t + (uint32_t)u * kBlock < nq;
-            put(v[u].x, ok);
-            put(v[u].y, ok);
-            put(v[u].z, ok);
-            put(v[u].w, ok);
-        }
-        put(vh, has_h);
-        put(vt, has_t);
-        if (MODE == 1 && lane == 0) comp[8 * (size_t)q + w] = packed;
+    for (int u = 0; u < kPerThread / 4; ++u) {
+        const bool ok = (uint32_t)t + (uint32_t)u * kBlock < nq;
+        put(v[u].x, ok);
+        put(v[u].y, ok);
+        put(v[u].z, ok);
+        put(v[u].w, ok);
     }
+    put(vh, has_h);
+    put(vt, has_t);
     const uint32_t ba = block_sum(below | (above << 16), sm);  // each <= 4096
-    if (MODE == 1 && t == 0) comp[8 * (size_t)q + 4] = ba & 0xffffu;
     uint32_t sum = hist[0][t] + hist[1][t] + hist[2][t] + hist[3][t];  // <= 4096
     if (t == 0) sum += ba & 0xffffu;
     if (t == kBins - 1) sum += ba >> 16;
@@ -357,19 +332,108 @@ __device__ inline void k_b_hist_chunk(uint32_t q, const CellRec *recs, const uin
     chunk_tie[(size_t)q * kBins + t] = (uint16_t)ties[t];
 }
 
-template <int LE, int MODE>
+template <int LE>
 __global__ __launch_bounds__(kBlock) void k_b_hist(const CellRec *recs, int32_t nCells,
                                                    const uint32_t *first,
                                                    const uint32_t *chunk_cell, const float *x,
                                                    const float *y, const float *z,
-                                                   uint16_t *chunk_hist, uint16_t *chunk_tie,
-                                                   float *ckeys, uint32_t *comp)
+                                                   uint16_t *chunk_hist, uint16_t *chunk_tie)
 {
     const uint32_t total = first[nCells];
     for (uint32_t q = blockIdx.x; q < total; q += gridDim.x) {
-        k_b_hist_chunk<LE, MODE>(q, recs, first, chunk_cell, x, y, z, chunk_hist, chunk_tie,
-                                 ckeys, comp);
+        k_b_hist_chunk<LE>(q, recs, first, chunk_cell, x, y, z, chunk_hist, chunk_tie);
         __syncthreads();  // the next chunk reuses this block's LDS
+    }
+}
+
+// Second histogram round.  Wave w takes the quarter [1024w, 1024w + 1024)
+// of the chunk, 16 keys per lane (64 consecutive keys per load).  Keys below
+// lo and above hi are counted in registers, the few in the window [lo, hi]
+// are binned, and each wave packs its window keys, in position order, into
+// its quarter's slots of `pack` (comp[8q + w] = how many; comp[8q + 4] = the
+// chunk's keys below lo).  Later windows lie inside this one, so later
+// rounds need only these keys.  The wave also leaves two of the chunk's
+// bitmasks for the Hoare partition (bit j of word k = chunk position
+// 64k + j): keys above hi (masks[q][0]) and keys in the window (masks[q][1]).
+// The cell's final cut lies in [lo, hi], so these decide every stopper bit
+// but those of the packed keys.
+template <int LE>
+__global__ __launch_bounds__(kBlock) void k_b_hist_window(
+    const CellRec *recs, int32_t nCells, const uint32_t *first, const uint32_t *chunk_cell,
+    const float *x, const float *y, const float *z, uint16_t *chunk_hist, uint16_t *chunk_tie,
+    float *pack, uint32_t *comp, uint64_t *masks)
+{
+    constexpr int kW = kBlock / 64;
+    constexpr int kPerWave = kMaskWords / kW;  // 16 loads of 64 keys
+    __shared__ float cand[kBins + 1];
+    __shared__ uint32_t hist[kBins];  // window keys are few: one histogram
+    __shared__ uint32_t ties[kBins];
+    __shared__ uint32_t sm[4];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const uint64_t lanes_below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t total = first[nCells];
+    for (uint32_t q = blockIdx.x; q < total; q += gridDim.x) {
+        const uint32_t cell = chunk_cell[q];
+        const CellRec r = recs[cell];
+        if (r.found || r.flags || r.axis < 0 || r.axis > 2) continue;  // uniform
+        const uint32_t b = r.begin + (q - first[cell]) * (uint32_t)kChunk;
+        uint32_t e = b + (uint32_t)kChunk;
+        if (e > r.end) e = r.end;
+        const float lo = r.margin_l, hi = r.margin_r;
+        const float *key = axis_ptr(x, y, z, r.axis);
+        const uint32_t sb = b + (uint32_t)(w * kPerWave * 64);
+        float v[kPerWave];
+#pragma unroll
+        for (int j = 0; j < kPerWave; ++j) {
+            const uint32_t p = sb + (uint32_t)(64 * j + lane);
+            v[j] = p < e ? key[p] : 0.0f;
+        }
+        if (t >= 1) cand[t] = candidate(t, lo, hi);
+        else cand[0] = -INFINITY;
+        if (t == 0) cand[kBins] = INFINITY;
+        hist[t] = 0u;
+        ties[t] = 0u;
+        __syncthreads();
+        const float width = hi - lo;
+        const float scale = width > 0.0f ? (float)kBins / width : 0.0f;
+        uint32_t below = 0, above = 0, packed = 0;
+        uint64_t mineG = 0ull, mineW = 0ull;
+        float *dst = pack + sb;
+#pragma unroll
+        for (int j = 0; j < kPerWave; ++j) {
+            const bool ok = sb + (uint32_t)(64 * j + lane) < e;
+            const float vj = v[j];
+            const bool bl = ok && vj < lo, abv = ok && vj > hi;
+            const bool in = ok && !(vj < lo) && !(vj > hi);
+            below += bl ? 1u : 0u;
+            above += abv ? 1u : 0u;
+            const uint64_t mG = __ballot(abv), mW = __ballot(in);
+            mineG = lane == j ? mG : mineG;
+            mineW = lane == j ? mW : mineW;
+            if (mW) {  // wave-uniform: most loads have no key in the window
+                if (in) {
+                    dst[packed + (uint32_t)__popcll(mW & lanes_below)] = vj;
+                    int tj;
+                    atomicAdd(&hist[bin_of<LE>(cand, vj, lo, scale, tj)], 1u);
+                    if (tj) atomicAdd(&ties[tj], 1u);
+                }
+                packed += (uint32_t)__popcll(mW);
+            }
+        }
+        uint64_t *mq = masks + (size_t)q * (2 * kMaskWords) + w * kPerWave;
+        if (lane < kPerWave) {
+            mq[lane] = mineG;
+            mq[kMaskWords + lane] = mineW;
+        }
+        if (lane == 0) comp[8 * (size_t)q + w] = packed;
+        const uint32_t ba = block_sum(below | (above << 16), sm);  // each <= 4096
+        if (t == 0) comp[8 * (size_t)q + 4] = ba & 0xffffu;
+        uint32_t sum = hist[t];
+        if (t == 0) sum += ba & 0xffffu;
+        if (t == kBins - 1) sum += ba >> 16;
+        chunk_hist[(size_t)q * kBins + t] = (uint16_t)sum;
+        chunk_tie[(size_t)q * kBins + t] = (uint16_t)ties[t];
+        __syncthreads();  // the next chunk rewrites cand, hist and ties
     }
 }
 
@@ -1156,27 +1220,31 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
         // rounds: all keys, all keys + packing the window, then the packed keys
         float *ckeys = c.packed.p;
         const int mode = r == 0 ? 0 : (r == 1 ? 1 : 2);
-#define ORB_HIST(L, M)                                                                           \
-    hipLaunchKernelGGL((k_b_hist<L, M>), dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,  \
-                       c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(), chunk_hist,          \
-                       chunk_tie, ckeys, c.chunk_comp.p)
+#define ORB_HIST(L)                                                                              \
+    hipLaunchKernelGGL((k_b_hist<L>), dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,     \
+                       c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(), chunk_hist, chunk_tie)
+#define ORB_HIST_WINDOW(L)                                                                       \
+    hipLaunchKernelGGL((k_b_hist_window<L>), dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p,      \
+                       nCells, c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(), chunk_hist,  \
+                       chunk_tie, ckeys, c.chunk_comp.p, c.masks.p)
         const unsigned gp = (gq + 3) / 4;  // a wave per chunk
         if (le) {
-            if (mode == 0) ORB_HIST(1, 0);
-            else if (mode == 1) ORB_HIST(1, 1);
+            if (mode == 0) ORB_HIST(1);
+            else if (mode == 1) ORB_HIST_WINDOW(1);
             else
                 hipLaunchKernelGGL(k_b_hist_packed<1>, dim3(gp), dim3(kBlock), 0, c.stream, c.recs.p,
                                    nCells, c.cell_first.p, c.chunk_cell.p, chunk_hist, chunk_tie,
                                    ckeys, c.chunk_comp.p);
         } else {
-            if (mode == 0) ORB_HIST(0, 0);
-            else if (mode == 1) ORB_HIST(0, 1);
+            if (mode == 0) ORB_HIST(0);
+            else if (mode == 1) ORB_HIST_WINDOW(0);
             else
                 hipLaunchKernelGGL(k_b_hist_packed<0>, dim3(gp), dim3(kBlock), 0, c.stream, c.recs.p,
                                    nCells, c.cell_first.p, c.chunk_cell.p, chunk_hist, chunk_tie,
                                    ckeys, c.chunk_comp.p);
         }
 #undef ORB_HIST
+#undef ORB_HIST_WINDOW
         ORB_HIP(hipGetLastError());
         ORB_TRY(timing_pair(c, tb, kTimeCount));
         hipLaunchKernelGGL(k_b_hist_reduce, dim3(gr), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
@@ -1192,7 +1260,7 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
     hc.le = le;
     hc.pack = c.packed.p;
     hc.comp = c.chunk_comp.p;
-    if (c.part == ORB_PART_HOARE) ORB_TRY(launch_hoare(c, nCells, status));
+    if (c.part == ORB_PART_HOARE) ORB_TRY(launch_hoare(c, nCells, status, &hc));
     else ORB_TRY(launch_stable_le(c, nCells, status, full_cover, &hc));
     ORB_TRY(timing_pair(c, tp, kTimePartition));
     return ORB_OK;

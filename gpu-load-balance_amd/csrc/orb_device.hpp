@@ -66,31 +66,14 @@ __device__ inline void store_root_or_zero(orb_cell *dst, const orb_cell &root, b
     for (int j = 0; j < (int)(sizeof(orb_cell) / 4); ++j) w[j] = is_root ? rw[j] : 0u;
 }
 
-// The keys of chunk [b, e) that wave w of a histogram block handles: 16-byte
-// groups g = 64w + i + 256u (i < 64, u < 4) of the aligned body, plus the
-// misaligned head and tail for the last wave.  Their number, and an offset
-// such that the four waves' shares tile [b, e) exactly: each wave packs its
-// keys into its own share of the chunk's slots.
+// The share of wave w in a chunk [b, e) whose window keys the second
+// histogram round packs: the quarter [1024w, 1024w + 1024) of the chunk
+// (offset from b, and length), which holds that wave's keys and its slots.
 __device__ inline void wave_share(uint32_t b, uint32_t e, int w, uint32_t &off, uint32_t &cnt)
 {
-    uint32_t ab = (b + 3u) & ~3u;
-    if (ab > e) ab = e;
-    uint32_t ae = e & ~3u;
-    if (ae < ab) ae = ab;
-    const uint32_t nq = (ae - ab) >> 2;
-    auto keys_of = [&](int wv) {
-        uint32_t k = 0;
-        for (int u = 0; u < kPerThread / 4; ++u) {
-            const uint32_t g0 = (uint32_t)(64 * wv + 256 * u);
-            k += g0 >= nq ? 0u : (nq - g0 < 64u ? nq - g0 : 64u);
-        }
-        k *= 4u;
-        if (wv == kBlock / 64 - 1) k += (ab - b) + (e - ae);
-        return k;
-    };
-    off = 0;
-    for (int v = 0; v < w; ++v) off += keys_of(v);
-    cnt = keys_of(w);
+    const uint32_t len = e - b, q = (uint32_t)(kChunk / (kBlock / 64));
+    off = (uint32_t)w * q < len ? (uint32_t)w * q : len;
+    cnt = len - off < q ? len - off : q;
 }
 
 __device__ inline const float *axis_ptr(const float *x, const float *y, const float *z, int a)

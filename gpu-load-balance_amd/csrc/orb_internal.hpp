@@ -442,7 +442,8 @@ struct HistCounts {
     const float *pack = nullptr;      // keys packed by the second round, per chunk share
     const uint32_t *comp = nullptr;   // [chunks][8] packed counts per share, keys below
 };
-int launch_hoare(Ctx &c, int32_t nCells, int32_t *status = nullptr);
+int launch_hoare(Ctx &c, int32_t nCells, int32_t *status = nullptr,
+                 const HistCounts *hc = nullptr);
 int launch_stopper_counts(Ctx &c, int32_t nCells, bool le_only, const HistCounts *hc = nullptr);
 int invalidate_stamps(Ctx &c, int32_t nCells);
 int launch_stable_le(Ctx &c, int32_t nCells, int32_t *status = nullptr, bool full_cover = false,

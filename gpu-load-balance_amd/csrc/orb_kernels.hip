@@ -617,6 +617,34 @@ __global__ __launch_bounds__(kBlock) void k_stopper_counts(const CellRec *recs, 
 // The masks are 2 bits per key, so the positions never go through HBM.
 // ---------------------------------------------------------------------------
 
+// Stopper masks from the keys: one block per chunk, wave w builds words
+// 16w .. 16w + 15 (one ballot per 64 keys), lane j keeps word 16w + j.
+__device__ inline void masks_from_keys(const CellRec &r, uint32_t b, uint32_t e, const float *x,
+                                       const float *y, const float *z, int w, int lane,
+                                       uint64_t &mineL, uint64_t &mineR)
+{
+    constexpr int kPerWave = kMaskWords / (kBlock / 64);
+    const float cut = r.cut;
+    const float *key = axis_ptr(x, y, z, r.axis);
+    const uint32_t sb = b + (uint32_t)(w * kPerWave * 64);
+    float v[kPerWave];
+#pragma unroll
+    for (int j = 0; j < kPerWave; ++j) {
+        const uint32_t p = sb + (uint32_t)(64 * j + lane);
+        v[j] = p < e ? key[p] : 0.0f;
+    }
+    mineL = 0ull;
+    mineR = 0ull;
+#pragma unroll
+    for (int j = 0; j < kPerWave; ++j) {
+        const bool in = sb + (uint32_t)(64 * j + lane) < e;
+        const uint64_t mL = __ballot(in && !(v[j] < cut));
+        const uint64_t mR = __ballot(in && !(v[j] > cut));
+        mineL = lane == j ? mL : mineL;
+        mineR = lane == j ? mR : mineR;
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_hoare_masks(const CellRec *recs, int32_t nCells,
                                                         const uint32_t *first,
                                                         const uint32_t *chunk_cell,
@@ -633,32 +661,15 @@ __global__ __launch_bounds__(kBlock) void k_hoare_masks(const CellRec *recs, int
         uint32_t cell, b, e;
         chunk_range(q, first, nCells, chunk_cell, recs, cell, b, e);
         const CellRec r = recs[cell];
-        const float *key = axis_ptr(x, y, z, r.axis);
-        const float cut = r.cut;
-        const uint32_t sb = b + (uint32_t)(w * kPerWave * 64);
-        float v[kPerWave];
-#pragma unroll
-        for (int j = 0; j < kPerWave; ++j) {
-            const uint32_t p = sb + (uint32_t)(64 * j + lane);
-            v[j] = p < e ? key[p] : 0.0f;
-        }
-        uint64_t mineL = 0ull, mineR = 0ull;
-        uint32_t nl = 0, nr = 0;
-#pragma unroll
-        for (int j = 0; j < kPerWave; ++j) {
-            const bool in = sb + (uint32_t)(64 * j + lane) < e;
-            const uint64_t mL = __ballot(in && !(v[j] < cut));
-            const uint64_t mR = __ballot(in && !(v[j] > cut));
-            nl += (uint32_t)__popcll(mL);
-            nr += (uint32_t)__popcll(mR);
-            mineL = lane == j ? mL : mineL;
-            mineR = lane == j ? mR : mineR;
-        }
-        uint64_t *dst = masks + (size_t)q * (2 * kMaskWords) + w * kPerWave;
+        uint64_t mineL, mineR;
+        masks_from_keys(r, b, e, x, y, z, w, lane, mineL, mineR);
+        uint64_t *mq = masks + (size_t)q * (2 * kMaskWords) + w * kPerWave;
         if (lane < kPerWave) {
-            dst[lane] = mineL;
-            dst[kMaskWords + lane] = mineR;
+            mq[lane] = mineL;
+            mq[kMaskWords + lane] = mineR;
         }
+        const uint32_t nl = wave_sum((uint32_t)__popcll(mineL));
+        const uint32_t nr = wave_sum((uint32_t)__popcll(mineR));
         if (lane == 0) {
             s_cnt[0][w] = nl;
             s_cnt[1][w] = nr;
@@ -673,6 +684,78 @@ __global__ __launch_bounds__(kBlock) void k_hoare_masks(const CellRec *recs, int
             chunk_cnt[q] = (uint64_t)L | ((uint64_t)R << 32);
         }
         __syncthreads();  // the next chunk reuses s_cnt
+    }
+}
+
+// Batched form: a wave per chunk (the work is a few dependent loads).  A
+// cell that went through the window round (passes > kHistK) finishes that
+// round's masks: keys above the window are L-stoppers only, keys below
+// R-stoppers only, and the window keys, packed in position order in the
+// quarter's slots, are compared with the cut; lane j owns word j.  Other
+// cells read their keys, 64 per load.
+__global__ __launch_bounds__(kBlock) void k_hoare_masks_window(
+    const CellRec *recs, int32_t nCells, const uint32_t *first, const uint32_t *chunk_cell,
+    const float *x, const float *y, const float *z, const float *pack, uint64_t *masks,
+    uint64_t *chunk_cnt)
+{
+    constexpr int kW = kBlock / 64;
+    constexpr int kQuarter = kMaskWords / kW;  // words per packing quarter
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t total = first[nCells];
+    for (uint32_t q = blockIdx.x * kW + (uint32_t)w; q < total; q += gridDim.x * kW) {
+        const uint32_t cell = chunk_cell[q];
+        const CellRec r = recs[cell];
+        const uint32_t b = r.begin + (q - first[cell]) * (uint32_t)kChunk;
+        uint32_t e = b + (uint32_t)kChunk;
+        if (e > r.end) e = r.end;
+        const uint32_t len = e - b, p0 = 64u * (uint32_t)lane;  // chunk position of bit 0
+        const uint64_t valid =
+            p0 >= len ? 0ull : (len - p0 >= 64u ? ~0ull : (1ull << (len - p0)) - 1ull);
+        uint64_t *mq = masks + (size_t)q * (2 * kMaskWords);
+        const float cut = r.cut;
+        uint64_t L = 0ull, R = 0ull;
+        if (r.passes > kHistK && r.axis >= 0 && r.axis <= 2) {  // uniform
+            const uint64_t G = mq[lane] & valid;
+            const uint64_t W = mq[kMaskWords + lane] & valid;  // slots stay inside the cell
+            const uint32_t cw = (uint32_t)__popcll(W);
+            const uint32_t ex = wave_inclusive_scan(cw) - cw;
+            const int q0 = lane & ~(kQuarter - 1);  // first word of this lane's quarter
+            uint32_t rank = ex - (uint32_t)__shfl((int)ex, q0, 64);
+            const float *pk = pack + b + 64u * (uint32_t)q0;
+            uint64_t wd = W;
+            while (wd) {
+                const uint32_t bit = (uint32_t)__builtin_ctzll(wd);
+                wd &= wd - 1ull;
+                const float v = pk[rank++];
+                L |= (!(v < cut) ? 1ull : 0ull) << bit;
+                R |= (!(v > cut) ? 1ull : 0ull) << bit;
+            }
+            L |= G;
+            R |= valid & ~G & ~W;
+        } else {
+            const float *key = axis_ptr(x, y, z, r.axis);
+            for (int j0 = 0; j0 < kMaskWords; j0 += 16) {
+                float v[16];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    const uint32_t p = b + 64u * (uint32_t)(j0 + j) + (uint32_t)lane;
+                    v[j] = p < e ? key[p] : 0.0f;
+                }
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    const bool in = b + 64u * (uint32_t)(j0 + j) + (uint32_t)lane < e;
+                    const uint64_t mL = __ballot(in && !(v[j] < cut));
+                    const uint64_t mR = __ballot(in && !(v[j] > cut));
+                    L = lane == j0 + j ? mL : L;
+                    R = lane == j0 + j ? mR : R;
+                }
+            }
+        }
+        mq[lane] = L;
+        mq[kMaskWords + lane] = R;
+        const uint32_t nl = wave_sum((uint32_t)__popcll(L));
+        const uint32_t nr = wave_sum((uint32_t)__popcll(R));
+        if (lane == 0) chunk_cnt[q] = (uint64_t)nl | ((uint64_t)nr << 32);
     }
 }
 
@@ -1217,13 +1300,18 @@ int launch_stopper_counts(Ctx &c, int32_t nCells, bool le_only, const HistCounts
                               c.cell_first.p + nCells);
 }
 
-int launch_hoare(Ctx &c, int32_t nCells, int32_t *status)
+int launch_hoare(Ctx &c, int32_t nCells, int32_t *status, const HistCounts *hc)
 {
     if (!status) status = c.status.p;
     const unsigned gq = chunk_grid(c);
-    hipLaunchKernelGGL(k_hoare_masks, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
-                       c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(), c.masks.p,
-                       c.chunk_cnt.p);
+    if (hc && hc->pack)  // batched form: the window round left masks
+        hipLaunchKernelGGL(k_hoare_masks_window, dim3((gq + 3) / 4), dim3(kBlock), 0, c.stream,
+                           c.recs.p, nCells, c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(),
+                           hc->pack, c.masks.p, c.chunk_cnt.p);
+    else
+        hipLaunchKernelGGL(k_hoare_masks, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
+                           c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(), c.masks.p,
+                           c.chunk_cnt.p);
     ORB_HIP(hipGetLastError());
     ORB_TRY(scan_exclusive_u64(c, c.chunk_cnt.p, c.chunk_pre.p, (size_t)c.max_chunks + 1,
                                c.cell_first.p + nCells));
