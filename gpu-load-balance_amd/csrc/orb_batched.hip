@@ -369,25 +369,28 @@ __global__ __launch_bounds__(kBlock) void k_b_hist_window(
     __shared__ uint32_t hist[kBins];  // window keys are few: one histogram
     __shared__ uint32_t ties[kBins];
     __shared__ uint32_t sm[4];
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    __shared__ float wkeys[kW][kChunk / kW];  // each wave's window keys, packed
+    const int t = threadIdx.x, lane = t & 63, w = (int)uniform((uint32_t)t >> 6);
     const uint64_t lanes_below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const uint32_t total = first[nCells];
     for (uint32_t q = blockIdx.x; q < total; q += gridDim.x) {
         const uint32_t cell = chunk_cell[q];
         const CellRec r = recs[cell];
         if (r.found || r.flags || r.axis < 0 || r.axis > 2) continue;  // uniform
-        const uint32_t b = r.begin + (q - first[cell]) * (uint32_t)kChunk;
-        uint32_t e = b + (uint32_t)kChunk;
-        if (e > r.end) e = r.end;
-        const float lo = r.margin_l, hi = r.margin_r;
-        const float *key = axis_ptr(x, y, z, r.axis);
-        const uint32_t sb = b + (uint32_t)(w * kPerWave * 64);
+        const uint32_t b = uniform(r.begin + (q - first[cell]) * (uint32_t)kChunk);
+        const uint32_t e = uniform(r.end - b < (uint32_t)kChunk ? r.end : b + (uint32_t)kChunk);
+        const float lo = uniform(r.margin_l), hi = uniform(r.margin_r);
+        // the chunk's keys through a range-checked descriptor: loads past the
+        // chunk's end return 0 and are masked out below, with no branches
+        const __amdgpu_buffer_rsrc_t kr = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<float *>(axis_ptr(x, y, z, r.axis) + b), 0, (int)((e - b) * 4u),
+            0x00020000);
+        const uint32_t sb = (uint32_t)(w * kPerWave * 64);  // chunk position of the quarter
         float v[kPerWave];
 #pragma unroll
-        for (int j = 0; j < kPerWave; ++j) {
-            const uint32_t p = sb + (uint32_t)(64 * j + lane);
-            v[j] = p < e ? key[p] : 0.0f;
-        }
+        for (int j = 0; j < kPerWave; ++j)
+            v[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+                kr, (int)((sb + (uint32_t)lane) * 4u), 256 * j, 0));
         if (t >= 1) cand[t] = candidate(t, lo, hi);
         else cand[0] = -INFINITY;
         if (t == 0) cand[kBins] = INFINITY;
@@ -396,44 +399,64 @@ __global__ __launch_bounds__(kBlock) void k_b_hist_window(
         __syncthreads();
         const float width = hi - lo;
         const float scale = width > 0.0f ? (float)kBins / width : 0.0f;
-        uint32_t below = 0, above = 0, packed = 0;
-        uint64_t mineG = 0ull, mineW = 0ull;
-        float *dst = pack + sb;
-#pragma unroll
-        for (int j = 0; j < kPerWave; ++j) {
-            const bool ok = sb + (uint32_t)(64 * j + lane) < e;
+        const uint32_t len = e - b;
+        uint32_t below = 0, above = 0, packed = 0;  // wave-uniform
+        // lane j collects word 16w + j of each mask
+        uint32_t gl = 0, gh = 0, wl = 0, wh = 0;
+        float *dst = pack + b + sb;
+        float *wk = wkeys[w];
+        static_for<0, kPerWave>([&](auto jc) __attribute__((always_inline)) {
+            constexpr int j = decltype(jc)::value;
+            const uint32_t p0 = sb + 64u * (uint32_t)j;  // chunk position of lane 0
+            const uint64_t valid =
+                p0 >= len ? 0ull : (len - p0 >= 64u ? ~0ull : (1ull << (len - p0)) - 1ull);
             const float vj = v[j];
-            const bool bl = ok && vj < lo, abv = ok && vj > hi;
-            const bool in = ok && !(vj < lo) && !(vj > hi);
-            below += bl ? 1u : 0u;
-            above += abv ? 1u : 0u;
-            const uint64_t mG = __ballot(abv), mW = __ballot(in);
-            mineG = lane == j ? mG : mineG;
-            mineW = lane == j ? mW : mineW;
+            const uint64_t mB = __ballot(vj < lo) & valid;
+            const uint64_t mG = __ballot(vj > hi) & valid;
+            const uint64_t mW = valid & ~mB & ~mG;
+            below += (uint32_t)__popcll(mB);
+            above += (uint32_t)__popcll(mG);
+            gl = write_lane<j>(gl, (uint32_t)mG);
+            gh = write_lane<j>(gh, (uint32_t)(mG >> 32));
+            wl = write_lane<j>(wl, (uint32_t)mW);
+            wh = write_lane<j>(wh, (uint32_t)(mW >> 32));
             if (mW) {  // wave-uniform: most loads have no key in the window
-                if (in) {
-                    dst[packed + (uint32_t)__popcll(mW & lanes_below)] = vj;
-                    int tj;
-                    atomicAdd(&hist[bin_of<LE>(cand, vj, lo, scale, tj)], 1u);
-                    if (tj) atomicAdd(&ties[tj], 1u);
+                if ((mW >> lane) & 1ull) {
+                    const uint32_t slot = packed + (uint32_t)__popcll(mW & lanes_below);
+                    dst[slot] = vj;
+                    wk[slot] = vj;
                 }
                 packed += (uint32_t)__popcll(mW);
             }
+        });
+        const uint64_t mineG = ((uint64_t)gh << 32) | gl;
+        const uint64_t mineW = ((uint64_t)wh << 32) | wl;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t k = (uint32_t)lane; k < packed; k += 64) {
+            const float vk = wk[k];
+            int tj;
+            atomicAdd(&hist[bin_of<LE>(cand, vk, lo, scale, tj)], 1u);
+            if (tj) atomicAdd(&ties[tj], 1u);
         }
         uint64_t *mq = masks + (size_t)q * (2 * kMaskWords) + w * kPerWave;
         if (lane < kPerWave) {
             mq[lane] = mineG;
             mq[kMaskWords + lane] = mineW;
         }
-        if (lane == 0) comp[8 * (size_t)q + w] = packed;
-        const uint32_t ba = block_sum(below | (above << 16), sm);  // each <= 4096
+        if (lane == 0) {
+            comp[8 * (size_t)q + w] = packed;
+            sm[w] = below | (above << 16);  // each <= 1024
+        }
+        __syncthreads();
+        const uint32_t ba = sm[0] + sm[1] + sm[2] + sm[3];
         if (t == 0) comp[8 * (size_t)q + 4] = ba & 0xffffu;
         uint32_t sum = hist[t];
         if (t == 0) sum += ba & 0xffffu;
         if (t == kBins - 1) sum += ba >> 16;
         chunk_hist[(size_t)q * kBins + t] = (uint16_t)sum;
         chunk_tie[(size_t)q * kBins + t] = (uint16_t)ties[t];
-        __syncthreads();  // the next chunk rewrites cand, hist and ties
+        __syncthreads();  // the next chunk rewrites cand, hist, ties and sm
     }
 }
 

@@ -1,6 +1,8 @@
 // orb_device.hpp -- wave64 and block primitives shared by the kernels.
 #pragma once
 
+#include <type_traits>
+
 #include "orb_internal.hpp"
 
 namespace orb {
@@ -8,6 +10,14 @@ namespace orb {
 // ---------------------------------------------------------------------------
 // wave64 / block primitives
 // ---------------------------------------------------------------------------
+// A value the same in every lane, kept in a scalar register (the compiler
+// cannot tell that a value loaded for a block-uniform index is uniform).
+__device__ inline uint32_t uniform(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ inline float uniform(float v)
+{
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
 template <class T>
 __device__ inline T wave_sum(T v)
 {
@@ -26,6 +36,25 @@ __device__ inline T wave_inclusive_scan(T v)
         if (lane >= off) v += t;
     }
     return v;
+}
+
+// f(std::integral_constant<int, I>) for I in [I0, N), unrolled at compile time.
+template <int I0, int N, class F>
+__device__ inline void static_for(F &&f)
+{
+    if constexpr (I0 < N) {
+        f(std::integral_constant<int, I0>{});
+        static_for<I0 + 1, N>(f);
+    }
+}
+
+// acc with lane L replaced by the wave-uniform v (one v_writelane_b32; the
+// compiler would otherwise keep every v until the last select).
+template <int L>
+__device__ inline uint32_t write_lane(uint32_t acc, uint32_t v)
+{
+    asm("v_writelane_b32 %0, %1, %2" : "+v"(acc) : "s"(v), "n"(L));
+    return acc;
 }
 
 // Sum over a 256-thread block; every thread gets the result.
