@@ -482,6 +482,7 @@ int orb_finalize(orb_ctx *ctx)
     c->packed.release();
     c->masks.release();
     c->chunk_aux.release();
+    c->cell_cand.release();
     c->splits.release();
     c->chunk_comp.release();
     c->status.release();

@@ -242,107 +242,101 @@ __device__ inline int bin_of(const float *cand, float v, float lo, float scale, 
     return e;
 }
 
-// First histogram round: one block per chunk bins every key of the chunk
-// over the cell's 255 candidate cuts (keys outside the margins are counted
-// in registers) and writes the chunk's 256-bin row and its tie row.
-template <int LE>
-__device__ inline void k_b_hist_chunk(uint32_t q, const CellRec *recs, const uint32_t *first,
-                                      const uint32_t *chunk_cell, const float *x, const float *y,
-                                      const float *z, uint16_t *chunk_hist, uint16_t *chunk_tie)
+// Each searching cell's 255 candidate cuts for a histogram round (entry 0
+// unused), computed once per cell instead of once per chunk.
+__global__ __launch_bounds__(kBlock) void k_b_cands(const CellRec *recs, int32_t nCells,
+                                                    float *cell_cand)
 {
-    __shared__ float cand[kBins + 1];
-    __shared__ uint32_t hist[4][kBins];
-    __shared__ uint32_t ties[kBins];  // keys equal to candidate j (rare)
-    __shared__ uint32_t sm[4];
-    const uint32_t cell = chunk_cell[q];
-    const CellRec r = recs[cell];
-    if (r.found || r.flags || r.axis < 0 || r.axis > 2) return;
-    const uint32_t b = r.begin + (q - first[cell]) * (uint32_t)kChunk;
-    uint32_t e = b + (uint32_t)kChunk;
-    if (e > r.end) e = r.end;
-    const int t = threadIdx.x, w = t >> 6;
-    const float lo = r.margin_l, hi = r.margin_r;
-    uint32_t below = 0, above = 0;
-
-    // the keys first: their memory latency overlaps the candidate set-up
-    const float *key = r.axis == 0 ? x : (r.axis == 1 ? y : z);
-    uint32_t ab = (b + 3u) & ~3u;
-    if (ab > e) ab = e;
-    uint32_t ae = e & ~3u;
-    if (ae < ab) ae = ab;
-    const float4 *k4 = reinterpret_cast<const float4 *>(key + ab);
-    const uint32_t nq = (ae - ab) >> 2;
-    float4 v[kPerThread / 4];
-#pragma unroll
-    for (int u = 0; u < kPerThread / 4; ++u) {
-        uint32_t j = (uint32_t)t + (uint32_t)u * kBlock;
-        if (j < nq) v[u] = k4[j];
-        else v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int32_t c = blockIdx.x; c < nCells; c += gridDim.x) {
+        const CellRec r = recs[c];
+        if (r.found || r.flags || r.axis < 0 || r.axis > 2) continue;  // uniform
+        const int t = threadIdx.x;
+        cell_cand[(size_t)c * kBins + t] = t ? candidate(t, r.margin_l, r.margin_r) : 0.0f;
     }
-    // the misaligned head (<= 3 keys) and tail (<= 3 keys) go to the last
-    // wave, whose share of 16-byte groups is short by at least as many
-    const int ht = t - (kBlock - 64);
-    const bool has_h = ht >= 0 && (uint32_t)ht < ab - b;
-    const bool has_t = ht >= 3 && (uint32_t)(ht - 3) < e - ae;
-    const float vh = has_h ? key[b + ht] : 0.0f;
-    const float vt = has_t ? key[ae + (ht - 3)] : 0.0f;
-
-    if (t >= 1) cand[t] = candidate(t, lo, hi);
-    else cand[0] = -INFINITY;
-    if (t == 0) cand[kBins] = INFINITY;
-    hist[0][t] = 0u;
-    hist[1][t] = 0u;
-    hist[2][t] = 0u;
-    hist[3][t] = 0u;
-    ties[t] = 0u;
-    __syncthreads();
-    const float width = hi - lo;
-    const float scale = width > 0.0f ? (float)kBins / width : 0.0f;
-    uint32_t *h = hist[w];
-    // keys below lo fall in bin 0 and keys above hi in bin 255 for both
-    // comparisons: counted in registers, the rest binned.  Branch-free
-    // counters (an if/else chain over counters can make the compiler pick a
-    // counter by address, which puts them in scratch memory)
-    auto put = [&](float v, bool ok) __attribute__((always_inline)) {
-        const bool bl = ok && v < lo, abv = ok && v > hi;
-        const bool in = ok && !(v < lo) && !(v > hi);
-        below += bl ? 1u : 0u;
-        above += abv ? 1u : 0u;
-        if (in) {
-            int tj;
-            atomicAdd(&h[bin_of<LE>(cand, v, lo, scale, tj)], 1u);
-            if (tj) atomicAdd(&ties[tj], 1u);
-        }
-    };
-#pragma unroll
-    for (int u = 0; u < kPerThread / 4; ++u) {
-        const bool ok = (uint32_t)t + (uint32_t)u * kBlock < nq;
-        put(v[u].x, ok);
-        put(v[u].y, ok);
-        put(v[u].z, ok);
-        put(v[u].w, ok);
-    }
-    put(vh, has_h);
-    put(vt, has_t);
-    const uint32_t ba = block_sum(below | (above << 16), sm);  // each <= 4096
-    uint32_t sum = hist[0][t] + hist[1][t] + hist[2][t] + hist[3][t];  // <= 4096
-    if (t == 0) sum += ba & 0xffffu;
-    if (t == kBins - 1) sum += ba >> 16;
-    chunk_hist[(size_t)q * kBins + t] = (uint16_t)sum;
-    chunk_tie[(size_t)q * kBins + t] = (uint16_t)ties[t];
 }
 
+// The candidate table of a cell into LDS, with the -inf / +inf sentinels
+// at 0 and kBins.
+__device__ inline void load_cands(float *cand, const float *cell_cand, uint32_t cell)
+{
+    const int t = threadIdx.x;
+    cand[t] = t ? cell_cand[(size_t)cell * kBins + t] : -INFINITY;
+    if (t == 0) cand[kBins] = INFINITY;
+}
+
+// First histogram round: one block per chunk bins every key of the chunk
+// over the cell's 255 candidate cuts and writes the chunk's 256-bin row and
+// its tie row.  Wave w takes the quarter [1024w, 1024w + 1024) of the
+// chunk, 64 consecutive keys per load through a range-checked descriptor
+// (loads past the chunk's end return 0 and are masked out); keys outside
+// the margins are counted with ballots, the rest binned.
 template <int LE>
 __global__ __launch_bounds__(kBlock) void k_b_hist(const CellRec *recs, int32_t nCells,
                                                    const uint32_t *first,
                                                    const uint32_t *chunk_cell, const float *x,
                                                    const float *y, const float *z,
-                                                   uint16_t *chunk_hist, uint16_t *chunk_tie)
+                                                   const float *cell_cand, uint16_t *chunk_hist,
+                                                   uint16_t *chunk_tie)
 {
+    constexpr int kW = kBlock / 64;
+    constexpr int kPerWave = kMaskWords / kW;  // 16 loads of 64 keys
+    __shared__ float cand[kBins + 1];
+    __shared__ uint32_t hist[kW][kBins];
+    __shared__ uint32_t ties[kBins];  // keys equal to candidate j (rare)
+    __shared__ uint32_t sm[kW];
+    const int t = threadIdx.x, lane = t & 63, w = (int)uniform((uint32_t)t >> 6);
     const uint32_t total = first[nCells];
     for (uint32_t q = blockIdx.x; q < total; q += gridDim.x) {
-        k_b_hist_chunk<LE>(q, recs, first, chunk_cell, x, y, z, chunk_hist, chunk_tie);
-        __syncthreads();  // the next chunk reuses this block's LDS
+        const uint32_t cell = chunk_cell[q];
+        const CellRec r = recs[cell];
+        if (r.found || r.flags || r.axis < 0 || r.axis > 2) continue;  // uniform
+        const uint32_t b = uniform(r.begin + (q - first[cell]) * (uint32_t)kChunk);
+        const uint32_t e = uniform(r.end - b < (uint32_t)kChunk ? r.end : b + (uint32_t)kChunk);
+        const float lo = uniform(r.margin_l), hi = uniform(r.margin_r);
+        const __amdgpu_buffer_rsrc_t kr = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<float *>(axis_ptr(x, y, z, r.axis) + b), 0, (int)((e - b) * 4u),
+            0x00020000);
+        const uint32_t sb = (uint32_t)(w * kPerWave * 64);  // chunk position of the quarter
+        float v[kPerWave];
+#pragma unroll
+        for (int j = 0; j < kPerWave; ++j)
+            v[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+                kr, (int)((sb + (uint32_t)lane) * 4u), 256 * j, 0));
+        load_cands(cand, cell_cand, cell);
+#pragma unroll
+        for (int u = 0; u < kW; ++u) hist[u][t] = 0u;
+        ties[t] = 0u;
+        __syncthreads();
+        const float width = hi - lo;
+        const float scale = width > 0.0f ? (float)kBins / width : 0.0f;
+        const uint32_t len = e - b;
+        uint32_t *h = hist[w];
+        uint32_t below = 0, above = 0;  // wave-uniform
+        static_for<0, kPerWave>([&](auto jc) __attribute__((always_inline)) {
+            constexpr int j = decltype(jc)::value;
+            const uint32_t p0 = sb + 64u * (uint32_t)j;  // chunk position of lane 0
+            const uint64_t valid =
+                p0 >= len ? 0ull : (len - p0 >= 64u ? ~0ull : (1ull << (len - p0)) - 1ull);
+            const float vj = v[j];
+            const uint64_t mB = __ballot(vj < lo) & valid;
+            const uint64_t mA = __ballot(vj > hi) & valid;
+            below += (uint32_t)__popcll(mB);
+            above += (uint32_t)__popcll(mA);
+            if (((valid & ~mB & ~mA) >> lane) & 1ull) {
+                int tj;
+                atomicAdd(&h[bin_of<LE>(cand, vj, lo, scale, tj)], 1u);
+                if (tj) atomicAdd(&ties[tj], 1u);
+            }
+        });
+        if (lane == 0) sm[w] = below | (above << 16);  // each <= 1024
+        __syncthreads();
+        const uint32_t ba = sm[0] + sm[1] + sm[2] + sm[3];
+        uint32_t sum = hist[0][t] + hist[1][t] + hist[2][t] + hist[3][t];  // <= 4096
+        if (t == 0) sum += ba & 0xffffu;
+        if (t == kBins - 1) sum += ba >> 16;
+        chunk_hist[(size_t)q * kBins + t] = (uint16_t)sum;
+        chunk_tie[(size_t)q * kBins + t] = (uint16_t)ties[t];
+        __syncthreads();  // the next chunk rewrites cand, hist, ties and sm
     }
 }
 
@@ -360,8 +354,8 @@ __global__ __launch_bounds__(kBlock) void k_b_hist(const CellRec *recs, int32_t 
 template <int LE>
 __global__ __launch_bounds__(kBlock) void k_b_hist_window(
     const CellRec *recs, int32_t nCells, const uint32_t *first, const uint32_t *chunk_cell,
-    const float *x, const float *y, const float *z, uint16_t *chunk_hist, uint16_t *chunk_tie,
-    float *pack, uint32_t *comp, uint64_t *masks)
+    const float *x, const float *y, const float *z, const float *cell_cand, uint16_t *chunk_hist,
+    uint16_t *chunk_tie, float *pack, uint32_t *comp, uint64_t *masks)
 {
     constexpr int kW = kBlock / 64;
     constexpr int kPerWave = kMaskWords / kW;  // 16 loads of 64 keys
@@ -391,9 +385,7 @@ __global__ __launch_bounds__(kBlock) void k_b_hist_window(
         for (int j = 0; j < kPerWave; ++j)
             v[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
                 kr, (int)((sb + (uint32_t)lane) * 4u), 256 * j, 0));
-        if (t >= 1) cand[t] = candidate(t, lo, hi);
-        else cand[0] = -INFINITY;
-        if (t == 0) cand[kBins] = INFINITY;
+        load_cands(cand, cell_cand, cell);
         hist[t] = 0u;
         ties[t] = 0u;
         __syncthreads();
@@ -1238,18 +1230,25 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
     const unsigned gq = chunk_grid(c);
     const unsigned gr = (unsigned)((gq + kReduceRows - 1) / kReduceRows * 4);
     const unsigned gw = (unsigned)((nCells + 3) / 4);
+    const unsigned gc = (unsigned)(nCells < 4096 ? nCells : 4096);
     for (int r = 0; r < kRounds; ++r) {
         const int tb = timing_mark(c);
         // rounds: all keys, all keys + packing the window, then the packed keys
         float *ckeys = c.packed.p;
         const int mode = r == 0 ? 0 : (r == 1 ? 1 : 2);
+        if (mode < 2) {  // the block-per-chunk rounds read the candidates from a table
+            hipLaunchKernelGGL(k_b_cands, dim3(gc), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
+                               c.cell_cand.p);
+            ORB_HIP(hipGetLastError());
+        }
 #define ORB_HIST(L)                                                                              \
     hipLaunchKernelGGL((k_b_hist<L>), dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,     \
-                       c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(), chunk_hist, chunk_tie)
+                       c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(), c.cell_cand.p,       \
+                       chunk_hist, chunk_tie)
 #define ORB_HIST_WINDOW(L)                                                                       \
     hipLaunchKernelGGL((k_b_hist_window<L>), dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p,      \
-                       nCells, c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(), chunk_hist,  \
-                       chunk_tie, ckeys, c.chunk_comp.p, c.masks.p)
+                       nCells, c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(),              \
+                       c.cell_cand.p, chunk_hist, chunk_tie, ckeys, c.chunk_comp.p, c.masks.p)
         const unsigned gp = (gq + 3) / 4;  // a wave per chunk
         if (le) {
             if (mode == 0) ORB_HIST(1);

@@ -315,6 +315,7 @@ int ensure_level_capacity(Ctx &c, int64_t nCells)
     ORB_TRY(c.chunk_pre.alloc(c.max_chunks + 1));
     ORB_TRY(c.masks.alloc(2 * (size_t)kMaskWords * (size_t)c.max_chunks));
     ORB_TRY(c.chunk_aux.alloc(c.max_chunks + 1));
+    ORB_TRY(c.cell_cand.alloc((size_t)m * kBins));
     ORB_TRY(c.splits.alloc(m));
     ORB_TRY(c.chunk_comp.alloc(8 * ((size_t)c.max_chunks + 1)));
     ORB_TRY(c.ids_dev.alloc(m));

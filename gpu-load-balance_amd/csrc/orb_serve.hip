@@ -543,6 +543,7 @@ int orb_server_destroy(orb_server *o)
     c.packed.release();
     c.masks.release();
     c.chunk_aux.release();
+    c.cell_cand.release();
     c.status.release();
     c.hstatus.release();
     c.hranges.release();
