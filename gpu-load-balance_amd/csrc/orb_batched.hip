@@ -312,22 +312,29 @@ __global__ __launch_bounds__(kBlock) void k_b_hist(const CellRec *recs, int32_t 
         const uint32_t len = e - b;
         uint32_t *h = hist[w];
         uint32_t below = 0, above = 0;  // wave-uniform
-        static_for<0, kPerWave>([&](auto jc) __attribute__((always_inline)) {
-            constexpr int j = decltype(jc)::value;
-            const uint32_t p0 = sb + 64u * (uint32_t)j;  // chunk position of lane 0
-            const uint64_t valid =
-                p0 >= len ? 0ull : (len - p0 >= 64u ? ~0ull : (1ull << (len - p0)) - 1ull);
-            const float vj = v[j];
-            const uint64_t mB = __ballot(vj < lo) & valid;
-            const uint64_t mA = __ballot(vj > hi) & valid;
-            below += (uint32_t)__popcll(mB);
-            above += (uint32_t)__popcll(mA);
-            if (((valid & ~mB & ~mA) >> lane) & 1ull) {
-                int tj;
-                atomicAdd(&h[bin_of<LE>(cand, vj, lo, scale, tj)], 1u);
-                if (tj) atomicAdd(&ties[tj], 1u);
-            }
-        });
+        auto scan = [&](auto full) __attribute__((always_inline)) {
+            static_for<0, kPerWave>([&](auto jc) __attribute__((always_inline)) {
+                constexpr int j = decltype(jc)::value;
+                const uint32_t p0 = sb + 64u * (uint32_t)j;  // chunk position of lane 0
+                const uint64_t valid =
+                    decltype(full)::value
+                        ? ~0ull
+                        : (p0 >= len ? 0ull
+                                     : (len - p0 >= 64u ? ~0ull : (1ull << (len - p0)) - 1ull));
+                const float vj = v[j];
+                const uint64_t mB = __ballot(vj < lo) & valid;
+                const uint64_t mA = __ballot(vj > hi) & valid;
+                below += (uint32_t)__popcll(mB);
+                above += (uint32_t)__popcll(mA);
+                if (((valid & ~mB & ~mA) >> lane) & 1ull) {
+                    int tj;
+                    atomicAdd(&h[bin_of<LE>(cand, vj, lo, scale, tj)], 1u);
+                    if (tj) atomicAdd(&ties[tj], 1u);
+                }
+            });
+        };
+        if (len == (uint32_t)kChunk) scan(std::true_type{});  // uniform
+        else scan(std::false_type{});
         if (lane == 0) sm[w] = below | (above << 16);  // each <= 1024
         __syncthreads();
         const uint32_t ba = sm[0] + sm[1] + sm[2] + sm[3];
@@ -351,8 +358,10 @@ __global__ __launch_bounds__(kBlock) void k_b_hist(const CellRec *recs, int32_t 
 // 64k + j): keys above hi (masks[q][0]) and keys in the window (masks[q][1]).
 // The cell's final cut lies in [lo, hi], so these decide every stopper bit
 // but those of the packed keys.
+// Eight waves per SIMD (the compiler would keep the load offsets in scalar
+// registers and stop at seven).
 template <int LE>
-__global__ __launch_bounds__(kBlock) void k_b_hist_window(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_b_hist_window(
     const CellRec *recs, int32_t nCells, const uint32_t *first, const uint32_t *chunk_cell,
     const float *x, const float *y, const float *z, const float *cell_cand, uint16_t *chunk_hist,
     uint16_t *chunk_tie, float *pack, uint32_t *comp, uint64_t *masks)
