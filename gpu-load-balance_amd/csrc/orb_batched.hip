@@ -364,13 +364,12 @@ template <int LE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_b_hist_window(
     const CellRec *recs, int32_t nCells, const uint32_t *first, const uint32_t *chunk_cell,
     const float *x, const float *y, const float *z, const float *cell_cand, uint16_t *chunk_hist,
-    uint16_t *chunk_tie, float *pack, uint32_t *comp, uint64_t *masks)
+    float *pack, uint32_t *comp, uint64_t *masks)
 {
     constexpr int kW = kBlock / 64;
     constexpr int kPerWave = kMaskWords / kW;  // 16 loads of 64 keys
     __shared__ float cand[kBins + 1];
     __shared__ uint32_t hist[kBins];  // window keys are few: one histogram
-    __shared__ uint32_t ties[kBins];
     __shared__ uint32_t sm[4];
     __shared__ float wkeys[kW][kChunk / kW];  // each wave's window keys, packed
     const int t = threadIdx.x, lane = t & 63, w = (int)uniform((uint32_t)t >> 6);
@@ -396,7 +395,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
                 kr, (int)((sb + (uint32_t)lane) * 4u), 256 * j, 0));
         load_cands(cand, cell_cand, cell);
         hist[t] = 0u;
-        ties[t] = 0u;
         __syncthreads();
         const float width = hi - lo;
         const float scale = width > 0.0f ? (float)kBins / width : 0.0f;
@@ -436,9 +434,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
         __builtin_amdgcn_wave_barrier();
         for (uint32_t k = (uint32_t)lane; k < packed; k += 64) {
             const float vk = wk[k];
-            int tj;
+            int tj;  // ties: the stopper counts of cells past this round use the packed keys
             atomicAdd(&hist[bin_of<LE>(cand, vk, lo, scale, tj)], 1u);
-            if (tj) atomicAdd(&ties[tj], 1u);
         }
         uint64_t *mq = masks + (size_t)q * (2 * kMaskWords) + w * kPerWave;
         if (lane < kPerWave) {
@@ -456,8 +453,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
         if (t == 0) sum += ba & 0xffffu;
         if (t == kBins - 1) sum += ba >> 16;
         chunk_hist[(size_t)q * kBins + t] = (uint16_t)sum;
-        chunk_tie[(size_t)q * kBins + t] = (uint16_t)ties[t];
-        __syncthreads();  // the next chunk rewrites cand, hist, ties and sm
+        __syncthreads();  // the next chunk rewrites cand, hist and sm
     }
 }
 
@@ -470,14 +466,12 @@ template <int LE>
 __global__ __launch_bounds__(kBlock) void k_b_hist_packed(const CellRec *recs, int32_t nCells,
                                                           const uint32_t *first,
                                                           const uint32_t *chunk_cell,
-                                                          uint16_t *chunk_hist,
-                                                          uint16_t *chunk_tie, const float *ckeys,
+                                                          uint16_t *chunk_hist, const float *ckeys,
                                                           const uint32_t *comp)
 {
     constexpr int kW = kBlock / 64;
     __shared__ float cand[kW][kBins + 1];
     __shared__ uint32_t hist[kW][kBins];
-    __shared__ uint32_t ties[kW][kBins];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t total = first[nCells];
     for (uint32_t q = blockIdx.x * kW + (uint32_t)w; q < total; q += gridDim.x * kW) {
@@ -505,25 +499,21 @@ __global__ __launch_bounds__(kBlock) void k_b_hist_packed(const CellRec *recs, i
         above = wave_sum(above);
         below += comp[8 * (size_t)q + 4];  // keys below the packing window
         uint16_t *hrow = chunk_hist + (size_t)q * kBins;
-        uint16_t *trow = chunk_tie + (size_t)q * kBins;
         if (__ballot(any) == 0ull) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int bin = 4 * lane + i;
                 hrow[bin] = (uint16_t)(bin == 0 ? below : (bin == kBins - 1 ? above : 0u));
-                trow[bin] = 0;
             }
             continue;
         }
         float *cw = cand[w];
         uint32_t *hw = hist[w];
-        uint32_t *tw = ties[w];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int j = 4 * lane + i;
             cw[j] = j == 0 ? -INFINITY : candidate(j, lo, hi);
             hw[j] = 0u;
-            tw[j] = 0u;
         }
         if (lane == 0) cw[kBins] = INFINITY;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
@@ -539,7 +529,6 @@ __global__ __launch_bounds__(kBlock) void k_b_hist_packed(const CellRec *recs, i
                 if (!(v < lo) && !(v > hi)) {
                     int tj;
                     atomicAdd(&hw[bin_of<LE>(cw, v, lo, scale, tj)], 1u);
-                    if (tj) atomicAdd(&tw[tj], 1u);
                 }
             }
         }
@@ -552,7 +541,6 @@ __global__ __launch_bounds__(kBlock) void k_b_hist_packed(const CellRec *recs, i
             if (bin == 0) v += below;
             if (bin == kBins - 1) v += above;
             hrow[bin] = (uint16_t)v;
-            trow[bin] = (uint16_t)tw[bin];
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         __builtin_amdgcn_wave_barrier();  // the rows are rewritten for the next chunk
@@ -560,8 +548,8 @@ __global__ __launch_bounds__(kBlock) void k_b_hist_packed(const CellRec *recs, i
 }
 
 // Sums chunk histograms into cell histograms: block g takes chunks
-// [g*64, g*64+64), thread t owns bin t; the 64 rows are loaded 16 at a time
-// (independent loads), and one atomic per (block, cell, bin) flushes a run.
+// [g*64, g*64+64), thread t owns bin t; all 64 row loads are in flight at
+// once, and one atomic per (block, cell, bin) flushes a run.
 __global__ __launch_bounds__(kBlock) void k_b_hist_reduce(const CellRec *recs, int32_t nCells,
                                                           const uint32_t *first,
                                                           const uint32_t *chunk_cell,
@@ -586,24 +574,19 @@ __global__ __launch_bounds__(kBlock) void k_b_hist_reduce(const CellRec *recs, i
     __syncthreads();
     int32_t cur = -1;
     uint32_t acc = 0;
-    for (uint32_t r0 = 0; r0 < nrows; r0 += 16) {
-        uint32_t v[16];
+    uint32_t v[kReduceRows];  // every row's load in flight at once
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
-            uint32_t row = r0 + u;
-            v[u] = (row < nrows && scell[row] >= 0) ? chunk_hist[(size_t)(q0 + row) * kBins + t] : 0u;
-        }
+    for (int u = 0; u < kReduceRows; ++u)
+        v[u] = ((uint32_t)u < nrows && scell[u] >= 0) ? chunk_hist[(size_t)(q0 + u) * kBins + t] : 0u;
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
-            uint32_t row = r0 + u;
-            const int32_t cl = row < nrows ? scell[row] : cur;
-            if (cl != cur) {
-                if (cur >= 0 && acc) atomicAdd(&cell_hist[(size_t)cur * kBins + t], acc);
-                cur = cl;
-                acc = 0;
-            }
-            acc += v[u];
+    for (int u = 0; u < kReduceRows; ++u) {
+        const int32_t cl = (uint32_t)u < nrows ? scell[u] : cur;
+        if (cl != cur) {
+            if (cur >= 0 && acc) atomicAdd(&cell_hist[(size_t)cur * kBins + t], acc);
+            cur = cl;
+            acc = 0;
         }
+        acc += v[u];
     }
     if (cur >= 0 && acc) atomicAdd(&cell_hist[(size_t)cur * kBins + t], acc);
     __syncthreads();  // scell is rewritten for the next tile
@@ -1257,22 +1240,22 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
 #define ORB_HIST_WINDOW(L)                                                                       \
     hipLaunchKernelGGL((k_b_hist_window<L>), dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p,      \
                        nCells, c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(),              \
-                       c.cell_cand.p, chunk_hist, chunk_tie, ckeys, c.chunk_comp.p, c.masks.p)
+                       c.cell_cand.p, chunk_hist, ckeys, c.chunk_comp.p, c.masks.p)
         const unsigned gp = (gq + 3) / 4;  // a wave per chunk
         if (le) {
             if (mode == 0) ORB_HIST(1);
             else if (mode == 1) ORB_HIST_WINDOW(1);
             else
                 hipLaunchKernelGGL(k_b_hist_packed<1>, dim3(gp), dim3(kBlock), 0, c.stream, c.recs.p,
-                                   nCells, c.cell_first.p, c.chunk_cell.p, chunk_hist, chunk_tie,
-                                   ckeys, c.chunk_comp.p);
+                                   nCells, c.cell_first.p, c.chunk_cell.p, chunk_hist, ckeys,
+                                   c.chunk_comp.p);
         } else {
             if (mode == 0) ORB_HIST(0);
             else if (mode == 1) ORB_HIST_WINDOW(0);
             else
                 hipLaunchKernelGGL(k_b_hist_packed<0>, dim3(gp), dim3(kBlock), 0, c.stream, c.recs.p,
-                                   nCells, c.cell_first.p, c.chunk_cell.p, chunk_hist, chunk_tie,
-                                   ckeys, c.chunk_comp.p);
+                                   nCells, c.cell_first.p, c.chunk_cell.p, chunk_hist, ckeys,
+                                   c.chunk_comp.p);
         }
 #undef ORB_HIST
 #undef ORB_HIST_WINDOW

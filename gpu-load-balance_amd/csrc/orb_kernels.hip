@@ -1227,25 +1227,7 @@ __global__ __launch_bounds__(kBlock) void k_b_stopper_counts(
         uint32_t e = b + (uint32_t)kChunk;
         if (e > r.end) e = r.end;
         uint32_t lt = 0, le = 0;
-        if (r.found >= 2) {
-            const int j = r.found - 1;
-            const uint16_t *row = hist + (size_t)q * kBins;
-            uint32_t s = 0;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int bin = 4 * lane + i;
-                if (bin < j) s += row[bin];
-            }
-            const uint32_t below = wave_sum(s);
-            const uint32_t eq = tie[(size_t)q * kBins + (uint32_t)j];
-            if (hist_le) {
-                le = below;
-                lt = below - eq;
-            } else {
-                lt = below;
-                le = below + eq;
-            }
-        } else if (r.passes > kHistK) {
+        if (r.passes > kHistK) {
             // the cell went through the packing round: its keys outside the
             // packing window lie below or above every later cut, so the
             // packed keys and the count below decide
@@ -1263,6 +1245,26 @@ __global__ __launch_bounds__(kBlock) void k_b_stopper_counts(
             const uint32_t bel = comp[8 * (size_t)q + 4];
             lt = wave_sum(lt) + bel;
             le = wave_sum(le) + bel;
+        } else if (r.found >= 2) {
+            // found by the first round: its rows and tie rows are this
+            // chunk's counts around the cut
+            const int j = r.found - 1;
+            const uint16_t *row = hist + (size_t)q * kBins;
+            uint32_t s = 0;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int bin = 4 * lane + i;
+                if (bin < j) s += row[bin];
+            }
+            const uint32_t below = wave_sum(s);
+            const uint32_t eq = tie[(size_t)q * kBins + (uint32_t)j];
+            if (hist_le) {
+                le = below;
+                lt = below - eq;
+            } else {
+                lt = below;
+                le = below + eq;
+            }
         } else {
             const int axis = r.axis < 0 ? 0 : (r.axis > 2 ? 2 : r.axis);
             const float *key = axis_ptr(x, y, z, axis);
