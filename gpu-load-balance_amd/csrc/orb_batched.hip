@@ -154,18 +154,13 @@ __global__ void k_b_init(orb_cell *heap, uint8_t *exists, uint8_t *sub, int64_t 
     if (i < 2) status[i] = 0;
 }
 
-// Level records from the device heap; small cells are marked for the
-// subtree kernel, cells inside a finished subtree are skipped.
-__global__ void k_b_recs(const LevelEntry *entries, int32_t nCells, const orb_cell *heap,
-                         const uint32_t *ranges, const uint8_t *sub, int use_subtree,
-                         uint32_t sub_max, CellRec *recs, uint32_t *nchunks)
+// Level record of entry i from the device heap (small cells are marked
+// for the subtree kernel, cells inside a finished subtree are skipped);
+// returns its chunk count.
+__device__ inline uint32_t level_rec(const LevelEntry *entries, int32_t i, const orb_cell *heap,
+                                     const uint32_t *ranges, const uint8_t *sub, int use_subtree,
+                                     uint32_t sub_max, CellRec *recs)
 {
-    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > nCells) return;
-    if (i == nCells) {
-        nchunks[nCells] = 0u;
-        return;
-    }
     const LevelEntry en = entries[i];
     const int32_t id = en.id;
     const int64_t g = (int64_t)en.hb + id;
@@ -193,7 +188,17 @@ __global__ void k_b_recs(const LevelEntry *entries, int32_t nCells, const orb_ce
         r.flags = 2;
     else r.flags = 0;
     recs[i] = r;
-    nchunks[i] = r.flags ? 0u : (n + (uint32_t)kChunk - 1u) / (uint32_t)kChunk;
+    return r.flags ? 0u : (n + (uint32_t)kChunk - 1u) / (uint32_t)kChunk;
+}
+
+__global__ void k_b_recs(const LevelEntry *entries, int32_t nCells, const orb_cell *heap,
+                         const uint32_t *ranges, const uint8_t *sub, int use_subtree,
+                         uint32_t sub_max, CellRec *recs, uint32_t *nchunks)
+{
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > nCells) return;
+    nchunks[i] = i == nCells ? 0u
+                             : level_rec(entries, i, heap, ranges, sub, use_subtree, sub_max, recs);
 }
 
 // in-order index t in [1, 255] of the depth-8 candidate tree on [lo, hi]
@@ -675,7 +680,8 @@ __device__ inline RoundOut replay_round(const uint32_t *P, float lo, float hi, f
 __global__ __launch_bounds__(kBlock) void k_b_replay(CellRec *recs, int32_t nCells,
                                                      uint32_t *cell_hist, orb_cell *heap,
                                                      uint32_t *trace, int32_t *level_passes,
-                                                     uint8_t *exists, int32_t *status)
+                                                     uint8_t *exists, int32_t *status,
+                                                     float *next_cand)
 {
     __shared__ uint32_t P[4][kBins + 1];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -710,6 +716,13 @@ __global__ __launch_bounds__(kBlock) void k_b_replay(CellRec *recs, int32_t nCel
     const RoundOut ro = replay_round(P[w], r.margin_l, r.margin_r, target, r.passes, lane,
                                      trace ? trace + ((size_t)r.hb + r.id) * ORB_MAX_PASSES
                                            : nullptr);
+    if (next_cand && !ro.found) {  // the next round's candidate table (uniform)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int j = 4 * lane + i;
+            next_cand[(size_t)c * kBins + j] = j ? candidate(j, ro.L, ro.R) : 0.0f;
+        }
+    }
     if (lane != 0) return;
     const float L = ro.L, R = ro.R;
     const int32_t passes = ro.passes;
@@ -1228,7 +1241,8 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
         // rounds: all keys, all keys + packing the window, then the packed keys
         float *ckeys = c.packed.p;
         const int mode = r == 0 ? 0 : (r == 1 ? 1 : 2);
-        if (mode < 2) {  // the block-per-chunk rounds read the candidates from a table
+        if (mode == 0) {  // the block-per-chunk rounds read the candidates from a table; the
+                          // first round's replay writes the second round's
             hipLaunchKernelGGL(k_b_cands, dim3(gc), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
                                c.cell_cand.p);
             ORB_HIP(hipGetLastError());
@@ -1264,7 +1278,8 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
         hipLaunchKernelGGL(k_b_hist_reduce, dim3(gr), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
                            c.cell_first.p, c.chunk_cell.p, chunk_hist, cell_hist);
         hipLaunchKernelGGL(k_b_replay, dim3(gw), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
-                           cell_hist, heap, trace, level_passes, exists, status);
+                           cell_hist, heap, trace, level_passes, exists, status,
+                           mode == 0 ? c.cell_cand.p : nullptr);
         ORB_HIP(hipGetLastError());
     }
     const int tp = timing_mark(c);
