@@ -33,42 +33,49 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_form(orb, ctx, form, levels, steps, warmup, timed_events):
-    """Times `steps` builds.  With timed_events the kernels are bracketed by HIP
-    events inside the timed builds (the plain form: the count-kernel roofline is
-    measured live); without, the batched form replays its captured graph and
-    one extra, untimed build with events gives the kernel breakdown."""
-    for _ in range(warmup):
-        ctx.restore_particles()
-        ctx.build(form, levels)
-    ctx.enable_timing(timed_events)
-    times = []
-    tm_total = {"count_ms": 0.0, "partition_ms": 0.0, "other_ms": 0.0, "count_bytes": 0,
-                "count_launches": 0}
-    info = None
+TM_KEYS = ("count_ms", "partition_ms", "other_ms", "count_bytes", "count_launches", "swap_ms",
+           "swap_launches")
+
+
+def timed_builds(ctx, form, levels, steps):
+    times, info = [], None
     for _ in range(steps):
         ctx.restore_particles()          # device copy + range reset, synchronized
         t0 = time.perf_counter()
         info = ctx.build(form, levels)   # synchronizes the stream before returning
         t1 = time.perf_counter()
         times.append((t1 - t0) * 1e3)
-        if timed_events:
-            tm = ctx.timing()
-            for k in tm_total:
-                tm_total[k] += tm[k]
-    if not timed_events:
-        ctx.enable_timing(True)
+    return times, info
+
+
+def run_form(orb, ctx, form, levels, steps, warmup, instrumented_steps):
+    """Times `steps` builds with no instrumentation (the batched form replays its
+    captured graph).  Then `instrumented_steps` more timed builds run with HIP
+    events recorded on the library's stream around the kernels (the plain form's
+    count launches: the count-pass roofline is measured live in these; the
+    swap launches of either form); their own mean time is reported beside."""
+    for _ in range(warmup):
         ctx.restore_particles()
         ctx.build(form, levels)
-        tm = ctx.timing()
-        for k in tm_total:
-            tm_total[k] = tm[k] * steps   # per-step figures below divide by steps
     ctx.enable_timing(False)
+    times, info = timed_builds(ctx, form, levels, steps)
+    ctx.enable_timing(True)
+    tm_total = {k: 0 for k in TM_KEYS}
+    itimes = []
+    for _ in range(instrumented_steps):
+        t, _ = timed_builds(ctx, form, levels, 1)
+        itimes += t
+        tm = ctx.timing()
+        for k in TM_KEYS:
+            tm_total[k] += tm[k]
+    ctx.enable_timing(False)
+    tm_total["instrumented_ms"] = sum(itimes) / max(len(itimes), 1)
+    tm_total["instrumented_steps"] = instrumented_steps
     return times, tm_total, info
 
 
 def committed_traffic(kernel="orb::k_count"):
-    """HBM bytes per launch of the count kernel from the committed FETCH_SIZE /
+    """HBM bytes per launch of a kernel from the committed FETCH_SIZE /
     WRITE_SIZE summary (rocprofv3 counter passes, scaled by the copy-kernel
     calibration stated in that file); None if absent."""
     path = os.path.join(ROOT, "profiles", "orb_build_c2_pmc_traffic.md")
@@ -329,7 +336,7 @@ def main():
                 f"{tuning['subtree_max_keys']}")
         try:
             times, tm, info = run_form(orb, ctx, form, levels, args.steps, args.warmup,
-                                       timed_events=(name == "plain"))
+                                       instrumented_steps=(args.steps if name == "plain" else 1))
         except orb.OrbError as e:
             if e.code == orb.ERR_STATE:
                 log(f"[bench] form={name} unavailable")
@@ -346,8 +353,10 @@ def main():
             res[name]["level_ms"] = [round(v, 4) for v in ctx.level_timing()]
             ctx.enable_timing(False)
         log(f"[bench] form={name} mean {res[name]['ms']:.3f} ms min {min(times):.3f} ms "
-            f"passes={info['total_passes']} count_ms/step={tm['count_ms'] / args.steps:.3f} "
-            f"partition_ms/step={tm['partition_ms'] / args.steps:.3f}")
+            f"passes={info['total_passes']} (instrumented build {tm['instrumented_ms']:.3f} ms: "
+            f"count {tm['count_ms'] / tm['instrumented_steps']:.3f} ms, partition "
+            f"{tm['partition_ms'] / tm['instrumented_steps']:.3f} ms, swap "
+            f"{tm['swap_ms'] / tm['instrumented_steps']:.3f} ms)")
     ctx.close()
 
     if args.config != "c2":
@@ -408,8 +417,35 @@ def main():
             "launches": tm["count_launches"],
             "avg_launch_us": round(tm["count_ms"] * 1e3 / max(tm["count_launches"], 1), 2),
             "bytes_per_launch": int(tm["count_bytes"] / max(tm["count_launches"], 1)),
+            "timed_in": f"{tm['instrumented_steps']} timed plain builds with HIP events around "
+                        f"each count launch (mean {tm['instrumented_ms']:.3f} ms; plain_ms is "
+                        f"timed without events)",
         },
     }
+    # the batched build's dominant kernel: the Hoare swap, one launch per level
+    tb = res.get("batched", {}).get("tm")
+    if tb and tb["swap_launches"] > 0:
+        # each level of the full tree partitions every particle; at the 50%
+        # stopper density of uniform keys every line of x, y, z is read and
+        # written once: 24 B per particle per launch
+        sw_bytes = 24 * n
+        sw_us = tb["swap_ms"] * 1e3 / tb["swap_launches"]
+        sw_ach = sw_bytes / (sw_us * 1e-6) / 1e9
+        out["roofline_partition"] = {
+            "bound": "hbm",
+            "kernel": "k_hoare_swap (batched build)",
+            "achieved": round(sw_ach, 1),
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": round(sw_ach / HBM_PEAK_GBPS, 4),
+            "traffic": (committed_traffic("orb::k_hoare_swap<true>")
+                        if (args.log2n, args.d, args.levels) == (27, 1024, "full") else None),
+            "launches": tb["swap_launches"],
+            "avg_launch_us": round(sw_us, 2),
+            "bytes_per_launch": sw_bytes,
+            "timed_in": "one batched build with HIP events around each swap launch, after the "
+                        "timed (graph-replayed, uninstrumented) builds",
+        }
     if not args.no_cpu:
         scale = n / (1 << args.cpu_log2n)
         nt = host_threads()

@@ -104,7 +104,7 @@ int timing_pair(Ctx &c, int begin_mark, int kind)
     return ORB_OK;
 }
 
-int timing_collect(Ctx &c)
+int timing_collect(Ctx &c, bool reuse_events)
 {
     for (const auto &p : c.tpairs) {
         float ms = 0.f;
@@ -118,12 +118,15 @@ int timing_collect(Ctx &c)
             c.tm.count_launches += 1;
         } else if (p.kind == kTimePartition) {
             c.tm.partition_ms += ms;
+        } else if (p.kind == kTimeSwap) {
+            c.tm.swap_ms += ms;
+            c.tm.swap_launches += 1;
         } else {
             c.tm.other_ms += ms;
         }
     }
     c.tpairs.clear();
-    c.tnext = 0;
+    if (reuse_events) c.tnext = 0;
     return ORB_OK;
 }
 
@@ -229,6 +232,7 @@ static int partition_service(Ctx &c, const orb_cell *cells, int32_t nCells, bool
         float ms = 0.f;
         ORB_HIP(hipEventElapsedTime(&ms, c.ev[0], c.ev[1]));
         c.tm.partition_ms += ms;
+        ORB_TRY(timing_collect(c, !c.building));  // the swap kernel's own pair
     }
     c.table_ids.clear();  // ranges of the children changed; next level rebuilds
     if (rc < 0) return rc;
@@ -267,6 +271,11 @@ static int start_build(Ctx &c)
 
 static int build_plain(Ctx &c, int32_t levels_mode, orb_build_info *info)
 {
+    struct Building {
+        Ctx &c;
+        explicit Building(Ctx &cc) : c(cc) { c.building = true; }
+        ~Building() { c.building = false; }
+    } building(c);
     ORB_TRY(start_build(c));
     const int32_t D = n_levels(c.d);
     std::vector<int32_t> ids;

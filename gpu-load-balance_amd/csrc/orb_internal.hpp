@@ -389,6 +389,7 @@ struct Ctx {
     std::vector<uint32_t> trace_counts;
     bool trace = false;
     bool timing = false;
+    bool building = false;     // inside a plain build (its level marks are pending)
     bool use_graph = true;    // batched form: capture the build once, replay it
     bool results_on_device = false;  // batched form: heap/exists/trace not yet mirrored
     std::vector<double> level_ms;    // per-level time of the last timed build
@@ -418,11 +419,11 @@ struct Ctx {
 };
 
 // timing helpers (orb_api.hip): marks are -1 when timing is off
-enum { kTimeCount = 0, kTimePartition = 1, kTimeOther = 2, kTimeLevel = 100 /* + level */ };
+enum { kTimeCount = 0, kTimePartition = 1, kTimeOther = 2, kTimeSwap = 3, kTimeLevel = 100 /* + level */ };
 void timing_reset(Ctx &c);
 int timing_mark(Ctx &c);
 int timing_pair(Ctx &c, int begin_mark, int kind);
-int timing_collect(Ctx &c);
+int timing_collect(Ctx &c, bool reuse_events = true);
 
 // kernels / launchers (orb_kernels.hip)
 int launch_zero_u32(hipStream_t s, uint32_t *p, size_t n);

@@ -1328,10 +1328,12 @@ int launch_hoare(Ctx &c, int32_t nCells, int32_t *status, const HistCounts *hc)
     ORB_HIP(hipGetLastError());
     // byte offsets inside any cell fit 32 bits below 2^30 particles
     auto swap = c.stride < (int64_t(1) << 30) ? k_hoare_swap<true> : k_hoare_swap<false>;
+    const int ts = timing_mark(c);
     hipLaunchKernelGGL(swap, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
                        c.cell_first.p, c.chunk_cell.p, c.chunk_pre.p, c.masks.p, c.splits.p,
                        c.chunk_aux.p, c.x(), c.y(), c.z(), c.ids());
     ORB_HIP(hipGetLastError());
+    ORB_TRY(timing_pair(c, ts, kTimeSwap));
     unsigned gc = (unsigned)((nCells + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_hoare_final, dim3(gc), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
                        c.splits.p, c.x(), c.y(), c.z(), c.ids());

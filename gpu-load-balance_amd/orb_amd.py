@@ -58,7 +58,8 @@ class BuildInfo(C.Structure):
 
 class Timing(C.Structure):
     _fields_ = [("count_ms", C.c_double), ("partition_ms", C.c_double), ("other_ms", C.c_double),
-                ("count_bytes", C.c_int64), ("count_launches", C.c_int32), ("pad", C.c_int32)]
+                ("count_bytes", C.c_int64), ("count_launches", C.c_int32), ("pad", C.c_int32),
+                ("swap_ms", C.c_double), ("swap_launches", C.c_int32), ("pad2", C.c_int32)]
 
 
 class ServerConfig(C.Structure):
@@ -434,7 +435,8 @@ class Context:
         t = Timing()
         _check(lib().orb_get_timing(self.h, C.byref(t)), "orb_get_timing")
         return {"count_ms": t.count_ms, "partition_ms": t.partition_ms, "other_ms": t.other_ms,
-                "count_bytes": t.count_bytes, "count_launches": t.count_launches}
+                "count_bytes": t.count_bytes, "count_launches": t.count_launches,
+                "swap_ms": t.swap_ms, "swap_launches": t.swap_launches}
 
 
 _hip = None

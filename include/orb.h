@@ -283,6 +283,9 @@ typedef struct orb_timing {
     int64_t count_bytes;   /* algorithmic bytes read by count kernels      */
     int32_t count_launches;
     int32_t pad;
+    double swap_ms;        /* the Hoare swap kernel alone (within partition_ms) */
+    int32_t swap_launches;
+    int32_t pad2;
 } orb_timing;
 int orb_get_timing(orb_ctx *ctx, orb_timing *t);
 /* Per-level time of the last build run with timing on: ms[l-1] for level l
