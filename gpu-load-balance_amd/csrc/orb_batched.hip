@@ -377,6 +377,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
     __shared__ uint32_t hist[kBins];  // window keys are few: one histogram
     __shared__ uint32_t sm[4];
     __shared__ float wkeys[kW][kChunk / kW];  // each wave's window keys, packed
+    __shared__ uint64_t words[kW][2][kPerWave];  // each wave's mask words, staged
     const int t = threadIdx.x, lane = t & 63, w = (int)uniform((uint32_t)t >> 6);
     const uint64_t lanes_below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const uint32_t total = first[nCells];
@@ -405,8 +406,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
         const float scale = width > 0.0f ? (float)kBins / width : 0.0f;
         const uint32_t len = e - b;
         uint32_t below = 0, above = 0, packed = 0;  // wave-uniform
-        // lane j collects word 16w + j of each mask
-        uint32_t gl = 0, gh = 0, wl = 0, wh = 0;
         float *dst = pack + b + sb;
         float *wk = wkeys[w];
         static_for<0, kPerWave>([&](auto jc) __attribute__((always_inline)) {
@@ -420,10 +419,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
             const uint64_t mW = valid & ~mB & ~mG;
             below += (uint32_t)__popcll(mB);
             above += (uint32_t)__popcll(mG);
-            gl = write_lane<j>(gl, (uint32_t)mG);
-            gh = write_lane<j>(gh, (uint32_t)(mG >> 32));
-            wl = write_lane<j>(wl, (uint32_t)mW);
-            wh = write_lane<j>(wh, (uint32_t)(mW >> 32));
+            // staged by one lane: a cross-lane write into a register would
+            // be lost if the compiler moved that register under a partial
+            // exec mask (the binning below is divergent)
+            if (lane == 0) {
+                words[w][0][j] = mG;
+                words[w][1][j] = mW;
+            }
             if (mW) {  // wave-uniform: most loads have no key in the window
                 if ((mW >> lane) & 1ull) {
                     const uint32_t slot = packed + (uint32_t)__popcll(mW & lanes_below);
@@ -433,10 +435,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
                 packed += (uint32_t)__popcll(mW);
             }
         });
-        const uint64_t mineG = ((uint64_t)gh << 32) | gl;
-        const uint64_t mineW = ((uint64_t)wh << 32) | wl;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         __builtin_amdgcn_wave_barrier();
+        const uint64_t mineG = lane < kPerWave ? words[w][0][lane] : 0ull;
+        const uint64_t mineW = lane < kPerWave ? words[w][1][lane] : 0ull;
         for (uint32_t k = (uint32_t)lane; k < packed; k += 64) {
             const float vk = wk[k];
             int tj;  // ties: the stopper counts of cells past this round use the packed keys

@@ -48,15 +48,6 @@ __device__ inline void static_for(F &&f)
     }
 }
 
-// acc with lane L replaced by the wave-uniform v (one v_writelane_b32; the
-// compiler would otherwise keep every v until the last select).
-template <int L>
-__device__ inline uint32_t write_lane(uint32_t acc, uint32_t v)
-{
-    asm("v_writelane_b32 %0, %1, %2" : "+v"(acc) : "s"(v), "n"(L));
-    return acc;
-}
-
 // Sum over a 256-thread block; every thread gets the result.
 template <class T>
 __device__ inline T block_sum(T v, T *smem4)
