@@ -481,6 +481,7 @@ int orb_finalize(orb_ctx *ctx)
     c->cell_nchunks.release();
     c->cell_first.release();
     c->chunk_cell.release();
+    c->chunk_desc.release();
     c->total_chunks.release();
     c->cnt.release();
     c->stamps.release();

@@ -247,6 +247,36 @@ __device__ inline int bin_of(const float *cand, float v, float lo, float scale, 
     return e;
 }
 
+// How far (in bins) the estimate (v - lo) * scale must stay from both edges
+// of its bin for that bin to be exact without the table.  With
+// M = max(|lo|, |hi|): a candidate is at most 8 float midpoints deep, each
+// midpoint of values in [lo, hi] off the exact one by at most 2^-24 M (the
+// sum's rounding) plus 2^-150 (a halving into the subnormals), so candidate
+// j sits within 2^-21 M + 2^-147 of lo + j (hi - lo) / 256, i.e. within
+// (2^-21 M + 2^-147) * scale (1 + 3 * 2^-24) bins of j.  The estimate itself
+// carries four roundings (v - lo, hi - lo, 256 / width, the product), at
+// most 2^-14 bins anywhere in [0, 256].  The returned bound doubles both
+// terms; M >= 2^126 (sums that could overflow) returns 1: never trusted.
+__device__ inline float bin_tolerance(float lo, float hi, float scale)
+{
+    const float m = fmaxf(fabsf(lo), fabsf(hi));
+    return m < 0x1p126f ? 0x1p-13f + (m + 0x1p-120f) * (0x1p-20f * scale) : 1.0f;
+}
+
+// Bin e of key v in [lo, hi] when the estimate lies more than tol from both
+// edges of [e, e + 1) (lim = 0.5 - tol: the fraction's distance from 0.5 is
+// below lim): candidate e is then strictly below v and candidate e + 1
+// strictly above, so e is the bin for both comparisons and v ties no
+// candidate.  False when the table has to decide (NaN estimates included).
+// An estimate at or past 256 needs no test of its own: v <= hi keeps it
+// within 2^-14 of 256, too close to that edge to pass.
+__device__ inline bool bin_estimate(float v, float lo, float scale, float lim, int &e)
+{
+    const float ef = (v - lo) * scale;
+    e = (int)ef;
+    return fabsf(__builtin_amdgcn_fractf(ef) - 0.5f) < lim;
+}
+
 // Each searching cell's 255 candidate cuts for a histogram round (entry 0
 // unused), computed once per cell instead of once per chunk.
 __global__ __launch_bounds__(kBlock) void k_b_cands(const CellRec *recs, int32_t nCells,
@@ -261,11 +291,17 @@ __global__ __launch_bounds__(kBlock) void k_b_cands(const CellRec *recs, int32_t
 }
 
 // The candidate table of a cell into LDS, with the -inf / +inf sentinels
-// at 0 and kBins.
-__device__ inline void load_cands(float *cand, const float *cell_cand, uint32_t cell)
+// at 0 and kBins: each thread loads its entry (cand_entry, issued before the
+// chunk's keys so that waiting for it does not wait for them), then stores it.
+__device__ inline float cand_entry(const float *cell_cand, uint32_t cell)
 {
     const int t = threadIdx.x;
-    cand[t] = t ? cell_cand[(size_t)cell * kBins + t] : -INFINITY;
+    return t ? cell_cand[(size_t)cell * kBins + t] : -INFINITY;
+}
+__device__ inline void store_cands(float *cand, float entry)
+{
+    const int t = threadIdx.x;
+    cand[t] = entry;
     if (t == 0) cand[kBins] = INFINITY;
 }
 
@@ -276,12 +312,11 @@ __device__ inline void load_cands(float *cand, const float *cell_cand, uint32_t 
 // (loads past the chunk's end return 0 and are masked out); keys outside
 // the margins are counted with ballots, the rest binned.
 template <int LE>
-__global__ __launch_bounds__(kBlock) void k_b_hist(const CellRec *recs, int32_t nCells,
-                                                   const uint32_t *first,
-                                                   const uint32_t *chunk_cell, const float *x,
-                                                   const float *y, const float *z,
-                                                   const float *cell_cand, uint16_t *chunk_hist,
-                                                   uint16_t *chunk_tie)
+__global__ __launch_bounds__(kBlock) void k_b_hist(
+    const CellRec *__restrict__ recs, const uint4 *__restrict__ chunk_desc, uint32_t max_chunks,
+    const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
+    const float *__restrict__ cell_cand, uint16_t *__restrict__ chunk_hist,
+    uint16_t *__restrict__ chunk_tie)
 {
     constexpr int kW = kBlock / 64;
     constexpr int kPerWave = kMaskWords / kW;  // 16 loads of 64 keys
@@ -290,30 +325,31 @@ __global__ __launch_bounds__(kBlock) void k_b_hist(const CellRec *recs, int32_t 
     __shared__ uint32_t ties[kBins];  // keys equal to candidate j (rare)
     __shared__ uint32_t sm[kW];
     const int t = threadIdx.x, lane = t & 63, w = (int)uniform((uint32_t)t >> 6);
-    const uint32_t total = first[nCells];
-    for (uint32_t q = blockIdx.x; q < total; q += gridDim.x) {
-        const uint32_t cell = chunk_cell[q];
-        const CellRec r = recs[cell];
-        if (r.found || r.flags || r.axis < 0 || r.axis > 2) continue;  // uniform
-        const uint32_t b = uniform(r.begin + (q - first[cell]) * (uint32_t)kChunk);
-        const uint32_t e = uniform(r.end - b < (uint32_t)kChunk ? r.end : b + (uint32_t)kChunk);
-        const float lo = uniform(r.margin_l), hi = uniform(r.margin_r);
-        const __amdgpu_buffer_rsrc_t kr = __builtin_amdgcn_make_buffer_rsrc(
-            const_cast<float *>(axis_ptr(x, y, z, r.axis) + b), 0, (int)((e - b) * 4u),
-            0x00020000);
+    for (uint32_t q = blockIdx.x; q < max_chunks; q += gridDim.x) {
+        const uint4 d = chunk_desc[q];  // idle past the level's last chunk
+        const uint32_t b = uniform(d.x), e = uniform(d.y), cell = uniform(d.z);
+        const uint32_t axis = uniform(d.w);
+        keep_issued();  // the whole record in one load
+        if (axis > 2u) continue;  // uniform: the cell is not searched
+        const float *kp = axis_ptr(x, y, z, (int)axis) + b;
+        const float ce = cand_entry(cell_cand, cell);
+        const __amdgpu_buffer_rsrc_t kr =
+            __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(kp), 0, (int)((e - b) * 4u), 0x00020000);
         const uint32_t sb = (uint32_t)(w * kPerWave * 64);  // chunk position of the quarter
         float v[kPerWave];
 #pragma unroll
         for (int j = 0; j < kPerWave; ++j)
             v[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
                 kr, (int)((sb + (uint32_t)lane) * 4u), 256 * j, 0));
-        load_cands(cand, cell_cand, cell);
+        const float lo = uniform(recs[cell].margin_l), hi = uniform(recs[cell].margin_r);
+        store_cands(cand, ce);
 #pragma unroll
         for (int u = 0; u < kW; ++u) hist[u][t] = 0u;
         ties[t] = 0u;
         __syncthreads();
         const float width = hi - lo;
         const float scale = width > 0.0f ? (float)kBins / width : 0.0f;
+        const float lim = 0.5f - bin_tolerance(lo, hi, scale);
         const uint32_t len = e - b;
         uint32_t *h = hist[w];
         uint32_t below = 0, above = 0;  // wave-uniform
@@ -321,20 +357,19 @@ __global__ __launch_bounds__(kBlock) void k_b_hist(const CellRec *recs, int32_t 
             static_for<0, kPerWave>([&](auto jc) __attribute__((always_inline)) {
                 constexpr int j = decltype(jc)::value;
                 const uint32_t p0 = sb + 64u * (uint32_t)j;  // chunk position of lane 0
-                const uint64_t valid =
-                    decltype(full)::value
-                        ? ~0ull
-                        : (p0 >= len ? 0ull
-                                     : (len - p0 >= 64u ? ~0ull : (1ull << (len - p0)) - 1ull));
+                const bool ok = decltype(full)::value || p0 + (uint32_t)lane < len;
                 const float vj = v[j];
-                const uint64_t mB = __ballot(vj < lo) & valid;
-                const uint64_t mA = __ballot(vj > hi) & valid;
-                below += (uint32_t)__popcll(mB);
-                above += (uint32_t)__popcll(mA);
-                if (((valid & ~mB & ~mA) >> lane) & 1ull) {
-                    int tj;
-                    atomicAdd(&h[bin_of<LE>(cand, vj, lo, scale, tj)], 1u);
-                    if (tj) atomicAdd(&ties[tj], 1u);
+                const bool isB = vj < lo, isA = vj > hi;
+                below += (uint32_t)__popcll(__ballot(ok && isB));
+                above += (uint32_t)__popcll(__ballot(ok && isA));
+                if (ok && !isB && !isA) {
+                    int bin;
+                    if (!bin_estimate(vj, lo, scale, lim, bin)) {  // near a bin edge: the table
+                        int tj;
+                        bin = bin_of<LE>(cand, vj, lo, scale, tj);
+                        if (tj) atomicAdd(&ties[tj], 1u);
+                    }
+                    atomicAdd(&h[bin], 1u);
                 }
             });
         };
@@ -367,9 +402,10 @@ __global__ __launch_bounds__(kBlock) void k_b_hist(const CellRec *recs, int32_t 
 // registers and stop at seven).
 template <int LE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_b_hist_window(
-    const CellRec *recs, int32_t nCells, const uint32_t *first, const uint32_t *chunk_cell,
-    const float *x, const float *y, const float *z, const float *cell_cand, uint16_t *chunk_hist,
-    float *pack, uint32_t *comp, uint64_t *masks)
+    const CellRec *__restrict__ recs, const uint4 *__restrict__ chunk_desc, uint32_t max_chunks,
+    const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
+    const float *__restrict__ cell_cand, uint16_t *__restrict__ chunk_hist,
+    float *__restrict__ pack, uint32_t *__restrict__ comp, uint64_t *__restrict__ masks)
 {
     constexpr int kW = kBlock / 64;
     constexpr int kPerWave = kMaskWords / kW;  // 16 loads of 64 keys
@@ -380,61 +416,73 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
     __shared__ uint64_t words[kW][2][kPerWave];  // each wave's mask words, staged
     const int t = threadIdx.x, lane = t & 63, w = (int)uniform((uint32_t)t >> 6);
     const uint64_t lanes_below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    const uint32_t total = first[nCells];
-    for (uint32_t q = blockIdx.x; q < total; q += gridDim.x) {
-        const uint32_t cell = chunk_cell[q];
-        const CellRec r = recs[cell];
-        if (r.found || r.flags || r.axis < 0 || r.axis > 2) continue;  // uniform
-        const uint32_t b = uniform(r.begin + (q - first[cell]) * (uint32_t)kChunk);
-        const uint32_t e = uniform(r.end - b < (uint32_t)kChunk ? r.end : b + (uint32_t)kChunk);
-        const float lo = uniform(r.margin_l), hi = uniform(r.margin_r);
+    for (uint32_t q = blockIdx.x; q < max_chunks; q += gridDim.x) {
+        const uint4 d = chunk_desc[q];  // idle past the level's last chunk
+        const uint32_t b = uniform(d.x), e = uniform(d.y), cell = uniform(d.z);
+        const uint32_t axis = uniform(d.w);
+        keep_issued();  // the whole record in one load
+        if (axis > 2u) continue;  // uniform: the cell is not searched
         // the chunk's keys through a range-checked descriptor: loads past the
         // chunk's end return 0 and are masked out below, with no branches
-        const __amdgpu_buffer_rsrc_t kr = __builtin_amdgcn_make_buffer_rsrc(
-            const_cast<float *>(axis_ptr(x, y, z, r.axis) + b), 0, (int)((e - b) * 4u),
-            0x00020000);
+        const float *kp = axis_ptr(x, y, z, (int)axis) + b;
+        const float ce = cand_entry(cell_cand, cell);
+        const __amdgpu_buffer_rsrc_t kr =
+            __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(kp), 0, (int)((e - b) * 4u), 0x00020000);
         const uint32_t sb = (uint32_t)(w * kPerWave * 64);  // chunk position of the quarter
         float v[kPerWave];
 #pragma unroll
         for (int j = 0; j < kPerWave; ++j)
             v[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
                 kr, (int)((sb + (uint32_t)lane) * 4u), 256 * j, 0));
-        load_cands(cand, cell_cand, cell);
+        // a cell the first round finished is binned too and its results
+        // ignored: the rows go to their own region, and a check here would
+        // wait for the cell record before the keys are requested
+        const float lo = uniform(recs[cell].margin_l), hi = uniform(recs[cell].margin_r);
+        store_cands(cand, ce);
         hist[t] = 0u;
         __syncthreads();
         const float width = hi - lo;
         const float scale = width > 0.0f ? (float)kBins / width : 0.0f;
         const uint32_t len = e - b;
-        uint32_t below = 0, above = 0, packed = 0;  // wave-uniform
+        uint32_t below = 0, packed = 0;  // wave-uniform
         float *dst = pack + b + sb;
         float *wk = wkeys[w];
-        static_for<0, kPerWave>([&](auto jc) __attribute__((always_inline)) {
-            constexpr int j = decltype(jc)::value;
-            const uint32_t p0 = sb + 64u * (uint32_t)j;  // chunk position of lane 0
-            const uint64_t valid =
-                p0 >= len ? 0ull : (len - p0 >= 64u ? ~0ull : (1ull << (len - p0)) - 1ull);
-            const float vj = v[j];
-            const uint64_t mB = __ballot(vj < lo) & valid;
-            const uint64_t mG = __ballot(vj > hi) & valid;
-            const uint64_t mW = valid & ~mB & ~mG;
-            below += (uint32_t)__popcll(mB);
-            above += (uint32_t)__popcll(mG);
-            // staged by one lane: a cross-lane write into a register would
-            // be lost if the compiler moved that register under a partial
-            // exec mask (the binning below is divergent)
-            if (lane == 0) {
-                words[w][0][j] = mG;
-                words[w][1][j] = mW;
-            }
-            if (mW) {  // wave-uniform: most loads have no key in the window
-                if ((mW >> lane) & 1ull) {
-                    const uint32_t slot = packed + (uint32_t)__popcll(mW & lanes_below);
-                    dst[slot] = vj;
-                    wk[slot] = vj;
+        auto scan = [&](auto full) __attribute__((always_inline)) {
+            static_for<0, kPerWave>([&](auto jc) __attribute__((always_inline)) {
+                constexpr int j = decltype(jc)::value;
+                uint64_t valid = ~0ull;
+                if constexpr (!decltype(full)::value) {
+                    const uint32_t p0 = sb + 64u * (uint32_t)j;  // chunk position of lane 0
+                    valid = p0 >= len ? 0ull
+                                      : (len - p0 >= 64u ? ~0ull : (1ull << (len - p0)) - 1ull);
                 }
-                packed += (uint32_t)__popcll(mW);
-            }
-        });
+                const float vj = v[j];
+                const uint64_t mB = __ballot(vj < lo) & valid;
+                const uint64_t mG = __ballot(vj > hi) & valid;
+                const uint64_t mW = valid & ~mB & ~mG;
+                below += (uint32_t)__popcll(mB);
+                // staged by one lane: a cross-lane write into a register would
+                // be lost if the compiler moved that register under a partial
+                // exec mask (the binning below is divergent)
+                if (lane == 0) {
+                    words[w][0][j] = mG;
+                    words[w][1][j] = mW;
+                }
+                if (mW) {  // wave-uniform: most loads have no key in the window
+                    if ((mW >> lane) & 1ull) {
+                        const uint32_t slot = packed + (uint32_t)__popcll(mW & lanes_below);
+                        dst[slot] = vj;
+                        wk[slot] = vj;
+                    }
+                    packed += (uint32_t)__popcll(mW);
+                }
+            });
+        };
+        if (len == (uint32_t)kChunk) scan(std::true_type{});  // uniform
+        else scan(std::false_type{});
+        // the quarter's keys that are neither below nor in the window
+        const uint32_t mine = len > sb ? (len - sb < 1024u ? len - sb : 1024u) : 0u;
+        const uint32_t above = mine - below - packed;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         __builtin_amdgcn_wave_barrier();
         const uint64_t mineG = lane < kPerWave ? words[w][0][lane] : 0ull;
@@ -1173,7 +1221,7 @@ static int ensure_state(Ctx &c, int32_t levels_mode, bool *complete)
     ORB_TRY(b.heap.alloc((size_t)c.cap));
     ORB_TRY(b.exists.alloc((size_t)c.cap));
     ORB_TRY(b.sub.alloc((size_t)c.cap));
-    ORB_TRY(b.chunk_hist.alloc(2 * (size_t)(c.max_chunks + 1) * kBins));  // bins | ties
+    ORB_TRY(b.chunk_hist.alloc(kHistRegions * (size_t)(c.max_chunks + 1) * kBins));
     if (!b.cell_hist.p) b.hist_clean = false;
     ORB_TRY(b.cell_hist.alloc((size_t)c.max_level_cells * kBins));
     ORB_TRY(b.level_passes.alloc(64));
@@ -1218,6 +1266,7 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
     ORB_HIP(hipGetLastError());
     ORB_TRY(build_chunk_table(c, nCells, false));
     uint16_t *chunk_tie = chunk_hist + (size_t)(c.max_chunks + 1) * kBins;
+    uint16_t *later_hist = chunk_hist + 2 * (size_t)(c.max_chunks + 1) * kBins;
     if (use_subtree) {
         int32_t sb = 4096;
         if (level >= 0 && level < (int32_t)c.tune_sub_blocks.size() &&
@@ -1250,27 +1299,27 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
             ORB_HIP(hipGetLastError());
         }
 #define ORB_HIST(L)                                                                              \
-    hipLaunchKernelGGL((k_b_hist<L>), dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,     \
-                       c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(), c.cell_cand.p,       \
+    hipLaunchKernelGGL((k_b_hist<L>), dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p,             \
+                       c.chunk_desc.p, (uint32_t)c.max_chunks, c.x(), c.y(), c.z(), c.cell_cand.p, \
                        chunk_hist, chunk_tie)
 #define ORB_HIST_WINDOW(L)                                                                       \
     hipLaunchKernelGGL((k_b_hist_window<L>), dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p,      \
-                       nCells, c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(),              \
-                       c.cell_cand.p, chunk_hist, ckeys, c.chunk_comp.p, c.masks.p)
+                       c.chunk_desc.p, (uint32_t)c.max_chunks, c.x(), c.y(), c.z(),              \
+                       c.cell_cand.p, later_hist, ckeys, c.chunk_comp.p, c.masks.p)
         const unsigned gp = (gq + 3) / 4;  // a wave per chunk
         if (le) {
             if (mode == 0) ORB_HIST(1);
             else if (mode == 1) ORB_HIST_WINDOW(1);
             else
                 hipLaunchKernelGGL(k_b_hist_packed<1>, dim3(gp), dim3(kBlock), 0, c.stream, c.recs.p,
-                                   nCells, c.cell_first.p, c.chunk_cell.p, chunk_hist, ckeys,
+                                   nCells, c.cell_first.p, c.chunk_cell.p, later_hist, ckeys,
                                    c.chunk_comp.p);
         } else {
             if (mode == 0) ORB_HIST(0);
             else if (mode == 1) ORB_HIST_WINDOW(0);
             else
                 hipLaunchKernelGGL(k_b_hist_packed<0>, dim3(gp), dim3(kBlock), 0, c.stream, c.recs.p,
-                                   nCells, c.cell_first.p, c.chunk_cell.p, chunk_hist, ckeys,
+                                   nCells, c.cell_first.p, c.chunk_cell.p, later_hist, ckeys,
                                    c.chunk_comp.p);
         }
 #undef ORB_HIST
@@ -1278,7 +1327,8 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
         ORB_HIP(hipGetLastError());
         ORB_TRY(timing_pair(c, tb, kTimeCount));
         hipLaunchKernelGGL(k_b_hist_reduce, dim3(gr), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
-                           c.cell_first.p, c.chunk_cell.p, chunk_hist, cell_hist);
+                           c.cell_first.p, c.chunk_cell.p, mode == 0 ? chunk_hist : later_hist,
+                           cell_hist);
         hipLaunchKernelGGL(k_b_replay, dim3(gw), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
                            cell_hist, heap, trace, level_passes, exists, status,
                            mode == 0 ? c.cell_cand.p : nullptr);

@@ -18,6 +18,12 @@ __device__ inline float uniform(float v)
     return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
+// A compiler-only barrier (no instruction): loads above it stay above it.
+// Without it the compiler splits a record load around a later uniform branch
+// (the field the branch tests first, the rest after it): two round trips
+// where one does.
+__device__ inline void keep_issued() { asm volatile("" ::: "memory"); }
+
 template <class T>
 __device__ inline T wave_sum(T v)
 {

@@ -30,6 +30,10 @@ constexpr int kHistK = 8;                   // bisection steps per histogram rou
 constexpr int kBins = 1 << kHistK;          // 256 bins (255 candidate cuts)
 constexpr int kMaskWords = kChunk / 64;     // 64-bit stopper-mask words per chunk and side
 static_assert(kBins == kBlock, "one thread per histogram bin");
+constexpr uint32_t kChunkIdle = 0xffffffffu;  // chunk record axis: no search this level
+// batched form, per-chunk 256-bin rows: the first round's bins, the first
+// round's ties (both read again by the partition), the later rounds' bins
+constexpr int kHistRegions = 3;
 
 // ---------------------------------------------------------------------------
 // error plumbing
@@ -357,6 +361,8 @@ struct Ctx {
     DevBuf<uint32_t> cell_nchunks;   // [cells+1]
     DevBuf<uint32_t> cell_first;     // [cells+1] exclusive scan
     DevBuf<uint32_t> chunk_cell;     // [max_chunks]
+    DevBuf<uint4> chunk_desc;        // [max_chunks] first key, end, cell, axis (kChunkIdle:
+                                     // the cell is not searched this level)
     DevBuf<uint32_t> total_chunks;   // [1]
     DevBuf<uint32_t> cnt;            // [cells][2] lt, le
     DevBuf<CountStamp> stamps;       // [cells] cut/axis of the chunk counts

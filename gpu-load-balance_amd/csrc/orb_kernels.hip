@@ -287,14 +287,31 @@ __global__ void k_level_recs(const orb_cell *cells, int32_t nCells, const uint32
     nchunks[i] = (r.end - r.begin + (uint32_t)kChunk - 1u) / (uint32_t)kChunk;
 }
 
-__global__ void k_fill_chunks(const uint32_t *first, int32_t nCells, uint32_t *chunk_cell,
-                              uint32_t max_chunks)
+// Each chunk's cell, and its record: first key, one past the last, cell,
+// axis (kChunkIdle when the cell has its cut already, belongs to a subtree
+// or has no axis; and for every record past the level's last chunk).  A
+// kernel that takes a chunk per block or wave reaches its keys after one
+// load of the record instead of the chain chunk count -> chunk -> cell ->
+// cell record.
+__global__ void k_fill_chunks(const uint32_t *first, int32_t nCells, const CellRec *recs,
+                              uint32_t *chunk_cell, uint4 *chunk_desc, uint32_t max_chunks)
 {
     uint32_t total = first[nCells];
     if (total > max_chunks) total = max_chunks;
-    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < total;
-         q += gridDim.x * blockDim.x)
-        chunk_cell[q] = chunk_to_cell(first, nCells, q);
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < max_chunks;
+         q += gridDim.x * blockDim.x) {
+        if (q >= total) {
+            chunk_desc[q] = make_uint4(0u, 0u, 0u, kChunkIdle);
+            continue;
+        }
+        const uint32_t cell = chunk_to_cell(first, nCells, q);
+        chunk_cell[q] = cell;
+        const CellRec &r = recs[cell];
+        const uint32_t b = r.begin + (q - first[cell]) * (uint32_t)kChunk;
+        const uint32_t e = r.end - b < (uint32_t)kChunk ? r.end : b + (uint32_t)kChunk;
+        const bool live = !r.found && !r.flags && r.axis >= 0 && r.axis <= 2;
+        chunk_desc[q] = make_uint4(b, e, cell, live ? (uint32_t)r.axis : kChunkIdle);
+    }
 }
 
 int ensure_level_capacity(Ctx &c, int64_t nCells)
@@ -308,6 +325,7 @@ int ensure_level_capacity(Ctx &c, int64_t nCells)
     ORB_TRY(c.cell_nchunks.alloc(m + 1));
     ORB_TRY(c.cell_first.alloc(m + 1));
     ORB_TRY(c.chunk_cell.alloc(c.max_chunks));
+    ORB_TRY(c.chunk_desc.alloc(c.max_chunks));
     ORB_TRY(c.cnt.alloc(2 * m));
     ORB_TRY(c.stamps.alloc(m));
     ORB_TRY(c.chunk_ltle.alloc(c.max_chunks + 1));
@@ -349,7 +367,8 @@ int build_chunk_table(Ctx &c, int32_t nCells, bool with_stamps)
 {
     ORB_TRY(scan_exclusive_u32(c, c.cell_nchunks.p, c.cell_first.p, (size_t)nCells + 1));
     hipLaunchKernelGGL(k_fill_chunks, dim3(grid_for((size_t)c.max_chunks, 16384)), dim3(kBlock), 0,
-                       c.stream, c.cell_first.p, nCells, c.chunk_cell.p, (uint32_t)c.max_chunks);
+                       c.stream, c.cell_first.p, nCells, c.recs.p, c.chunk_cell.p, c.chunk_desc.p,
+                       (uint32_t)c.max_chunks);
     ORB_HIP(hipGetLastError());
     return with_stamps ? invalidate_stamps(c, nCells) : ORB_OK;
 }

@@ -225,7 +225,7 @@ static int server_alloc(Server &S)
     ORB_TRY(S.heap.alloc((size_t)H));
     ORB_TRY(S.exists.alloc((size_t)H));
     ORB_TRY(S.sub.alloc((size_t)H));
-    ORB_TRY(S.chunk_hist.alloc(2 * (size_t)(c.max_chunks + 1) * kBins));  // bins | ties
+    ORB_TRY(S.chunk_hist.alloc(kHistRegions * (size_t)(c.max_chunks + 1) * kBins));
     ORB_TRY(S.cell_hist.alloc((size_t)c.max_level_cells * 256));
     ORB_TRY(S.level_passes.alloc((size_t)slots * 64));
     ORB_TRY(S.status.alloc((size_t)slots * 2));
@@ -552,6 +552,7 @@ int orb_server_destroy(orb_server *o)
     c.cell_nchunks.release();
     c.cell_first.release();
     c.chunk_cell.release();
+    c.chunk_desc.release();
     c.cnt.release();
     c.stamps.release();
     c.chunk_ltle.release();
