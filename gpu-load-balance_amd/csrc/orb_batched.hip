@@ -518,31 +518,37 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
 // A chunk with no packed key in the current window writes bins 0 and 255
 // only; otherwise the wave builds its own candidates and LDS histogram.
 template <int LE>
-__global__ __launch_bounds__(kBlock) void k_b_hist_packed(const CellRec *recs, int32_t nCells,
-                                                          const uint32_t *first,
-                                                          const uint32_t *chunk_cell,
-                                                          uint16_t *chunk_hist, const float *ckeys,
-                                                          const uint32_t *comp)
+__global__ __launch_bounds__(kBlock) void k_b_hist_packed(const CellRec *__restrict__ recs,
+                                                          const uint4 *__restrict__ chunk_desc,
+                                                          uint32_t max_chunks,
+                                                          uint16_t *__restrict__ chunk_hist,
+                                                          const float *__restrict__ ckeys,
+                                                          const uint32_t *__restrict__ comp)
 {
     constexpr int kW = kBlock / 64;
+    static_assert(kW == 4, "one packed count per quarter, four in a 16-byte load");
     __shared__ float cand[kW][kBins + 1];
     __shared__ uint32_t hist[kW][kBins];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const uint32_t total = first[nCells];
-    for (uint32_t q = blockIdx.x * kW + (uint32_t)w; q < total; q += gridDim.x * kW) {
-        const uint32_t cell = chunk_cell[q];
-        const CellRec r = recs[cell];
-        if (r.found || r.flags || r.axis < 0 || r.axis > 2) continue;  // uniform per wave
-        const uint32_t b = r.begin + (q - first[cell]) * (uint32_t)kChunk;
-        uint32_t e = b + (uint32_t)kChunk;
-        if (e > r.end) e = r.end;
+    const int lane = threadIdx.x & 63, w = (int)uniform((uint32_t)threadIdx.x >> 6);
+    for (uint32_t q = blockIdx.x * kW + (uint32_t)w; q < max_chunks; q += gridDim.x * kW) {
+        // the record and the packed counts together, then the cell record
+        const uint4 d = chunk_desc[q];
+        const uint4 ncs = *reinterpret_cast<const uint4 *>(comp + 8 * (size_t)q);
+        const uint32_t nbelow = uniform(comp[8 * (size_t)q + 4]);
+        const uint32_t b = uniform(d.x), e = uniform(d.y), cell = uniform(d.z);
+        const uint32_t axis = uniform(d.w);
+        const uint32_t nc4[kW] = {uniform(ncs.x), uniform(ncs.y), uniform(ncs.z), uniform(ncs.w)};
+        keep_issued();  // the record and the counts in one round trip
+        if (axis > 2u) continue;  // uniform: the cell is not searched
+        const CellRec &r = recs[cell];
+        if (r.found) continue;  // uniform per wave
         const float lo = r.margin_l, hi = r.margin_r;
         uint32_t below = 0, above = 0;
         bool any = false;
         for (int sh = 0; sh < kW; ++sh) {
             uint32_t off, cnt;
             wave_share(b, e, sh, off, cnt);
-            const uint32_t nc = comp[8 * (size_t)q + sh];
+            const uint32_t nc = nc4[sh];
             for (uint32_t k = (uint32_t)lane; k < nc; k += 64) {
                 const float v = ckeys[b + off + k];
                 below += v < lo ? 1u : 0u;
@@ -552,7 +558,7 @@ __global__ __launch_bounds__(kBlock) void k_b_hist_packed(const CellRec *recs, i
         }
         below = wave_sum(below);
         above = wave_sum(above);
-        below += comp[8 * (size_t)q + 4];  // keys below the packing window
+        below += nbelow;  // keys below the packing window
         uint16_t *hrow = chunk_hist + (size_t)q * kBins;
         if (__ballot(any) == 0ull) {
 #pragma unroll
@@ -578,7 +584,7 @@ __global__ __launch_bounds__(kBlock) void k_b_hist_packed(const CellRec *recs, i
         for (int sh = 0; sh < kW; ++sh) {
             uint32_t off, cnt;
             wave_share(b, e, sh, off, cnt);
-            const uint32_t nc = comp[8 * (size_t)q + sh];
+            const uint32_t nc = nc4[sh];
             for (uint32_t k = (uint32_t)lane; k < nc; k += 64) {
                 const float v = ckeys[b + off + k];
                 if (!(v < lo) && !(v > hi)) {
@@ -1312,14 +1318,14 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
             else if (mode == 1) ORB_HIST_WINDOW(1);
             else
                 hipLaunchKernelGGL(k_b_hist_packed<1>, dim3(gp), dim3(kBlock), 0, c.stream, c.recs.p,
-                                   nCells, c.cell_first.p, c.chunk_cell.p, later_hist, ckeys,
+                                   c.chunk_desc.p, (uint32_t)c.max_chunks, later_hist, ckeys,
                                    c.chunk_comp.p);
         } else {
             if (mode == 0) ORB_HIST(0);
             else if (mode == 1) ORB_HIST_WINDOW(0);
             else
                 hipLaunchKernelGGL(k_b_hist_packed<0>, dim3(gp), dim3(kBlock), 0, c.stream, c.recs.p,
-                                   nCells, c.cell_first.p, c.chunk_cell.p, later_hist, ckeys,
+                                   c.chunk_desc.p, (uint32_t)c.max_chunks, later_hist, ckeys,
                                    c.chunk_comp.p);
         }
 #undef ORB_HIST

@@ -713,30 +713,40 @@ __global__ __launch_bounds__(kBlock) void k_hoare_masks(const CellRec *recs, int
 // R-stoppers only, and the window keys, packed in position order in the
 // quarter's slots, are compared with the cut; lane j owns word j.  Other
 // cells read their keys, 64 per load.
+//
+// The chunk's record and its two window-round mask words per lane are
+// requested together; the cell record next; the packed keys last.  Chunks of
+// cells that are not searched (subtree cells, no axis) and records past the
+// level's last chunk are skipped: nothing reads their masks, and their count
+// slot is written as zero.
 __global__ __launch_bounds__(kBlock) void k_hoare_masks_window(
-    const CellRec *recs, int32_t nCells, const uint32_t *first, const uint32_t *chunk_cell,
+    const CellRec *__restrict__ recs, const uint4 *__restrict__ chunk_desc, uint32_t max_chunks,
     const float *x, const float *y, const float *z, const float *pack, uint64_t *masks,
     uint64_t *chunk_cnt)
 {
     constexpr int kW = kBlock / 64;
     constexpr int kQuarter = kMaskWords / kW;  // words per packing quarter
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const uint32_t total = first[nCells];
-    for (uint32_t q = blockIdx.x * kW + (uint32_t)w; q < total; q += gridDim.x * kW) {
-        const uint32_t cell = chunk_cell[q];
-        const CellRec r = recs[cell];
-        const uint32_t b = r.begin + (q - first[cell]) * (uint32_t)kChunk;
-        uint32_t e = b + (uint32_t)kChunk;
-        if (e > r.end) e = r.end;
+    for (uint32_t q = blockIdx.x * kW + (uint32_t)w; q < max_chunks; q += gridDim.x * kW) {
+        const uint4 d = chunk_desc[q];
+        uint64_t *mq = masks + (size_t)q * (2 * kMaskWords);
+        const uint64_t Gw = mq[lane], Ww = mq[kMaskWords + lane];  // in flight with the record
+        const uint32_t b = uniform(d.x), e = uniform(d.y), cell = uniform(d.z);
+        const uint32_t axis = uniform(d.w);
+        keep_issued();  // the whole record in one load
+        if (axis > 2u) {  // uniform
+            if (lane == 0) chunk_cnt[q] = 0ull;
+            continue;
+        }
+        const CellRec &r = recs[cell];
         const uint32_t len = e - b, p0 = 64u * (uint32_t)lane;  // chunk position of bit 0
         const uint64_t valid =
             p0 >= len ? 0ull : (len - p0 >= 64u ? ~0ull : (1ull << (len - p0)) - 1ull);
-        uint64_t *mq = masks + (size_t)q * (2 * kMaskWords);
         const float cut = r.cut;
         uint64_t L = 0ull, R = 0ull;
-        if (r.passes > kHistK && r.axis >= 0 && r.axis <= 2) {  // uniform
-            const uint64_t G = mq[lane] & valid;
-            const uint64_t W = mq[kMaskWords + lane] & valid;  // slots stay inside the cell
+        if (r.passes > kHistK) {  // uniform
+            const uint64_t G = Gw & valid;
+            const uint64_t W = Ww & valid;  // slots stay inside the cell
             const uint32_t cw = (uint32_t)__popcll(W);
             const uint32_t ex = wave_inclusive_scan(cw) - cw;
             const int q0 = lane & ~(kQuarter - 1);  // first word of this lane's quarter
@@ -753,7 +763,7 @@ __global__ __launch_bounds__(kBlock) void k_hoare_masks_window(
             L |= G;
             R |= valid & ~G & ~W;
         } else {
-            const float *key = axis_ptr(x, y, z, r.axis);
+            const float *key = axis_ptr(x, y, z, (int)axis);
             for (int j0 = 0; j0 < kMaskWords; j0 += 16) {
                 float v[16];
 #pragma unroll
@@ -1328,7 +1338,7 @@ int launch_hoare(Ctx &c, int32_t nCells, int32_t *status, const HistCounts *hc)
     const unsigned gq = chunk_grid(c);
     if (hc && hc->pack)  // batched form: the window round left masks
         hipLaunchKernelGGL(k_hoare_masks_window, dim3((gq + 3) / 4), dim3(kBlock), 0, c.stream,
-                           c.recs.p, nCells, c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(),
+                           c.recs.p, c.chunk_desc.p, (uint32_t)c.max_chunks, c.x(), c.y(), c.z(),
                            hc->pack, c.masks.p, c.chunk_cnt.p);
     else
         hipLaunchKernelGGL(k_hoare_masks, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
