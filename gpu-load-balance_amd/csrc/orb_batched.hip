@@ -40,6 +40,7 @@ struct BatchedState {
     std::vector<int32_t> level_n;      // [D+1] cells per level (index l)
     std::vector<uint8_t> level_cover;  // [D+1] the level's cells hold every particle
     std::vector<int64_t> level_off;    // [D+1] offset into level_ids
+    std::vector<int32_t> level_min_leaf;  // [D+1] fewest leaves under a cell of the level
     DevBuf<LevelEntry> level_ids;      // all levels' cells, concatenated
     DevBuf<orb_cell> heap;             // [cap]
     DevBuf<uint8_t> exists;            // [cap]
@@ -63,7 +64,8 @@ struct BatchedState {
 // schedule (host): ids per level, data-independent
 // ---------------------------------------------------------------------------
 int schedule_levels(int32_t d, int32_t levels_mode, int64_t cap,
-                    std::vector<std::vector<int32_t>> &levels, bool *complete)
+                    std::vector<std::vector<int32_t>> &levels, bool *complete,
+                    std::vector<int32_t> *min_leaf)
 {
     // existence and nLeafCells follow from d alone (left = ceil(n/2),
     // right = n - left), so the schedule needs no particle data
@@ -75,6 +77,7 @@ int schedule_levels(int32_t d, int32_t levels_mode, int64_t cap,
     ex[0] = 1;
     std::vector<int32_t> ids;
     levels.assign((size_t)D + 1, {});
+    if (min_leaf) min_leaf->assign((size_t)D + 2, 0);
     *complete = true;
     for (int32_t l = 1; l <= D; ++l) {
         int32_t bad = -1;
@@ -88,6 +91,8 @@ int schedule_levels(int32_t d, int32_t levels_mode, int64_t cap,
         if (expect != (int32_t)ids.size()) *complete = false;
         for (int32_t id : ids) {
             int32_t nl = hp[(size_t)id].nLeafCells;
+            if (min_leaf && ((*min_leaf)[(size_t)l] == 0 || nl < (*min_leaf)[(size_t)l]))
+                (*min_leaf)[(size_t)l] = nl;
             int32_t left = (nl + 1) / 2;
             int32_t lid = left_child_id(id), rid = right_child_id(id);
             if (rid >= cap) return ORB_ERR_ARG;
@@ -107,7 +112,7 @@ static int make_schedule(Ctx &c, BatchedState &b, int32_t levels_mode, bool *com
     b.cap = c.cap;
     b.D = n_levels(c.d);
     std::vector<std::vector<int32_t>> levels;
-    ORB_TRY(schedule_levels(c.d, levels_mode, c.cap, levels, complete));
+    ORB_TRY(schedule_levels(c.d, levels_mode, c.cap, levels, complete, &b.level_min_leaf));
     const int32_t lmax = levels_mode == ORB_LEVELS_REFERENCE ? b.D - 1 : b.D;
     std::vector<LevelEntry> all;
     b.level_n.assign((size_t)b.D + 2, 0);
@@ -1228,7 +1233,8 @@ static int ensure_state(Ctx &c, int32_t levels_mode, bool *complete)
     ORB_TRY(b.exists.alloc((size_t)c.cap));
     ORB_TRY(b.sub.alloc((size_t)c.cap));
     ORB_TRY(b.chunk_hist.alloc(kHistRegions * (size_t)(c.max_chunks + 1) * kBins));
-    if (!b.cell_hist.p) b.hist_clean = false;
+    // a fresh (or grown, hence reallocated) buffer is cleared before use
+    if (b.cell_hist.n < (size_t)c.max_level_cells * kBins) b.hist_clean = false;
     ORB_TRY(b.cell_hist.alloc((size_t)c.max_level_cells * kBins));
     ORB_TRY(b.level_passes.alloc(64));
     ORB_TRY(b.hpasses.alloc(64));
@@ -1370,9 +1376,16 @@ static int enqueue_build(Ctx &c, int32_t levels_mode)
         const int32_t nCells = b.level_n[(size_t)l];
         if (nCells == 0) continue;
         const int tlevel = timing_mark(c);
+        // the subtree kernel runs where a cell may be small enough for it:
+        // cells split by count, so one with nLeaf leaves holds about
+        // n * nLeaf / d keys; where even the smallest expected cell is several
+        // times the subtree's limit, the launch (and its flags) are left out
+        // -- only a speed choice, every cell is built either way
+        const int64_t few = c.n * (int64_t)b.level_min_leaf[(size_t)l] / (c.d > 0 ? c.d : 1);
+        const bool may_sub = few <= 4 * (int64_t)subtree_max_keys(c);
         ORB_TRY(enqueue_level(c, b.level_ids.p + b.level_off[(size_t)l], nCells, b.heap.p,
                               b.exists.p, b.sub.p, b.chunk_hist.p, b.cell_hist.p, trace,
-                              b.level_passes.p, c.status.p, b.complete,
+                              b.level_passes.p, c.status.p, b.complete && may_sub,
                               b.level_cover[(size_t)l] != 0, l));
         ORB_TRY(timing_pair(c, tlevel, kTimeLevel + l));
     }

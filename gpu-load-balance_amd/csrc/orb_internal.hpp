@@ -462,7 +462,8 @@ int launch_copy_u32(hipStream_t s, const uint32_t *src, uint32_t *dst, size_t n)
 int launch_iota(hipStream_t s, uint32_t *p, size_t n);
 int particles_home(Ctx &c);
 int schedule_levels(int32_t d, int32_t levels_mode, int64_t cap,
-                    std::vector<std::vector<int32_t>> &levels, bool *complete);
+                    std::vector<std::vector<int32_t>> &levels, bool *complete,
+                    std::vector<int32_t> *min_leaf = nullptr);
 int launch_make_axis(Ctx &c, int32_t nCells);
 int launch_cell_sizes(Ctx &c, int32_t nCells, uint32_t *dst);
 // one level step of the batched form over `entries` (device array)
