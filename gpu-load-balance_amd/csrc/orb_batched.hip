@@ -318,10 +318,10 @@ __device__ inline void store_cands(float *cand, float entry)
 // the margins are counted with ballots, the rest binned.
 template <int LE>
 __global__ __launch_bounds__(kBlock) void k_b_hist(
-    const CellRec *__restrict__ recs, const uint4 *__restrict__ chunk_desc, uint32_t max_chunks,
-    const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
-    const float *__restrict__ cell_cand, uint16_t *__restrict__ chunk_hist,
-    uint16_t *__restrict__ chunk_tie)
+    const CellRec *__restrict__ recs, int32_t nCells, const uint32_t *__restrict__ first,
+    const uint4 *__restrict__ chunk_desc, uint32_t max_chunks, const float *__restrict__ x,
+    const float *__restrict__ y, const float *__restrict__ z, const float *__restrict__ cell_cand,
+    uint16_t *__restrict__ chunk_hist, uint16_t *__restrict__ chunk_tie)
 {
     constexpr int kW = kBlock / 64;
     constexpr int kPerWave = kMaskWords / kW;  // 16 loads of 64 keys
@@ -330,11 +330,13 @@ __global__ __launch_bounds__(kBlock) void k_b_hist(
     __shared__ uint32_t ties[kBins];  // keys equal to candidate j (rare)
     __shared__ uint32_t sm[kW];
     const int t = threadIdx.x, lane = t & 63, w = (int)uniform((uint32_t)t >> 6);
-    for (uint32_t q = blockIdx.x; q < max_chunks; q += gridDim.x) {
-        const uint4 d = chunk_desc[q];  // idle past the level's last chunk
+    const uint32_t total = first[nCells];
+    for (uint32_t q = blockIdx.x;; q += gridDim.x) {
+        const uint4 d = chunk_record(chunk_desc, q, max_chunks);
         const uint32_t b = uniform(d.x), e = uniform(d.y), cell = uniform(d.z);
         const uint32_t axis = uniform(d.w);
-        keep_issued();  // the whole record in one load
+        keep_values(b, e, cell, axis);  // the whole record, in flight with total
+        if (q >= total) break;
         if (axis > 2u) continue;  // uniform: the cell is not searched
         const float *kp = axis_ptr(x, y, z, (int)axis) + b;
         const float ce = cand_entry(cell_cand, cell);
@@ -407,10 +409,11 @@ __global__ __launch_bounds__(kBlock) void k_b_hist(
 // registers and stop at seven).
 template <int LE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_b_hist_window(
-    const CellRec *__restrict__ recs, const uint4 *__restrict__ chunk_desc, uint32_t max_chunks,
-    const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
-    const float *__restrict__ cell_cand, uint16_t *__restrict__ chunk_hist,
-    float *__restrict__ pack, uint32_t *__restrict__ comp, uint64_t *__restrict__ masks)
+    const CellRec *__restrict__ recs, int32_t nCells, const uint32_t *__restrict__ first,
+    const uint4 *__restrict__ chunk_desc, uint32_t max_chunks, const float *__restrict__ x,
+    const float *__restrict__ y, const float *__restrict__ z, const float *__restrict__ cell_cand,
+    uint16_t *__restrict__ chunk_hist, float *__restrict__ pack, uint32_t *__restrict__ comp,
+    uint64_t *__restrict__ masks)
 {
     constexpr int kW = kBlock / 64;
     constexpr int kPerWave = kMaskWords / kW;  // 16 loads of 64 keys
@@ -421,12 +424,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
     __shared__ uint64_t words[kW][2][kPerWave];  // each wave's mask words, staged
     const int t = threadIdx.x, lane = t & 63, w = (int)uniform((uint32_t)t >> 6);
     const uint64_t lanes_below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    for (uint32_t q = blockIdx.x; q < max_chunks; q += gridDim.x) {
-        const uint4 d = chunk_desc[q];  // idle past the level's last chunk
+    const uint32_t total = first[nCells];
+    for (uint32_t q = blockIdx.x;; q += gridDim.x) {
+        const uint4 d = chunk_record(chunk_desc, q, max_chunks);
         const uint32_t b = uniform(d.x), e = uniform(d.y), cell = uniform(d.z);
         const uint32_t axis = uniform(d.w);
-        keep_issued();  // the whole record in one load
+        keep_values(b, e, cell, axis);  // the whole record, in flight with total
+        if (q >= total) break;
         if (axis > 2u) continue;  // uniform: the cell is not searched
+        if (recs[cell].found) continue;  // uniform: the first round found the cut
         // the chunk's keys through a range-checked descriptor: loads past the
         // chunk's end return 0 and are masked out below, with no branches
         const float *kp = axis_ptr(x, y, z, (int)axis) + b;
@@ -439,9 +445,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
         for (int j = 0; j < kPerWave; ++j)
             v[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
                 kr, (int)((sb + (uint32_t)lane) * 4u), 256 * j, 0));
-        // a cell the first round finished is binned too and its results
-        // ignored: the rows go to their own region, and a check here would
-        // wait for the cell record before the keys are requested
         const float lo = uniform(recs[cell].margin_l), hi = uniform(recs[cell].margin_r);
         store_cands(cand, ce);
         hist[t] = 0u;
@@ -524,6 +527,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
 // only; otherwise the wave builds its own candidates and LDS histogram.
 template <int LE>
 __global__ __launch_bounds__(kBlock) void k_b_hist_packed(const CellRec *__restrict__ recs,
+                                                          int32_t nCells,
+                                                          const uint32_t *__restrict__ first,
                                                           const uint4 *__restrict__ chunk_desc,
                                                           uint32_t max_chunks,
                                                           uint16_t *__restrict__ chunk_hist,
@@ -535,15 +540,18 @@ __global__ __launch_bounds__(kBlock) void k_b_hist_packed(const CellRec *__restr
     __shared__ float cand[kW][kBins + 1];
     __shared__ uint32_t hist[kW][kBins];
     const int lane = threadIdx.x & 63, w = (int)uniform((uint32_t)threadIdx.x >> 6);
-    for (uint32_t q = blockIdx.x * kW + (uint32_t)w; q < max_chunks; q += gridDim.x * kW) {
-        // the record and the packed counts together, then the cell record
-        const uint4 d = chunk_desc[q];
-        const uint4 ncs = *reinterpret_cast<const uint4 *>(comp + 8 * (size_t)q);
-        const uint32_t nbelow = uniform(comp[8 * (size_t)q + 4]);
+    const uint32_t total = first[nCells];
+    for (uint32_t q = blockIdx.x * kW + (uint32_t)w;; q += gridDim.x * kW) {
+        // the record and the packed counts together (with total), then the cell record
+        const uint32_t qc = q < max_chunks ? q : 0u;  // counts slot in bounds
+        const uint4 ncs = *reinterpret_cast<const uint4 *>(comp + 8 * (size_t)qc);
+        const uint32_t nbelow = uniform(comp[8 * (size_t)qc + 4]);
+        const uint4 d = chunk_record(chunk_desc, q, max_chunks);
         const uint32_t b = uniform(d.x), e = uniform(d.y), cell = uniform(d.z);
         const uint32_t axis = uniform(d.w);
         const uint32_t nc4[kW] = {uniform(ncs.x), uniform(ncs.y), uniform(ncs.z), uniform(ncs.w)};
-        keep_issued();  // the record and the counts in one round trip
+        keep_values(b, e, cell, axis, nc4[0], nc4[1], nc4[2], nc4[3]);  // one round trip
+        if (q >= total) break;
         if (axis > 2u) continue;  // uniform: the cell is not searched
         const CellRec &r = recs[cell];
         if (r.found) continue;  // uniform per wave
@@ -1311,28 +1319,28 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
             ORB_HIP(hipGetLastError());
         }
 #define ORB_HIST(L)                                                                              \
-    hipLaunchKernelGGL((k_b_hist<L>), dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p,             \
-                       c.chunk_desc.p, (uint32_t)c.max_chunks, c.x(), c.y(), c.z(), c.cell_cand.p, \
-                       chunk_hist, chunk_tie)
+    hipLaunchKernelGGL((k_b_hist<L>), dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,     \
+                       c.cell_first.p, c.chunk_desc.p, (uint32_t)c.max_chunks, c.x(), c.y(), c.z(), \
+                       c.cell_cand.p, chunk_hist, chunk_tie)
 #define ORB_HIST_WINDOW(L)                                                                       \
     hipLaunchKernelGGL((k_b_hist_window<L>), dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p,      \
-                       c.chunk_desc.p, (uint32_t)c.max_chunks, c.x(), c.y(), c.z(),              \
-                       c.cell_cand.p, later_hist, ckeys, c.chunk_comp.p, c.masks.p)
+                       nCells, c.cell_first.p, c.chunk_desc.p, (uint32_t)c.max_chunks, c.x(), c.y(), \
+                       c.z(), c.cell_cand.p, later_hist, ckeys, c.chunk_comp.p, c.masks.p)
         const unsigned gp = (gq + 3) / 4;  // a wave per chunk
         if (le) {
             if (mode == 0) ORB_HIST(1);
             else if (mode == 1) ORB_HIST_WINDOW(1);
             else
                 hipLaunchKernelGGL(k_b_hist_packed<1>, dim3(gp), dim3(kBlock), 0, c.stream, c.recs.p,
-                                   c.chunk_desc.p, (uint32_t)c.max_chunks, later_hist, ckeys,
-                                   c.chunk_comp.p);
+                                   nCells, c.cell_first.p, c.chunk_desc.p, (uint32_t)c.max_chunks,
+                                   later_hist, ckeys, c.chunk_comp.p);
         } else {
             if (mode == 0) ORB_HIST(0);
             else if (mode == 1) ORB_HIST_WINDOW(0);
             else
                 hipLaunchKernelGGL(k_b_hist_packed<0>, dim3(gp), dim3(kBlock), 0, c.stream, c.recs.p,
-                                   c.chunk_desc.p, (uint32_t)c.max_chunks, later_hist, ckeys,
-                                   c.chunk_comp.p);
+                                   nCells, c.cell_first.p, c.chunk_desc.p, (uint32_t)c.max_chunks,
+                                   later_hist, ckeys, c.chunk_comp.p);
         }
 #undef ORB_HIST
 #undef ORB_HIST_WINDOW

@@ -18,11 +18,29 @@ __device__ inline float uniform(float v)
     return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
-// A compiler-only barrier (no instruction): loads above it stay above it.
-// Without it the compiler splits a record load around a later uniform branch
-// (the field the branch tests first, the rest after it): two round trips
-// where one does.
-__device__ inline void keep_issued() { asm volatile("" ::: "memory"); }
+// A compiler-only barrier (no instruction) on four values: the loads that
+// produce them complete above it, so a later branch cannot pull them below
+// itself (the compiler would otherwise split a record load around a branch
+// on one of its fields, or issue it only after an unrelated check: one more
+// memory round trip per chunk).
+__device__ inline void keep_values(uint32_t a, uint32_t b, uint32_t c, uint32_t d)
+{
+    asm volatile("" ::"s"(a), "s"(b), "s"(c), "s"(d));
+}
+__device__ inline void keep_values(uint32_t a, uint32_t b, uint32_t c, uint32_t d, uint32_t e,
+                                   uint32_t f, uint32_t g, uint32_t h)
+{
+    asm volatile("" ::"s"(a), "s"(b), "s"(c), "s"(d), "s"(e), "s"(f), "s"(g), "s"(h));
+}
+
+// The record of chunk q of the level.  A loop over the level's chunks loads
+// it before it compares q with the chunk count (so both loads are in flight
+// at once) and uses it only when q is below that count; an index past the
+// allocation reads its last record instead (no branch around the load).
+__device__ inline uint4 chunk_record(const uint4 *chunk_desc, uint32_t q, uint32_t max_chunks)
+{
+    return chunk_desc[q < max_chunks ? q : max_chunks - 1u];
+}
 
 template <class T>
 __device__ inline T wave_sum(T v)

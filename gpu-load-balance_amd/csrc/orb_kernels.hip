@@ -289,21 +289,17 @@ __global__ void k_level_recs(const orb_cell *cells, int32_t nCells, const uint32
 
 // Each chunk's cell, and its record: first key, one past the last, cell,
 // axis (kChunkIdle when the cell has its cut already, belongs to a subtree
-// or has no axis; and for every record past the level's last chunk).  A
-// kernel that takes a chunk per block or wave reaches its keys after one
-// load of the record instead of the chain chunk count -> chunk -> cell ->
-// cell record.
+// or has no axis).  A kernel that takes a chunk per block or wave reaches
+// its keys after one load of the record (issued together with the level's
+// chunk count) instead of the chain chunk count -> chunk -> cell -> cell
+// record.
 __global__ void k_fill_chunks(const uint32_t *first, int32_t nCells, const CellRec *recs,
                               uint32_t *chunk_cell, uint4 *chunk_desc, uint32_t max_chunks)
 {
     uint32_t total = first[nCells];
     if (total > max_chunks) total = max_chunks;
-    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < max_chunks;
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < total;
          q += gridDim.x * blockDim.x) {
-        if (q >= total) {
-            chunk_desc[q] = make_uint4(0u, 0u, 0u, kChunkIdle);
-            continue;
-        }
         const uint32_t cell = chunk_to_cell(first, nCells, q);
         chunk_cell[q] = cell;
         const CellRec &r = recs[cell];
@@ -716,24 +712,25 @@ __global__ __launch_bounds__(kBlock) void k_hoare_masks(const CellRec *recs, int
 //
 // The chunk's record and its two window-round mask words per lane are
 // requested together; the cell record next; the packed keys last.  Chunks of
-// cells that are not searched (subtree cells, no axis) and records past the
-// level's last chunk are skipped: nothing reads their masks, and their count
-// slot is written as zero.
+// cells that are not searched (subtree cells, no axis) are skipped: nothing
+// reads their masks, and their count slot is written as zero.
 __global__ __launch_bounds__(kBlock) void k_hoare_masks_window(
-    const CellRec *__restrict__ recs, const uint4 *__restrict__ chunk_desc, uint32_t max_chunks,
-    const float *x, const float *y, const float *z, const float *pack, uint64_t *masks,
-    uint64_t *chunk_cnt)
+    const CellRec *__restrict__ recs, int32_t nCells, const uint32_t *__restrict__ first,
+    const uint4 *__restrict__ chunk_desc, uint32_t max_chunks, const float *x, const float *y,
+    const float *z, const float *pack, uint64_t *masks, uint64_t *chunk_cnt)
 {
     constexpr int kW = kBlock / 64;
     constexpr int kQuarter = kMaskWords / kW;  // words per packing quarter
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (uint32_t q = blockIdx.x * kW + (uint32_t)w; q < max_chunks; q += gridDim.x * kW) {
-        const uint4 d = chunk_desc[q];
-        uint64_t *mq = masks + (size_t)q * (2 * kMaskWords);
+    const uint32_t total = first[nCells];
+    for (uint32_t q = blockIdx.x * kW + (uint32_t)w;; q += gridDim.x * kW) {
+        const uint4 d = chunk_record(chunk_desc, q, max_chunks);
+        uint64_t *mq = masks + (size_t)(q < max_chunks ? q : 0u) * (2 * kMaskWords);
         const uint64_t Gw = mq[lane], Ww = mq[kMaskWords + lane];  // in flight with the record
         const uint32_t b = uniform(d.x), e = uniform(d.y), cell = uniform(d.z);
         const uint32_t axis = uniform(d.w);
-        keep_issued();  // the whole record in one load
+        keep_values(b, e, cell, axis);  // the whole record, in flight with total
+        if (q >= total) break;
         if (axis > 2u) {  // uniform
             if (lane == 0) chunk_cnt[q] = 0ull;
             continue;
@@ -1338,8 +1335,9 @@ int launch_hoare(Ctx &c, int32_t nCells, int32_t *status, const HistCounts *hc)
     const unsigned gq = chunk_grid(c);
     if (hc && hc->pack)  // batched form: the window round left masks
         hipLaunchKernelGGL(k_hoare_masks_window, dim3((gq + 3) / 4), dim3(kBlock), 0, c.stream,
-                           c.recs.p, c.chunk_desc.p, (uint32_t)c.max_chunks, c.x(), c.y(), c.z(),
-                           hc->pack, c.masks.p, c.chunk_cnt.p);
+                           c.recs.p, nCells, c.cell_first.p, c.chunk_desc.p,
+                           (uint32_t)c.max_chunks, c.x(), c.y(), c.z(), hc->pack, c.masks.p,
+                           c.chunk_cnt.p);
     else
         hipLaunchKernelGGL(k_hoare_masks, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
                            c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(), c.masks.p,
