@@ -282,19 +282,6 @@ __device__ inline bool bin_estimate(float v, float lo, float scale, float lim, i
     return fabsf(__builtin_amdgcn_fractf(ef) - 0.5f) < lim;
 }
 
-// Each searching cell's 255 candidate cuts for a histogram round (entry 0
-// unused), computed once per cell instead of once per chunk.
-__global__ __launch_bounds__(kBlock) void k_b_cands(const CellRec *recs, int32_t nCells,
-                                                    float *cell_cand)
-{
-    for (int32_t c = blockIdx.x; c < nCells; c += gridDim.x) {
-        const CellRec r = recs[c];
-        if (r.found || r.flags || r.axis < 0 || r.axis > 2) continue;  // uniform
-        const int t = threadIdx.x;
-        cell_cand[(size_t)c * kBins + t] = t ? candidate(t, r.margin_l, r.margin_r) : 0.0f;
-    }
-}
-
 // The candidate table of a cell into LDS, with the -inf / +inf sentinels
 // at 0 and kBins: each thread loads its entry (cand_entry, issued before the
 // chunk's keys so that waiting for it does not wait for them), then stores it.
@@ -320,8 +307,8 @@ template <int LE>
 __global__ __launch_bounds__(kBlock) void k_b_hist(
     const CellRec *__restrict__ recs, int32_t nCells, const uint32_t *__restrict__ first,
     const uint4 *__restrict__ chunk_desc, uint32_t max_chunks, const float *__restrict__ x,
-    const float *__restrict__ y, const float *__restrict__ z, const float *__restrict__ cell_cand,
-    uint16_t *__restrict__ chunk_hist, uint16_t *__restrict__ chunk_tie)
+    const float *__restrict__ y, const float *__restrict__ z, uint16_t *__restrict__ chunk_hist,
+    uint16_t *__restrict__ chunk_tie)
 {
     constexpr int kW = kBlock / 64;
     constexpr int kPerWave = kMaskWords / kW;  // 16 loads of 64 keys
@@ -339,7 +326,6 @@ __global__ __launch_bounds__(kBlock) void k_b_hist(
         if (q >= total) break;
         if (axis > 2u) continue;  // uniform: the cell is not searched
         const float *kp = axis_ptr(x, y, z, (int)axis) + b;
-        const float ce = cand_entry(cell_cand, cell);
         const __amdgpu_buffer_rsrc_t kr =
             __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(kp), 0, (int)((e - b) * 4u), 0x00020000);
         const uint32_t sb = (uint32_t)(w * kPerWave * 64);  // chunk position of the quarter
@@ -349,7 +335,9 @@ __global__ __launch_bounds__(kBlock) void k_b_hist(
             v[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
                 kr, (int)((sb + (uint32_t)lane) * 4u), 256 * j, 0));
         const float lo = uniform(recs[cell].margin_l), hi = uniform(recs[cell].margin_r);
-        store_cands(cand, ce);
+        // the candidate table from the margins while the keys arrive (no
+        // table load: the first round starts from each cell's fresh margins)
+        store_cands(cand, t ? candidate(t, lo, hi) : -INFINITY);
 #pragma unroll
         for (int u = 0; u < kW; ++u) hist[u][t] = 0u;
         ties[t] = 0u;
@@ -1306,22 +1294,15 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
     const unsigned gq = chunk_grid(c);
     const unsigned gr = (unsigned)((gq + kReduceRows - 1) / kReduceRows * 4);
     const unsigned gw = (unsigned)((nCells + 3) / 4);
-    const unsigned gc = (unsigned)(nCells < 4096 ? nCells : 4096);
     for (int r = 0; r < kRounds; ++r) {
         const int tb = timing_mark(c);
         // rounds: all keys, all keys + packing the window, then the packed keys
         float *ckeys = c.packed.p;
         const int mode = r == 0 ? 0 : (r == 1 ? 1 : 2);
-        if (mode == 0) {  // the block-per-chunk rounds read the candidates from a table; the
-                          // first round's replay writes the second round's
-            hipLaunchKernelGGL(k_b_cands, dim3(gc), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
-                               c.cell_cand.p);
-            ORB_HIP(hipGetLastError());
-        }
 #define ORB_HIST(L)                                                                              \
     hipLaunchKernelGGL((k_b_hist<L>), dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,     \
                        c.cell_first.p, c.chunk_desc.p, (uint32_t)c.max_chunks, c.x(), c.y(), c.z(), \
-                       c.cell_cand.p, chunk_hist, chunk_tie)
+                       chunk_hist, chunk_tie)
 #define ORB_HIST_WINDOW(L)                                                                       \
     hipLaunchKernelGGL((k_b_hist_window<L>), dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p,      \
                        nCells, c.cell_first.p, c.chunk_desc.p, (uint32_t)c.max_chunks, c.x(), c.y(), \

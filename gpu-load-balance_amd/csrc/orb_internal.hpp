@@ -375,7 +375,8 @@ struct Ctx {
     DevBuf<uint64_t> masks;          // [max_chunks][2][64] Hoare stopper masks per chunk
     DevBuf<uint32_t> chunk_aux;      // [max_chunks+1] Hoare swap: each unit's partner chunk
     DevBuf<float> cell_cand;         // [max_level_cells][256] batched form: candidate cuts of
-                                     // each searching cell for the current histogram round
+                                     // each searching cell for the second histogram round
+                                     // (written by the first round's replay)
     DevBuf<uint32_t> chunk_comp;     // [max_chunks+1][8] packed keys per wave, keys below
     DevBuf<SplitRec> splits;         // [cells]
     DevBuf<int32_t> status;          // [2]: error flag, failing heap id
