@@ -206,6 +206,29 @@ __global__ void k_b_recs(const LevelEntry *entries, int32_t nCells, const orb_ce
                              : level_rec(entries, i, heap, ranges, sub, use_subtree, sub_max, recs);
 }
 
+// A level of fewer than kRecsScan cells in one block of kRecsScan threads,
+// a cell per thread: the cell records and the exclusive scan of their chunk
+// counts into first[] (first[nCells] = the level's chunks), in place of a
+// record launch and a scan launch.
+constexpr int kRecsScan = 1024;
+__global__ __launch_bounds__(kRecsScan) void k_b_recs_scan(
+    const LevelEntry *__restrict__ entries, int32_t nCells, const orb_cell *__restrict__ heap,
+    const uint32_t *__restrict__ ranges, const uint8_t *__restrict__ sub, int use_subtree,
+    uint32_t sub_max, CellRec *__restrict__ recs, uint32_t *__restrict__ first)
+{
+    constexpr int kW = kRecsScan / 64;
+    __shared__ uint32_t wsum[kW];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const uint32_t v =
+        t < nCells ? level_rec(entries, t, heap, ranges, sub, use_subtree, sub_max, recs) : 0u;
+    const uint32_t inc = wave_inclusive_scan(v);
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    uint32_t pre = 0;
+    for (int i = 0; i < w; ++i) pre += wsum[i];
+    if (t <= nCells) first[t] = pre + inc - v;  // t = nCells: the level's chunk count
+}
+
 // in-order index t in [1, 255] of the depth-8 candidate tree on [lo, hi]
 __device__ inline float candidate(int t, float lo, float hi)
 {
@@ -1267,12 +1290,20 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
         Ctx &c;
         ~LevelScope() { c.cur_level = -1; }
     } scope{c};
-    unsigned g1 = (unsigned)((nCells + 1 + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(k_b_recs, dim3(g1), dim3(kBlock), 0, c.stream, entries, nCells, heap,
-                       c.ranges.p, sub, use_subtree ? 1 : 0, (uint32_t)sub_max, c.recs.p,
-                       c.cell_nchunks.p);
-    ORB_HIP(hipGetLastError());
-    ORB_TRY(build_chunk_table(c, nCells, false));
+    if (nCells < kRecsScan) {
+        hipLaunchKernelGGL(k_b_recs_scan, dim3(1), dim3(kRecsScan), 0, c.stream, entries, nCells, heap,
+                           c.ranges.p, sub, use_subtree ? 1 : 0, (uint32_t)sub_max, c.recs.p,
+                           c.cell_first.p);
+        ORB_HIP(hipGetLastError());
+        ORB_TRY(build_chunk_table(c, nCells, false, true));
+    } else {
+        unsigned g1 = (unsigned)((nCells + 1 + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL(k_b_recs, dim3(g1), dim3(kBlock), 0, c.stream, entries, nCells, heap,
+                           c.ranges.p, sub, use_subtree ? 1 : 0, (uint32_t)sub_max, c.recs.p,
+                           c.cell_nchunks.p);
+        ORB_HIP(hipGetLastError());
+        ORB_TRY(build_chunk_table(c, nCells, false));
+    }
     uint16_t *chunk_tie = chunk_hist + (size_t)(c.max_chunks + 1) * kBins;
     uint16_t *later_hist = chunk_hist + 2 * (size_t)(c.max_chunks + 1) * kBins;
     if (use_subtree) {

@@ -440,7 +440,8 @@ int scan_exclusive_u64(Ctx &c, const uint64_t *in, uint64_t *out, size_t m,
                        const uint32_t *lim = nullptr);
 unsigned chunk_grid(const Ctx &c);
 int upload_level(Ctx &c, const orb_cell *cells, int32_t nCells);
-int build_chunk_table(Ctx &c, int32_t nCells, bool with_stamps = true);
+// scanned: cell_first already holds the exclusive scan of the chunk counts
+int build_chunk_table(Ctx &c, int32_t nCells, bool with_stamps = true, bool scanned = false);
 int launch_count(Ctx &c, int32_t nCells, bool lt_skip_found, const float *axis_override,
                  hipEvent_t after_count = nullptr);
 // Per-chunk histograms of the round that found each cell's cut (batched

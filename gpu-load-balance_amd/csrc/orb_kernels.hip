@@ -359,9 +359,10 @@ int invalidate_stamps(Ctx &c, int32_t nCells);
 // with_stamps: the plain form reuses a count pass's chunk counts for the
 // partition when the stamp matches; the batched form takes them from its
 // histograms and needs no stamps.
-int build_chunk_table(Ctx &c, int32_t nCells, bool with_stamps)
+int build_chunk_table(Ctx &c, int32_t nCells, bool with_stamps, bool scanned)
 {
-    ORB_TRY(scan_exclusive_u32(c, c.cell_nchunks.p, c.cell_first.p, (size_t)nCells + 1));
+    if (!scanned)
+        ORB_TRY(scan_exclusive_u32(c, c.cell_nchunks.p, c.cell_first.p, (size_t)nCells + 1));
     hipLaunchKernelGGL(k_fill_chunks, dim3(grid_for((size_t)c.max_chunks, 16384)), dim3(kBlock), 0,
                        c.stream, c.cell_first.p, nCells, c.recs.p, c.chunk_cell.p, c.chunk_desc.p,
                        (uint32_t)c.max_chunks);
