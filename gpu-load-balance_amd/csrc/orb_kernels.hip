@@ -62,7 +62,7 @@ int launch_zero_u64(hipStream_t s, uint64_t *p, size_t n)
 }
 
 // ---------------------------------------------------------------------------
-// exclusive scan (three kernels; m <= kScanTile^2)
+// exclusive scan (two kernels; m <= kScanTile^2)
 // ---------------------------------------------------------------------------
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlock * kScanItems;  // 2048
@@ -90,31 +90,28 @@ __global__ void k_scan_reduce(const T *in, size_t m, T *partials, const uint32_t
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
+// raw: partials holds each tile's sum (not yet scanned); the block adds up
+// those of the tiles before it (at most kScanTile of them, kScanItems per
+// thread), which saves a launch over scanning them first.
 template <class T>
-__global__ void k_scan_partials(T *partials, int np)
+__global__ void k_scan_apply(const T *in, T *out, size_t m, const T *partials, const uint32_t *lim,
+                             int raw)
 {
     __shared__ T sm[4];
-    T v[kScanItems];
-    int base = threadIdx.x * kScanItems;
-    T s = 0;
+    T offset = 0;
+    if (partials) {
+        if (raw) {
+            T p = 0;
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        v[k] = (base + k < np) ? partials[base + k] : T(0);
-        s += v[k];
+            for (int k = 0; k < kScanItems; ++k) {
+                const uint32_t i = threadIdx.x * kScanItems + k;
+                if (i < blockIdx.x) p += partials[i];
+            }
+            offset = block_sum(p, sm);
+        } else {
+            offset = partials[blockIdx.x];
+        }
     }
-    T total;
-    T pre = block_exclusive_scan(s, sm, &total);
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        if (base + k < np) partials[base + k] = pre;
-        pre += v[k];
-    }
-}
-
-template <class T>
-__global__ void k_scan_apply(const T *in, T *out, size_t m, const T *partials, const uint32_t *lim)
-{
-    __shared__ T sm[4];
     size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
     const size_t mr = scan_limit(m, lim);
     const size_t mw = lim ? (mr + 1 < m ? mr + 1 : m) : m;
@@ -126,7 +123,7 @@ __global__ void k_scan_apply(const T *in, T *out, size_t m, const T *partials, c
         s += v[k];
     }
     T total;
-    T pre = block_exclusive_scan(s, sm, &total) + (partials ? partials[blockIdx.x] : T(0));
+    T pre = block_exclusive_scan(s, sm, &total) + offset;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         if (base + k < mw) out[base + k] = pre;
@@ -147,15 +144,14 @@ static int scan_exclusive(Ctx &c, const T *in, T *out, size_t m, DevBuf<uint64_t
     if (np == 1) {
         // one tile: the block scan alone (offset 0)
         hipLaunchKernelGGL(k_scan_apply<T>, dim3(1), dim3(kBlock), 0, c.stream, in, out, m,
-                           (const T *)nullptr, lim);
+                           (const T *)nullptr, lim, 0);
         ORB_HIP(hipGetLastError());
         return ORB_OK;
     }
     hipLaunchKernelGGL(k_scan_reduce<T>, dim3((unsigned)np), dim3(kBlock), 0, c.stream, in, m, partials,
                        lim);
-    hipLaunchKernelGGL(k_scan_partials<T>, dim3(1), dim3(kBlock), 0, c.stream, partials, (int)np);
     hipLaunchKernelGGL(k_scan_apply<T>, dim3((unsigned)np), dim3(kBlock), 0, c.stream, in, out, m,
-                       (const T *)partials, lim);
+                       (const T *)partials, lim, 1);
     ORB_HIP(hipGetLastError());
     return ORB_OK;
 }
