@@ -357,6 +357,11 @@ def main():
             f"count {tm['count_ms'] / tm['instrumented_steps']:.3f} ms, partition "
             f"{tm['partition_ms'] / tm['instrumented_steps']:.3f} ms, swap "
             f"{tm['swap_ms'] / tm['instrumented_steps']:.3f} ms)")
+    # this GPU's own streaming bandwidth over the same arrays, to read the
+    # roofline fractions against what HBM delivers in practice
+    stream_read, stream_copy = ctx.measure_stream(10)
+    log(f"[bench] stream over the particle arrays: read {stream_read:.0f} GB/s, "
+        f"copy {stream_copy:.0f} GB/s (read + write)")
     ctx.close()
 
     if args.config != "c2":
@@ -420,6 +425,8 @@ def main():
             "timed_in": f"{tm['instrumented_steps']} timed plain builds with HIP events around "
                         f"each count launch (mean {tm['instrumented_ms']:.3f} ms; plain_ms is "
                         f"timed without events)",
+            "measured_read_GBps": round(stream_read, 1),
+            "frac_of_measured_read": round(achieved / stream_read, 4) if stream_read > 0 else None,
         },
     }
     # the batched build's dominant kernel: the Hoare swap, one launch per level
@@ -445,6 +452,8 @@ def main():
             "bytes_per_launch": sw_bytes,
             "timed_in": "one batched build with HIP events around each swap launch, after the "
                         "timed (graph-replayed, uninstrumented) builds",
+            "measured_copy_GBps": round(stream_copy, 1),
+            "frac_of_measured_copy": round(sw_ach / stream_copy, 4) if stream_copy > 0 else None,
         }
     if not args.no_cpu:
         scale = n / (1 << args.cpu_log2n)

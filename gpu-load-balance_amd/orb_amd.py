@@ -114,6 +114,7 @@ PROTOTYPES = {
     "orb_reset_ranges": (C.c_int, [_vp]),
     "orb_save_particles": (C.c_int, [_vp]),
     "orb_restore_particles": (C.c_int, [_vp]),
+    "orb_measure_stream": (C.c_int, [_vp, _i32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "orb_copy_particles": (C.c_int, [_vp]),
     "orb_copy_cells": (C.c_int, [_vp, _vp, _i32]),
     "orb_count": (C.c_int, [_vp, _vp, _i32, _vp]),
@@ -424,6 +425,13 @@ class Context:
         ms = np.zeros(64, np.float64)
         k = _check(lib().orb_get_level_timing(self.h, _p(ms), 64), "orb_get_level_timing")
         return ms[:k].copy()
+
+    def measure_stream(self, reps=10):
+        """(read GB/s, copy GB/s) of 16-byte streams over the particle arrays."""
+        r, w = C.c_double(0.0), C.c_double(0.0)
+        _check(lib().orb_measure_stream(self.h, int(reps), C.byref(r), C.byref(w)),
+               "orb_measure_stream")
+        return r.value, w.value
 
     def enable_graph(self, on=True):
         _check(lib().orb_enable_graph(self.h, bool(on)), "orb_enable_graph")

@@ -152,3 +152,18 @@ def test_undefined_input_is_reported(orb, oracle):
     info = ctx.build(orb.FORM_PLAIN, orb.LEVELS_REFERENCE)
     assert info["nlevels"] == 1
     ctx.close()
+
+
+def test_measure_stream_leaves_particles_alone(orb, oracle):
+    """The bandwidth reference reads the particle arrays and copies them to
+    scratch: plausible rates, and the context's particles unchanged."""
+    n = (1 << 22) + 64
+    ctx = orb.Context(n, 16, generate=True)
+    before = ctx.get_particles()
+    read_gbs, copy_gbs = ctx.measure_stream(3)
+    assert 100.0 < read_gbs < 8000.0
+    assert 100.0 < copy_gbs < 8000.0
+    after = ctx.get_particles()
+    for a, b in zip(before, after):
+        assert np.array_equal(fbits(a), fbits(b))
+    ctx.close()
