@@ -170,3 +170,26 @@ def test_batched_capped_and_found_cells(orb, oracle, grid_bits):
                for _ in range(3))
     g, r = run_both(orb, oracle, x, y, z, 2048, 1, form=BATCHED)
     assert_same_build(g, r)
+
+
+def _bin_edge_keys(n, rng):
+    """Keys at the root cell's first-round candidate cuts (-0.5 + j/256, exact
+    midpoints of [-0.5, 0.5]) and one float step either side: the arithmetic
+    bin estimate of every key sits on or next to a bin edge, so the table
+    decides every bin and every tie row is used (the first level then runs
+    into the 32-pass cap)."""
+    F = np.float32
+    c = (F(-0.5) + np.arange(1, 256, dtype=F) / F(256)).astype(F)
+    pool = np.concatenate([c, np.nextafter(c, F(-1)), np.nextafter(c, F(1))])
+    return rng.choice(pool, n).astype(F)
+
+
+@pytest.mark.parametrize("cmp", [0, 1])
+def test_batched_keys_on_bin_edges(orb, oracle, cmp):
+    rng = np.random.default_rng(11)
+    n = (1 << 17) + 3
+    x = _bin_edge_keys(n, rng)
+    y = (rng.random(n) - 0.5).astype(np.float32)
+    z = (rng.random(n) - 0.5).astype(np.float32)
+    g, r = run_both(orb, oracle, x, y, z, 64, 1, cmp, 0, form=BATCHED)
+    assert_same_build(g, r)
