@@ -810,34 +810,40 @@ int orb_get_level_timing(orb_ctx *ctx, double *ms, int32_t max_levels)
     return n;
 }
 
-// Stream kernels for the bandwidth reference: 16-byte accesses, four in
-// flight per thread per step, a grid-stride walk over the whole array.
+// Stream kernels for the bandwidth reference, shaped like the count pass:
+// each block streams one tile of kStreamTile float4 (16 KB at four per
+// thread), all of a thread's accesses in flight at once, blocks per tile.
+constexpr int kStreamPer = 4;
+constexpr size_t kStreamTile = (size_t)kBlock * kStreamPer;
 __global__ __launch_bounds__(kBlock) void k_stream_read(const float4 *a, size_t n4, float *out)
 {
-    float s = 0.0f;
-    const size_t stride = (size_t)gridDim.x * kBlock;
-    size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    for (; i + 3 * stride < n4; i += 4 * stride) {
-        const float4 v0 = a[i], v1 = a[i + stride], v2 = a[i + 2 * stride], v3 = a[i + 3 * stride];
-        s += (v0.x + v1.x + v2.x + v3.x) + (v0.y + v1.y + v2.y + v3.y) +
-             (v0.z + v1.z + v2.z + v3.z) + (v0.w + v1.w + v2.w + v3.w);
+    const size_t base = (size_t)blockIdx.x * kStreamTile + threadIdx.x;
+    float4 v[kStreamPer];
+#pragma unroll
+    for (int k = 0; k < kStreamPer; ++k) {
+        const size_t i = base + (size_t)k * kBlock;
+        v[k] = i < n4 ? a[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
-    for (; i < n4; i += stride) s += a[i].x + a[i].y + a[i].z + a[i].w;
-    if (s == 1.0e30f) out[blockIdx.x] = s;  // keeps the loads; practically never taken
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kStreamPer; ++k) s += (v[k].x + v[k].y) + (v[k].z + v[k].w);
+    if (s == 1.0e30f) out[blockIdx.x & 4095u] = s;  // keeps the loads; practically never taken
 }
 
 __global__ __launch_bounds__(kBlock) void k_stream_copy(const float4 *a, float4 *b, size_t n4)
 {
-    const size_t stride = (size_t)gridDim.x * kBlock;
-    size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    for (; i + 3 * stride < n4; i += 4 * stride) {
-        const float4 v0 = a[i], v1 = a[i + stride], v2 = a[i + 2 * stride], v3 = a[i + 3 * stride];
-        b[i] = v0;
-        b[i + stride] = v1;
-        b[i + 2 * stride] = v2;
-        b[i + 3 * stride] = v3;
+    const size_t base = (size_t)blockIdx.x * kStreamTile + threadIdx.x;
+    float4 v[kStreamPer];
+#pragma unroll
+    for (int k = 0; k < kStreamPer; ++k) {
+        const size_t i = base + (size_t)k * kBlock;
+        if (i < n4) v[k] = a[i];
     }
-    for (; i < n4; i += stride) b[i] = a[i];
+#pragma unroll
+    for (int k = 0; k < kStreamPer; ++k) {
+        const size_t i = base + (size_t)k * kBlock;
+        if (i < n4) b[i] = v[k];
+    }
 }
 
 int orb_measure_stream(orb_ctx *ctx, int32_t reps, double *read_gbs, double *copy_gbs)
@@ -848,9 +854,7 @@ int orb_measure_stream(orb_ctx *ctx, int32_t reps, double *read_gbs, double *cop
     DevBuf<float> dst, sink;
     ORB_TRY(dst.alloc(4 * n4));
     ORB_TRY(sink.alloc(4096));
-    int cus = 0;
-    ORB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
-    const unsigned per_cu[3] = {4u, 8u, 16u};  // grids tried; the fastest counts
+    const unsigned grid = (unsigned)((n4 + kStreamTile - 1) / kStreamTile);
     const float4 *a = reinterpret_cast<const float4 *>(c->pos.p);
     float4 *b = reinterpret_cast<float4 *>(dst.p);
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -861,7 +865,6 @@ int orb_measure_stream(orb_ctx *ctx, int32_t reps, double *read_gbs, double *cop
     }
     int rc = ORB_OK;
     float ms_read = 0.0f, ms_copy = 0.0f;
-    unsigned grid = 0;
     auto run = [&](bool copy, float *ms) -> int {
         for (int32_t r = -1; r < reps; ++r) {  // one untimed launch first
             if (r == 0) ORB_HIP(hipEventRecord(e0, c->stream));
@@ -877,14 +880,8 @@ int orb_measure_stream(orb_ctx *ctx, int32_t reps, double *read_gbs, double *cop
         ORB_HIP(hipEventElapsedTime(ms, e0, e1));
         return ORB_OK;
     };
-    for (int k = 0; k < 3 && rc == ORB_OK; ++k) {
-        grid = (unsigned)(cus > 0 ? cus : 256) * per_cu[k];
-        float mr = 0.0f, mc = 0.0f;
-        rc = run(false, &mr);
-        if (rc == ORB_OK) rc = run(true, &mc);
-        if (rc == ORB_OK && (ms_read == 0.0f || mr < ms_read)) ms_read = mr;
-        if (rc == ORB_OK && (ms_copy == 0.0f || mc < ms_copy)) ms_copy = mc;
-    }
+    rc = run(false, &ms_read);
+    if (rc == ORB_OK) rc = run(true, &ms_copy);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc != ORB_OK) return rc;

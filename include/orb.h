@@ -294,10 +294,10 @@ int orb_get_timing(orb_ctx *ctx, orb_timing *t);
 int orb_get_level_timing(orb_ctx *ctx, double *ms, int32_t max_levels);
 /* HBM bandwidth reference on this GPU (no counterpart in the reference; for
  * reading roofline fractions): a 16-byte streaming read and a copy of the
- * context's particle arrays (3 x n floats), `reps` timed launches each after
- * an untimed one at 4, 8 and 16 blocks per CU, the fastest grid's rate in
- * GB/s (copy counts bytes read plus bytes written).  Allocates a scratch
- * copy of the arrays for the duration of the call.                        */
+ * context's particle arrays (3 x n floats), one 16 KB tile per block like
+ * the count pass, `reps` timed launches each after an untimed one, in GB/s
+ * (copy counts bytes read plus bytes written).  Allocates a scratch copy of
+ * the arrays for the duration of the call.                                 */
 int orb_measure_stream(orb_ctx *ctx, int32_t reps, double *read_gbs, double *copy_gbs);
 int orb_enable_timing(orb_ctx *ctx, bool on);
 
