@@ -353,10 +353,12 @@ __global__ __launch_bounds__(kBlock) void k_b_hist(
             __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(kp), 0, (int)((e - b) * 4u), 0x00020000);
         const uint32_t sb = (uint32_t)(w * kPerWave * 64);  // chunk position of the quarter
         float v[kPerWave];
+        // the whole offset in the per-lane operand: the range check covers
+        // it, not the scalar offset
 #pragma unroll
         for (int j = 0; j < kPerWave; ++j)
             v[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
-                kr, (int)((sb + (uint32_t)lane) * 4u), 256 * j, 0));
+                kr, (int)((sb + (uint32_t)lane) * 4u + 256u * (uint32_t)j), 0, 0));
         const float lo = uniform(recs[cell].margin_l), hi = uniform(recs[cell].margin_r);
         // the candidate table from the margins while the keys arrive (no
         // table load: the first round starts from each cell's fresh margins)
@@ -452,10 +454,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
             __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(kp), 0, (int)((e - b) * 4u), 0x00020000);
         const uint32_t sb = (uint32_t)(w * kPerWave * 64);  // chunk position of the quarter
         float v[kPerWave];
+        // the whole offset in the per-lane operand: the range check covers
+        // it, not the scalar offset
 #pragma unroll
         for (int j = 0; j < kPerWave; ++j)
             v[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
-                kr, (int)((sb + (uint32_t)lane) * 4u), 256 * j, 0));
+                kr, (int)((sb + (uint32_t)lane) * 4u + 256u * (uint32_t)j), 0, 0));
         const float lo = uniform(recs[cell].margin_l), hi = uniform(recs[cell].margin_r);
         store_cands(cand, ce);
         hist[t] = 0u;
