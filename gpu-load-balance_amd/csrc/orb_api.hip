@@ -491,7 +491,6 @@ int orb_finalize(orb_ctx *ctx)
     c->scan_partials.release();
     c->packed.release();
     c->masks.release();
-    c->chunk_aux.release();
     c->cell_cand.release();
     c->splits.release();
     c->chunk_comp.release();

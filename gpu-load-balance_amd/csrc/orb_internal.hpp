@@ -373,7 +373,6 @@ struct Ctx {
     DevBuf<float> packed;            // [n] batched form: window keys packed by the second
                                      // histogram round, per chunk share
     DevBuf<uint64_t> masks;          // [max_chunks][2][64] Hoare stopper masks per chunk
-    DevBuf<uint32_t> chunk_aux;      // [max_chunks+1] Hoare swap: each unit's partner chunk
     DevBuf<float> cell_cand;         // [max_level_cells][256] batched form: candidate cuts of
                                      // each searching cell for the second histogram round
                                      // (written by the first round's replay)

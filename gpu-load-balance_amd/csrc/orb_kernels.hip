@@ -324,7 +324,6 @@ int ensure_level_capacity(Ctx &c, int64_t nCells)
     ORB_TRY(c.chunk_cnt.alloc(c.max_chunks + 1));
     ORB_TRY(c.chunk_pre.alloc(c.max_chunks + 1));
     ORB_TRY(c.masks.alloc(2 * (size_t)kMaskWords * (size_t)c.max_chunks));
-    ORB_TRY(c.chunk_aux.alloc(c.max_chunks + 1));
     ORB_TRY(c.cell_cand.alloc((size_t)m * kBins));
     ORB_TRY(c.splits.alloc(m));
     ORB_TRY(c.chunk_comp.alloc(8 * ((size_t)c.max_chunks + 1)));
@@ -997,43 +996,6 @@ __device__ inline void side_range(int side, const SwapUnit &u, uint32_t TR, uint
     a1 = side ? TR - u.t0 : u.t1;
 }
 
-// One thread per chunk: the cell-local chunk that holds the first partner
-// index of the chunk's swap unit (binary search over the cell's chunk
-// offsets, which sit in L2).
-__global__ __launch_bounds__(kBlock) void k_hoare_partners(const CellRec *recs, int32_t nCells,
-                                                           const uint32_t *first,
-                                                           const uint32_t *chunk_cell,
-                                                           const uint64_t *chunk_pre,
-                                                           const SplitRec *splits,
-                                                           uint32_t *partner)
-{
-    const uint32_t total = first[nCells];
-    for (uint32_t q = blockIdx.x * kBlock + threadIdx.x; q < total; q += gridDim.x * kBlock) {
-        const uint32_t cell = chunk_cell[q];
-        const SplitRec s = splits[cell];
-        uint32_t kp = 0;
-        if (s.K != 0u) {
-            const CellRec r = recs[cell];
-            const uint32_t f = first[cell], nch = first[cell + 1] - f;
-            const CellOffsets co{chunk_pre + f, chunk_pre[f]};
-            const SwapUnit u = swap_unit(co, s, r.end - r.begin, q - f);
-            if (u.t0 < u.t1) {
-                const int side = 1 - u.owner;
-                uint32_t a0, a1;
-                side_range(side, u, s.TR, a0, a1);
-                uint32_t lo = 0, hi = nch;  // last k with P(k) <= a0
-                while (hi - lo > 1u) {
-                    const uint32_t mid = lo + (hi - lo) / 2u;
-                    if (co.at(side, mid) <= a0) lo = mid;
-                    else hi = mid;
-                }
-                kp = lo;
-            }
-        }
-        partner[q] = kp;
-    }
-}
-
 // Swaps m pairs: positions sl[j] and sr[j] (cell-local) for j < m, with
 // kSwapBatch pairs per thread in flight (all loads, then all stores).
 constexpr int kSwapBatch = 8;
@@ -1143,7 +1105,6 @@ __global__ __launch_bounds__(kBlock) void k_hoare_swap(const CellRec *recs, int3
                                                        const uint64_t *chunk_pre,
                                                        const uint64_t *masks,
                                                        const SplitRec *splits,
-                                                       const uint32_t *partner,
                                                        float *x, float *y, float *z,
                                                        uint32_t *pid)
 {
@@ -1178,7 +1139,9 @@ __global__ __launch_bounds__(kBlock) void k_hoare_swap(const CellRec *recs, int3
                 ++idx;
             }
         };
-        uint32_t kp = partner[q];
+        // the chunk holding the unit's first partner index (a 64-way search
+        // over the cell's chunk offsets, each wave on its own)
+        uint32_t kp = co.chunk_of(sp, ap0, 0u, nch);
         bool own = true;
         while (true) {  // uniform
             const uint32_t bo =
@@ -1346,16 +1309,12 @@ int launch_hoare(Ctx &c, int32_t nCells, int32_t *status, const HistCounts *hc)
     hipLaunchKernelGGL(k_hoare_split, dim3(gw), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
                        c.cell_first.p, c.chunk_pre.p, c.masks.p, c.splits.p, c.ranges.p, status);
     ORB_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_hoare_partners, dim3(grid_for((size_t)c.max_chunks, 4096)), dim3(kBlock),
-                       0, c.stream, c.recs.p, nCells, c.cell_first.p, c.chunk_cell.p,
-                       c.chunk_pre.p, c.splits.p, c.chunk_aux.p);
-    ORB_HIP(hipGetLastError());
     // byte offsets inside any cell fit 32 bits below 2^30 particles
     auto swap = c.stride < (int64_t(1) << 30) ? k_hoare_swap<true> : k_hoare_swap<false>;
     const int ts = timing_mark(c);
     hipLaunchKernelGGL(swap, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
                        c.cell_first.p, c.chunk_cell.p, c.chunk_pre.p, c.masks.p, c.splits.p,
-                       c.chunk_aux.p, c.x(), c.y(), c.z(), c.ids());
+                       c.x(), c.y(), c.z(), c.ids());
     ORB_HIP(hipGetLastError());
     ORB_TRY(timing_pair(c, ts, kTimeSwap));
     unsigned gc = (unsigned)((nCells + kBlock - 1) / kBlock);

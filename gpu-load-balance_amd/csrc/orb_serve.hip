@@ -542,7 +542,6 @@ int orb_server_destroy(orb_server *o)
     c.ranges.release();
     c.packed.release();
     c.masks.release();
-    c.chunk_aux.release();
     c.cell_cand.release();
     c.status.release();
     c.hstatus.release();
