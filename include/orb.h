@@ -199,8 +199,8 @@ typedef struct orb_build_info {
     int32_t nlevels;          /* levels processed                          */
     int32_t total_passes;     /* bisection passes summed over levels       */
     int32_t level_passes[64]; /* passes per level (<= 32 each)             */
-    int32_t error_cell;       /* heap id that raised ORB_ERR_UB, else -1 (batched form: one of a
-                                 level's undefined cells, not the first in order) */
+    int32_t error_cell;       /* heap id that raised ORB_ERR_UB, else -1 (one of a level's
+                                 undefined cells, not necessarily the first in order) */
     int32_t pad;
     double ms_total;          /* host wall time of the build               */
 } orb_build_info;
