@@ -47,33 +47,6 @@ bool level_cells(int32_t d, int32_t levels_mode, int32_t l, const std::vector<ui
     return true;
 }
 
-// The reference generator (src/services/init.cu:11-25, 47-53): xorshf96 on
-// 64-bit unsigned long with the seeds of init.cu:11, i-major draws, SoA.
-static inline float xorshf96(uint64_t &x, uint64_t &y, uint64_t &z)
-{
-    uint64_t t;
-    x ^= x << 16;
-    x ^= x >> 5;
-    x ^= x << 1;
-    t = x;
-    x = y;
-    y = z;
-    z = t ^ x ^ y;
-    float f = (float)z;                  // uint64 -> float, round to nearest even
-    float q = f / (float)UINT64_MAX;     // ULONG_MAX converts to 2^64
-    return (float)((double)q - 0.5);     // float - double 0.5 in double, narrowed
-}
-
-static void generate_reference(int64_t n, float *x, float *y, float *z)
-{
-    uint64_t sx = 123456789ull, sy = 362436069ull, sz = 521288629ull;
-    for (int64_t i = 0; i < n; ++i) {
-        x[i] = xorshf96(sx, sy, sz);
-        y[i] = xorshf96(sx, sy, sz);
-        z[i] = xorshf96(sx, sy, sz);
-    }
-}
-
 void timing_reset(Ctx &c)
 {
     c.tm = orb_timing{};
@@ -152,10 +125,7 @@ static int read_status(Ctx &c, int32_t *bad_id)
     ORB_HIP(hipMemcpyAsync(c.hstatus.p, c.status.p, 2 * sizeof(int32_t), hipMemcpyDeviceToHost,
                            c.stream));
     ORB_HIP(hipStreamSynchronize(c.stream));
-    if (c.hstatus.p[0] != 0) {
-        if (bad_id) *bad_id = c.hstatus.p[1];
-        return ORB_ERR_UB;
-    }
+    if (status_error(c.hstatus.p, bad_id)) return ORB_ERR_UB;
     return ORB_OK;
 }
 
@@ -446,7 +416,9 @@ int orb_init(orb_ctx **out, int64_t nParticles, int32_t d, bool generate, orb_me
     if (generate && nParticles > 0) {
         PinBuf<float> h;
         if ((rc = h.alloc(3 * (size_t)c.stride)) < 0) return fail(rc);
-        generate_reference(nParticles, h.p, h.p + c.stride, h.p + 2 * c.stride);
+        // the reference generator (init.cu:11-25, 47-53): seed 0
+        if ((rc = orb_generate(0, nParticles, h.p, h.p + c.stride, h.p + 2 * c.stride)) < 0)
+            return fail(rc);
         e = hipMemcpyAsync(c.pos.p, h.p, 3 * sizeof(float) * (size_t)c.stride,
                            hipMemcpyHostToDevice, c.stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
@@ -531,6 +503,19 @@ int orb_set_particles(orb_ctx *ctx, const float *x, const float *y, const float 
         ORB_HIP(hipMemcpyAsync(c->x(), x, sizeof(float) * (size_t)n, hipMemcpyHostToDevice, c->stream));
         ORB_HIP(hipMemcpyAsync(c->y(), y, sizeof(float) * (size_t)n, hipMemcpyHostToDevice, c->stream));
         ORB_HIP(hipMemcpyAsync(c->z(), z, sizeof(float) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+        // a NaN coordinate is refused (ORB_ERR_ARG): checked on the device,
+        // through the status pair, which every build clears before use
+        ORB_TRY(clear_status(*c));
+        ORB_TRY(launch_find_nan(c->stream, c->x(), c->y(), c->z(), (size_t)n, c->status.p));
+        ORB_HIP(hipMemcpyAsync(c->hstatus.p, c->status.p, 2 * sizeof(int32_t),
+                               hipMemcpyDeviceToHost, c->stream));
+        ORB_HIP(hipStreamSynchronize(c->stream));
+        const bool has_nan = status_error(c->hstatus.p, nullptr);
+        ORB_TRY(clear_status(*c));
+        if (has_nan) {
+            ORB_TRY(reset_ranges(*c));
+            return ORB_ERR_ARG;
+        }
     }
     return reset_ranges(*c);
 }

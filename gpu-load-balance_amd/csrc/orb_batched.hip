@@ -50,10 +50,10 @@ struct BatchedState {
     DevBuf<uint32_t> trace;            // [cap][32]
     DevBuf<int32_t> level_passes;      // [64]
     PinBuf<int32_t> hpasses;           // [64]
-    // captured build: valid for one (levels mode, comparison, partition, trace)
+    // captured build: valid while its signature (graph_signature) holds
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    int32_t gkey[5] = {-1, -1, -1, -1, -1};
+    std::vector<uint64_t> gsig;
     // cell_hist rows are added to by k_b_hist_reduce and consumed and
     // re-zeroed by k_b_replay in the same round, so the buffer is all zero
     // between builds; it is cleared after allocation and after a failure
@@ -845,7 +845,7 @@ __global__ __launch_bounds__(kBlock) void k_b_replay(CellRec *recs, int32_t nCel
     for (int j = 0; j < (int)(sizeof(orb_cell) / 4); ++j) wl[j] = wr[j] = 0u;
     const int64_t hb = r.hb;
     if (right_child_id(r.id) >= r.capr || !cell_split(p, lc, rc)) {
-        if (atomicCAS(&status[r.st], 0, 1) == 0) status[r.st + 1] = r.id;
+        report_error(status + r.st, r.lp & 63, kErrSplit, r.id);
         return;
     }
     heap[hb + lc.id] = lc;
@@ -993,7 +993,7 @@ __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32
         if (2 * ncur > kSmallLeaves) {
             // cannot happen while the root has <= kSmallLeaves leaves; refuse
             // rather than leave the subtree half built
-            if (t == 0 && atomicCAS(&st[0], 0, 1) == 0) st[1] = rr.id;
+            if (t == 0) report_error(st, l, kErrSplit, rr.id);
             break;
         }
         const int g = ncur >= 4 ? 1 : 4 / ncur;          // waves per cell
@@ -1171,10 +1171,11 @@ __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32
 #pragma unroll
                 for (int j = 0; j < (int)(sizeof(orb_cell) / 4); ++j) wl[j] = wr[j] = 0u;
                 const uint32_t i = sI[slot];
-                const bool ok = !bad && right_child_id(cl.id) < rr.capr && cell_split(cl, lc, rc);
-                if (!ok) {
+                // the reference splits a level's cells before it partitions any
+                const bool split_ok = right_child_id(cl.id) < rr.capr && cell_split(cl, lc, rc);
+                if (!split_ok || bad) {
                     s_err = 1;
-                    if (atomicCAS(&st[0], 0, 1) == 0) st[1] = cl.id;
+                    report_error(st, l, split_ok ? kErrPartition : kErrSplit, cl.id);
                 } else {
                     if (lb + i != le - 1) {
                         const uint32_t a = lb + i, b2 = le - 1;
@@ -1278,7 +1279,33 @@ static void drop_graph(BatchedState &b)
     if (b.graph) (void)hipGraphDestroy(b.graph);
     b.exec = nullptr;
     b.graph = nullptr;
-    for (auto &k : b.gkey) k = -1;
+    b.gsig.clear();
+}
+
+// What a captured build holds fixed: the settings that choose its launches,
+// the launch shapes, and the address of every device buffer it touches (a
+// buffer freed and allocated again -- orb_enable_ids off and on, a level
+// table grown by a service call with more cells -- has a new address, and a
+// replay through the old one would write freed memory).
+static std::vector<uint64_t> graph_signature(const Ctx &c, const BatchedState &b,
+                                             int32_t levels_mode)
+{
+    std::vector<uint64_t> s = {(uint64_t)levels_mode, (uint64_t)c.cmp, (uint64_t)c.part,
+                               (uint64_t)c.trace, (uint64_t)c.track_ids, (uint64_t)c.n,
+                               (uint64_t)c.d, (uint64_t)c.max_chunks,
+                               (uint64_t)c.max_level_cells, (uint64_t)(int64_t)c.sub_max_keys};
+    const void *bufs[] = {c.pos.p, c.tmp.p, c.pid.p, c.pid_tmp.p, c.ranges.p, c.recs.p,
+                          c.cell_nchunks.p, c.cell_first.p, c.chunk_cell.p, c.chunk_desc.p,
+                          c.chunk_ltle.p, c.chunk_cnt.p, c.chunk_pre.p, c.scan_partials.p,
+                          c.packed.p, c.masks.p, c.cell_cand.p, c.splits.p, c.chunk_comp.p,
+                          c.status.p, c.stamps.p, c.cnt.p, b.level_ids.p, b.heap.p, b.exists.p,
+                          b.sub.p, b.chunk_hist.p, b.cell_hist.p, b.trace.p, b.level_passes.p};
+    for (const void *p : bufs) s.push_back((uint64_t)(uintptr_t)p);
+    s.push_back((uint64_t)c.tune_chunk_blocks.size());
+    for (int32_t v : c.tune_chunk_blocks) s.push_back((uint64_t)(int64_t)v);
+    s.push_back((uint64_t)c.tune_sub_blocks.size());
+    for (int32_t v : c.tune_sub_blocks) s.push_back((uint64_t)(int64_t)v);
+    return s;
 }
 
 int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *heap,
@@ -1429,12 +1456,10 @@ int build_batched(Ctx &c, int32_t levels_mode, orb_build_info *info)
         b.hist_clean = true;
     }
     ORB_TRY(particles_home(c));
-    const int32_t key[5] = {levels_mode, c.cmp, c.part, c.trace ? 1 : 0, c.track_ids ? 1 : 0};
     const bool graph = c.use_graph && !c.timing;
     if (graph) {
-        bool same = b.exec != nullptr;
-        for (int k = 0; k < 5 && same; ++k) same = b.gkey[k] == key[k];
-        if (!same) {
+        std::vector<uint64_t> sig = graph_signature(c, b, levels_mode);
+        if (!b.exec || sig != b.gsig) {
             drop_graph(b);
             ORB_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
             int rc = enqueue_build(c, levels_mode);
@@ -1447,7 +1472,7 @@ int build_batched(Ctx &c, int32_t levels_mode, orb_build_info *info)
             ORB_HIP(e);
             b.graph = g;
             ORB_HIP(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
-            for (int k = 0; k < 5; ++k) b.gkey[k] = key[k];
+            b.gsig = std::move(sig);
         }
         ORB_HIP(hipGraphLaunch(b.exec, c.stream));
     } else {
@@ -1471,8 +1496,7 @@ int build_batched(Ctx &c, int32_t levels_mode, orb_build_info *info)
         info->level_passes[l - 1] = b.hpasses.p[l];
         info->total_passes += b.hpasses.p[l];
     }
-    if (c.hstatus.p[0] != 0) {
-        info->error_cell = c.hstatus.p[1];
+    if (status_error(c.hstatus.p, &info->error_cell)) {
         b.hist_clean = false;  // an aborted subtree may leave rows behind
         return ORB_ERR_UB;
     }

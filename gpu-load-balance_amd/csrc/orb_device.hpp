@@ -120,6 +120,14 @@ __device__ inline void wave_share(uint32_t b, uint32_t e, int w, uint32_t &off, 
     cnt = len - off < q ? len - off : q;
 }
 
+// Records an undefined cell in a build's status pair (the earliest in the
+// reference's order wins, whatever the order the checks land in).
+__device__ inline void report_error(int32_t *status_pair, int32_t level, int32_t kind, int32_t id)
+{
+    atomicMax(reinterpret_cast<unsigned long long *>(status_pair),
+              (unsigned long long)error_word(level, kind, id));
+}
+
 __device__ inline const float *axis_ptr(const float *x, const float *y, const float *z, int a)
 {
     return a == 0 ? x : (a == 1 ? y : z);

@@ -56,6 +56,38 @@ void set_last_hip_error(hipError_t e);
     } while (0)
 
 // ---------------------------------------------------------------------------
+// undefined-input report
+// ---------------------------------------------------------------------------
+// A build's status pair (two int32 words, 8-byte aligned) holds one 64-bit
+// word: 0 while no cell was undefined, else the complement of the key of the
+// first undefined cell in the reference's order.  orbit.cpp:102-275 walks the
+// levels in order; within a level it splits every cell (orbit.cpp:235-250)
+// before it partitions any (orbit.cpp:252-274); within each of those loops
+// it takes the cells in heap-id order (the level slices are id ranges).  So
+// the key is (level, split before partition, heap id), and every site that
+// finds an undefined cell takes the maximum of the complemented words: the
+// result does not depend on which check lands first.
+enum { kErrSplit = 0, kErrPartition = 1 };
+__host__ __device__ inline uint64_t error_word(int32_t level, int32_t kind, int32_t id)
+{
+    const uint64_t key = ((uint64_t)(uint32_t)level << 40) | ((uint64_t)(uint32_t)kind << 32) |
+                         (uint64_t)(uint32_t)id;
+    return ~key;
+}
+// The word of an input the library refuses (a non-finite coordinate in a
+// device request): above every cell's word, so it is what the build reports.
+constexpr uint64_t kErrInputWord = ~0ull;
+
+// Host side: true if the pair records an undefined cell; *id gets its heap id.
+inline bool status_error(const int32_t *pair, int32_t *id)
+{
+    const uint64_t w = (uint64_t)(uint32_t)pair[0] | ((uint64_t)(uint32_t)pair[1] << 32);
+    if (w == 0) return false;
+    if (id) *id = w == kErrInputWord ? -1 : (int32_t)(uint32_t)(~w);
+    return true;
+}
+
+// ---------------------------------------------------------------------------
 // Cell rules (src/cell.h), host and device
 // ---------------------------------------------------------------------------
 __host__ __device__ inline float cell_cut(float left, float right)
@@ -109,6 +141,59 @@ __host__ __device__ inline int32_t difference_from_target(uint32_t count_left, f
     return (int32_t)__fsub_rn((float)count_left, target);
 #else
     return (int32_t)((float)count_left - target);
+#endif
+}
+
+// uint64 -> float32, round to nearest even, by integer operations alone (the
+// same bits on the host and the device): the top 24 significant bits, then
+// the round bit and the sticky bits below it.  For values below 2^32 it is
+// the hardware conversion of the uint32 value (a correctly rounded result is
+// unique).
+__host__ __device__ inline float u64_to_f32(uint64_t v)
+{
+    if (v == 0) return 0.0f;
+    const int msb = 63 - __builtin_clzll(v);  // value in [2^msb, 2^(msb+1))
+    uint64_t m;               // 24-bit significand
+    if (msb <= 23) {
+        m = v << (23 - msb);
+    } else {
+        const int sh = msb - 23;
+        m = v >> sh;
+        const uint64_t rest = v & ((1ull << sh) - 1ull), half = 1ull << (sh - 1);
+        if (rest > half || (rest == half && (m & 1ull))) ++m;
+    }
+    int e = msb;
+    if (m == (1ull << 24)) {  // rounded up to the next power of two
+        m >>= 1;
+        ++e;
+    }
+    const uint32_t bits = ((uint32_t)(e + 127) << 23) | ((uint32_t)m & 0x7fffffu);
+    float f;
+    memcpy(&f, &bits, 4);
+    return f;
+}
+
+// The termination rule of orbit.cpp:204-208 on weight sums (workload
+// weights, documentation/notes.tex:259-271, 326-336): the weight left of the
+// cut against the cell's weight times the left share, in the same float
+// roundings; the truncation keeps 64 bits.  With every weight 1 the sums
+// are the counts and the result is difference_rule's.
+__host__ __device__ inline float difference_target_w(uint64_t weight, int32_t nLeafCells)
+{
+    float ratio = (float)(ceil(nLeafCells / 2.0) / (double)nLeafCells);
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __fmul_rn(u64_to_f32(weight), ratio);
+#else
+    return u64_to_f32(weight) * ratio;
+#endif
+}
+
+__host__ __device__ inline int64_t difference_from_target_w(uint64_t weight_left, float target)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return (int64_t)__fsub_rn(u64_to_f32(weight_left), target);
+#else
+    return (int64_t)(u64_to_f32(weight_left) - target);
 #endif
 }
 
@@ -461,6 +546,9 @@ int launch_stable_le(Ctx &c, int32_t nCells, int32_t *status = nullptr, bool ful
 int ensure_stable_buffers(Ctx &c);
 int launch_copy_u32(hipStream_t s, const uint32_t *src, uint32_t *dst, size_t n);
 int launch_iota(hipStream_t s, uint32_t *p, size_t n);
+// raises the status pair to kErrInputWord if x, y or z hold a NaN
+int launch_find_nan(hipStream_t s, const float *x, const float *y, const float *z, size_t n,
+                    int32_t *status_pair);
 int particles_home(Ctx &c);
 int schedule_levels(int32_t d, int32_t levels_mode, int64_t cap,
                     std::vector<std::vector<int32_t>> &levels, bool *complete,

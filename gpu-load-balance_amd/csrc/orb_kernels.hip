@@ -225,6 +225,31 @@ int launch_iota(hipStream_t s, uint32_t *p, size_t n)
     return ORB_OK;
 }
 
+// Non-finite input check: NaN compares false both ways, so it is neither
+// below nor above any cut, which the histogram rounds cannot count the way
+// the reference's comparisons treat it.  Builds refuse such input: any NaN
+// in x, y or z raises *word to kErrInputWord (one atomic per wave that saw
+// one, none otherwise).
+__global__ void k_find_nan(const float *x, const float *y, const float *z, size_t n,
+                           unsigned long long *word)
+{
+    bool any = false;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride)
+        any = any || isnan(x[k]) || isnan(y[k]) || isnan(z[k]);
+    if (__ballot(any) != 0ull && (threadIdx.x & 63) == 0) atomicMax(word, (unsigned long long)kErrInputWord);
+}
+
+int launch_find_nan(hipStream_t s, const float *x, const float *y, const float *z, size_t n,
+                    int32_t *status_pair)
+{
+    if (n == 0) return ORB_OK;
+    hipLaunchKernelGGL(k_find_nan, dim3(grid_for(n, 8192)), dim3(kBlock), 0, s, x, y, z, n,
+                       reinterpret_cast<unsigned long long *>(status_pair));
+    ORB_HIP(hipGetLastError());
+    return ORB_OK;
+}
+
 // Brings the particles (and ids) back to the primary buffer after a stable
 // build that ended in the other one: one copy per build instead of one per
 // level.
@@ -782,11 +807,6 @@ __global__ __launch_bounds__(kBlock) void k_hoare_masks_window(
     }
 }
 
-__device__ inline void report_error(int32_t *status, int32_t id)
-{
-    if (atomicCAS(&status[0], 0, 1) == 0) status[1] = id;
-}
-
 __device__ inline uint32_t lo32(uint64_t v) { return (uint32_t)v; }
 __device__ inline uint32_t hi32(uint64_t v) { return (uint32_t)(v >> 32); }
 
@@ -916,7 +936,7 @@ __global__ __launch_bounds__(kBlock) void k_hoare_split(const CellRec *recs, int
     if (n == 0 || TL == 0 || r.axis < 0 || r.axis > 2 || rid >= r.capr) {
         // empty cell or no key >= cut: the reference's loop leaves the cell
         if (lane == 0) {
-            report_error(status + r.st, r.id);
+            report_error(status + r.st, r.lp & 63, kErrPartition, r.id);
             s.TL = 0u;  // the swap kernels skip the cell
             splits[c] = s;
         }
@@ -1453,7 +1473,7 @@ __global__ void k_stable_ranges(const CellRec *recs, int32_t nCells, const uint3
     (void)cap;
     int64_t lid = left_child_id(r.id), rid = right_child_id(r.id);
     if (r.axis < 0 || r.axis > 2 || rid >= r.capr) {
-        report_error(status + r.st, r.id);
+        report_error(status + r.st, r.lp & 63, kErrPartition, r.id);
         return;
     }
     splits[c] = SplitRec{nle, (uint32_t)(tot >> 32), 0u, nle};

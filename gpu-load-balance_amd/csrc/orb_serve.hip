@@ -146,6 +146,7 @@ struct Request {
     int32_t next_level = 1;
     int state = kPending;
     int error = 0;
+    int32_t error_cell = -1;
     float *dev[3] = {nullptr, nullptr, nullptr};  // device request: the caller's arrays
     // pinned blocks: particles (3n floats: in at submit, out at finish; host
     // requests only) and the tree (ranges [cap][2] pool-absolute, heap,
@@ -276,6 +277,10 @@ static int admit(Server &S)
                            (uint32_t)r->n, S.level_passes.p + 64 * r->slot,
                            S.status.p + 2 * r->slot);
         ORB_HIP(hipGetLastError());
+        // a NaN coordinate fails the request with ORB_ERR_ARG (status word
+        // kErrInputWord, above every undefined-cell word)
+        ORB_TRY(launch_find_nan(c.stream, c.x() + pb, c.y() + pb, c.z() + pb, (size_t)r->n,
+                                S.status.p + 2 * r->slot));
         r->state = kActive;
         r->t_admit = now_ms();
         S.active.push_back(r);
@@ -327,7 +332,10 @@ static int poll_finishing(Server &S)
             r->state = kDone;
             r->t_done = now_ms();
             S.done_ids.push_back(r->id);
-            if (r->counters[64] != 0) r->error = ORB_ERR_UB;
+            int32_t bad = -1;
+            if (status_error(r->counters + 64, &bad))
+                r->error = bad < 0 ? ORB_ERR_ARG : ORB_ERR_UB;
+            r->error_cell = bad;
             S.stats.completed++;
             S.finishing[i] = S.finishing.back();
             S.finishing.pop_back();
@@ -655,7 +663,7 @@ int orb_server_fetch(orb_server *o, uint64_t id, float *x, float *y, float *z, u
         info->n = r->n;
         info->d = r->d;
         info->error = r->error;
-        info->error_cell = r->error ? r->counters[65] : -1;
+        info->error_cell = r->error ? r->error_cell : -1;
         for (int32_t l = 1; l <= r->lmax && l < 64; ++l) {
             info->nlevels = l;
             if (r->levels[(size_t)l].empty()) continue;  // as orb_build reports it

@@ -157,6 +157,9 @@ PROTOTYPES = {
     "orb_server_get_stats": (C.c_int, [_vp, C.POINTER(ServerStats)]),
     "orb_read_tipsy": (C.c_int, [C.c_char_p, C.POINTER(TipsyHeader), _vp, _vp, _vp, _i64]),
     "orb_init_tipsy": (C.c_int, [C.POINTER(_vp), C.c_char_p, _i32, MetaParams]),
+    "orb_generate": (C.c_int, [C.c_uint64, _i64, _vp, _vp, _vp]),
+    "orb_generate_plummer": (C.c_int, [C.c_uint64, _i64, _i32, C.c_float, C.c_float, _vp, _vp,
+                                       _vp]),
     "orb_error_string": (C.c_char_p, [C.c_int]),
     "orb_last_hip_error": (C.c_int, []),
     "orb_device_count": (C.c_int, []),
@@ -576,10 +579,10 @@ class Server:
         info = RequestInfo()
         rc = lib().orb_server_fetch(self.h, rid, _p(x), _p(y), _p(z), _p(ranges), _p(heap),
                                     _p(exists), cap, C.byref(info))
-        if rc < 0 and rc != ERR_UB:
+        if rc < 0 and rc != info.error:  # a failed call, not the build's own status
             raise OrbError(rc, "orb_server_fetch")
         return {"x": x, "y": y, "z": z, "ranges": ranges, "heap": heap, "exists": exists,
-                "error": info.error, "nlevels": info.nlevels,
+                "error": info.error, "error_cell": info.error_cell, "nlevels": info.nlevels,
                 "level_passes": list(info.level_passes[:info.nlevels]),
                 "total_passes": info.total_passes, "submit_ms": info.submit_ms,
                 "admit_ms": info.admit_ms, "done_ms": info.done_ms}
@@ -588,6 +591,24 @@ class Server:
         st = ServerStats()
         _check(lib().orb_server_get_stats(self.h, C.byref(st)), "orb_server_get_stats")
         return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+
+def generate(n, seed=0):
+    """xorshf96 particles (host code, no GPU): seed 0 is the reference's
+    stream (src/services/init.cu:11-25, 47-53)."""
+    x, y, z = (np.empty(int(n), np.float32) for _ in range(3))
+    _check(lib().orb_generate(int(seed) & (2**64 - 1), int(n), _p(x), _p(y), _p(z)),
+           "orb_generate")
+    return x, y, z
+
+
+def generate_plummer(n, seed=0, n_spheres=64, a=0.01, background=0.1):
+    """Plummer spheres plus a uniform background (host code, no GPU)."""
+    x, y, z = (np.empty(int(n), np.float32) for _ in range(3))
+    _check(lib().orb_generate_plummer(int(seed) & (2**64 - 1), int(n), int(n_spheres), float(a),
+                                      float(background), _p(x), _p(y), _p(z)),
+           "orb_generate_plummer")
+    return x, y, z
 
 
 def read_tipsy(path):

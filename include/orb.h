@@ -137,7 +137,10 @@ int orb_set_semantics(orb_ctx *ctx, int32_t cmp, int32_t partition);
 
 /* The generate=false input path (init.cu:54-59 loads a Tipsy file):
  * copies host SoA arrays of ctx's particle count to the device and resets
- * the range map to range(0) = [0, n).                                    */
+ * the range map to range(0) = [0, n).  A NaN coordinate is refused with
+ * ORB_ERR_ARG (it compares false both ways, which the batched form's
+ * histogram rounds cannot count as the reference's comparisons do);
+ * infinities are accepted.                                               */
 int orb_set_particles(orb_ctx *ctx, const float *x, const float *y, const float *z, int64_t n);
 /* Copies the device particles (current order) back to host arrays.       */
 int orb_get_particles(orb_ctx *ctx, float *x, float *y, float *z, int64_t n);
@@ -199,8 +202,11 @@ typedef struct orb_build_info {
     int32_t nlevels;          /* levels processed                          */
     int32_t total_passes;     /* bisection passes summed over levels       */
     int32_t level_passes[64]; /* passes per level (<= 32 each)             */
-    int32_t error_cell;       /* heap id that raised ORB_ERR_UB, else -1 (one of a level's
-                                 undefined cells, not necessarily the first in order) */
+    int32_t error_cell;       /* heap id that raised ORB_ERR_UB, else -1: the first
+                                 undefined cell in the reference's order (the lowest
+                                 level; there a cell whose split fails before one
+                                 whose partition does, orbit.cpp:235-274; then the
+                                 lowest heap id), in both build forms              */
     int32_t pad;
     double ms_total;          /* host wall time of the build               */
 } orb_build_info;
@@ -319,6 +325,24 @@ int orb_read_tipsy(const char *path, orb_tipsy_header *hdr, float *x, float *y, 
                    int64_t cap);
 /* orb_init(generate=false) with the snapshot's positions as the particles. */
 int orb_init_tipsy(orb_ctx **ctx, const char *path, int32_t d, orb_meta_params params);
+
+/* ------------------------------------------------------------------ */
+/* synthetic inputs (host code).  No counterpart in the reference beyond */
+/* its generator (src/services/init.cu:11-25, 47-53), which orb_generate */
+/* extends with a seed; the Plummer sets are the clustered workloads.    */
+/* ------------------------------------------------------------------ */
+/* xorshf96, i-major draws into SoA arrays: seed 0 is the reference's
+ * stream (the particles of orb_init(generate=true)); another seed derives
+ * the three state words from splitmix64.                                 */
+int orb_generate(uint64_t seed, int64_t n, float *x, float *y, float *z);
+/* round(n * background) particles uniform in [-0.5, 0.5)^3, the rest from
+ * n_spheres Plummer spheres of scale radius a (radius a / sqrt(u^(-2/3) - 1)
+ * for u uniform, isotropic direction), centres uniform in the box, equal
+ * split; draws that leave [-0.5, 0.5)^3 as float32 are drawn again.
+ * Spheres are spread over the array.  Deterministic in the seed, whatever
+ * the number of host threads used.                                       */
+int orb_generate_plummer(uint64_t seed, int64_t n, int32_t n_spheres, float a,
+                         float background, float *x, float *y, float *z);
 
 /* ------------------------------------------------------------------ */
 /* errors / device                                                     */

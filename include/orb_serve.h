@@ -45,7 +45,8 @@ typedef struct orb_server_stats {
 typedef struct orb_request_info {
     int64_t n;
     int32_t d;
-    int32_t error;              /* 0 or ORB_ERR_UB                                */
+    int32_t error;              /* 0, ORB_ERR_UB (error_cell as orb_build reports
+                                   it) or ORB_ERR_ARG (a NaN coordinate)         */
     int32_t error_cell;
     int32_t nlevels;
     int32_t total_passes;

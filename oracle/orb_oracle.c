@@ -200,38 +200,56 @@ int32_t orc_difference(uint32_t count_left, uint32_t count, int32_t nLeafCells)
     return (int32_t)diff;
 }
 
+int64_t orc_difference_w(uint64_t weight_left, uint64_t weight, int32_t nLeafCells)
+{
+    /* orbit.cpp:204-205 with the counts replaced by weight sums
+     * (notes.tex:326-336 states the weighted split target; the reference's
+     * controller reaches it by the same bisection).  uint64 -> float rounds
+     * to nearest even, as uint32 -> float does.                             */
+    float ratio = (float)(ceil(nLeafCells / 2.0) / (double)nLeafCells);
+    float prod = (float)weight * ratio;
+    float diff = (float)weight_left - prod;
+    return (int64_t)diff;
+}
+
 /* ------------------------------------------------------------------------ */
 /* partitions                                                               */
 /* ------------------------------------------------------------------------ */
 
-static void swap_rows(float *x, float *y, float *z, uint32_t *ids, int64_t a, int64_t b)
+/* payload that moves with the rows: particle ids and weights (optional) */
+typedef struct rows_payload { uint32_t *ids, *w; } rows_payload;
+
+static void swap_rows(float *x, float *y, float *z, rows_payload pl, int64_t a, int64_t b)
 {
-    /* partition.cpp:10-16; ids (optional) move with the row */
+    /* partition.cpp:10-16; ids and weights (optional) move with the row */
     float t;
     t = x[a]; x[a] = x[b]; x[b] = t;
     t = y[a]; y[a] = y[b]; y[b] = t;
     t = z[a]; z[a] = z[b]; z[b] = t;
-    if (ids) { uint32_t u = ids[a]; ids[a] = ids[b]; ids[b] = u; }
+    if (pl.ids) { uint32_t u = pl.ids[a]; pl.ids[a] = pl.ids[b]; pl.ids[b] = u; }
+    if (pl.w) { uint32_t u = pl.w[a]; pl.w[a] = pl.w[b]; pl.w[b] = u; }
 }
 
-static int hoare_impl(float *x, float *y, float *z, uint32_t *ids, int64_t begin, int64_t end,
+static int hoare_impl(float *x, float *y, float *z, rows_payload pl, int64_t begin, int64_t end,
                       int axis, float cut, int64_t *split);
-static int stable_impl(float *x, float *y, float *z, uint32_t *ids, int64_t begin, int64_t end,
+static int stable_impl(float *x, float *y, float *z, rows_payload pl, int64_t begin, int64_t end,
                        int axis, float cut, int64_t *split);
 
 int orc_hoare(float *x, float *y, float *z, int64_t begin, int64_t end, int axis,
               float cut, int64_t *split)
 {
-    return hoare_impl(x, y, z, NULL, begin, end, axis, cut, split);
+    rows_payload none = {NULL, NULL};
+    return hoare_impl(x, y, z, none, begin, end, axis, cut, split);
 }
 
 int orc_stable_le(float *x, float *y, float *z, int64_t begin, int64_t end, int axis,
                   float cut, int64_t *split)
 {
-    return stable_impl(x, y, z, NULL, begin, end, axis, cut, split);
+    rows_payload none = {NULL, NULL};
+    return stable_impl(x, y, z, none, begin, end, axis, cut, split);
 }
 
-static int hoare_impl(float *x, float *y, float *z, uint32_t *ids, int64_t begin, int64_t end,
+static int hoare_impl(float *x, float *y, float *z, rows_payload pl, int64_t begin, int64_t end,
                       int axis, float cut, int64_t *split)
 {
     /* partition.cpp:28-60.  The reference's bounds tests come after the key
@@ -250,9 +268,9 @@ static int hoare_impl(float *x, float *y, float *z, uint32_t *ids, int64_t begin
         do { i++; } while (i < end && k[i] < cut);
         do { j--; } while (j >= begin && k[j] > cut);
         if (i >= j) break;
-        swap_rows(x, y, z, ids, i, j);
+        swap_rows(x, y, z, pl, i, j);
     }
-    swap_rows(x, y, z, ids, i, end - 1);
+    swap_rows(x, y, z, pl, i, end - 1);
     *split = i;
     return ORC_OK;
 }
@@ -297,7 +315,7 @@ int orc_hoare_closed_form(const float *key, int64_t n, float cut, int64_t *K_out
     return ORC_OK;
 }
 
-static int stable_impl(float *x, float *y, float *z, uint32_t *ids, int64_t begin, int64_t end,
+static int stable_impl(float *x, float *y, float *z, rows_payload pl, int64_t begin, int64_t end,
                        int axis, float cut, int64_t *split)
 {
     /* Keys <= cut first, then keys > cut, each in input order.  The reference
@@ -307,7 +325,9 @@ static int stable_impl(float *x, float *y, float *z, uint32_t *ids, int64_t begi
     int64_t n = end - begin;
     float *k = axis == 0 ? x : axis == 1 ? y : z;
     float *tmp = (float *)malloc(sizeof(float) * 3 * (size_t)(n > 0 ? n : 1));
+    uint32_t *ids = pl.ids, *wts = pl.w;
     uint32_t *tid = ids ? (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1)) : NULL;
+    uint32_t *tw = wts ? (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1)) : NULL;
     int64_t nl = 0;
     for (int64_t p = begin; p < end; ++p)
         if (k[p] <= cut) nl++;
@@ -318,13 +338,16 @@ static int stable_impl(float *x, float *y, float *z, uint32_t *ids, int64_t begi
         tmp[n + w] = y[p];
         tmp[2 * n + w] = z[p];
         if (tid) tid[w] = ids[p];
+        if (tw) tw[w] = wts[p];
     }
     memcpy(x + begin, tmp, sizeof(float) * (size_t)n);
     memcpy(y + begin, tmp + n, sizeof(float) * (size_t)n);
     memcpy(z + begin, tmp + 2 * n, sizeof(float) * (size_t)n);
     if (tid) memcpy(ids + begin, tid, sizeof(uint32_t) * (size_t)n);
+    if (tw) memcpy(wts + begin, tw, sizeof(uint32_t) * (size_t)n);
     free(tmp);
     free(tid);
+    free(tw);
     *split = begin + nl;
     return ORC_OK;
 }
@@ -407,8 +430,14 @@ int orc_build(int64_t n, int32_t d, const orc_opts *opts, float *x, float *y, fl
     tr->ranges[1] = (uint32_t)n;
 
     int32_t *ids = (int32_t *)malloc(sizeof(int32_t) * (size_t)cap);
-    uint32_t *counts = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)cap);
-    uint32_t *left = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)cap);
+    /* weight sums; with no weights given every weight is 1 and these are the
+     * counts of count.cpp:14-17 and countLeft.cpp:33-36                     */
+    uint64_t *counts = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)cap);
+    uint64_t *left = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)cap);
+    const uint32_t *wt = tr->weights;
+    rows_payload pl = {tr->ids, tr->weights};
+    if (tr->pass_weights)
+        memset(tr->pass_weights, 0xff, sizeof(uint64_t) * ORC_MAX_PASSES * (size_t)cap);
     const int nt = opts->threads > 1 ? opts->threads : 1;
     int rc = ORC_OK;
     int32_t D = orc_n_levels(d);
@@ -424,10 +453,17 @@ int orc_build(int64_t n, int32_t d, const orc_opts *opts, float *x, float *y, fl
             if (!tr->exists[ids[i]]) { tr->error_cell = ids[i]; rc = ORC_ERR_UB; goto done; }
         }
 
-        /* ServiceCount (count.cpp:14-17) */
+        /* ServiceCount (count.cpp:14-17); weighted: the cell's weight */
         for (int64_t i = 0; i < nCells; ++i) {
             int32_t id = ids[i];
-            counts[i] = tr->ranges[2 * id + 1] - tr->ranges[2 * id];
+            uint32_t b = tr->ranges[2 * id], e = tr->ranges[2 * id + 1];
+            if (!wt) {
+                counts[i] = e - b;
+            } else {
+                uint64_t s = 0;
+                for (uint32_t p = b; p < e; ++p) s += wt[p];
+                counts[i] = s;
+            }
         }
 
         /* bisection controller (orbit.cpp:146-232).  The counts of a pass
@@ -446,24 +482,34 @@ int orc_build(int64_t n, int32_t d, const orc_opts *opts, float *x, float *y, fl
                 uint32_t b = tr->ranges[2 * c->id], e = tr->ranges[2 * c->id + 1];
                 const float *col = c->cutAxis == 0 ? x : c->cutAxis == 1 ? y : z;
                 float cut = orc_get_cut(c);
-                uint32_t nLeft = 0;
+                uint64_t nLeft = 0;
                 const int by_keys = nt > 1 && !by_cells && (e - b) > 65536u;
-                if (opts->cmp == ORC_CMP_LT) {
+                if (!wt && opts->cmp == ORC_CMP_LT) {
 #pragma omp parallel for reduction(+ : nLeft) num_threads(nt) if (by_keys)
                     for (uint32_t p = b; p < e; ++p) nLeft += col[p] < cut;   /* countLeft.cpp:33-36 */
-                } else {
+                } else if (!wt) {
 #pragma omp parallel for reduction(+ : nLeft) num_threads(nt) if (by_keys)
                     for (uint32_t p = b; p < e; ++p) nLeft += col[p] <= cut;  /* countLeftGPUAxis.cu:159-162 */
+                } else if (opts->cmp == ORC_CMP_LT) {
+#pragma omp parallel for reduction(+ : nLeft) num_threads(nt) if (by_keys)
+                    for (uint32_t p = b; p < e; ++p) nLeft += col[p] < cut ? wt[p] : 0u;
+                } else {
+#pragma omp parallel for reduction(+ : nLeft) num_threads(nt) if (by_keys)
+                    for (uint32_t p = b; p < e; ++p) nLeft += col[p] <= cut ? wt[p] : 0u;
                 }
                 left[i] = nLeft;
             }
             for (int64_t i = 0; i < nCells; ++i) {
                 orc_cell *c = &tr->heap[ids[i]];
                 if (c->foundCut) continue;
-                const uint32_t nLeft = left[i];
-                if (tr->pass_counts) tr->pass_counts[(int64_t)c->id * ORC_MAX_PASSES + (j - 1)] = nLeft;
-                int32_t difference = orc_difference(nLeft, counts[i], c->nLeafCells);
-                if (abs(difference) < 3) {
+                const uint64_t nLeft = left[i];
+                const int64_t slot = (int64_t)c->id * ORC_MAX_PASSES + (j - 1);
+                if (tr->pass_counts && !wt) tr->pass_counts[slot] = (uint32_t)nLeft;
+                if (tr->pass_weights) tr->pass_weights[slot] = nLeft;
+                int64_t difference = wt ? orc_difference_w(nLeft, counts[i], c->nLeafCells)
+                                        : orc_difference((uint32_t)nLeft, (uint32_t)counts[i],
+                                                         c->nLeafCells);
+                if (difference < 3 && difference > -3) {
                     c->foundCut = true;
                 } else if (difference > 0) {
                     c->cutMarginRight = orc_get_cut(c);
@@ -498,9 +544,9 @@ int orc_build(int64_t n, int32_t d, const orc_opts *opts, float *x, float *y, fl
             int64_t split = 0;
             int r;
             if (opts->partition == ORC_PART_HOARE)
-                r = hoare_impl(x, y, z, tr->ids, b, e, c->cutAxis, orc_get_cut(c), &split);
+                r = hoare_impl(x, y, z, pl, b, e, c->cutAxis, orc_get_cut(c), &split);
             else
-                r = stable_impl(x, y, z, tr->ids, b, e, c->cutAxis, orc_get_cut(c), &split);
+                r = stable_impl(x, y, z, pl, b, e, c->cutAxis, orc_get_cut(c), &split);
             if (r != ORC_OK) {
 #pragma omp critical
                 {

@@ -71,6 +71,11 @@ orc_cell orc_root(int32_t d);
 
 /* ---- termination rule (src/orbit.cpp:204-208) ---- */
 int32_t orc_difference(uint32_t count_left, uint32_t count, int32_t nLeafCells);
+/* The same rule on weight sums (weighted build): the weight left of the cut
+ * against the cell's weight times the left share, in the same float
+ * roundings; the truncation keeps 64 bits.  With every weight 1 the sums are
+ * the counts and the result is orc_difference's.                           */
+int64_t orc_difference_w(uint64_t weight_left, uint64_t weight, int32_t nLeafCells);
 
 /* ---- partitions ---- */
 /* Serial Hoare loop of src/services/partition.cpp:30-52 on SoA columns,
@@ -113,6 +118,11 @@ typedef struct orc_trace {
     int32_t pad;
     uint32_t *ids;             /* optional [n]: set to 0..n-1, then moved with
                                   the particles by every partition            */
+    uint32_t *weights;         /* optional [n]: the workload weight w(p_i) of
+                                  each particle (documentation/notes.tex:259-271),
+                                  moved with the particles; NULL: every weight 1 */
+    uint64_t *pass_weights;    /* optional [heap_cap][32]: the left weight sum of
+                                  each pass (the left count when unweighted)  */
 } orc_trace;
 
 /* Capacity of the heap for the given d and level mode (ids are 0-based).   */

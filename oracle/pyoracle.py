@@ -67,6 +67,7 @@ class Trace(C.Structure):
         ("pass_counts", C.c_void_p), ("level_passes", C.c_void_p),
         ("heap_cap", C.c_int64), ("nlevels", C.c_int32), ("total_passes", C.c_int32),
         ("error_cell", C.c_int32), ("pad", C.c_int32), ("ids", C.c_void_p),
+        ("weights", C.c_void_p), ("pass_weights", C.c_void_p),
     ]
 
 
@@ -103,6 +104,8 @@ def lib():
         L.orc_right_child_id.argtypes = [C.c_int32]
         L.orc_difference.argtypes = [C.c_uint32, C.c_uint32, C.c_int32]
         L.orc_difference.restype = C.c_int32
+        L.orc_difference_w.argtypes = [C.c_uint64, C.c_uint64, C.c_int32]
+        L.orc_difference_w.restype = C.c_int64
         L.orc_hoare.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int64,
                                 C.c_int, C.c_float, C.POINTER(C.c_int64)]
         L.orc_stable_le.argtypes = L.orc_hoare.argtypes
@@ -137,8 +140,10 @@ class BuildResult:
 
 
 def build(x, y, z, d, levels_mode=LEVELS_REFERENCE, cmp=CMP_LT, partition=PART_HOARE,
-          max_passes=MAX_PASSES, pass_counts=False, ids=False, threads=1):
-    """Runs the oracle build in place on copies of x, y, z."""
+          max_passes=MAX_PASSES, pass_counts=False, ids=False, threads=1, weights=None,
+          pass_weights=False):
+    """Runs the oracle build in place on copies of x, y, z (and of the
+    weights, when given: uint32 per particle, moved with the particles)."""
     L = lib()
     n = len(x)
     x = np.ascontiguousarray(x, np.float32).copy()
@@ -151,10 +156,17 @@ def build(x, y, z, d, levels_mode=LEVELS_REFERENCE, cmp=CMP_LT, partition=PART_H
     pc = np.zeros((cap, MAX_PASSES), np.uint32) if pass_counts else None
     lp = np.zeros(64, np.int32)
     pid = np.zeros(max(n, 1), np.uint32) if ids else None
+    wts = None
+    if weights is not None:
+        wts = np.ascontiguousarray(weights, np.uint32).copy()
+        assert len(wts) == n
+    pw = np.zeros((cap, MAX_PASSES), np.uint64) if pass_weights else None
     tr = Trace(heap=heap.ctypes.data, ranges=ranges.ctypes.data, exists=exists.ctypes.data,
                pass_counts=(pc.ctypes.data if pc is not None else None),
                level_passes=lp.ctypes.data, heap_cap=cap,
-               ids=(pid.ctypes.data if pid is not None else None))
+               ids=(pid.ctypes.data if pid is not None else None),
+               weights=(wts.ctypes.data if wts is not None and n > 0 else None),
+               pass_weights=(pw.ctypes.data if pw is not None else None))
     opts = Opts(levels_mode, cmp, partition, max_passes, threads)
     rc = L.orc_build(n, d, C.byref(opts), ptr(x), ptr(y), ptr(z), C.byref(tr))
     r = BuildResult()
@@ -163,6 +175,8 @@ def build(x, y, z, d, levels_mode=LEVELS_REFERENCE, cmp=CMP_LT, partition=PART_H
     r.heap, r.ranges, r.exists = heap, ranges, exists
     r.pass_counts = pc
     r.ids = pid[:n] if pid is not None else None
+    r.weights = wts
+    r.pass_weights = pw
     r.level_passes = lp[:max(tr.nlevels, 0)].copy()
     r.nlevels = tr.nlevels
     r.total_passes = tr.total_passes
