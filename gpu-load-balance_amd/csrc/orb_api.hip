@@ -140,6 +140,18 @@ static int prepare(Ctx &c, const orb_cell *cells, int32_t nCells)
     for (int32_t i = 0; same && i < nCells; ++i) same = c.table_ids[i] == cells[i].id;
     if (!same) {
         ORB_TRY(build_chunk_table(c, nCells));
+        // a service call's cells may overlap (a level's never do): when they
+        // hold more chunks than the table has room for, it grows and is built
+        // again (every chunk kernel reads up to first[nCells])
+        ORB_HIP(hipMemcpyAsync(c.hcnt.p, c.cell_first.p + nCells, sizeof(uint32_t),
+                               hipMemcpyDeviceToHost, c.stream));
+        ORB_HIP(hipStreamSynchronize(c.stream));
+        const int64_t total = c.hcnt.p[0];
+        if (total > c.max_chunks) {
+            ORB_TRY(ensure_level_capacity(c, nCells, total + 1));
+            ORB_TRY(upload_level(c, cells, nCells));
+            ORB_TRY(build_chunk_table(c, nCells));
+        }
         c.table_ids.resize((size_t)nCells);
         for (int32_t i = 0; i < nCells; ++i) c.table_ids[i] = cells[i].id;
     }

@@ -561,7 +561,8 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
                   uint32_t *trace, int32_t *level_passes, int32_t *status, bool any_subtree,
                   bool full_cover, int32_t level);
 int preallocate_scratch(Ctx &c);
-int ensure_level_capacity(Ctx &c, int64_t nCells);
+// level tables for nCells cells and (at least) `chunks` chunks
+int ensure_level_capacity(Ctx &c, int64_t nCells, int64_t chunks = 0);
 
 // batched build (orb_batched.hip)
 int build_batched(Ctx &c, int32_t levels_mode, orb_build_info *info);

@@ -331,12 +331,16 @@ __global__ void k_fill_chunks(const uint32_t *first, int32_t nCells, const CellR
     }
 }
 
-int ensure_level_capacity(Ctx &c, int64_t nCells)
+int ensure_level_capacity(Ctx &c, int64_t nCells, int64_t chunks)
 {
-    if (nCells <= c.max_level_cells && c.recs.p) return ORB_OK;
+    if (nCells <= c.max_level_cells && chunks <= c.max_chunks && c.recs.p) return ORB_OK;
     int64_t m = nCells > c.max_level_cells ? nCells : c.max_level_cells;
     c.max_level_cells = m;
-    c.max_chunks = c.n / kChunk + m + 1;
+    // disjoint cells (a level's) hold at most n / kChunk + m chunks; a
+    // service call's cells may overlap and ask for more
+    const int64_t level_chunks = c.n / kChunk + m + 1;
+    int64_t mc = level_chunks > c.max_chunks ? level_chunks : c.max_chunks;
+    c.max_chunks = chunks > mc ? chunks : mc;
     ORB_TRY(c.dcells.alloc(m));
     ORB_TRY(c.recs.alloc(m));
     ORB_TRY(c.cell_nchunks.alloc(m + 1));
@@ -461,7 +465,10 @@ __device__ inline void k_count_chunk(uint32_t q, const CellRec *recs, int32_t nC
     if (!chunk_range(q, first, nCells, chunk_cell, recs, cell, b, e)) return;
     const CellRec r = recs[cell];
     if (skip_found && r.found) return;
-    if (r.axis < 0 || r.axis > 2) return;
+    if (r.axis < 0 || r.axis > 2) {  // no cut axis: nothing is left of the cut
+        if (threadIdx.x == 0) chunk_ltle[q] = 0u;
+        return;
+    }
     const float *key = axisT ? axisT : axis_ptr(x, y, z, r.axis);
     uint32_t lt = 0, le = 0;
     count_range(key, b, e, r.cut, lt, le);
@@ -476,7 +483,7 @@ __global__ __launch_bounds__(kBlock) void k_count(const CellRec *recs, int32_t n
                                                   const float *axisT, int skip_found,
                                                   uint32_t *chunk_ltle)
 {
-    const uint32_t total = first[nCells];
+    const uint32_t total = first[nCells];  // the caller keeps it within the table
     for (uint32_t q = blockIdx.x; q < total; q += gridDim.x) {
         k_count_chunk(q, recs, nCells, first, chunk_cell, x, y, z, axisT, skip_found, chunk_ltle);
         __syncthreads();  // the next chunk reuses this block's LDS
