@@ -522,12 +522,23 @@ int launch_zero_u64(hipStream_t s, uint64_t *p, size_t n);
 int scan_exclusive_u32(Ctx &c, const uint32_t *in, uint32_t *out, size_t m);
 int scan_exclusive_u64(Ctx &c, const uint64_t *in, uint64_t *out, size_t m,
                        const uint32_t *lim = nullptr);
+// the same with caller-provided partials (one per 2048 entries), no allocation
+int scan_exclusive_u32_into(hipStream_t s, const uint32_t *in, uint32_t *out, size_t m,
+                            uint32_t *partials);
 unsigned chunk_grid(const Ctx &c);
 int upload_level(Ctx &c, const orb_cell *cells, int32_t nCells);
 // scanned: cell_first already holds the exclusive scan of the chunk counts
 int build_chunk_table(Ctx &c, int32_t nCells, bool with_stamps = true, bool scanned = false);
 int launch_count(Ctx &c, int32_t nCells, bool lt_skip_found, const float *axis_override,
                  hipEvent_t after_count = nullptr);
+// The count pass over caller-provided tables (no context, no allocation):
+// the chunk table from the exclusive chunk offsets first[] (first[nCells] at
+// most max_chunks), the chunk counts, the per-cell lt/le sums into cnt.
+int launch_count_explicit(hipStream_t s, const CellRec *recs, int32_t nCells, const uint32_t *first,
+                          uint32_t *chunk_cell, uint4 *chunk_desc, uint32_t max_chunks,
+                          const float *x, const float *y, const float *z, bool skip_found,
+                          uint32_t *chunk_ltle, uint32_t *cnt, CountStamp *stamps,
+                          unsigned chunk_blocks);
 // Per-chunk histograms of the round that found each cell's cut (batched
 // form): the stopper counts come from them instead of a key read.
 struct HistCounts {

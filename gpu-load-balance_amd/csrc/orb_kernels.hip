@@ -131,6 +131,29 @@ __global__ void k_scan_apply(const T *in, T *out, size_t m, const T *partials, c
     }
 }
 
+// partials: room for one T per tile of kScanTile entries
+template <class T>
+static int scan_exclusive_into(hipStream_t stream, const T *in, T *out, size_t m, T *partials,
+                               const uint32_t *lim)
+{
+    if (m == 0) return ORB_OK;
+    size_t np = (m + kScanTile - 1) / kScanTile;
+    if (np > (size_t)kScanTile) return ORB_ERR_ARG;
+    if (np == 1) {
+        // one tile: the block scan alone (offset 0)
+        hipLaunchKernelGGL(k_scan_apply<T>, dim3(1), dim3(kBlock), 0, stream, in, out, m,
+                           (const T *)nullptr, lim, 0);
+        ORB_HIP(hipGetLastError());
+        return ORB_OK;
+    }
+    hipLaunchKernelGGL(k_scan_reduce<T>, dim3((unsigned)np), dim3(kBlock), 0, stream, in, m, partials,
+                       lim);
+    hipLaunchKernelGGL(k_scan_apply<T>, dim3((unsigned)np), dim3(kBlock), 0, stream, in, out, m,
+                       (const T *)partials, lim, 1);
+    ORB_HIP(hipGetLastError());
+    return ORB_OK;
+}
+
 template <class T>
 static int scan_exclusive(Ctx &c, const T *in, T *out, size_t m, DevBuf<uint64_t> &scratch,
                           const uint32_t *lim = nullptr)
@@ -140,20 +163,13 @@ static int scan_exclusive(Ctx &c, const T *in, T *out, size_t m, DevBuf<uint64_t
     if (np > (size_t)kScanTile) return ORB_ERR_ARG;
     size_t words = (np * sizeof(T) + sizeof(uint64_t) - 1) / sizeof(uint64_t);
     ORB_TRY(scratch.alloc(words));
-    T *partials = reinterpret_cast<T *>(scratch.p);
-    if (np == 1) {
-        // one tile: the block scan alone (offset 0)
-        hipLaunchKernelGGL(k_scan_apply<T>, dim3(1), dim3(kBlock), 0, c.stream, in, out, m,
-                           (const T *)nullptr, lim, 0);
-        ORB_HIP(hipGetLastError());
-        return ORB_OK;
-    }
-    hipLaunchKernelGGL(k_scan_reduce<T>, dim3((unsigned)np), dim3(kBlock), 0, c.stream, in, m, partials,
-                       lim);
-    hipLaunchKernelGGL(k_scan_apply<T>, dim3((unsigned)np), dim3(kBlock), 0, c.stream, in, out, m,
-                       (const T *)partials, lim, 1);
-    ORB_HIP(hipGetLastError());
-    return ORB_OK;
+    return scan_exclusive_into<T>(c.stream, in, out, m, reinterpret_cast<T *>(scratch.p), lim);
+}
+
+int scan_exclusive_u32_into(hipStream_t s, const uint32_t *in, uint32_t *out, size_t m,
+                            uint32_t *partials)
+{
+    return scan_exclusive_into<uint32_t>(s, in, out, m, partials, nullptr);
 }
 
 unsigned chunk_grid(const Ctx &c)
@@ -577,6 +593,25 @@ int launch_count(Ctx &c, int32_t nCells, bool skip_found, const float *axis_over
     hipLaunchKernelGGL(k_cell_totals, dim3(g), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
                        c.cell_first.p, c.chunk_ltle.p, skip_found ? 1 : 0,
                        axis_override ? 0 : 1, c.cnt.p, c.stamps.p);
+    ORB_HIP(hipGetLastError());
+    return ORB_OK;
+}
+
+int launch_count_explicit(hipStream_t s, const CellRec *recs, int32_t nCells, const uint32_t *first,
+                          uint32_t *chunk_cell, uint4 *chunk_desc, uint32_t max_chunks,
+                          const float *x, const float *y, const float *z, bool skip_found,
+                          uint32_t *chunk_ltle, uint32_t *cnt, CountStamp *stamps,
+                          unsigned chunk_blocks)
+{
+    if (nCells <= 0) return ORB_OK;
+    hipLaunchKernelGGL(k_fill_chunks, dim3(grid_for((size_t)max_chunks, 16384)), dim3(kBlock), 0, s,
+                       first, nCells, recs, chunk_cell, chunk_desc, max_chunks);
+    hipLaunchKernelGGL(k_count, dim3(chunk_blocks > 0 ? chunk_blocks : 1u), dim3(kBlock), 0, s, recs,
+                       nCells, first, chunk_cell, x, y, z, (const float *)nullptr, skip_found ? 1 : 0,
+                       chunk_ltle);
+    const unsigned g = (unsigned)(nCells < 65536 ? nCells : 65536);
+    hipLaunchKernelGGL(k_cell_totals, dim3(g), dim3(kBlock), 0, s, recs, nCells, first, chunk_ltle,
+                       skip_found ? 1 : 0, 0, cnt, stamps);
     ORB_HIP(hipGetLastError());
     return ORB_OK;
 }

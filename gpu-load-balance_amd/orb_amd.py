@@ -125,6 +125,8 @@ PROTOTYPES = {
     "orb_partition": (C.c_int, [_vp, _vp, _i32]),
     "orb_partition_gpu": (C.c_int, [_vp, _vp, _i32]),
     "orb_run_service": (C.c_int, [_vp, _i32, _vp, _i32, _vp, _i32]),
+    "orb_count_left_static": (C.c_int, [_vp, _vp, _vp, _i64, _vp, _i64, _vp, _i32, _i32, _vp,
+                                        _vp]),
     "orb_heap_capacity": (C.c_int64, [_i32]),
     "orb_build": (C.c_int, [_vp, _i32, _i32, C.POINTER(BuildInfo)]),
     "orb_get_heap": (C.c_int, [_vp, _vp, _i64]),

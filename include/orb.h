@@ -188,6 +188,25 @@ int orb_partition(orb_ctx *ctx, const orb_cell *cells, int32_t nCells);
  * cut the partition uses.                                                */
 int orb_partition_gpu(orb_ctx *ctx, const orb_cell *cells, int32_t nCells);
 
+/* No counterpart in the reference: orb_count_left (cmp ORB_CMP_LT: #{x < cut},
+ * found cells skipped, their out[i] untouched) or orb_count_left_gpu
+ * (ORB_CMP_LE: #{x <= cut} for every cell) for callers that cannot
+ * allocate (code under stream capture, tight serving loops).  It allocates
+ * nothing: every argument is caller-owned DEVICE memory -- x, y, z [n]
+ * (axis-major SoA), the cell-to-range map ranges [cap][2], cells [nCells],
+ * out [nCells] -- and the work is enqueued on `stream` (a hipStream_t;
+ * NULL: the default stream) without synchronising; out is ready when the
+ * stream reaches this point.  The tables live in one static device array
+ * per GPU (7.2 MB, held until the process ends), so at most 8096 cells
+ * (MAX_CELLS, src/constants.h:11) and 2^30 particles; larger sizes return
+ * ORB_ERR_ARG.  Calls that overlap on one GPU share the array and race.
+ * Cell ranges are clamped into [0, n) and an id outside [0, cap) is an
+ * empty cell; cells whose ranges overlap so much that their chunks exceed
+ * the table get UINT32_MAX counts.  Returns nCells * 4.                  */
+int orb_count_left_static(const float *x, const float *y, const float *z, int64_t n,
+                          const uint32_t *ranges, int64_t cap, const orb_cell *cells,
+                          int32_t nCells, int32_t cmp, uint32_t *out, void *stream);
+
 /* mdl->RunService(id, nIn, vin, vout) for the services above (the ids of
  * src/services/pst.h:66-81); vin = Cell[nIn/52] except for COPYPARTICLES
  * and FINALIZE (ignored).  Returns the service's byte count.             */
