@@ -9,9 +9,13 @@
  * `-O2 -ffp-contract=off` (no -ffast-math): see oracle/Makefile.
  *
  * Parity pins (what checks each part independently of this file):
- *   generator       tests/test_oracle.py: uint64->float32 rounding against
- *                   numpy's conversion, first draws against the closed-form
- *                   bound of the seeds, and uniform moments.
+ *   generator       against the reference's own xorshf96 (init.cu:11-25,
+ *                   compiled where it lies into oracle/_ref) and the fixture
+ *                   it wrote (first 2^20 particles); uint64->float32 rounding
+ *                   against exact arithmetic; uniform moments.
+ *   weights         unit weights give the unweighted build; a particle of
+ *                   weight k builds the same tree as k copies of it
+ *                   (tests/test_oracle_weights.py).
  *   Cell rules      against oracle/_ref (the reference's own src/cell.h compiled
  *                   by oracle/Makefile) and the committed fixture it wrote.
  *   termination     brute force over small counts against exact rational

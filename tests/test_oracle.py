@@ -1,8 +1,10 @@
 """Pins the CPU oracle (oracle/orb_oracle.c) against sources independent of it.
 
-- generator: the uint64 -> float32 -> double chain against exact rational
-  rounding; the first draws against bounds that follow from the seeds and the
-  64-bit word size; uniform moments.
+- generator: the reference's own xorshf96 (init.cu:11-25, compiled where it
+  lies into oracle/_ref; the committed fixture it wrote); the uint64 ->
+  float32 -> double chain against exact rational rounding; the first draws
+  against bounds that follow from the seeds and the 64-bit word size; uniform
+  moments.
 - Cell rules: the reference's own src/cell.h (oracle/_ref, and the committed
   fixture it wrote).
 - termination rule: numpy's IEEE float32 arithmetic.
@@ -101,6 +103,41 @@ def test_rng_uniform_moments(oracle):
     # three axes are uncorrelated
     assert abs(np.corrcoef(x, y)[0, 1]) < 0.01
     assert abs(np.corrcoef(x[1:], x[:-1])[0, 1]) < 0.01
+
+
+REF_RNG = os.path.join(ORACLE, "_ref", "libref_rng.so")
+
+
+def _digest(x, y, z):
+    import hashlib
+    h = hashlib.sha256()
+    for a in (x, y, z):
+        h.update(np.ascontiguousarray(a, "<f4").tobytes())
+    return h.hexdigest()
+
+
+def test_rng_matches_reference_fixture(oracle):
+    """The first 2^20 particles against the reference's own generator (the
+    fixture tests/golden/make_rng_vectors.py wrote from init.cu:11-25 compiled
+    where it lies): the first 1024 bit for bit, all of them by digest.  A
+    changed shift, seed, divisor or draw order fails it."""
+    fx = json.load(open(os.path.join(GOLDEN, "rng_vectors.json")))
+    x, y, z = oracle.generate(fx["n_digest"])
+    for k, a in (("x", x), ("y", y), ("z", z)):
+        assert a[:1024].view(np.uint32).tolist() == fx["first_1024"][k]
+    assert _digest(x, y, z) == fx["sha256_xyz"]
+
+
+@pytest.mark.skipif(not os.path.exists(REF_RNG), reason="oracle/_ref not built (no reference tree)")
+def test_rng_against_live_reference(oracle):
+    ref = C.CDLL(REF_RNG)
+    ref.ref_generate.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]
+    n = (1 << 18) + 7
+    rx, ry, rz = (np.empty(n, np.float32) for _ in range(3))
+    ref.ref_generate(n, rx.ctypes.data, ry.ctypes.data, rz.ctypes.data)
+    x, y, z = oracle.generate(n)
+    for a, b in ((x, rx), (y, ry), (z, rz)):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
 # ---------------------------------------------------------------------------
