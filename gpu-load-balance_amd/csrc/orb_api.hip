@@ -234,6 +234,32 @@ static int count_sizes(Ctx &c, const orb_cell *cells, int32_t nCells, uint32_t *
     return (int)(nCells * sizeof(uint32_t));
 }
 
+// Weighted services (workload weights on): per-cell weight sums in 64 bits.
+// total: the cell's weight (ServiceCount with weights); otherwise the weight
+// left of the cut, < with found cells skipped (their out[i] untouched) or <=
+// for every cell.
+static int weight_service(Ctx &c, const orb_cell *cells, int32_t nCells, uint64_t *out,
+                          bool total, bool lt_skip_found)
+{
+    if (!c.weighted) return ORB_ERR_STATE;
+    if (!out && nCells > 0) return ORB_ERR_ARG;
+    ORB_TRY(prepare(c, cells, nCells));
+    if (nCells == 0) return 0;
+    if (c.timing && !total) ORB_HIP(hipEventRecord(c.ev[0], c.stream));
+    ORB_TRY(launch_count_w(c, nCells, lt_skip_found, total));
+    if (c.timing && !total) ORB_HIP(hipEventRecord(c.ev[1], c.stream));
+    ORB_HIP(hipMemcpyAsync(c.hcnt_w.p, c.cnt_w.p, 2 * sizeof(uint64_t) * (size_t)nCells,
+                           hipMemcpyDeviceToHost, c.stream));
+    ORB_HIP(hipStreamSynchronize(c.stream));
+    if (!total) ORB_TRY(finish_timed_count(c));
+    for (int32_t i = 0; i < nCells; ++i) {
+        if (total) out[i] = c.hcnt_w.p[2 * i];
+        else if (!lt_skip_found) out[i] = c.hcnt_w.p[2 * i + 1];
+        else if (!cells[i].foundCut) out[i] = c.hcnt_w.p[2 * i];
+    }
+    return (int)(nCells * sizeof(uint64_t));
+}
+
 // ---------------------------------------------------------------------------
 // plain build
 // ---------------------------------------------------------------------------
@@ -246,7 +272,7 @@ static int start_build(Ctx &c)
     c.exists.assign((size_t)c.cap, 0);
     c.heap[0] = cell_root(c.d);
     c.exists[0] = 1;
-    if (c.trace) c.trace_counts.assign((size_t)c.cap * ORB_MAX_PASSES, 0xffffffffu);
+    if (c.trace) c.trace_counts.assign((size_t)c.cap * ORB_MAX_PASSES, ~0ull);
     timing_reset(c);
     return reset_ranges(c);
 }
@@ -263,6 +289,8 @@ static int build_plain(Ctx &c, int32_t levels_mode, orb_build_info *info)
     std::vector<int32_t> ids;
     std::vector<orb_cell> cells;
     std::vector<uint32_t> counts, left;
+    std::vector<uint64_t> wcounts, wleft;  // weighted: weight sums
+    c.trace_weighted = c.weighted;
     for (int32_t l = 1; l <= D; ++l) {
         int32_t bad = -1;
         if (!level_cells(c.d, levels_mode, l, c.exists, c.heap, ids, &bad)) {
@@ -281,6 +309,11 @@ static int build_plain(Ctx &c, int32_t levels_mode, orb_build_info *info)
 
         ORB_TRY(prepare(c, cells.data(), nCells));          // ServiceCopyCells
         ORB_TRY(count_sizes(c, cells.data(), nCells, counts.data()));  // ServiceCount
+        if (c.weighted) {  // the cells' weights
+            wcounts.assign((size_t)nCells, 0ull);
+            wleft.assign((size_t)nCells, 0ull);
+            ORB_TRY(weight_service(c, cells.data(), nCells, wcounts.data(), true, false));
+        }
 
         // bisection controller (orbit.cpp:146-232)
         bool foundAll = false;
@@ -289,20 +322,27 @@ static int build_plain(Ctx &c, int32_t levels_mode, orb_build_info *info)
             j++;
             foundAll = true;
             int64_t bytes = 0;
+            const int64_t per_key = c.weighted ? 8 : 4;  // key (+ weight)
             for (int32_t i = 0; i < nCells; ++i)
-                if (c.cmp != ORB_CMP_LT || !cells[i].foundCut) bytes += 4 * (int64_t)counts[i];
-            if (c.cmp == ORB_CMP_LT)
-                ORB_TRY(count_service(c, cells.data(), nCells, left.data(), true, nullptr));
+                if (c.cmp != ORB_CMP_LT || !cells[i].foundCut) bytes += per_key * (int64_t)counts[i];
+            const bool lt = c.cmp == ORB_CMP_LT;
+            if (c.weighted)
+                ORB_TRY(weight_service(c, cells.data(), nCells, wleft.data(), false, lt));
             else
-                ORB_TRY(count_service(c, cells.data(), nCells, left.data(), false, nullptr));
+                ORB_TRY(count_service(c, cells.data(), nCells, left.data(), lt, nullptr));
             c.tm.count_bytes += bytes;
             for (int32_t i = 0; i < nCells; ++i) {
                 orb_cell &cl = cells[i];
                 if (cl.foundCut) continue;
                 if (c.trace)
-                    c.trace_counts[(size_t)cl.id * ORB_MAX_PASSES + (size_t)(j - 1)] = left[i];
-                int32_t difference = difference_rule(left[i], counts[i], cl.nLeafCells);
-                if (abs(difference) < 3) {
+                    c.trace_counts[(size_t)cl.id * ORB_MAX_PASSES + (size_t)(j - 1)] =
+                        c.weighted ? wleft[i] : left[i];
+                // orbit.cpp:204-205; weighted: the same rule on weight sums
+                const int64_t difference =
+                    c.weighted ? difference_from_target_w(
+                                     wleft[i], difference_target_w(wcounts[i], cl.nLeafCells))
+                               : difference_rule(left[i], counts[i], cl.nLeafCells);
+                if (difference < 3 && difference > -3) {
                     cl.foundCut = true;
                 } else if (difference > 0) {
                     cl.cutMarginRight = cell_get_cut(cl);
@@ -459,6 +499,12 @@ int orb_finalize(orb_ctx *ctx)
     c->pid.release();
     c->pid_tmp.release();
     c->pid_saved.release();
+    c->wt.release();
+    c->wt_tmp.release();
+    c->wt_saved.release();
+    c->chunk_w.release();
+    c->cnt_w.release();
+    c->hcnt_w.release();
     c->ranges.release();
     c->dcells.release();
     c->recs.release();
@@ -510,6 +556,7 @@ int orb_set_particles(orb_ctx *ctx, const float *x, const float *y, const float 
     Ctx *c = C(ctx);
     if (!c || n != c->n || (n > 0 && (!x || !y || !z))) return ORB_ERR_ARG;
     c->in_tmp = false;
+    c->weighted = false;  // new particles: their weights come with orb_set_weights
     if (c->track_ids) ORB_TRY(launch_iota(c->stream, c->pid.p, (size_t)c->stride));
     if (n > 0) {
         ORB_HIP(hipMemcpyAsync(c->x(), x, sizeof(float) * (size_t)n, hipMemcpyHostToDevice, c->stream));
@@ -567,6 +614,13 @@ int orb_save_particles(orb_ctx *ctx)
     } else {
         c->pid_saved.release();
     }
+    if (c->weighted) {
+        ORB_TRY(c->wt_saved.alloc((size_t)c->stride));
+        ORB_HIP(hipMemcpyAsync(c->wt_saved.p, c->wt.p, sizeof(uint32_t) * (size_t)c->stride,
+                               hipMemcpyDeviceToDevice, c->stream));
+    } else {
+        c->wt_saved.release();
+    }
     ORB_HIP(hipStreamSynchronize(c->stream));
     return ORB_OK;
 }
@@ -587,7 +641,61 @@ int orb_restore_particles(orb_ctx *ctx)
         else
             ORB_TRY(launch_iota(c->stream, c->pid.p, (size_t)c->stride));
     }
+    // the weights saved with the particles come back with them; weights set
+    // after the snapshot have no saved order to come back in
+    if (c->weighted) {
+        if (!c->wt_saved.p) return ORB_ERR_STATE;
+        ORB_HIP(hipMemcpyAsync(c->wt.p, c->wt_saved.p, sizeof(uint32_t) * (size_t)c->stride,
+                               hipMemcpyDeviceToDevice, c->stream));
+    }
     return reset_ranges(*c);
+}
+
+int orb_set_weights(orb_ctx *ctx, const uint32_t *w, int64_t n)
+{
+    Ctx *c = C(ctx);
+    if (!c || n != c->n) return ORB_ERR_ARG;
+    if (!w) {
+        c->weighted = false;
+        return ORB_OK;
+    }
+    ORB_TRY(particles_home(*c));
+    ORB_TRY(c->wt.alloc((size_t)c->stride));
+    if (n > 0)
+        ORB_HIP(hipMemcpyAsync(c->wt.p, w, sizeof(uint32_t) * (size_t)n, hipMemcpyHostToDevice,
+                               c->stream));
+    c->weighted = true;
+    if (c->part == ORB_PART_STABLE_LE) ORB_TRY(ensure_stable_buffers(*c));
+    c->wt_saved.release();  // a snapshot taken before holds other weights
+    ORB_HIP(hipStreamSynchronize(c->stream));
+    return ORB_OK;
+}
+
+int orb_get_weights(orb_ctx *ctx, uint32_t *w, int64_t n)
+{
+    Ctx *c = C(ctx);
+    if (!c || n != c->n || (n > 0 && !w)) return ORB_ERR_ARG;
+    if (!c->weighted) return ORB_ERR_STATE;
+    if (n > 0)
+        ORB_HIP(hipMemcpyAsync(w, c->weights(), sizeof(uint32_t) * (size_t)n,
+                               hipMemcpyDeviceToHost, c->stream));
+    ORB_HIP(hipStreamSynchronize(c->stream));
+    return ORB_OK;
+}
+
+int orb_count_weight(orb_ctx *ctx, const orb_cell *cells, int32_t nCells, uint64_t *out)
+{
+    Ctx *c = C(ctx);
+    if (!c) return ORB_ERR_ARG;
+    return weight_service(*c, cells, nCells, out, true, false);
+}
+
+int orb_count_left_weight(orb_ctx *ctx, const orb_cell *cells, int32_t nCells, int32_t cmp,
+                          uint64_t *out)
+{
+    Ctx *c = C(ctx);
+    if (!c || (cmp != ORB_CMP_LT && cmp != ORB_CMP_LE)) return ORB_ERR_ARG;
+    return weight_service(*c, cells, nCells, out, false, cmp == ORB_CMP_LT);
 }
 
 int orb_enable_ids(orb_ctx *ctx, bool on)
@@ -789,7 +897,21 @@ int orb_get_pass_counts(orb_ctx *ctx, uint32_t *counts, int64_t cap)
     if (!c || !counts || cap < c->cap) return ORB_ERR_ARG;
     if (c->results_on_device) ORB_TRY(batched_pull_results(*c));
     if (c->trace_counts.size() != (size_t)c->cap * ORB_MAX_PASSES) return ORB_ERR_STATE;
-    std::memcpy(counts, c->trace_counts.data(), sizeof(uint32_t) * c->trace_counts.size());
+    if (c->trace_weighted) return ORB_ERR_STATE;  // weight sums: orb_get_pass_weights
+    for (size_t k = 0; k < c->trace_counts.size(); ++k) {
+        const uint64_t v = c->trace_counts[k];
+        counts[k] = v == ~0ull ? 0xffffffffu : (uint32_t)v;
+    }
+    return ORB_OK;
+}
+
+int orb_get_pass_weights(orb_ctx *ctx, uint64_t *sums, int64_t cap)
+{
+    Ctx *c = C(ctx);
+    if (!c || !sums || cap < c->cap) return ORB_ERR_ARG;
+    if (c->results_on_device) ORB_TRY(batched_pull_results(*c));
+    if (c->trace_counts.size() != (size_t)c->cap * ORB_MAX_PASSES) return ORB_ERR_STATE;
+    std::memcpy(sums, c->trace_counts.data(), sizeof(uint64_t) * c->trace_counts.size());
     return ORB_OK;
 }
 

@@ -47,7 +47,10 @@ struct BatchedState {
     DevBuf<uint8_t> sub;               // [cap] cell handled inside a subtree kernel
     DevBuf<uint16_t> chunk_hist;       // [max_chunks][256]
     DevBuf<uint32_t> cell_hist;        // [max_level_cells][256]
-    DevBuf<uint32_t> trace;            // [cap][32]
+    DevBuf<uint64_t> chunk_hist_w;     // weighted builds: [max_chunks][256] weight sums
+    DevBuf<uint64_t> cell_hist_w;      // weighted builds: [max_level_cells][256]
+    bool hist_w_clean = false;         // cell_hist_w all zero (as cell_hist)
+    DevBuf<uint64_t> trace;            // [cap][32]
     DevBuf<int32_t> level_passes;      // [64]
     PinBuf<int32_t> hpasses;           // [64]
     // captured build: valid while its signature (graph_signature) holds
@@ -142,7 +145,7 @@ static int make_schedule(Ctx &c, BatchedState &b, int32_t levels_mode, bool *com
 // Fresh build state: root cell, empty heap and flags, range(0) = [0, n),
 // cleared status and pass counters (no memset or host copy: graph-safe).
 __global__ void k_b_init(orb_cell *heap, uint8_t *exists, uint8_t *sub, int64_t cap,
-                         orb_cell root, uint32_t *trace, int64_t trace_n, int32_t *level_passes,
+                         orb_cell root, uint64_t *trace, int64_t trace_n, int32_t *level_passes,
                          uint32_t *ranges, uint32_t n, int32_t *status)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -154,7 +157,7 @@ __global__ void k_b_init(orb_cell *heap, uint8_t *exists, uint8_t *sub, int64_t 
         exists[k] = k == 0 ? 1 : 0;
         sub[k] = 0;
     }
-    for (int64_t k = i; k < trace_n; k += stride) trace[k] = 0xffffffffu;
+    for (int64_t k = i; k < trace_n; k += stride) trace[k] = ~0ull;
     if (i < 64) level_passes[i] = 0;
     if (i < 2) status[i] = 0;
 }
@@ -704,14 +707,26 @@ struct RoundOut {
     int32_t node;  // the candidate the round stopped at (when found)
 };
 
-__device__ inline RoundOut replay_round(const uint32_t *P, float lo, float hi, float target,
-                                        int32_t passes0, int lane, uint32_t *trace_row)
+// The controller's difference for a left count (uint32) or a left weight
+// (uint64, weighted builds): weights grow with the candidate as counts do.
+__device__ inline int64_t round_diff(uint32_t left, float target)
+{
+    return difference_from_target(left, target);
+}
+__device__ inline int64_t round_diff(uint64_t left, float target)
+{
+    return difference_from_target_w(left, target);
+}
+
+template <class T>
+__device__ inline RoundOut replay_round(const T *P, float lo, float hi, float target,
+                                        int32_t passes0, int lane, uint64_t *trace_row)
 {
     uint32_t nle = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int j = 4 * lane + 1 + q;
-        const bool le0 = j < kBins && difference_from_target(P[j], target) <= 0;
+        const bool le0 = j < kBins && round_diff(P[j], target) <= 0;
         nle += (uint32_t)__popcll(__ballot(le0));
     }
     const int b = 1 + (int)nle;
@@ -726,13 +741,15 @@ __device__ inline RoundOut replay_round(const uint32_t *P, float lo, float hi, f
             step >>= 1;
         }
     }
-    uint32_t cnt[kHistK];
+    T cnt[kHistK];
 #pragma unroll
     for (int d = 0; d < kHistK; ++d) cnt[d] = P[nodes[d]];
     int s = steps;
 #pragma unroll
-    for (int d = kHistK - 1; d >= 0; --d)
-        if (d < steps && abs(difference_from_target(cnt[d], target)) < 3) s = d;
+    for (int d = kHistK - 1; d >= 0; --d) {
+        const int64_t df = round_diff(cnt[d], target);
+        if (d < steps && df < 3 && df > -3) s = d;
+    }
     const bool found = s < steps;
     const int moves = found ? s : steps;
     float L = lo, R = hi;
@@ -746,7 +763,7 @@ __device__ inline RoundOut replay_round(const uint32_t *P, float lo, float hi, f
     }
     const int used = found ? s + 1 : steps;
     if (trace_row && lane < used) {
-        uint32_t v = 0;
+        T v = 0;
 #pragma unroll
         for (int d = 0; d < kHistK; ++d)
             if (d == lane) v = cnt[d];
@@ -759,11 +776,51 @@ __device__ inline RoundOut replay_round(const uint32_t *P, float lo, float hi, f
     return RoundOut{L, R, passes0 + used, found, found ? stop : 0};
 }
 
+// Lane 0 of the replaying wave: the round's margins into the cell's record
+// and heap cell; a finished cell (cut found, or the pass cap with the last
+// midpoint) is split at once (orbit.cpp:235-250).
+__device__ inline void finish_round(CellRec r, int32_t c, const RoundOut &ro, int32_t found_enc,
+                                    CellRec *recs, orb_cell *heap, int32_t *level_passes,
+                                    uint8_t *exists, int32_t *status)
+{
+    const float L = ro.L, R = ro.R;
+    const int32_t passes = ro.passes;
+    const bool found = ro.found;
+    r.margin_l = L;
+    r.margin_r = R;
+    r.cut = cell_cut(L, R);
+    r.passes = passes;
+    r.found = found_enc;  // the stopper counts read this round's histograms
+    recs[c] = r;
+    orb_cell &hc = heap[(int64_t)r.hb + r.id];
+    hc.cutMarginLeft = L;
+    hc.cutMarginRight = R;
+    hc.foundCut = found;
+    record_level_passes(level_passes, r.lp, passes);
+    if (!found && passes < ORB_MAX_PASSES) return;
+    const orb_cell p = hc;
+    orb_cell lc, rc;
+    uint32_t *wl = reinterpret_cast<uint32_t *>(&lc);
+    uint32_t *wr = reinterpret_cast<uint32_t *>(&rc);
+#pragma unroll
+    for (int j = 0; j < (int)(sizeof(orb_cell) / 4); ++j) wl[j] = wr[j] = 0u;
+    const int64_t hb = r.hb;
+    if (right_child_id(r.id) >= r.capr || !cell_split(p, lc, rc)) {
+        report_error(status + r.st, r.lp & 63, kErrSplit, r.id);
+        return;
+    }
+    heap[hb + lc.id] = lc;
+    heap[hb + rc.id] = rc;
+    exists[hb + lc.id] = 1;
+    exists[hb + rc.id] = 1;
+}
+
+
 // One wave per cell: prefix sums of the cell histogram in LDS, then the
 // wave replays up to 8 passes of the bisection controller.
 __global__ __launch_bounds__(kBlock) void k_b_replay(CellRec *recs, int32_t nCells,
                                                      uint32_t *cell_hist, orb_cell *heap,
-                                                     uint32_t *trace, int32_t *level_passes,
+                                                     uint64_t *trace, int32_t *level_passes,
                                                      uint8_t *exists, int32_t *status,
                                                      float *next_cand)
 {
@@ -808,52 +865,161 @@ __global__ __launch_bounds__(kBlock) void k_b_replay(CellRec *recs, int32_t nCel
         }
     }
     if (lane != 0) return;
-    const float L = ro.L, R = ro.R;
-    const int32_t passes = ro.passes;
-    const bool found = ro.found;
     // The stopper counts can come from this round's histograms when the cut
     // is a candidate distinct from its neighbours (equal candidates -- a
     // window a few ulps wide -- would split the keys equal to the cut over
     // several tie bins): encode 1 + node, else 1 (found, keys are read).
     int32_t found_enc = 0;
-    if (found) {
+    if (ro.found) {
         const int j = ro.node;
         const float cj = candidate(j, r.margin_l, r.margin_r);
         const float cp = j > 1 ? candidate(j - 1, r.margin_l, r.margin_r) : -INFINITY;
         const float cn = j < kBins - 1 ? candidate(j + 1, r.margin_l, r.margin_r) : INFINITY;
         found_enc = (cj == cp || cj == cn) ? 1 : 1 + j;
     }
-    r.margin_l = L;
-    r.margin_r = R;
-    r.cut = cell_cut(L, R);
-    r.passes = passes;
-    r.found = found_enc;  // the stopper counts read this round's histograms
-    recs[c] = r;
-    orb_cell &hc = heap[(int64_t)r.hb + r.id];
-    hc.cutMarginLeft = L;
-    hc.cutMarginRight = R;
-    hc.foundCut = found;
-    record_level_passes(level_passes, r.lp, passes);
-    if (!found && passes < ORB_MAX_PASSES) return;
-    // the cell is finished (cut found, or the pass cap with the last
-    // midpoint): split it now (orbit.cpp:235-250)
-    const orb_cell p = hc;
-    orb_cell lc, rc;
-    uint32_t *wl = reinterpret_cast<uint32_t *>(&lc);
-    uint32_t *wr = reinterpret_cast<uint32_t *>(&rc);
-#pragma unroll
-    for (int j = 0; j < (int)(sizeof(orb_cell) / 4); ++j) wl[j] = wr[j] = 0u;
-    const int64_t hb = r.hb;
-    if (right_child_id(r.id) >= r.capr || !cell_split(p, lc, rc)) {
-        report_error(status + r.st, r.lp & 63, kErrSplit, r.id);
-        return;
-    }
-    heap[hb + lc.id] = lc;
-    heap[hb + rc.id] = rc;
-    exists[hb + lc.id] = 1;
-    exists[hb + rc.id] = 1;
+    finish_round(r, c, ro, found_enc, recs, heap, level_passes, exists, status);
 }
 
+// ---------------------------------------------------------------------------
+// weighted rounds (workload weights w(p_i), documentation/notes.tex:259-271,
+// 326-336): the bins hold weight sums (64-bit) instead of counts, and the
+// controller compares the weight left of each candidate with the cell's
+// weight times the left share.  Every round bins every key of a searching
+// cell (no window packing), so a row's total is the chunk's weight and a
+// cell histogram's total the cell's weight.
+// ---------------------------------------------------------------------------
+template <int LE>
+__global__ __launch_bounds__(kBlock) void k_bw_hist(
+    const CellRec *__restrict__ recs, int32_t nCells, const uint32_t *__restrict__ first,
+    const uint4 *__restrict__ chunk_desc, uint32_t max_chunks, const float *__restrict__ x,
+    const float *__restrict__ y, const float *__restrict__ z, const uint32_t *__restrict__ wgt,
+    uint64_t *__restrict__ chunk_hist_w)
+{
+    __shared__ float cand[kBins + 1];
+    __shared__ unsigned long long hist[kBins];
+    const int t = threadIdx.x;
+    const uint32_t total = first[nCells];
+    for (uint32_t q = blockIdx.x;; q += gridDim.x) {
+        const uint4 d = chunk_record(chunk_desc, q, max_chunks);
+        const uint32_t b = uniform(d.x), e = uniform(d.y), cell = uniform(d.z);
+        const uint32_t axis = uniform(d.w);
+        keep_values(b, e, cell, axis);
+        if (q >= total) break;
+        if (axis > 2u) continue;  // uniform: the cell is not searched
+        const CellRec &r = recs[cell];
+        if (r.found) continue;    // uniform: an earlier round found the cut
+        const float lo = uniform(r.margin_l), hi = uniform(r.margin_r);
+        store_cands(cand, t ? candidate(t, lo, hi) : -INFINITY);
+        hist[t] = 0ull;
+        __syncthreads();
+        const float width = hi - lo;
+        const float scale = width > 0.0f ? (float)kBins / width : 0.0f;
+        const float lim = 0.5f - bin_tolerance(lo, hi, scale);
+        const float *key = axis_ptr(x, y, z, (int)axis);
+        for (uint32_t p = b + (uint32_t)t; p < e; p += kBlock) {
+            const float v = key[p];
+            const unsigned long long wv = wgt[p];
+            int bin;
+            if (v < lo) {
+                bin = 0;                  // below every candidate
+            } else if (v > hi) {
+                bin = kBins - 1;          // above every candidate
+            } else if (!bin_estimate(v, lo, scale, lim, bin)) {
+                int tj;
+                bin = bin_of<LE>(cand, v, lo, scale, tj);
+            }
+            atomicAdd(&hist[bin], wv);
+        }
+        __syncthreads();
+        chunk_hist_w[(size_t)q * kBins + t] = hist[t];
+        __syncthreads();  // the next chunk rewrites cand and hist
+    }
+}
+
+// Chunk rows into cell rows (64-bit): block g takes chunks [64g, 64g + 64),
+// thread t bin t, one atomic per (block, cell, bin).
+__global__ __launch_bounds__(kBlock) void k_bw_reduce(const CellRec *recs, int32_t nCells,
+                                                      const uint32_t *first,
+                                                      const uint32_t *chunk_cell,
+                                                      const uint64_t *chunk_hist_w,
+                                                      uint64_t *cell_hist_w)
+{
+    __shared__ int32_t scell[kReduceRows];
+    const uint32_t total = first[nCells];
+    const int t = threadIdx.x;
+    for (uint32_t q0 = blockIdx.x * (uint32_t)kReduceRows; q0 < total;
+         q0 += gridDim.x * (uint32_t)kReduceRows) {
+        const uint32_t nrows =
+            total - q0 < (uint32_t)kReduceRows ? total - q0 : (uint32_t)kReduceRows;
+        if (t < kReduceRows) {
+            int32_t cl = -1;
+            if ((uint32_t)t < nrows) {
+                const uint32_t cell = chunk_cell[q0 + t];
+                const CellRec &r = recs[cell];
+                if (!(r.found || r.flags) && r.axis >= 0 && r.axis <= 2) cl = (int32_t)cell;
+            }
+            scell[t] = cl;
+        }
+        __syncthreads();
+        int32_t cur = -1;
+        unsigned long long acc = 0;
+        for (uint32_t u = 0; u < nrows; ++u) {
+            const int32_t cl = scell[u];
+            if (cl != cur) {
+                if (cur >= 0 && acc)
+                    atomicAdd(reinterpret_cast<unsigned long long *>(cell_hist_w) +
+                                  (size_t)cur * kBins + t,
+                              acc);
+                cur = cl;
+                acc = 0;
+            }
+            if (cl >= 0) acc += chunk_hist_w[(size_t)(q0 + u) * kBins + t];
+        }
+        if (cur >= 0 && acc)
+            atomicAdd(reinterpret_cast<unsigned long long *>(cell_hist_w) + (size_t)cur * kBins + t,
+                      acc);
+        __syncthreads();  // scell is rewritten for the next tile
+    }
+}
+
+// One wave per cell: prefix sums of the cell's weight histogram, the cell's
+// weight (the row's total), then up to 8 passes of the controller.
+__global__ __launch_bounds__(kBlock) void k_bw_replay(CellRec *recs, int32_t nCells,
+                                                      uint64_t *cell_hist_w, orb_cell *heap,
+                                                      uint64_t *trace, int32_t *level_passes,
+                                                      uint8_t *exists, int32_t *status)
+{
+    __shared__ uint64_t P[4][kBins + 1];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int32_t c = blockIdx.x * 4 + w;
+    if (c >= nCells) return;  // uniform per wave
+    CellRec r = recs[c];
+    if (r.found || r.flags || r.axis < 0 || r.axis > 2) return;
+    uint64_t *row = cell_hist_w + (size_t)c * kBins;
+    uint64_t h[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        h[i] = row[4 * lane + i];
+        row[4 * lane + i] = 0ull;  // the next round starts from zero
+    }
+    const uint64_t s0 = h[0], s1 = s0 + h[1], s2 = s1 + h[2], s3 = s2 + h[3];
+    const uint64_t base = wave_inclusive_scan(s3) - s3;
+    P[w][4 * lane + 1] = base + s0;
+    P[w][4 * lane + 2] = base + s1;
+    P[w][4 * lane + 3] = base + s2;
+    P[w][4 * lane + 4] = base + s3;
+    if (lane == 0) P[w][0] = 0ull;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    const uint64_t weight = P[w][kBins];  // every key of the cell was binned
+    const float target = difference_target_w(weight, r.nLeaf);
+    const RoundOut ro = replay_round(P[w], r.margin_l, r.margin_r, target, r.passes, lane,
+                                     trace ? trace + ((size_t)r.hb + r.id) * ORB_MAX_PASSES
+                                           : nullptr);
+    if (lane != 0) return;
+    // found: the partition reads the keys (no stopper counts from the bins)
+    finish_round(r, c, ro, ro.found ? 1 : 0, recs, heap, level_passes, exists, status);
+}
 
 // ---------------------------------------------------------------------------
 // LDS-resident subtree kernel
@@ -893,6 +1059,45 @@ __device__ inline uint32_t wave_count_below(const float *key, uint32_t pb, uint3
     return cnt;
 }
 
+// The weight of the keys of [pb, pe) below the cut (weighted build), summed
+// over the wave in 64 bits.
+__device__ inline uint64_t wave_weight_below(const float *key, const uint32_t *wgt, uint32_t pb,
+                                             uint32_t pe, float cut, int le_cmp, int lane)
+{
+    uint64_t s = 0;
+    for (uint32_t p = pb + (uint32_t)lane; p < pe; p += 64u) {
+        const float v = key[p];
+        const bool below = le_cmp ? (v <= cut) : (v < cut);
+        s += below ? (uint64_t)wgt[p] : 0ull;
+    }
+    return wave_sum(s);
+}
+
+// The total weight of [pb, pe), summed over the wave.
+__device__ inline uint64_t wave_weight_total(const uint32_t *wgt, uint32_t pb, uint32_t pe, int lane)
+{
+    uint64_t s = 0;
+    for (uint32_t p = pb + (uint32_t)lane; p < pe; p += 64u) s += wgt[p];
+    return wave_sum(s);
+}
+
+// The left count (unweighted) or left weight (weighted) of [pb, pe).
+template <bool W>
+__device__ inline uint64_t wave_left(const float *key, const uint32_t *wgt, uint32_t pb, uint32_t pe,
+                                     float cut, int le_cmp, int lane)
+{
+    if constexpr (W) return wave_weight_below(key, wgt, pb, pe, cut, le_cmp, lane);
+    else return wave_count_below(key, pb, pe, cut, le_cmp, lane);
+}
+
+// The termination rule's difference for a left count or weight.
+template <bool W>
+__device__ inline int64_t left_difference(uint64_t left, float target)
+{
+    if constexpr (W) return difference_from_target_w(left, target);
+    else return difference_from_target((uint32_t)left, target);
+}
+
 // Every remaining level of the subtree under a small cell, in LDS.  Depth by
 // depth, the cells of one depth are taken four at a time, one per wave; at
 // depths with fewer than four cells the waves of the block split each cell's
@@ -903,23 +1108,27 @@ __device__ inline uint32_t wave_count_below(const float *key, uint32_t pb, uint3
 // scans, the crossing K by a 64-way search, K swaps, the final swap.  Every
 // barrier is reached by all waves the same number of times.
 // IDS: particle ids move with the particles; a 16-bit local index rides along
-// in LDS and the ids are permuted once at the write-back.
-template <bool IDS>
+// in LDS and the ids are permuted once at the write-back.  W: workload
+// weights ride along in LDS and the passes sum weights instead of counting.
+template <bool IDS, bool W>
 __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32_t nCells,
                                                       orb_cell *heap, uint8_t *exists,
                                                       uint8_t *sub, uint32_t *ranges, float *x,
                                                       float *y, float *z, int le_cmp,
-                                                      uint32_t *trace, int32_t *level_passes,
-                                                      int32_t *status, uint32_t *pid)
+                                                      uint64_t *trace, int32_t *level_passes,
+                                                      int32_t *status, uint32_t *pid,
+                                                      uint32_t *wgt)
 {
     __shared__ float sx[kSmall], sy[kSmall], sz[kSmall];
+    __shared__ uint32_t sw[W ? kSmall : 1];
+    __shared__ uint64_t sWt[4];
     // the two stopper lists (16-bit), reused as the cell's 32-bit ids at the end
     __shared__ uint32_t spos_raw[kSmall];
     __shared__ uint16_t sidx[IDS ? kSmall : 1];
     uint16_t *const slpos = reinterpret_cast<uint16_t *>(spos_raw);
     uint16_t *const srpos = slpos + kSmall;
     __shared__ SubCell cur[kSmallLeaves], nxt[kSmallLeaves];
-    __shared__ uint32_t sCnt[2][4];
+    __shared__ uint64_t sCnt[2][4];
     __shared__ int32_t sAct[2][4];
     __shared__ uint32_t sTL[4], sTR[4], sK[4], sI[4];
     __shared__ int32_t s_ncur, s_nvalid, s_err;
@@ -957,6 +1166,7 @@ __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32
         // memory latency per cell instead of one per 256 keys
         constexpr int kIt = (kSmall + kBlock - 1) / kBlock;
         float vx[kIt], vy[kIt], vz[kIt];
+        uint32_t vw[W ? kIt : 1];
 #pragma unroll
         for (int u = 0; u < kIt; ++u) {
             const uint32_t k = (uint32_t)(t + u * kBlock);
@@ -964,6 +1174,7 @@ __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32
                 vx[u] = x[begin + k];
                 vy[u] = y[begin + k];
                 vz[u] = z[begin + k];
+                if constexpr (W) vw[u] = wgt[begin + k];
             }
         }
 #pragma unroll
@@ -973,6 +1184,7 @@ __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32
                 sx[k] = vx[u];
                 sy[k] = vy[u];
                 sz[k] = vz[u];
+                if constexpr (W) sw[k] = vw[u];
                 if (IDS) sidx[k] = (uint16_t)k;
             }
         }
@@ -1017,19 +1229,31 @@ __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32
             uint32_t pb = lb + (uint32_t)part * seg, pe = pb + seg;
             if (pb > le) pb = le;
             if (pe > le) pe = le;
-            const float target = difference_target(m, cl.nLeafCells);
-            uint32_t *trow = trace ? trace + ((size_t)hb + cl.id) * ORB_MAX_PASSES : nullptr;
+            float target;
+            if constexpr (W) {
+                // the cell's weight (count.cpp:14-17 with weights): the g waves
+                // of the cell add their parts through LDS
+                const uint64_t pw = valid ? wave_weight_total(sw, pb, pe, lane) : 0ull;
+                if (lane == 0) sWt[w] = pw;
+                __syncthreads();
+                uint64_t tot = 0;
+                for (int q = 0; q < g; ++q) tot += sWt[w0 + q];
+                target = difference_target_w(tot, cl.nLeafCells);
+            } else {
+                target = difference_target(m, cl.nLeafCells);
+            }
+            uint64_t *trow = trace ? trace + ((size_t)hb + cl.id) * ORB_MAX_PASSES : nullptr;
             float L = cl.cutMarginLeft, R = cl.cutMarginRight;
             int32_t passes = 0;
             bool found = false;
             if (g == 1) {
                 while (valid && !found && passes < ORB_MAX_PASSES) {
                     const float cut = cell_cut(L, R);
-                    const uint32_t cnt = wave_count_below(key, lb, le, cut, le_cmp, lane);
+                    const uint64_t cnt = wave_left<W>(key, sw, lb, le, cut, le_cmp, lane);
                     if (trow && lane == 0) trow[passes] = cnt;
                     passes++;
-                    const int32_t diff = difference_from_target(cnt, target);
-                    if (abs(diff) < 3) found = true;
+                    const int64_t diff = left_difference<W>(cnt, target);
+                    if (diff < 3 && diff > -3) found = true;
                     else if (diff > 0) R = cut;
                     else L = cut;
                 }
@@ -1037,7 +1261,7 @@ __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32
                 for (int ps = 0;; ++ps) {
                     const bool act = valid && !found && passes < ORB_MAX_PASSES;
                     const float cut = cell_cut(L, R);
-                    const uint32_t part_cnt = act ? wave_count_below(key, pb, pe, cut, le_cmp, lane) : 0u;
+                    const uint64_t part_cnt = act ? wave_left<W>(key, sw, pb, pe, cut, le_cmp, lane) : 0ull;
                     if (lane == 0) {
                         sCnt[ps & 1][w] = part_cnt;
                         sAct[ps & 1][w] = act ? 1 : 0;
@@ -1047,12 +1271,12 @@ __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32
                                       sAct[ps & 1][3]) != 0;
                     if (!any) break;  // the same words for every wave
                     if (act) {
-                        uint32_t cnt = 0;
+                        uint64_t cnt = 0;
                         for (int q = 0; q < g; ++q) cnt += sCnt[ps & 1][w0 + q];
                         if (trow && part == 0 && lane == 0) trow[passes] = cnt;
                         passes++;
-                        const int32_t diff = difference_from_target(cnt, target);
-                        if (abs(diff) < 3) found = true;
+                        const int64_t diff = left_difference<W>(cnt, target);
+                        if (diff < 3 && diff > -3) found = true;
                         else if (diff > 0) R = cut;
                         else L = cut;
                     }
@@ -1150,6 +1374,11 @@ __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32
                     sx[b2] = ax;
                     sy[b2] = ay;
                     sz[b2] = az;
+                    if constexpr (W) {
+                        const uint32_t wa = sw[a];
+                        sw[a] = sw[b2];
+                        sw[b2] = wa;
+                    }
                     if (IDS) {
                         const uint16_t ia = sidx[a];
                         sidx[a] = sidx[b2];
@@ -1186,6 +1415,11 @@ __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32
                         sx[b2] = ax;
                         sy[b2] = ay;
                         sz[b2] = az;
+                        if constexpr (W) {
+                            const uint32_t wa = sw[a];
+                            sw[a] = sw[b2];
+                            sw[b2] = wa;
+                        }
                         if (IDS) {
                             const uint16_t ia = sidx[a];
                             sidx[a] = sidx[b2];
@@ -1230,6 +1464,7 @@ __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32
         x[begin + k] = sx[k];
         y[begin + k] = sy[k];
         z[begin + k] = sz[k];
+        if constexpr (W) wgt[begin + k] = sw[k];
     }
     if (IDS) {
         // the stopper lists are free: stage the cell's ids, then permute them
@@ -1263,6 +1498,11 @@ static int ensure_state(Ctx &c, int32_t levels_mode, bool *complete)
     ORB_TRY(b.level_passes.alloc(64));
     ORB_TRY(b.hpasses.alloc(64));
     if (c.trace) ORB_TRY(b.trace.alloc((size_t)c.cap * ORB_MAX_PASSES));
+    if (c.weighted) {
+        ORB_TRY(b.chunk_hist_w.alloc((size_t)(c.max_chunks + 1) * kBins));
+        if (b.cell_hist_w.n < (size_t)c.max_level_cells * kBins) b.hist_w_clean = false;
+        ORB_TRY(b.cell_hist_w.alloc((size_t)c.max_level_cells * kBins));
+    }
     // every buffer the launches touch exists before a capture starts
     ORB_TRY(preallocate_scratch(c));
     return ORB_OK;
@@ -1291,7 +1531,8 @@ static std::vector<uint64_t> graph_signature(const Ctx &c, const BatchedState &b
                                              int32_t levels_mode)
 {
     std::vector<uint64_t> s = {(uint64_t)levels_mode, (uint64_t)c.cmp, (uint64_t)c.part,
-                               (uint64_t)c.trace, (uint64_t)c.track_ids, (uint64_t)c.n,
+                               (uint64_t)c.trace, (uint64_t)c.track_ids, (uint64_t)c.weighted,
+                               (uint64_t)c.n,
                                (uint64_t)c.d, (uint64_t)c.max_chunks,
                                (uint64_t)c.max_level_cells, (uint64_t)(int64_t)c.sub_max_keys};
     const void *bufs[] = {c.pos.p, c.tmp.p, c.pid.p, c.pid_tmp.p, c.ranges.p, c.recs.p,
@@ -1299,7 +1540,8 @@ static std::vector<uint64_t> graph_signature(const Ctx &c, const BatchedState &b
                           c.chunk_ltle.p, c.chunk_cnt.p, c.chunk_pre.p, c.scan_partials.p,
                           c.packed.p, c.masks.p, c.cell_cand.p, c.splits.p, c.chunk_comp.p,
                           c.status.p, c.stamps.p, c.cnt.p, b.level_ids.p, b.heap.p, b.exists.p,
-                          b.sub.p, b.chunk_hist.p, b.cell_hist.p, b.trace.p, b.level_passes.p};
+                          b.sub.p, b.chunk_hist.p, b.cell_hist.p, b.trace.p, b.level_passes.p,
+                          c.wt.p, c.wt_tmp.p, b.chunk_hist_w.p, b.cell_hist_w.p};
     for (const void *p : bufs) s.push_back((uint64_t)(uintptr_t)p);
     s.push_back((uint64_t)c.tune_chunk_blocks.size());
     for (int32_t v : c.tune_chunk_blocks) s.push_back((uint64_t)(int64_t)v);
@@ -1310,9 +1552,10 @@ static std::vector<uint64_t> graph_signature(const Ctx &c, const BatchedState &b
 
 int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *heap,
                   uint8_t *exists, uint8_t *sub, uint16_t *chunk_hist, uint32_t *cell_hist,
-                  uint32_t *trace, int32_t *level_passes, int32_t *status, bool any_subtree,
-                  bool full_cover, int32_t level)
+                  uint64_t *trace, int32_t *level_passes, int32_t *status, bool any_subtree,
+                  bool full_cover, int32_t level, uint64_t *chunk_hist_w, uint64_t *cell_hist_w)
 {
+    if (c.weighted && (!chunk_hist_w || !cell_hist_w)) return ORB_ERR_STATE;
     const int32_t sub_max = subtree_max_keys(c);
     const bool use_subtree = any_subtree && c.part == ORB_PART_HOARE && sub_max > 0;
     const int le = c.cmp == ORB_CMP_LE ? 1 : 0;
@@ -1343,19 +1586,46 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
             c.tune_sub_blocks[(size_t)level] > 0)
             sb = c.tune_sub_blocks[(size_t)level];
         const unsigned gs = (unsigned)(nCells < sb ? nCells : sb);
-        if (c.track_ids)
-            hipLaunchKernelGGL(k_b_subtree<true>, dim3(gs), dim3(kBlock), 0, c.stream, c.recs.p,
-                               nCells, heap, exists, sub, c.ranges.p, c.x(), c.y(), c.z(), le, trace,
-                               level_passes, status, c.ids());
-        else
-            hipLaunchKernelGGL(k_b_subtree<false>, dim3(gs), dim3(kBlock), 0, c.stream, c.recs.p,
-                               nCells, heap, exists, sub, c.ranges.p, c.x(), c.y(), c.z(), le, trace,
-                               level_passes, status, nullptr);
+        auto subtree = c.track_ids ? (c.weighted ? k_b_subtree<true, true> : k_b_subtree<true, false>)
+                                   : (c.weighted ? k_b_subtree<false, true> : k_b_subtree<false, false>);
+        hipLaunchKernelGGL(subtree, dim3(gs), dim3(kBlock), 0, c.stream, c.recs.p, nCells, heap,
+                           exists, sub, c.ranges.p, c.x(), c.y(), c.z(), le, trace, level_passes,
+                           status, c.ids(), c.weights());
         ORB_HIP(hipGetLastError());
     }
     const unsigned gq = chunk_grid(c);
     const unsigned gr = (unsigned)((gq + kReduceRows - 1) / kReduceRows * 4);
     const unsigned gw = (unsigned)((nCells + 3) / 4);
+    if (c.weighted) {
+        // weight-sum rounds over every key, then the partition from the keys
+        for (int r = 0; r < kRounds; ++r) {
+            const int tb = timing_mark(c);
+            if (le)
+                hipLaunchKernelGGL(k_bw_hist<1>, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p,
+                                   nCells, c.cell_first.p, c.chunk_desc.p, (uint32_t)c.max_chunks,
+                                   c.x(), c.y(), c.z(), c.weights(), chunk_hist_w);
+            else
+                hipLaunchKernelGGL(k_bw_hist<0>, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p,
+                                   nCells, c.cell_first.p, c.chunk_desc.p, (uint32_t)c.max_chunks,
+                                   c.x(), c.y(), c.z(), c.weights(), chunk_hist_w);
+            ORB_HIP(hipGetLastError());
+            ORB_TRY(timing_pair(c, tb, kTimeCount));
+            hipLaunchKernelGGL(k_bw_reduce, dim3(gr), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
+                               c.cell_first.p, c.chunk_cell.p, chunk_hist_w, cell_hist_w);
+            hipLaunchKernelGGL(k_bw_replay, dim3(gw), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
+                               cell_hist_w, heap, trace, level_passes, exists, status);
+            ORB_HIP(hipGetLastError());
+        }
+        const int tp = timing_mark(c);
+        if (c.part == ORB_PART_HOARE) {
+            ORB_TRY(launch_hoare(c, nCells, status, nullptr));
+        } else {
+            ORB_TRY(invalidate_stamps(c, nCells));  // the stopper counts read the keys
+            ORB_TRY(launch_stable_le(c, nCells, status, full_cover, nullptr));
+        }
+        ORB_TRY(timing_pair(c, tp, kTimePartition));
+        return ORB_OK;
+    }
     for (int r = 0; r < kRounds; ++r) {
         const int tb = timing_mark(c);
         // rounds: all keys, all keys + packing the window, then the packed keys
@@ -1415,7 +1685,7 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
 static int enqueue_build(Ctx &c, int32_t levels_mode)
 {
     BatchedState &b = *c.batched;
-    uint32_t *trace = c.trace ? b.trace.p : nullptr;
+    uint64_t *trace = c.trace ? b.trace.p : nullptr;
     const int64_t trace_n = c.trace ? c.cap * ORB_MAX_PASSES : 0;
     const orb_cell root = cell_root(c.d);
     hipLaunchKernelGGL(k_b_init, dim3(1024), dim3(kBlock), 0, c.stream, b.heap.p, b.exists.p,
@@ -1437,7 +1707,7 @@ static int enqueue_build(Ctx &c, int32_t levels_mode)
         ORB_TRY(enqueue_level(c, b.level_ids.p + b.level_off[(size_t)l], nCells, b.heap.p,
                               b.exists.p, b.sub.p, b.chunk_hist.p, b.cell_hist.p, trace,
                               b.level_passes.p, c.status.p, b.complete && may_sub,
-                              b.level_cover[(size_t)l] != 0, l));
+                              b.level_cover[(size_t)l] != 0, l, b.chunk_hist_w.p, b.cell_hist_w.p));
         ORB_TRY(timing_pair(c, tlevel, kTimeLevel + l));
     }
     return particles_home(c);  // in-stream: the capture ends where it began
@@ -1455,6 +1725,11 @@ int build_batched(Ctx &c, int32_t levels_mode, orb_build_info *info)
         ORB_TRY(launch_zero_u32(c.stream, b.cell_hist.p, (size_t)c.max_level_cells * kBins));
         b.hist_clean = true;
     }
+    if (c.weighted && !b.hist_w_clean) {
+        ORB_TRY(launch_zero_u64(c.stream, b.cell_hist_w.p, (size_t)c.max_level_cells * kBins));
+        b.hist_w_clean = true;
+    }
+    c.trace_weighted = c.weighted;
     ORB_TRY(particles_home(c));
     const bool graph = c.use_graph && !c.timing;
     if (graph) {
@@ -1498,6 +1773,7 @@ int build_batched(Ctx &c, int32_t levels_mode, orb_build_info *info)
     }
     if (status_error(c.hstatus.p, &info->error_cell)) {
         b.hist_clean = false;  // an aborted subtree may leave rows behind
+        b.hist_w_clean = false;
         return ORB_ERR_UB;
     }
     return ORB_OK;
@@ -1516,7 +1792,7 @@ int batched_pull_results(Ctx &c)
     if (c.trace && b.trace.p) {
         c.trace_counts.resize((size_t)c.cap * ORB_MAX_PASSES);
         ORB_HIP(hipMemcpyAsync(c.trace_counts.data(), b.trace.p,
-                               sizeof(uint32_t) * c.trace_counts.size(), hipMemcpyDeviceToHost,
+                               sizeof(uint64_t) * c.trace_counts.size(), hipMemcpyDeviceToHost,
                                c.stream));
     }
     ORB_HIP(hipStreamSynchronize(c.stream));
@@ -1540,6 +1816,8 @@ void batched_release(Ctx &c)
     b.sub.release();
     b.chunk_hist.release();
     b.cell_hist.release();
+    b.chunk_hist_w.release();
+    b.cell_hist_w.release();
     b.trace.release();
     b.level_passes.release();
     b.hpasses.release();

@@ -436,6 +436,14 @@ struct Ctx {
     // optional particle ids: the input index of the particle at each position
     bool track_ids = false;
     DevBuf<uint32_t> pid, pid_tmp, pid_saved;
+    // optional workload weights w(p_i) (documentation/notes.tex:259-271):
+    // uint32 per particle, moved with the particles; the bisection counts
+    // become weight sums (uint64)
+    bool weighted = false;
+    DevBuf<uint32_t> wt, wt_tmp, wt_saved;
+    DevBuf<uint64_t> chunk_w;        // [max_chunks][2] plain form: weight < cut, <= cut
+    DevBuf<uint64_t> cnt_w;          // [cells][2] per-cell sums
+    PinBuf<uint64_t> hcnt_w;         // [cells][2]
     DevBuf<uint32_t> ranges;   // [cap][2] cellToRangeMap
 
     // level tables (sized for the widest level)
@@ -477,8 +485,9 @@ struct Ctx {
     // build results
     std::vector<orb_cell> heap;
     std::vector<uint8_t> exists;
-    std::vector<uint32_t> trace_counts;
+    std::vector<uint64_t> trace_counts;  // [cap][32] left count (weighted: weight) per pass
     bool trace = false;
+    bool trace_weighted = false;     // the last traced build summed weights
     bool timing = false;
     bool building = false;     // inside a plain build (its level marks are pending)
     bool use_graph = true;    // batched form: capture the build once, replay it
@@ -507,6 +516,8 @@ struct Ctx {
     float *z() const { return cur() + 2 * stride; }
     uint32_t *ids() const { return track_ids ? (in_tmp ? pid_tmp.p : pid.p) : nullptr; }
     uint32_t *other_ids() const { return track_ids ? (in_tmp ? pid.p : pid_tmp.p) : nullptr; }
+    uint32_t *weights() const { return weighted ? (in_tmp ? wt_tmp.p : wt.p) : nullptr; }
+    uint32_t *other_weights() const { return weighted ? (in_tmp ? wt.p : wt_tmp.p) : nullptr; }
 };
 
 // timing helpers (orb_api.hip): marks are -1 when timing is off
@@ -531,6 +542,9 @@ int upload_level(Ctx &c, const orb_cell *cells, int32_t nCells);
 int build_chunk_table(Ctx &c, int32_t nCells, bool with_stamps = true, bool scanned = false);
 int launch_count(Ctx &c, int32_t nCells, bool lt_skip_found, const float *axis_override,
                  hipEvent_t after_count = nullptr);
+// Weighted count pass into c.cnt_w ([cells][2]: weight < cut, <= cut; with
+// total_only both are the cell's weight); skip_found leaves found cells out.
+int launch_count_w(Ctx &c, int32_t nCells, bool skip_found, bool total_only);
 // The count pass over caller-provided tables (no context, no allocation):
 // the chunk table from the exclusive chunk offsets first[] (first[nCells] at
 // most max_chunks), the chunk counts, the per-cell lt/le sums into cnt.
@@ -567,10 +581,12 @@ int schedule_levels(int32_t d, int32_t levels_mode, int64_t cap,
 int launch_make_axis(Ctx &c, int32_t nCells);
 int launch_cell_sizes(Ctx &c, int32_t nCells, uint32_t *dst);
 // one level step of the batched form over `entries` (device array)
+// (chunk_hist_w, cell_hist_w: the weighted rounds' rows, when c.weighted)
 int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *heap,
                   uint8_t *exists, uint8_t *sub, uint16_t *chunk_hist, uint32_t *cell_hist,
-                  uint32_t *trace, int32_t *level_passes, int32_t *status, bool any_subtree,
-                  bool full_cover, int32_t level);
+                  uint64_t *trace, int32_t *level_passes, int32_t *status, bool any_subtree,
+                  bool full_cover, int32_t level, uint64_t *chunk_hist_w = nullptr,
+                  uint64_t *cell_hist_w = nullptr);
 int preallocate_scratch(Ctx &c);
 // level tables for nCells cells and (at least) `chunks` chunks
 int ensure_level_capacity(Ctx &c, int64_t nCells, int64_t chunks = 0);

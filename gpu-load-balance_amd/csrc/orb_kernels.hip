@@ -198,6 +198,7 @@ int ensure_stable_buffers(Ctx &c)
     // the other buffer has the particle buffer's shape, slack included
     ORB_TRY(c.tmp.alloc(3 * (size_t)c.stride + 64));
     if (c.track_ids) ORB_TRY(c.pid_tmp.alloc((size_t)c.stride));
+    if (c.weighted) ORB_TRY(c.wt_tmp.alloc((size_t)c.stride));
     return ORB_OK;
 }
 
@@ -275,6 +276,7 @@ int particles_home(Ctx &c)
     ORB_TRY(launch_copy_u32(c.stream, reinterpret_cast<const uint32_t *>(c.tmp.p),
                             reinterpret_cast<uint32_t *>(c.pos.p), 3 * (size_t)c.stride));
     if (c.track_ids) ORB_TRY(launch_copy_u32(c.stream, c.pid_tmp.p, c.pid.p, (size_t)c.stride));
+    if (c.weighted) ORB_TRY(launch_copy_u32(c.stream, c.wt_tmp.p, c.wt.p, (size_t)c.stride));
     c.in_tmp = false;
     return ORB_OK;
 }
@@ -593,6 +595,94 @@ int launch_count(Ctx &c, int32_t nCells, bool skip_found, const float *axis_over
     hipLaunchKernelGGL(k_cell_totals, dim3(g), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
                        c.cell_first.p, c.chunk_ltle.p, skip_found ? 1 : 0,
                        axis_override ? 0 : 1, c.cnt.p, c.stamps.p);
+    ORB_HIP(hipGetLastError());
+    return ORB_OK;
+}
+
+// Weighted count pass (workload weights w(p_i), documentation/notes.tex:
+// 259-271): per chunk the weight of the keys < cut and <= cut (total_only:
+// of every key, the cell's weight for ServiceCount), 64-bit sums.  One block
+// per chunk, coalesced loads of keys and weights, wave shuffles and LDS.
+// Roofline: HBM, 8 B per key (key + weight).
+__global__ __launch_bounds__(kBlock) void k_count_w(const CellRec *recs, int32_t nCells,
+                                                    const uint32_t *first,
+                                                    const uint32_t *chunk_cell, const float *x,
+                                                    const float *y, const float *z,
+                                                    const uint32_t *wgt, int skip_found,
+                                                    int total_only, uint64_t *chunk_w)
+{
+    __shared__ uint64_t sm[2][kBlock / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint32_t total = first[nCells];
+    for (uint32_t q = blockIdx.x; q < total; q += gridDim.x) {
+        uint32_t cell, b, e;
+        chunk_range(q, first, nCells, chunk_cell, recs, cell, b, e);
+        const CellRec r = recs[cell];
+        if (!total_only && skip_found && r.found) continue;  // uniform per block
+        const bool axis_ok = r.axis >= 0 && r.axis <= 2;  // no axis: nothing left of the cut
+        const float *key = axis_ptr(x, y, z, axis_ok ? r.axis : 0);
+        uint64_t lt = 0, le = 0;
+        for (uint32_t p = b + threadIdx.x; p < e; p += kBlock) {
+            const uint64_t w = wgt[p];
+            if (total_only) {
+                lt += w;
+            } else if (axis_ok) {
+                const float v = key[p];
+                lt += v < r.cut ? w : 0ull;
+                le += v <= r.cut ? w : 0ull;
+            }
+        }
+        if (total_only) le = lt;
+        lt = wave_sum(lt);
+        le = wave_sum(le);
+        if (lane == 0) {
+            sm[0][wv] = lt;
+            sm[1][wv] = le;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            chunk_w[2 * (size_t)q] = sm[0][0] + sm[0][1] + sm[0][2] + sm[0][3];
+            chunk_w[2 * (size_t)q + 1] = sm[1][0] + sm[1][1] + sm[1][2] + sm[1][3];
+        }
+        __syncthreads();  // the next chunk rewrites sm
+    }
+}
+
+// Per-cell sums of the weighted chunk slots (one block per cell).
+__global__ __launch_bounds__(kBlock) void k_cell_totals_w(const CellRec *recs, int32_t nCells,
+                                                          const uint32_t *first,
+                                                          const uint64_t *chunk_w, int skip_found,
+                                                          uint64_t *cnt_w)
+{
+    __shared__ uint64_t sm[4];
+    for (int32_t c = blockIdx.x; c < nCells; c += gridDim.x) {
+        if (skip_found && recs[c].found) continue;  // uniform per block
+        uint64_t lt = 0, le = 0;
+        for (uint32_t q = first[c] + threadIdx.x; q < first[c + 1]; q += kBlock) {
+            lt += chunk_w[2 * (size_t)q];
+            le += chunk_w[2 * (size_t)q + 1];
+        }
+        lt = block_sum(lt, sm);
+        le = block_sum(le, sm);
+        if (threadIdx.x == 0) {
+            cnt_w[2 * (size_t)c] = lt;
+            cnt_w[2 * (size_t)c + 1] = le;
+        }
+    }
+}
+
+int launch_count_w(Ctx &c, int32_t nCells, bool skip_found, bool total_only)
+{
+    if (!c.weighted) return ORB_ERR_STATE;
+    ORB_TRY(c.chunk_w.alloc(2 * ((size_t)c.max_chunks + 1)));
+    ORB_TRY(c.cnt_w.alloc(2 * (size_t)c.max_level_cells));
+    ORB_TRY(c.hcnt_w.alloc(2 * (size_t)c.max_level_cells));
+    hipLaunchKernelGGL(k_count_w, dim3(chunk_grid(c)), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
+                       c.cell_first.p, c.chunk_cell.p, c.x(), c.y(), c.z(), c.weights(),
+                       skip_found ? 1 : 0, total_only ? 1 : 0, c.chunk_w.p);
+    const unsigned g = (unsigned)(nCells < 65536 ? (nCells > 0 ? nCells : 1) : 65536);
+    hipLaunchKernelGGL(k_cell_totals_w, dim3(g), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
+                       c.cell_first.p, c.chunk_w.p, (skip_found && !total_only) ? 1 : 0, c.cnt_w.p);
     ORB_HIP(hipGetLastError());
     return ORB_OK;
 }
@@ -1062,23 +1152,29 @@ __device__ inline void side_range(int side, const SwapUnit &u, uint32_t TR, uint
 // kSwapBatch pairs per thread in flight (all loads, then all stores).
 constexpr int kSwapBatch = 8;
 
-// Particle arrays seen from a cell's first particle.  Narrow: raw buffer
-// accesses with 32-bit byte offsets over one descriptor per array (one
-// address register per access; the cell holds fewer than 2^30 particles).
-// Wide: plain 64-bit addressing.
+// Particle arrays seen from a cell's first particle: x, y, z, then the
+// payload arrays that move with them (ids, weights), whichever are on, in
+// that order (`count` of them).  Narrow: raw buffer accesses with 32-bit
+// byte offsets over one descriptor per array (one address register per
+// access; the cell holds fewer than 2^30 particles).  Wide: plain 64-bit
+// addressing.
 template <bool kNarrow>
 struct CellArrays;
 
 template <>
 struct CellArrays<true> {
-    __amdgpu_buffer_rsrc_t a[4];
-    __device__ CellArrays(float *x, float *y, float *z, uint32_t *pid, uint32_t begin)
+    __amdgpu_buffer_rsrc_t a[5];
+    int count;
+    __device__ CellArrays(float *x, float *y, float *z, uint32_t *pid, uint32_t *w, uint32_t begin)
     {
         // raw buffer, 32-bit data format, no range limit below 4 GiB
         a[0] = __builtin_amdgcn_make_buffer_rsrc(x + begin, 0, -1, 0x00020000);
         a[1] = __builtin_amdgcn_make_buffer_rsrc(y + begin, 0, -1, 0x00020000);
         a[2] = __builtin_amdgcn_make_buffer_rsrc(z + begin, 0, -1, 0x00020000);
-        a[3] = __builtin_amdgcn_make_buffer_rsrc(pid ? pid + begin : nullptr, 0, -1, 0x00020000);
+        uint32_t *p3 = pid ? pid : w, *p4 = pid ? w : nullptr;
+        a[3] = __builtin_amdgcn_make_buffer_rsrc(p3 ? p3 + begin : nullptr, 0, -1, 0x00020000);
+        a[4] = __builtin_amdgcn_make_buffer_rsrc(p4 ? p4 + begin : nullptr, 0, -1, 0x00020000);
+        count = 3 + (pid ? 1 : 0) + (w ? 1 : 0);
     }
     using Off = uint32_t;
     __device__ uint32_t load(int arr, Off o) const
@@ -1093,13 +1189,17 @@ struct CellArrays<true> {
 
 template <>
 struct CellArrays<false> {
-    char *a[4];
-    __device__ CellArrays(float *x, float *y, float *z, uint32_t *pid, uint32_t begin)
+    char *a[5];
+    int count;
+    __device__ CellArrays(float *x, float *y, float *z, uint32_t *pid, uint32_t *w, uint32_t begin)
     {
         a[0] = reinterpret_cast<char *>(x + begin);
         a[1] = reinterpret_cast<char *>(y + begin);
         a[2] = reinterpret_cast<char *>(z + begin);
-        a[3] = reinterpret_cast<char *>(pid ? pid + begin : nullptr);
+        uint32_t *p3 = pid ? pid : w, *p4 = pid ? w : nullptr;
+        a[3] = reinterpret_cast<char *>(p3 ? p3 + begin : nullptr);
+        a[4] = reinterpret_cast<char *>(p4 ? p4 + begin : nullptr);
+        count = 3 + (pid ? 1 : 0) + (w ? 1 : 0);
     }
     using Off = uint64_t;
     __device__ uint32_t load(int arr, Off o) const
@@ -1112,9 +1212,11 @@ struct CellArrays<false> {
     }
 };
 
-template <bool kNarrow>
+// kArrays: the most arrays this instantiation moves (3 without payload, up
+// to 5 with ids and weights): the register arrays are sized by it.
+template <bool kNarrow, int kArrays>
 __device__ inline void swap_pairs(const uint32_t *sl, const uint32_t *sr, uint32_t m,
-                                  const CellArrays<kNarrow> &ca, bool ids)
+                                  const CellArrays<kNarrow> &ca)
 {
     using Off = typename CellArrays<kNarrow>::Off;
     for (uint32_t j0 = 0; j0 < m; j0 += (uint32_t)(kBlock * kSwapBatch)) {
@@ -1126,12 +1228,12 @@ __device__ inline void swap_pairs(const uint32_t *sl, const uint32_t *sr, uint32
             pa[u] = jj < m ? (Off)sl[jj] * (Off)4 : (Off)0;
             pb[u] = jj < m ? (Off)sr[jj] * (Off)4 : (Off)0;
         }
-        const int na = ids ? 4 : 3;  // uniform
+        const int na = ca.count;  // uniform, at most kArrays
         // defined on every path: otherwise the arrays are carried from batch
         // to batch in registers
-        uint32_t va[4][kSwapBatch] = {}, vb[4][kSwapBatch] = {};
+        uint32_t va[kArrays][kSwapBatch] = {}, vb[kArrays][kSwapBatch] = {};
 #pragma unroll
-        for (int arr = 0; arr < 4; ++arr) {
+        for (int arr = 0; arr < kArrays; ++arr) {
             if (arr >= na) break;
 #pragma unroll
             for (int u = 0; u < kSwapBatch; ++u) {
@@ -1142,7 +1244,7 @@ __device__ inline void swap_pairs(const uint32_t *sl, const uint32_t *sr, uint32
             }
         }
 #pragma unroll
-        for (int arr = 0; arr < 4; ++arr) {
+        for (int arr = 0; arr < kArrays; ++arr) {
             if (arr >= na) break;
 #pragma unroll
             for (int u = 0; u < kSwapBatch; ++u) {
@@ -1160,7 +1262,7 @@ __device__ inline void swap_pairs(const uint32_t *sl, const uint32_t *sr, uint32
 // block scan ranks both, and every stopper whose index the unit uses goes
 // to its slot (L index t -> t - t0, R index j -> TR - 1 - j - t0).  Later
 // rounds continue on the partner side only, chunk by chunk.
-template <bool kNarrow>
+template <bool kNarrow, int kArrays>
 __global__ __launch_bounds__(kBlock) void k_hoare_swap(const CellRec *recs, int32_t nCells,
                                                        const uint32_t *first,
                                                        const uint32_t *chunk_cell,
@@ -1168,7 +1270,7 @@ __global__ __launch_bounds__(kBlock) void k_hoare_swap(const CellRec *recs, int3
                                                        const uint64_t *masks,
                                                        const SplitRec *splits,
                                                        float *x, float *y, float *z,
-                                                       uint32_t *pid)
+                                                       uint32_t *pid, uint32_t *w)
 {
     __shared__ uint32_t spos[2][kChunk];  // cell-local positions of L_t, R_t at t - t0
     __shared__ uint32_t sm[kBlock / 64];
@@ -1225,14 +1327,14 @@ __global__ __launch_bounds__(kBlock) void k_hoare_swap(const CellRec *recs, int3
             kp = co.at(sp, kp + 2u) > next ? kp + 1u : co.chunk_of(sp, next, kp + 1u, nch);
         }
         __syncthreads();
-        const CellArrays<kNarrow> ca(x, y, z, pid, r.begin);
-        swap_pairs<kNarrow>(spos[0], spos[1], u.t1 - u.t0, ca, pid != nullptr);
+        const CellArrays<kNarrow> ca(x, y, z, pid, w, r.begin);
+        swap_pairs<kNarrow, kArrays>(spos[0], spos[1], u.t1 - u.t0, ca);
         __syncthreads();  // the next unit overwrites spos
     }
 }
 
 __global__ void k_hoare_final(const CellRec *recs, int32_t nCells, const SplitRec *splits,
-                              float *x, float *y, float *z, uint32_t *pid)
+                              float *x, float *y, float *z, uint32_t *pid, uint32_t *w)
 {
     int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nCells) return;
@@ -1249,6 +1351,11 @@ __global__ void k_hoare_final(const CellRec *recs, int32_t nCells, const SplitRe
         const uint32_t ia = pid[pa];
         pid[pa] = pid[pb];
         pid[pb] = ia;
+    }
+    if (w) {
+        const uint32_t wa = w[pa];
+        w[pa] = w[pb];
+        w[pb] = wa;
     }
 }
 
@@ -1371,17 +1478,23 @@ int launch_hoare(Ctx &c, int32_t nCells, int32_t *status, const HistCounts *hc)
     hipLaunchKernelGGL(k_hoare_split, dim3(gw), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
                        c.cell_first.p, c.chunk_pre.p, c.masks.p, c.splits.p, c.ranges.p, status);
     ORB_HIP(hipGetLastError());
-    // byte offsets inside any cell fit 32 bits below 2^30 particles
-    auto swap = c.stride < (int64_t(1) << 30) ? k_hoare_swap<true> : k_hoare_swap<false>;
+    // byte offsets inside any cell fit 32 bits below 2^30 particles; the
+    // payload-free instantiation keeps the fewest registers
+    const bool narrow = c.stride < (int64_t(1) << 30);
+    const int arrays = 3 + (c.ids() ? 1 : 0) + (c.weights() ? 1 : 0);
+    auto swap = narrow ? (arrays == 3 ? k_hoare_swap<true, 3>
+                          : arrays == 4 ? k_hoare_swap<true, 4> : k_hoare_swap<true, 5>)
+                       : (arrays == 3 ? k_hoare_swap<false, 3>
+                          : arrays == 4 ? k_hoare_swap<false, 4> : k_hoare_swap<false, 5>);
     const int ts = timing_mark(c);
     hipLaunchKernelGGL(swap, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
                        c.cell_first.p, c.chunk_cell.p, c.chunk_pre.p, c.masks.p, c.splits.p,
-                       c.x(), c.y(), c.z(), c.ids());
+                       c.x(), c.y(), c.z(), c.ids(), c.weights());
     ORB_HIP(hipGetLastError());
     ORB_TRY(timing_pair(c, ts, kTimeSwap));
     unsigned gc = (unsigned)((nCells + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_hoare_final, dim3(gc), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
-                       c.splits.p, c.x(), c.y(), c.z(), c.ids());
+                       c.splits.p, c.x(), c.y(), c.z(), c.ids(), c.weights());
     ORB_HIP(hipGetLastError());
     return ORB_OK;
 }
@@ -1398,7 +1511,8 @@ __device__ inline void k_stable_scatter_chunk(uint32_t q, const CellRec *recs, i
                                               const uint32_t *first, const uint32_t *chunk_cell,
                                               const uint64_t *chunk_pre, const float *x,
                                               const float *y, const float *z, const uint32_t *pid,
-                                              float *ox, float *oy, float *oz, uint32_t *opid)
+                                              const uint32_t *wt, float *ox, float *oy, float *oz,
+                                              uint32_t *opid, uint32_t *ow)
 {
     constexpr int kSeg = kChunk / (kBlock / 64);
     constexpr int kIt = kSeg / 64;
@@ -1449,6 +1563,7 @@ __device__ inline void k_stable_scatter_chunk(uint32_t q, const CellRec *recs, i
             oy[d] = y[p];
             oz[d] = z[p];
             if (opid) opid[d] = pid[p];
+            if (ow) ow[d] = wt[p];
         }
         dl += (uint32_t)__popcll(mL);
         dr += (uint32_t)__popcll(mR);
@@ -1461,13 +1576,13 @@ __global__ __launch_bounds__(kBlock) void k_stable_scatter(const CellRec *recs, 
                                                            const uint64_t *chunk_pre,
                                                            const float *x, const float *y,
                                                            const float *z, const uint32_t *pid,
-                                                           float *ox, float *oy, float *oz,
-                                                           uint32_t *opid)
+                                                           const uint32_t *w, float *ox, float *oy,
+                                                           float *oz, uint32_t *opid, uint32_t *ow)
 {
     const uint32_t total = first[nCells];
     for (uint32_t q = blockIdx.x; q < total; q += gridDim.x) {
-        k_stable_scatter_chunk(q, recs, nCells, first, chunk_cell, chunk_pre, x, y, z, pid, ox,
-                               oy, oz, opid);
+        k_stable_scatter_chunk(q, recs, nCells, first, chunk_cell, chunk_pre, x, y, z, pid, w, ox,
+                               oy, oz, opid, ow);
         __syncthreads();  // the next chunk reuses this block's LDS
     }
 }
@@ -1477,8 +1592,8 @@ __global__ __launch_bounds__(kBlock) void k_stable_scatter(const CellRec *recs, 
 __device__ inline void k_copy_back_chunk(uint32_t q, const CellRec *recs, int32_t nCells,
                                          const uint32_t *first, const uint32_t *chunk_cell,
                                          const float *ox, const float *oy, const float *oz,
-                                         const uint32_t *opid, float *x, float *y, float *z,
-                                         uint32_t *pid)
+                                         const uint32_t *opid, const uint32_t *ow, float *x,
+                                         float *y, float *z, uint32_t *pid, uint32_t *w)
 {
     uint32_t cell, b, e;
     if (!chunk_range(q, first, nCells, chunk_cell, recs, cell, b, e)) return;
@@ -1487,6 +1602,7 @@ __device__ inline void k_copy_back_chunk(uint32_t q, const CellRec *recs, int32_
         y[p] = oy[p];
         z[p] = oz[p];
         if (pid) pid[p] = opid[p];
+        if (w) w[p] = ow[p];
     }
 }
 
@@ -1494,12 +1610,14 @@ __global__ __launch_bounds__(kBlock) void k_copy_back(const CellRec *recs, int32
                                                       const uint32_t *first,
                                                       const uint32_t *chunk_cell, const float *ox,
                                                       const float *oy, const float *oz,
-                                                      const uint32_t *opid, float *x, float *y,
-                                                      float *z, uint32_t *pid)
+                                                      const uint32_t *opid, const uint32_t *ow,
+                                                      float *x, float *y, float *z, uint32_t *pid,
+                                                      uint32_t *w)
 {
     const uint32_t total = first[nCells];
     for (uint32_t q = blockIdx.x; q < total; q += gridDim.x)
-        k_copy_back_chunk(q, recs, nCells, first, chunk_cell, ox, oy, oz, opid, x, y, z, pid);
+        k_copy_back_chunk(q, recs, nCells, first, chunk_cell, ox, oy, oz, opid, ow, x, y, z, pid,
+                          w);
 }
 
 __global__ void k_stable_ranges(const CellRec *recs, int32_t nCells, const uint32_t *first,
@@ -1537,15 +1655,15 @@ int launch_stable_le(Ctx &c, int32_t nCells, int32_t *status, bool full_cover,
     float *ox = ob, *oy = ob + c.stride, *oz = ob + 2 * c.stride;
     hipLaunchKernelGGL(k_stable_scatter, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
                        c.cell_first.p, c.chunk_cell.p, c.chunk_pre.p, c.x(), c.y(), c.z(), c.ids(),
-                       ox, oy, oz, c.other_ids());
+                       c.weights(), ox, oy, oz, c.other_ids(), c.other_weights());
     ORB_HIP(hipGetLastError());
     if (full_cover) {
         // every particle was written to the other buffer: it becomes current
         c.in_tmp = !c.in_tmp;
     } else {
         hipLaunchKernelGGL(k_copy_back, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
-                           c.cell_first.p, c.chunk_cell.p, ox, oy, oz, c.other_ids(), c.x(), c.y(),
-                           c.z(), c.ids());
+                           c.cell_first.p, c.chunk_cell.p, ox, oy, oz, c.other_ids(),
+                           c.other_weights(), c.x(), c.y(), c.z(), c.ids(), c.weights());
         ORB_HIP(hipGetLastError());
     }
     unsigned gc = (unsigned)((nCells + kBlock - 1) / kBlock);

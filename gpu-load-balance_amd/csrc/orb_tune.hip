@@ -21,6 +21,7 @@ namespace {
 struct Snapshot {
     DevBuf<float> pos;
     DevBuf<uint32_t> pid;
+    DevBuf<uint32_t> wt;
 };
 
 int take_snapshot(Ctx &c, Snapshot &s)
@@ -34,6 +35,11 @@ int take_snapshot(Ctx &c, Snapshot &s)
         ORB_HIP(hipMemcpyAsync(s.pid.p, c.pid.p, sizeof(uint32_t) * (size_t)c.stride,
                                hipMemcpyDeviceToDevice, c.stream));
     }
+    if (c.weighted) {
+        ORB_TRY(s.wt.alloc((size_t)c.stride));
+        ORB_HIP(hipMemcpyAsync(s.wt.p, c.wt.p, sizeof(uint32_t) * (size_t)c.stride,
+                               hipMemcpyDeviceToDevice, c.stream));
+    }
     ORB_HIP(hipStreamSynchronize(c.stream));
     return ORB_OK;
 }
@@ -45,6 +51,9 @@ int put_back(Ctx &c, const Snapshot &s)
                            hipMemcpyDeviceToDevice, c.stream));
     if (c.track_ids && s.pid.p)
         ORB_HIP(hipMemcpyAsync(c.pid.p, s.pid.p, sizeof(uint32_t) * (size_t)c.stride,
+                               hipMemcpyDeviceToDevice, c.stream));
+    if (c.weighted && s.wt.p)
+        ORB_HIP(hipMemcpyAsync(c.wt.p, s.wt.p, sizeof(uint32_t) * (size_t)c.stride,
                                hipMemcpyDeviceToDevice, c.stream));
     ORB_HIP(hipStreamSynchronize(c.stream));
     return ORB_OK;

@@ -141,6 +141,11 @@ PROTOTYPES = {
     "orb_get_membership": (C.c_int, [_vp, _vp, _i64, _b]),
     "orb_get_leaf_bounds": (C.c_int, [_vp, _vp, _vp, _i64]),
     "orb_enable_trace": (C.c_int, [_vp, _b]),
+    "orb_set_weights": (C.c_int, [_vp, _vp, _i64]),
+    "orb_get_weights": (C.c_int, [_vp, _vp, _i64]),
+    "orb_count_weight": (C.c_int, [_vp, _vp, _i32, _vp]),
+    "orb_count_left_weight": (C.c_int, [_vp, _vp, _i32, _i32, _vp]),
+    "orb_get_pass_weights": (C.c_int, [_vp, _vp, _i64]),
     "orb_get_pass_counts": (C.c_int, [_vp, _vp, _i64]),
     "orb_get_timing": (C.c_int, [_vp, C.POINTER(Timing)]),
     "orb_enable_timing": (C.c_int, [_vp, _b]),
@@ -417,6 +422,39 @@ class Context:
             _check(lib().orb_get_leaf_bounds(self.h, _p(lo), _p(hi), len(ids)),
                    "orb_get_leaf_bounds")
         return lo, hi
+
+    # ---- workload weights
+    def set_weights(self, w):
+        if w is None:
+            _check(lib().orb_set_weights(self.h, None, self.n), "orb_set_weights")
+            return
+        w = np.ascontiguousarray(w, np.uint32)
+        assert len(w) == self.n
+        _check(lib().orb_set_weights(self.h, _p(w), self.n), "orb_set_weights")
+
+    def weights(self):
+        out = np.zeros(self.n, np.uint32)
+        _check(lib().orb_get_weights(self.h, _p(out), self.n), "orb_get_weights")
+        return out
+
+    def count_weight(self, cells):
+        a = cells_array(cells)
+        out = np.zeros(len(a), np.uint64)
+        _check(lib().orb_count_weight(self.h, _p(a), len(a), _p(out)), "orb_count_weight")
+        return out
+
+    def count_left_weight(self, cells, cmp, out=None):
+        a = cells_array(cells)
+        if out is None:
+            out = np.zeros(len(a), np.uint64)
+        _check(lib().orb_count_left_weight(self.h, _p(a), len(a), int(cmp), _p(out)),
+               "orb_count_left_weight")
+        return out
+
+    def pass_weights(self):
+        pw = np.zeros((self.cap, MAX_PASSES), np.uint64)
+        _check(lib().orb_get_pass_weights(self.h, _p(pw), self.cap), "orb_get_pass_weights")
+        return pw
 
     def enable_trace(self, on=True):
         _check(lib().orb_enable_trace(self.h, bool(on)), "orb_enable_trace")

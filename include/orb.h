@@ -272,6 +272,38 @@ int orb_get_membership(orb_ctx *ctx, int32_t *leaf_of, int64_t n, bool by_input_
  * gets lower = +inf, upper = -inf.                                      */
 int orb_get_leaf_bounds(orb_ctx *ctx, float *lower, float *upper, int64_t cap);
 
+/* ------------------------------------------------------------------ */
+/* workload weights.  The thesis defines the weight w(p_i) of a particle */
+/* as its share of the work (the simulation steps it needs) and the      */
+/* split target as a weight quantile (documentation/notes.tex:259-271,   */
+/* 326-336); the reference's code counts particles only.  With weights   */
+/* on, both build forms run the reference's controller on weight sums:   */
+/* count(x < cut) becomes the weight of those keys and the cell's count  */
+/* its weight (uint64 sums), the termination rule of orbit.cpp:204-208   */
+/* is evaluated in the same float32 roundings (the uint64 sums rounded   */
+/* to nearest) and |difference| < 3 weight units ends the search.  With  */
+/* every weight 1 a weighted build equals the unweighted one.  Weights   */
+/* move with the particles through every partition.                      */
+/* ------------------------------------------------------------------ */
+/* Sets the weights (uint32, in the particles' current order; n = the
+ * context's particle count); NULL turns weights off.  orb_set_particles
+ * turns them off (new particles need their own).  orb_save_particles
+ * saves them with the particles; orb_restore_particles then restores them
+ * (ORB_ERR_STATE if weights were set after the snapshot).                */
+int orb_set_weights(orb_ctx *ctx, const uint32_t *w, int64_t n);
+int orb_get_weights(orb_ctx *ctx, uint32_t *w, int64_t n);
+/* ServiceCount with weights: out[i] = the weight of cells[i]'s particles. */
+int orb_count_weight(orb_ctx *ctx, const orb_cell *cells, int32_t nCells, uint64_t *out);
+/* ServiceCountLeft / ServiceCountLeftGPU with weights: out[i] = the weight
+ * of the keys < cut (ORB_CMP_LT, found cells skipped, out[i] untouched) or
+ * <= cut (ORB_CMP_LE, every cell).  Returns nCells * 8.                  */
+int orb_count_left_weight(orb_ctx *ctx, const orb_cell *cells, int32_t nCells, int32_t cmp,
+                          uint64_t *out);
+/* The trace of a weighted build: [cap][32] left weight per pass (UINT64_MAX
+ * where a cell was not counted; for an unweighted build the counts).
+ * orb_get_pass_counts refuses (ORB_ERR_STATE) after a weighted build.    */
+int orb_get_pass_weights(orb_ctx *ctx, uint64_t *sums, int64_t cap);
+
 /* Batched form: capture the whole build into a hipGraph on first use and
  * replay it afterwards (default on; a build with timing enabled runs
  * eagerly).  The capture is redone when the level mode, comparison,
