@@ -34,7 +34,7 @@ def log(msg):
 
 
 TM_KEYS = ("count_ms", "partition_ms", "other_ms", "count_bytes", "count_launches", "swap_ms",
-           "swap_launches")
+           "swap_launches", "swap_pairs", "swap_bytes")
 
 
 def timed_builds(ctx, form, levels, steps):
@@ -110,16 +110,21 @@ def cpu_model():
     return "unknown CPU"
 
 
-def cpu_baseline(log2n, d, levels, threads=1):
+def cpu_baseline(n, d, levels, thread_counts):
+    """The oracle (a port of the reference's o=0 path) building the same
+    workload on the host: wall ms per thread count, with its count and
+    partition phases (the reference's CountCopy and Partition columns)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
-    n = 1 << log2n
     x, y, z = pyoracle.generate(n)
-    t0 = time.perf_counter()
-    r = pyoracle.build(x, y, z, d, levels_mode=levels, threads=threads)
-    t1 = time.perf_counter()
-    assert r.rc == 0
-    return (t1 - t0) * 1e3, r
+    out = {}
+    for nt in thread_counts:
+        t0 = time.perf_counter()
+        r = pyoracle.build(x, y, z, d, levels_mode=levels, threads=nt)
+        t1 = time.perf_counter()
+        assert r.rc == 0
+        out[nt] = {"ms": (t1 - t0) * 1e3, "count_ms": r.count_ms, "partition_ms": r.partition_ms}
+    return out
 
 
 def pct(v, q):
@@ -132,8 +137,9 @@ def pct(v, q):
 
 def serve_bench(args):
     """Serving workload: requests with N log-uniform in [2^16, 2^24] and d
-    log-uniform in [2^4, 2^12], half uniform and half Plummer-clustered,
-    particles resident in HBM.  Run 1 (saturated): every request queued at
+    log-uniform in [2^4, 2^12], half uniform (the reference generator with the
+    request's seed) and half Plummer-clustered (64 spheres, a = 0.01, 10%
+    background, the request's seed), particles resident in HBM.  Run 1 (saturated): every request queued at
     t=0, builds/s = requests / makespan.  Run 2 (Poisson): arrivals at 70% of
     the saturated rate, latency = done - arrival (p50/p99).  The comparator is
     the same requests built one after another with the plain form (measured
@@ -144,24 +150,26 @@ def serve_bench(args):
 
     K = args.requests
     rng = np.random.default_rng(args.seed)
-    log(f"[serve] generating particle pools (2^24 uniform, 2^24 clustered)")
-    top = 1 << 24
-    pool = [np.stack([rng.uniform(-0.5, 0.5, top).astype(np.float32) for _ in range(3)]),
-            np.stack(inputs.clustered(top, n_spheres=8, a=0.02, background=0.05, seed=args.seed))]
     reqs = []
     for k in range(K):
         n = int(round(2.0 ** rng.uniform(16, 24)))
         d = int(round(2.0 ** rng.uniform(4, 12)))
         kind = k % 2
-        off = int(rng.integers(0, top - n + 1))
-        reqs.append((kind, off, n, d))
+        reqs.append((kind, int(rng.integers(1, 1 << 40)), n, d))
     tot_particles = sum(r[2] for r in reqs)
-    log(f"[serve] {K} requests, {tot_particles} particles in all")
+    log(f"[serve] {K} requests, {tot_particles} particles in all: generating their inputs")
+    host = []
+    for kind, s, n, d in reqs:
+        if kind == 0:
+            host.append(np.stack(orb.generate(n, seed=s)))
+        else:
+            host.append(np.stack(inputs.clustered(n, n_spheres=64, a=0.01, background=0.1,
+                                                  seed=s)))
 
     # comparator: serial plain builds, inputs in HBM, tree copied back
     serial_plain, serial_batched = [], []
-    for k, (kind, off, n, d) in enumerate(reqs):
-        src = pool[kind][:, off:off + n]
+    for k, (kind, s, n, d) in enumerate(reqs):
+        src = host[k]
         ctx = orb.Context(n, d, generate=False)
         ctx.set_particles(src[0], src[1], src[2])
         ctx.save_particles()
@@ -179,12 +187,12 @@ def serve_bench(args):
     dev = [None] * K
 
     def upload():
-        for k, (kind, off, n, d) in enumerate(reqs):
+        for k in range(K):
             if dev[k] is None:
-                dev[k] = [orb.DeviceArray(pool[kind][a, off:off + n]) for a in range(3)]
+                dev[k] = [orb.DeviceArray(host[k][a]) for a in range(3)]
             else:
                 for a in range(3):
-                    dev[k][a].upload(pool[kind][a, off:off + n])
+                    dev[k][a].upload(host[k][a])
         orb.hip().hipDeviceSynchronize()
 
     def run_server(arrivals):
@@ -248,8 +256,9 @@ def serve_bench(args):
     sb_lat, sb_span = simulate(serial_batched, arrivals)
     out = {
         "config": "c5", "workload": "serving: N log-uniform [2^16,2^24], d log-uniform "
-                                    "[2^4,2^12], 50% uniform / 50% Plummer-clustered, Poisson "
-                                    "arrivals, full trees, particles resident in HBM",
+                                    "[2^4,2^12], 50% uniform (xorshf96, request seed) / 50% "
+                                    "Plummer (64 spheres, a=0.01, 10% background, request seed), "
+                                    "Poisson arrivals, full trees, particles resident in HBM",
         "requests": K, "particles": tot_particles,
         "saturated": {"server_builds_per_s": round(mu, 2),
                       "serial_plain_builds_per_s": round(K / (sum(serial_plain) * 1e-3), 2),
@@ -281,7 +290,6 @@ def main():
     ap.add_argument("--d", type=int, default=1024)
     ap.add_argument("--levels", choices=["full", "reference"], default="full")
     ap.add_argument("--forms", default="plain,batched")
-    ap.add_argument("--cpu-log2n", type=int, default=26)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--config", choices=["c2", "c3", "c4", "c5"], default="c2",
                     help="c2: uniform 2^27 -> 2^10; c3: clustered 2^26 -> 2^14; "
@@ -432,12 +440,18 @@ def main():
     # the batched build's dominant kernel: the Hoare swap, one launch per level
     tb = res.get("batched", {}).get("tm")
     if tb and tb["swap_launches"] > 0:
-        # each level of the full tree partitions every particle; at the 50%
-        # stopper density of uniform keys every line of x, y, z is read and
-        # written once: 24 B per particle per launch
-        sw_bytes = 24 * n
-        sw_us = tb["swap_ms"] * 1e3 / tb["swap_launches"]
-        sw_ach = sw_bytes / (sw_us * 1e-6) / 1e9
+        # algorithmic bytes: every swapped pair's two rows read and written,
+        # 12 B each (x, y, z) -- 48 B x K, with K (the pairs, summed over the
+        # cells of every level) counted on the device in the timed build.
+        # Line bytes: at the 50% stopper density of uniform keys every line
+        # of x, y, z is touched, 24 B per particle per launch read + written.
+        sw_launches = tb["swap_launches"]
+        sw_alg = tb["swap_bytes"] / sw_launches
+        sw_line = 24 * n
+        sw_us = tb["swap_ms"] * 1e3 / sw_launches
+        sw_ach = sw_alg / (sw_us * 1e-6) / 1e9
+        traffic = (committed_traffic("orb::k_hoare_swap<true, 3>")
+                   if (args.log2n, args.d, args.levels) == (27, 1024, "full") else None)
         out["roofline_partition"] = {
             "bound": "hbm",
             "kernel": "k_hoare_swap (batched build)",
@@ -445,35 +459,36 @@ def main():
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(sw_ach / HBM_PEAK_GBPS, 4),
-            "traffic": (committed_traffic("orb::k_hoare_swap<true>")
-                        if (args.log2n, args.d, args.levels) == (27, 1024, "full") else None),
-            "launches": tb["swap_launches"],
+            "traffic": traffic,
+            "traffic_over_algorithmic": round(traffic / sw_alg, 3) if traffic else None,
+            "launches": sw_launches,
             "avg_launch_us": round(sw_us, 2),
-            "bytes_per_launch": sw_bytes,
+            "bytes_per_launch": int(sw_alg),
+            "pairs_per_launch": int(tb["swap_pairs"] / sw_launches),
+            "line_bytes_per_launch": sw_line,
+            "line_GBps": round(sw_line / (sw_us * 1e-6) / 1e9, 1),
             "timed_in": "one batched build with HIP events around each swap launch, after the "
                         "timed (graph-replayed, uninstrumented) builds",
             "measured_copy_GBps": round(stream_copy, 1),
-            "frac_of_measured_copy": round(sw_ach / stream_copy, 4) if stream_copy > 0 else None,
         }
     if not args.no_cpu:
-        scale = n / (1 << args.cpu_log2n)
         nt = host_threads()
-        log(f"[bench] cpu baseline: oracle build of 2^{args.cpu_log2n} particles, {nt} threads")
-        ms_mt, _ = cpu_baseline(args.cpu_log2n, args.d, levels, threads=nt)
-        log(f"[bench] cpu baseline: the same on 1 thread")
-        ms_1t, _ = cpu_baseline(args.cpu_log2n, args.d, levels, threads=1)
-        note = (f"scaled x{scale:g} to 2^{args.log2n} (linear in N at fixed d; passes grow "
-                f"~log N, so this understates the CPU time)")
+        log(f"[bench] cpu baseline: oracle build of the same 2^{args.log2n} particles, {nt} "
+            f"threads, then 1 thread")
+        cb = cpu_baseline(n, args.d, levels, (nt, 1))
+        mt, st = cb[nt], cb[1]
         out["cpu_baseline"] = {
-            "value": round(ms_mt * scale, 1), "unit": "ms", "cores": nt, "kind": "port",
-            "sample": f"oracle (o=0 port, OpenMP over cells and over a large cell's keys; "
-                      f"same result as 1 thread) full build of 2^{args.cpu_log2n} particles -> "
-                      f"{args.d} leaves took {ms_mt:.1f} ms on {nt} host threads "
-                      f"({cpu_model()}, {os.cpu_count()} logical CPUs visible); {note}",
+            "value": round(mt["ms"], 1), "unit": "ms", "cores": nt, "kind": "port",
+            "sample": f"the oracle (o=0 port, OpenMP over cells and over a large cell's keys; "
+                      f"same result as 1 thread) building this workload in full (2^{args.log2n} "
+                      f"particles -> {args.d} leaves) on {nt} host threads ({cpu_model()}, "
+                      f"{os.cpu_count()} logical CPUs visible)",
+            "count_ms": round(mt["count_ms"], 1), "partition_ms": round(mt["partition_ms"], 1),
         }
         out["cpu_baseline_1_thread"] = {
-            "value": round(ms_1t * scale, 1), "unit": "ms", "cores": 1, "kind": "port",
-            "sample": f"the same build on 1 thread took {ms_1t:.1f} ms; {note}",
+            "value": round(st["ms"], 1), "unit": "ms", "cores": 1, "kind": "port",
+            "sample": "the same build on 1 thread",
+            "count_ms": round(st["count_ms"], 1), "partition_ms": round(st["partition_ms"], 1),
         }
     print(json.dumps(out), flush=True)
 

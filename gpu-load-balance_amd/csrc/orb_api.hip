@@ -53,6 +53,8 @@ void timing_reset(Ctx &c)
     c.level_ms.clear();
     c.tnext = 0;
     c.tpairs.clear();
+    if (c.timing && c.swap_pairs.alloc(1) == ORB_OK)
+        (void)launch_zero_u64(c.stream, reinterpret_cast<uint64_t *>(c.swap_pairs.p), 1);
 }
 
 int timing_mark(Ctx &c)
@@ -506,6 +508,7 @@ int orb_finalize(orb_ctx *ctx)
     c->cnt_w.release();
     c->hcnt_w.release();
     c->ranges.release();
+    c->swap_pairs.release();
     c->dcells.release();
     c->recs.release();
     c->cell_nchunks.release();
@@ -1030,6 +1033,14 @@ int orb_get_timing(orb_ctx *ctx, orb_timing *t)
     Ctx *c = C(ctx);
     if (!c || !t) return ORB_ERR_ARG;
     *t = c->tm;
+    if (c->timing && c->swap_pairs.p) {
+        unsigned long long k = 0;
+        ORB_HIP(hipMemcpyAsync(&k, c->swap_pairs.p, sizeof k, hipMemcpyDeviceToHost, c->stream));
+        ORB_HIP(hipStreamSynchronize(c->stream));
+        const int64_t row = 12 + (c->track_ids ? 4 : 0) + (c->weighted ? 4 : 0);
+        t->swap_pairs = (int64_t)k;
+        t->swap_bytes = (int64_t)k * 2 * 2 * row;  // two rows, read and written
+    }
     return ORB_OK;
 }
 

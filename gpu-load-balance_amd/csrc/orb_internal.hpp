@@ -501,6 +501,7 @@ struct Ctx {
     size_t tnext = 0;
     struct TPair { int a, b, kind; };
     std::vector<TPair> tpairs;
+    DevBuf<unsigned long long> swap_pairs;  // timing: K summed over the timed build's swaps
     BatchedState *batched = nullptr;
     // batched-form launch shapes (chosen by orb_autotune; 0 / -1 = default)
     int32_t cur_level = -1;              // level being enqueued, -1 outside a build

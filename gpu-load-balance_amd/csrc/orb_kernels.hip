@@ -1458,6 +1458,18 @@ int launch_stopper_counts(Ctx &c, int32_t nCells, bool le_only, const HistCounts
                               c.cell_first.p + nCells);
 }
 
+// Timing only: the pairs this level's swap exchanges, added to *acc.
+__global__ void k_sum_swap_pairs(const CellRec *recs, const SplitRec *splits, int32_t nCells,
+                                 unsigned long long *acc)
+{
+    unsigned long long k = 0;
+    for (int32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nCells;
+         c += gridDim.x * blockDim.x)
+        if (!recs[c].flags) k += splits[c].K;
+    k = wave_sum(k);
+    if ((threadIdx.x & 63) == 0 && k) atomicAdd(acc, k);
+}
+
 int launch_hoare(Ctx &c, int32_t nCells, int32_t *status, const HistCounts *hc)
 {
     if (!status) status = c.status.p;
@@ -1478,6 +1490,12 @@ int launch_hoare(Ctx &c, int32_t nCells, int32_t *status, const HistCounts *hc)
     hipLaunchKernelGGL(k_hoare_split, dim3(gw), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
                        c.cell_first.p, c.chunk_pre.p, c.masks.p, c.splits.p, c.ranges.p, status);
     ORB_HIP(hipGetLastError());
+    if (c.timing && c.swap_pairs.p) {
+        const unsigned gs = (unsigned)((nCells + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL(k_sum_swap_pairs, dim3(gs < 1024 ? gs : 1024), dim3(kBlock), 0, c.stream,
+                           c.recs.p, c.splits.p, nCells, c.swap_pairs.p);
+        ORB_HIP(hipGetLastError());
+    }
     // byte offsets inside any cell fit 32 bits below 2^30 particles; the
     // payload-free instantiation keeps the fewest registers
     const bool narrow = c.stride < (int64_t(1) << 30);

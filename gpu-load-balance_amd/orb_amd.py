@@ -59,7 +59,8 @@ class BuildInfo(C.Structure):
 class Timing(C.Structure):
     _fields_ = [("count_ms", C.c_double), ("partition_ms", C.c_double), ("other_ms", C.c_double),
                 ("count_bytes", C.c_int64), ("count_launches", C.c_int32), ("pad", C.c_int32),
-                ("swap_ms", C.c_double), ("swap_launches", C.c_int32), ("pad2", C.c_int32)]
+                ("swap_ms", C.c_double), ("swap_launches", C.c_int32), ("pad2", C.c_int32),
+                ("swap_pairs", C.c_int64), ("swap_bytes", C.c_int64)]
 
 
 class ServerConfig(C.Structure):
@@ -487,7 +488,8 @@ class Context:
         _check(lib().orb_get_timing(self.h, C.byref(t)), "orb_get_timing")
         return {"count_ms": t.count_ms, "partition_ms": t.partition_ms, "other_ms": t.other_ms,
                 "count_bytes": t.count_bytes, "count_launches": t.count_launches,
-                "swap_ms": t.swap_ms, "swap_launches": t.swap_launches}
+                "swap_ms": t.swap_ms, "swap_launches": t.swap_launches,
+                "swap_pairs": t.swap_pairs, "swap_bytes": t.swap_bytes}
 
 
 _hip = None

@@ -344,6 +344,9 @@ typedef struct orb_timing {
     double swap_ms;        /* the Hoare swap kernel alone (within partition_ms) */
     int32_t swap_launches;
     int32_t pad2;
+    int64_t swap_pairs;    /* pairs the Hoare swaps exchanged (K summed over cells) */
+    int64_t swap_bytes;    /* their algorithmic bytes: each pair's two rows read and
+                              written, 12 B each (x, y, z) plus 4 B per id and weight */
 } orb_timing;
 int orb_get_timing(orb_ctx *ctx, orb_timing *t);
 /* Per-level time of the last build run with timing on: ms[l-1] for level l

@@ -31,6 +31,14 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
+
+static double now_ms(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec * 1e3 + ts.tv_nsec * 1e-6;
+}
 
 /* ------------------------------------------------------------------------ */
 /* generator: src/services/init.cu:11-25 (xorshf96 on 64-bit unsigned long)  */
@@ -425,6 +433,8 @@ int orc_build(int64_t n, int32_t d, const orc_opts *opts, float *x, float *y, fl
     tr->nlevels = 0;
     tr->total_passes = 0;
     tr->error_cell = -1;
+    tr->count_ms = 0.0;
+    tr->partition_ms = 0.0;
 
     if (tr->ids)
         for (int64_t p = 0; p < n; ++p) tr->ids[p] = (uint32_t)p;
@@ -458,6 +468,7 @@ int orc_build(int64_t n, int32_t d, const orc_opts *opts, float *x, float *y, fl
         }
 
         /* ServiceCount (count.cpp:14-17); weighted: the cell's weight */
+        const double t_count = now_ms();
         for (int64_t i = 0; i < nCells; ++i) {
             int32_t id = ids[i];
             uint32_t b = tr->ranges[2 * id], e = tr->ranges[2 * id + 1];
@@ -526,6 +537,7 @@ int orc_build(int64_t n, int32_t d, const orc_opts *opts, float *x, float *y, fl
         }
         tr->level_passes[l - 1] = j;
         tr->total_passes += j;
+        tr->count_ms += now_ms() - t_count;
 
         /* split (orbit.cpp:235-250) */
         for (int64_t i = 0; i < nCells; ++i) {
@@ -541,6 +553,7 @@ int orc_build(int64_t n, int32_t d, const orc_opts *opts, float *x, float *y, fl
 
         /* partition (partition.cpp:18-60); cells are disjoint ranges     */
         int bad = -1;
+        const double t_part = now_ms();
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nt) if (nt > 1 && nCells > 1)
         for (int64_t i = 0; i < nCells; ++i) {
             orc_cell *c = &tr->heap[ids[i]];
@@ -564,6 +577,7 @@ int orc_build(int64_t n, int32_t d, const orc_opts *opts, float *x, float *y, fl
             tr->ranges[2 * rid] = (uint32_t)split;
             tr->ranges[2 * rid + 1] = (uint32_t)e;
         }
+        tr->partition_ms += now_ms() - t_part;
         if (bad >= 0) { tr->error_cell = tr->heap[ids[bad]].id; goto done; }
     }
 done:

@@ -123,6 +123,11 @@ typedef struct orc_trace {
                                   moved with the particles; NULL: every weight 1 */
     uint64_t *pass_weights;    /* optional [heap_cap][32]: the left weight sum of
                                   each pass (the left count when unweighted)  */
+    double count_ms;           /* out: wall time of the counting passes and the
+                                  controller (the reference's CountCopy column,
+                                  orbit.cpp:164-188)                          */
+    double partition_ms;       /* out: wall time of the partitions (its
+                                  Partition column, orbit.cpp:252-274)        */
 } orc_trace;
 
 /* Capacity of the heap for the given d and level mode (ids are 0-based).   */

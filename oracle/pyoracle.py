@@ -68,6 +68,7 @@ class Trace(C.Structure):
         ("heap_cap", C.c_int64), ("nlevels", C.c_int32), ("total_passes", C.c_int32),
         ("error_cell", C.c_int32), ("pad", C.c_int32), ("ids", C.c_void_p),
         ("weights", C.c_void_p), ("pass_weights", C.c_void_p),
+        ("count_ms", C.c_double), ("partition_ms", C.c_double),
     ]
 
 
@@ -177,6 +178,8 @@ def build(x, y, z, d, levels_mode=LEVELS_REFERENCE, cmp=CMP_LT, partition=PART_H
     r.ids = pid[:n] if pid is not None else None
     r.weights = wts
     r.pass_weights = pw
+    r.count_ms = tr.count_ms
+    r.partition_ms = tr.partition_ms
     r.level_passes = lp[:max(tr.nlevels, 0)].copy()
     r.nlevels = tr.nlevels
     r.total_passes = tr.total_passes
