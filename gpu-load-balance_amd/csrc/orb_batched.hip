@@ -13,10 +13,11 @@
 //    every candidate, and a replay kernel walks the reference's termination
 //    rule over them: 8 passes of src/orbit.cpp:149-231 per HBM read, bit-exact
 //    by construction.  Four rounds cover the 32-pass cap;
-//  * a cell of at most kSmall keys is handed to a subtree kernel: one block
-//    loads its x, y, z into LDS once, runs every remaining level of the
-//    cell's subtree there (bisection by wave reductions, exact Hoare by
-//    ballot scans) and writes x, y, z back once;
+//  * a cell of at most 8832 keys (7680 with ids, 5120 with weights) is
+//    handed to a subtree kernel: one 1024-thread block loads its x, y, z
+//    into LDS once, runs every remaining level of the cell's subtree there
+//    (the same 8-pass histogram rounds and replay, exact Hoare by ballot
+//    scans) and writes x, y, z back once;
 //  * the partition of the remaining (large) cells uses the same exact Hoare
 //    kernels as the plain form.
 #include "orb_device.hpp"
@@ -24,12 +25,8 @@
 namespace orb {
 
 constexpr int kRounds = (ORB_MAX_PASSES + kHistK - 1) / kHistK;  // 4
-// Subtree kernel limits.  Cells of a level cluster around n / 2^(l-1); the key
-// budget sits 27% above a power of two so a whole level qualifies at once.
-// LDS: 12 B x kSmall (x, y, z) + 4 B x kSmall (positions) + 2 x kSmallLeaves
-// cell records ~ 50 KB, three blocks per CU.
-constexpr int kSmall = 2600;                   // keys per LDS-resident subtree
-constexpr int kSmallLeaves = 64;               // max leaves under a subtree root
+// Leaves under a subtree root (the subtree kernel's key capacity is SubCap).
+constexpr int kSmallLeaves = 64;
 constexpr int kReduceRows = 64;                // chunk histograms per reduce block
 
 struct BatchedState {
@@ -323,12 +320,42 @@ __device__ inline void store_cands(float *cand, float entry)
     if (t == 0) cand[kBins] = INFINITY;
 }
 
+// The keys of a chunk (record dd) that wave w reads, 16 loads of 64
+// consecutive keys, through a range-checked descriptor: loads past the
+// chunk's end return 0.  A chunk that is not live (past the level's count,
+// or of a cell that is not searched) gets an empty range: its loads return
+// 0 and touch no memory.
+template <int kPerWave>
+__device__ inline void issue_chunk_loads(const float *x, const float *y, const float *z, bool live,
+                                         uint4 dd, uint32_t sb, int lane, float (&vv)[kPerWave])
+{
+    const uint32_t ax = uniform(dd.w), b = uniform(dd.x), e = uniform(dd.y);
+    const float *kp = axis_ptr(x, y, z, live ? (int)ax : 0) + (live ? b : 0u);
+    const __amdgpu_buffer_rsrc_t kr = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float *>(kp), 0, live ? (int)((e - b) * 4u) : 0, 0x00020000);
+    // the whole offset in the per-lane operand: the range check covers it,
+    // not the scalar offset
+#pragma unroll
+    for (int j = 0; j < kPerWave; ++j)
+        vv[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+            kr, (int)((sb + (uint32_t)lane) * 4u + 256u * (uint32_t)j), 0, 0));
+}
+
+// Waits for the loads into vv (a use the compiler cannot drop).
+template <int kPerWave>
+__device__ inline void settle_loads(const float (&vv)[kPerWave])
+{
+#pragma unroll
+    for (int j = 0; j < kPerWave; ++j) asm volatile("" ::"v"(vv[j]));
+}
+
 // First histogram round: one block per chunk bins every key of the chunk
 // over the cell's 255 candidate cuts and writes the chunk's 256-bin row and
 // its tie row.  Wave w takes the quarter [1024w, 1024w + 1024) of the
-// chunk, 64 consecutive keys per load through a range-checked descriptor
-// (loads past the chunk's end return 0 and are masked out); keys outside
-// the margins are counted with ballots, the rest binned.
+// chunk, 64 consecutive keys per load; keys outside the margins are counted
+// with ballots, the rest binned.  Software-pipelined: the block's next
+// chunk (record and keys) is in flight while it bins the current one, so a
+// block waits for memory once, not once per chunk.
 template <int LE>
 __global__ __launch_bounds__(kBlock) void k_b_hist(
     const CellRec *__restrict__ recs, int32_t nCells, const uint32_t *__restrict__ first,
@@ -343,33 +370,37 @@ __global__ __launch_bounds__(kBlock) void k_b_hist(
     __shared__ uint32_t ties[kBins];  // keys equal to candidate j (rare)
     __shared__ uint32_t sm[kW];
     const int t = threadIdx.x, lane = t & 63, w = (int)uniform((uint32_t)t >> 6);
+    const uint32_t sb = (uint32_t)(w * kPerWave * 64);  // chunk position of the quarter
     const uint32_t total = first[nCells];
-    for (uint32_t q = blockIdx.x;; q += gridDim.x) {
-        const uint4 d = chunk_record(chunk_desc, q, max_chunks);
+    uint32_t q = blockIdx.x;
+    uint4 d = chunk_record(chunk_desc, q, max_chunks);
+    // one chunk with cur's keys, the next chunk's keys issued into nxt; false
+    // past the level's last chunk
+    auto step = [&](float (&cur)[kPerWave], float (&nxt)[kPerWave]) __attribute__((always_inline)) -> bool {
         const uint32_t b = uniform(d.x), e = uniform(d.y), cell = uniform(d.z);
         const uint32_t axis = uniform(d.w);
-        keep_values(b, e, cell, axis);  // the whole record, in flight with total
-        if (q >= total) break;
-        if (axis > 2u) continue;  // uniform: the cell is not searched
-        const float *kp = axis_ptr(x, y, z, (int)axis) + b;
-        const __amdgpu_buffer_rsrc_t kr =
-            __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(kp), 0, (int)((e - b) * 4u), 0x00020000);
-        const uint32_t sb = (uint32_t)(w * kPerWave * 64);  // chunk position of the quarter
-        float v[kPerWave];
-        // the whole offset in the per-lane operand: the range check covers
-        // it, not the scalar offset
-#pragma unroll
-        for (int j = 0; j < kPerWave; ++j)
-            v[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
-                kr, (int)((sb + (uint32_t)lane) * 4u + 256u * (uint32_t)j), 0, 0));
+        if (q >= total) return false;
+        const uint32_t qn = q + gridDim.x;
+        const uint4 dn = chunk_record(chunk_desc, qn, max_chunks);
+        if (axis > 2u) {  // uniform: the cell is not searched
+            // the empty loads of this chunk retire first: otherwise the
+            // compiler cannot tell, at the loop's head, which buffer still
+            // has loads in flight and waits for both
+            settle_loads(cur);
+            issue_chunk_loads(x, y, z, qn < total && uniform(dn.w) <= 2u, dn, sb, lane, nxt);
+            q = qn;
+            d = dn;
+            return true;
+        }
         const float lo = uniform(recs[cell].margin_l), hi = uniform(recs[cell].margin_r);
-        // the candidate table from the margins while the keys arrive (no
-        // table load: the first round starts from each cell's fresh margins)
+        // the candidate table from the margins (no table load: the first
+        // round starts from each cell's fresh margins)
         store_cands(cand, t ? candidate(t, lo, hi) : -INFINITY);
 #pragma unroll
         for (int u = 0; u < kW; ++u) hist[u][t] = 0u;
         ties[t] = 0u;
         __syncthreads();
+        issue_chunk_loads(x, y, z, qn < total && uniform(dn.w) <= 2u, dn, sb, lane, nxt);
         const float width = hi - lo;
         const float scale = width > 0.0f ? (float)kBins / width : 0.0f;
         const float lim = 0.5f - bin_tolerance(lo, hi, scale);
@@ -381,7 +412,7 @@ __global__ __launch_bounds__(kBlock) void k_b_hist(
                 constexpr int j = decltype(jc)::value;
                 const uint32_t p0 = sb + 64u * (uint32_t)j;  // chunk position of lane 0
                 const bool ok = decltype(full)::value || p0 + (uint32_t)lane < len;
-                const float vj = v[j];
+                const float vj = cur[j];
                 const bool isB = vj < lo, isA = vj > hi;
                 below += (uint32_t)__popcll(__ballot(ok && isB));
                 above += (uint32_t)__popcll(__ballot(ok && isA));
@@ -407,6 +438,17 @@ __global__ __launch_bounds__(kBlock) void k_b_hist(
         chunk_hist[(size_t)q * kBins + t] = (uint16_t)sum;
         chunk_tie[(size_t)q * kBins + t] = (uint16_t)ties[t];
         __syncthreads();  // the next chunk rewrites cand, hist, ties and sm
+        q = qn;
+        d = dn;
+        return true;
+    };
+    float va[kPerWave], vb[kPerWave];
+    issue_chunk_loads(x, y, z, q < total && uniform(d.w) <= 2u, d, sb, lane, va);
+    // two steps per trip with the buffers swapped: no register copies (a
+    // copy would wait for the loads in flight)
+    for (;;) {
+        if (!step(va, vb)) break;
+        if (!step(vb, va)) break;
     }
 }
 
@@ -421,10 +463,10 @@ __global__ __launch_bounds__(kBlock) void k_b_hist(
 // 64k + j): keys above hi (masks[q][0]) and keys in the window (masks[q][1]).
 // The cell's final cut lies in [lo, hi], so these decide every stopper bit
 // but those of the packed keys.
-// Eight waves per SIMD (the compiler would keep the load offsets in scalar
-// registers and stop at seven).
+// Software-pipelined as the first round: the block's next chunk (record,
+// candidate entry, keys) is in flight while it scans the current one.
 template <int LE>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_b_hist_window(
+__global__ __launch_bounds__(kBlock) void k_b_hist_window(
     const CellRec *__restrict__ recs, int32_t nCells, const uint32_t *__restrict__ first,
     const uint4 *__restrict__ chunk_desc, uint32_t max_chunks, const float *__restrict__ x,
     const float *__restrict__ y, const float *__restrict__ z, const float *__restrict__ cell_cand,
@@ -440,33 +482,39 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
     __shared__ uint64_t words[kW][2][kPerWave];  // each wave's mask words, staged
     const int t = threadIdx.x, lane = t & 63, w = (int)uniform((uint32_t)t >> 6);
     const uint64_t lanes_below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t sb = (uint32_t)(w * kPerWave * 64);  // chunk position of the quarter
     const uint32_t total = first[nCells];
-    for (uint32_t q = blockIdx.x;; q += gridDim.x) {
-        const uint4 d = chunk_record(chunk_desc, q, max_chunks);
+    // a chunk is scanned when it is within the level and its cell is still
+    // searched (a cell found by the first round is not)
+    auto live_of = [&](uint32_t qq, uint4 dd) __attribute__((always_inline)) -> bool {
+        if (qq >= total || uniform(dd.w) > 2u) return false;
+        return recs[uniform(dd.z)].found == 0;
+    };
+    uint32_t q = blockIdx.x;
+    uint4 d = chunk_record(chunk_desc, q, max_chunks);
+    bool live = live_of(q, d);
+    auto step = [&](float (&cur)[kPerWave], float (&nxt)[kPerWave], float &ce_cur,
+                    float &ce_nxt) __attribute__((always_inline)) -> bool {
         const uint32_t b = uniform(d.x), e = uniform(d.y), cell = uniform(d.z);
-        const uint32_t axis = uniform(d.w);
-        keep_values(b, e, cell, axis);  // the whole record, in flight with total
-        if (q >= total) break;
-        if (axis > 2u) continue;  // uniform: the cell is not searched
-        if (recs[cell].found) continue;  // uniform: the first round found the cut
-        // the chunk's keys through a range-checked descriptor: loads past the
-        // chunk's end return 0 and are masked out below, with no branches
-        const float *kp = axis_ptr(x, y, z, (int)axis) + b;
-        const float ce = cand_entry(cell_cand, cell);
-        const __amdgpu_buffer_rsrc_t kr =
-            __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(kp), 0, (int)((e - b) * 4u), 0x00020000);
-        const uint32_t sb = (uint32_t)(w * kPerWave * 64);  // chunk position of the quarter
-        float v[kPerWave];
-        // the whole offset in the per-lane operand: the range check covers
-        // it, not the scalar offset
-#pragma unroll
-        for (int j = 0; j < kPerWave; ++j)
-            v[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
-                kr, (int)((sb + (uint32_t)lane) * 4u + 256u * (uint32_t)j), 0, 0));
+        if (q >= total) return false;
+        const uint32_t qn = q + gridDim.x;
+        const uint4 dn = chunk_record(chunk_desc, qn, max_chunks);
+        if (!live) {  // uniform
+            settle_loads(cur);
+            live = live_of(qn, dn);
+            ce_nxt = live ? cand_entry(cell_cand, uniform(dn.z)) : 0.0f;
+            issue_chunk_loads(x, y, z, live, dn, sb, lane, nxt);
+            q = qn;
+            d = dn;
+            return true;
+        }
         const float lo = uniform(recs[cell].margin_l), hi = uniform(recs[cell].margin_r);
-        store_cands(cand, ce);
+        store_cands(cand, ce_cur);
         hist[t] = 0u;
         __syncthreads();
+        const bool live_n = live_of(qn, dn);
+        ce_nxt = live_n ? cand_entry(cell_cand, uniform(dn.z)) : 0.0f;
+        issue_chunk_loads(x, y, z, live_n, dn, sb, lane, nxt);
         const float width = hi - lo;
         const float scale = width > 0.0f ? (float)kBins / width : 0.0f;
         const uint32_t len = e - b;
@@ -482,7 +530,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
                     valid = p0 >= len ? 0ull
                                       : (len - p0 >= 64u ? ~0ull : (1ull << (len - p0)) - 1ull);
                 }
-                const float vj = v[j];
+                const float vj = cur[j];
                 const uint64_t mB = __ballot(vj < lo) & valid;
                 const uint64_t mG = __ballot(vj > hi) & valid;
                 const uint64_t mW = valid & ~mB & ~mG;
@@ -535,6 +583,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
         if (t == kBins - 1) sum += ba >> 16;
         chunk_hist[(size_t)q * kBins + t] = (uint16_t)sum;
         __syncthreads();  // the next chunk rewrites cand, hist and sm
+        live = live_n;
+        q = qn;
+        d = dn;
+        return true;
+    };
+    float va[kPerWave], vb[kPerWave];
+    float ca = live ? cand_entry(cell_cand, uniform(d.z)) : 0.0f, cb = 0.0f;
+    issue_chunk_loads(x, y, z, live, d, sb, lane, va);
+    for (;;) {
+        if (!step(va, vb, ca, cb)) break;
+        if (!step(vb, va, cb, ca)) break;
     }
 }
 
@@ -716,6 +775,11 @@ __device__ inline int64_t round_diff(uint32_t left, float target)
 __device__ inline int64_t round_diff(uint64_t left, float target)
 {
     return difference_from_target_w(left, target);
+}
+// (uint16: prefix sums of a cell of at most 8192 keys, LDS subtree kernel)
+__device__ inline int64_t round_diff(uint16_t left, float target)
+{
+    return difference_from_target((uint32_t)left, target);
 }
 
 template <class T>
@@ -1098,39 +1162,93 @@ __device__ inline int64_t left_difference(uint64_t left, float target)
     else return difference_from_target((uint32_t)left, target);
 }
 
-// Every remaining level of the subtree under a small cell, in LDS.  Depth by
-// depth, the cells of one depth are taken four at a time, one per wave; at
-// depths with fewer than four cells the waves of the block split each cell's
-// keys between them and add their counts through LDS, one barrier per pass.
-// The bisection controller (orbit.cpp:149-231) runs pass by pass: at these
-// sizes a pass is a few LDS reads per lane.  The Hoare partition
-// (partition.cpp:28-60) is the stopper closed form: stopper lists by ballot
-// scans, the crossing K by a 64-way search, K swaps, the final swap.  Every
-// barrier is reached by all waves the same number of times.
+// Exact bin of key v in [lo, hi] among the 255 candidate cuts of the next 8
+// bisection steps, without a table: the walk down the candidate tree
+// computes each midpoint with the cut arithmetic (cell.h:74-76) and counts
+// the candidates at or below v (LE = 0) or below v (LE = 1).  The
+// candidates are sorted, so a node that qualifies takes its whole left
+// subtree with it.
+template <int LE>
+__device__ inline int bin_walk(float v, float lo, float hi)
+{
+    int e = 0;
+#pragma unroll
+    for (int d = 0; d < kHistK; ++d) {
+        const float m = cell_cut(lo, hi);
+        const bool right = LE ? (m < v) : (m <= v);
+        if (right) {
+            e += 1 << (kHistK - 1 - d);
+            lo = m;
+        } else {
+            hi = m;
+        }
+    }
+    return e;
+}
+
+// The LDS capacity of the subtree kernel (keys of one subtree root) per
+// payload: x, y, z (12 B), the two 16-bit stopper lists (4 B), with ids a
+// 16-bit local index (2 B), with weights the weight (4 B) and 64-bit
+// histogram slots; 155-163 KB of gfx950's 160 KiB LDS in each variant.  A
+// level's cells sit within a few keys of n / 2^(l-1), so the plain
+// capacity stays a little above 8192 for a level of 8192-key cells to
+// qualify whole.
+template <bool IDS, bool W>
+struct SubCap {
+    static constexpr int value = W ? 5120 : (IDS ? 7680 : 8832);
+};
+constexpr int kSubBlock = 1024;                 // threads of the subtree kernel
+constexpr int kSubWaves = kSubBlock / 64;       // 16 histogram slots
+
+static int32_t subtree_capacity(const Ctx &c)
+{
+    if (c.weighted) return SubCap<true, true>::value;
+    return c.track_ids ? SubCap<true, false>::value : SubCap<false, false>::value;
+}
+
+// Every remaining level of the subtree under a small cell, in LDS: the cell's
+// x, y, z (and ids' local index, weights) are loaded once, every remaining
+// level is built there, and x, y, z are written back once.
+//
+// A depth's cells are taken up to 16 at a time (a slot each); a slot has
+// 16 / (cells) waves when the depth has fewer than 16 cells.  The bisection
+// controller (orbit.cpp:149-231) runs in rounds of 8 passes, as the chunk
+// path's histogram rounds do: the slot's waves bin the cell's keys over the
+// 255 candidate cuts of its margins into a 256-bin LDS histogram (two 16-bit
+// bins per word; 64-bit bins of weight sums with W), and the slot's first
+// wave prefix-sums it in place and replays up to 8 passes (replay_round),
+// exact by construction.  The Hoare partition (partition.cpp:28-60) is the
+// stopper closed form: stopper lists by ballot scans, the crossing K by a
+// 64-way search, K swaps, the final swap.  Every barrier is reached by all
+// waves the same number of times.
 // IDS: particle ids move with the particles; a 16-bit local index rides along
 // in LDS and the ids are permuted once at the write-back.  W: workload
-// weights ride along in LDS and the passes sum weights instead of counting.
+// weights ride along in LDS and the histograms sum weights.
 template <bool IDS, bool W>
-__global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32_t nCells,
-                                                      orb_cell *heap, uint8_t *exists,
-                                                      uint8_t *sub, uint32_t *ranges, float *x,
-                                                      float *y, float *z, int le_cmp,
-                                                      uint64_t *trace, int32_t *level_passes,
-                                                      int32_t *status, uint32_t *pid,
-                                                      uint32_t *wgt)
+__global__ __launch_bounds__(kSubBlock) void k_b_subtree(const CellRec *recs, int32_t nCells,
+                                                         orb_cell *heap, uint8_t *exists,
+                                                         uint8_t *sub, uint32_t *ranges, float *x,
+                                                         float *y, float *z, int le_cmp,
+                                                         uint64_t *trace, int32_t *level_passes,
+                                                         int32_t *status, uint32_t *pid,
+                                                         uint32_t *wgt)
 {
-    __shared__ float sx[kSmall], sy[kSmall], sz[kSmall];
-    __shared__ uint32_t sw[W ? kSmall : 1];
-    __shared__ uint64_t sWt[4];
+    constexpr int kCap = SubCap<IDS, W>::value;
+    using HistWord = typename std::conditional<W, unsigned long long, uint32_t>::type;
+    constexpr int kHistWords = W ? kBins : kBins / 2;  // W: a bin per word; else two
+    __shared__ float sx[kCap], sy[kCap], sz[kCap];
+    __shared__ uint32_t sw[W ? kCap : 1];
     // the two stopper lists (16-bit), reused as the cell's 32-bit ids at the end
-    __shared__ uint32_t spos_raw[kSmall];
-    __shared__ uint16_t sidx[IDS ? kSmall : 1];
+    __shared__ uint32_t spos_raw[kCap];
+    __shared__ uint16_t sidx[IDS ? kCap : 1];
     uint16_t *const slpos = reinterpret_cast<uint16_t *>(spos_raw);
-    uint16_t *const srpos = slpos + kSmall;
+    uint16_t *const srpos = slpos + kCap;
+    __shared__ HistWord shist[kSubWaves][kHistWords];
     __shared__ SubCell cur[kSmallLeaves], nxt[kSmallLeaves];
-    __shared__ uint64_t sCnt[2][4];
-    __shared__ int32_t sAct[2][4];
-    __shared__ uint32_t sTL[4], sTR[4], sK[4], sI[4];
+    __shared__ float sL[kSubWaves], sR[kSubWaves];
+    __shared__ int32_t sPasses[kSubWaves], sFound[kSubWaves], sAct[kSubWaves];
+    __shared__ uint64_t sWt[kSubWaves];
+    __shared__ uint32_t sTL[kSubWaves], sTR[kSubWaves], sK[kSubWaves], sI[kSubWaves];
     __shared__ int32_t s_ncur, s_nvalid, s_err;
     // the block's pass maxima per level of the build slot in s_lp0, flushed
     // with one atomicMax per level when the slot changes and at the end
@@ -1140,11 +1258,8 @@ __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32
     if (t < 64) s_lpmax[t] = 0;
     if (t == 0) s_lp0 = -1;
     __syncthreads();
-    CellRec rn;  // the next cell's record, loaded one cell ahead
-    if (blockIdx.x < (uint32_t)nCells) rn = recs[blockIdx.x];
     for (int32_t c = blockIdx.x; c < nCells; c += gridDim.x) {
-    const CellRec rr = rn;
-    if (c + (int32_t)gridDim.x < nCells) rn = recs[c + gridDim.x];
+    const CellRec rr = recs[c];
     if (rr.flags != 2) continue;  // uniform per block
     if ((rr.lp - (rr.lp & 63)) != s_lp0) {
         if (t < 64 && s_lp0 >= 0 && s_lpmax[t] > 0) atomicMax(&level_passes[s_lp0 + t], s_lpmax[t]);
@@ -1157,19 +1272,23 @@ __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32
     const int64_t hb = rr.hb;
     const int32_t level = rr.lp & 63, lmax = rr.lmax;
     int32_t *st = status + rr.st;
+    if (n > (uint32_t)kCap) {
+        // the level records hand over at most kCap keys; refuse anything else
+        if (t == 0) report_error(st, level, kErrSplit, rr.id);
+        continue;
+    }
     // the root cell's 13 words, one per lane of wave 0 (no struct copy)
     constexpr int kCellWords = (int)(sizeof(orb_cell) / 4);
     uint32_t root_w = 0u;
     if (t < kCellWords) root_w = reinterpret_cast<const uint32_t *>(heap + hb + rr.id)[t];
     {
-        // every load of the cell in flight before the first LDS store: one
-        // memory latency per cell instead of one per 256 keys
-        constexpr int kIt = (kSmall + kBlock - 1) / kBlock;
+        // every load of the cell in flight before the first LDS store
+        constexpr int kIt = (kCap + kSubBlock - 1) / kSubBlock;
         float vx[kIt], vy[kIt], vz[kIt];
         uint32_t vw[W ? kIt : 1];
 #pragma unroll
         for (int u = 0; u < kIt; ++u) {
-            const uint32_t k = (uint32_t)(t + u * kBlock);
+            const uint32_t k = (uint32_t)(t + u * kSubBlock);
             if (k < n) {
                 vx[u] = x[begin + k];
                 vy[u] = y[begin + k];
@@ -1179,7 +1298,7 @@ __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32
         }
 #pragma unroll
         for (int u = 0; u < kIt; ++u) {
-            const uint32_t k = (uint32_t)(t + u * kBlock);
+            const uint32_t k = (uint32_t)(t + u * kSubBlock);
             if (k < n) {
                 sx[k] = vx[u];
                 sy[k] = vy[u];
@@ -1208,9 +1327,9 @@ __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32
             if (t == 0) report_error(st, l, kErrSplit, rr.id);
             break;
         }
-        const int g = ncur >= 4 ? 1 : 4 / ncur;          // waves per cell
-        const int nslots = ncur >= 4 ? 4 : ncur;
-        const int nbatch = ncur >= 4 ? ncur / 4 : 1;      // ncur is a power of two
+        const int nslots = ncur >= kSubWaves ? kSubWaves : ncur;   // ncur is a power of two
+        const int g = kSubWaves / nslots;                           // waves per cell
+        const int nbatch = ncur >= kSubWaves ? ncur / kSubWaves : 1;
         const int slot = w / g, part = w % g, w0 = slot * g;
         for (int bt = 0; bt < nbatch; ++bt) {
             const int32_t k = bt * nslots + slot;
@@ -1223,8 +1342,7 @@ __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32
                 nxt[2 * k].valid = 0;
                 nxt[2 * k + 1].valid = 0;
             }
-            // ---- counting: the g waves of a cell run the same controller on
-            // the same summed counts
+            // this wave's part of the cell
             const uint32_t seg = ((m + (uint32_t)g - 1u) / (uint32_t)g + 63u) & ~63u;
             uint32_t pb = lb + (uint32_t)part * seg, pe = pb + seg;
             if (pb > le) pb = le;
@@ -1246,43 +1364,83 @@ __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32
             float L = cl.cutMarginLeft, R = cl.cutMarginRight;
             int32_t passes = 0;
             bool found = false;
-            if (g == 1) {
-                while (valid && !found && passes < ORB_MAX_PASSES) {
-                    const float cut = cell_cut(L, R);
-                    const uint64_t cnt = wave_left<W>(key, sw, lb, le, cut, le_cmp, lane);
-                    if (trow && lane == 0) trow[passes] = cnt;
-                    passes++;
-                    const int64_t diff = left_difference<W>(cnt, target);
-                    if (diff < 3 && diff > -3) found = true;
-                    else if (diff > 0) R = cut;
-                    else L = cut;
+            HistWord *hist = shist[slot];
+            for (;;) {
+                const bool act = valid && !found && passes < ORB_MAX_PASSES;
+                if (part == 0 && lane == 0) sAct[slot] = act ? 1 : 0;
+                if (act)
+                    for (int i = part * 64 + lane; i < kHistWords; i += g * 64) hist[i] = 0;
+                __syncthreads();  // zeros and flags in
+                int any = 0;
+                for (int q = 0; q < nslots; ++q) any |= sAct[q];
+                if (!any) break;  // the same words for every wave
+                if (act) {
+                    const float width = R - L;
+                    const float scale = width > 0.0f ? (float)kBins / width : 0.0f;
+                    const float lim = 0.5f - bin_tolerance(L, R, scale);
+                    for (uint32_t p = pb + (uint32_t)lane; p < pe; p += 64u) {
+                        const float v = key[p];
+                        int bin;
+                        if (v < L) bin = 0;                 // below every candidate
+                        else if (v > R) bin = kBins - 1;    // above every candidate
+                        else if (!bin_estimate(v, L, scale, lim, bin))
+                            bin = le_cmp ? bin_walk<1>(v, L, R) : bin_walk<0>(v, L, R);
+                        if constexpr (W) atomicAdd(&hist[bin], (unsigned long long)sw[p]);
+                        else atomicAdd(&hist[bin >> 1], (bin & 1) ? 0x10000u : 1u);
+                    }
                 }
-            } else {
-                for (int ps = 0;; ++ps) {
-                    const bool act = valid && !found && passes < ORB_MAX_PASSES;
-                    const float cut = cell_cut(L, R);
-                    const uint64_t part_cnt = act ? wave_left<W>(key, sw, pb, pe, cut, le_cmp, lane) : 0ull;
+                __syncthreads();  // the slot's histogram is complete
+                if (act && part == 0) {
+                    // prefix sums in place (P[j] = keys in bins below j, j < 256),
+                    // then up to 8 passes of the controller on them
+                    RoundOut ro;
+                    if constexpr (W) {
+                        uint64_t h[4];
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) h[i] = hist[4 * lane + i];
+                        const uint64_t s0 = h[0], s1 = s0 + h[1], s2 = s1 + h[2], s3 = s2 + h[3];
+                        const uint64_t base = wave_inclusive_scan(s3) - s3;
+                        lds_wave_sync();
+                        unsigned long long *P = hist;
+                        if (lane == 0) P[0] = 0ull;
+                        P[4 * lane + 1] = base + s0;
+                        P[4 * lane + 2] = base + s1;
+                        P[4 * lane + 3] = base + s2;
+                        if (lane < 63) P[4 * lane + 4] = base + s3;
+                        lds_wave_sync();
+                        ro = replay_round(reinterpret_cast<const uint64_t *>(P), L, R, target,
+                                          passes, lane, trow);
+                    } else {
+                        const uint32_t w01 = hist[2 * lane], w23 = hist[2 * lane + 1];
+                        const uint32_t s0 = w01 & 0xffffu, s1 = s0 + (w01 >> 16);
+                        const uint32_t s2 = s1 + (w23 & 0xffffu), s3 = s2 + (w23 >> 16);
+                        const uint32_t base = wave_inclusive_scan(s3) - s3;  // <= kCap
+                        lds_wave_sync();
+                        uint16_t *P = reinterpret_cast<uint16_t *>(hist);
+                        if (lane == 0) P[0] = 0;
+                        P[4 * lane + 1] = (uint16_t)(base + s0);
+                        P[4 * lane + 2] = (uint16_t)(base + s1);
+                        P[4 * lane + 3] = (uint16_t)(base + s2);
+                        if (lane < 63) P[4 * lane + 4] = (uint16_t)(base + s3);
+                        lds_wave_sync();
+                        ro = replay_round(static_cast<const uint16_t *>(P), L, R, target, passes,
+                                          lane, trow);
+                    }
                     if (lane == 0) {
-                        sCnt[ps & 1][w] = part_cnt;
-                        sAct[ps & 1][w] = act ? 1 : 0;
+                        sL[slot] = ro.L;
+                        sR[slot] = ro.R;
+                        sPasses[slot] = ro.passes;
+                        sFound[slot] = ro.found ? 1 : 0;
                     }
-                    __syncthreads();  // double-buffered: one barrier per pass
-                    const bool any = (sAct[ps & 1][0] | sAct[ps & 1][1] | sAct[ps & 1][2] |
-                                      sAct[ps & 1][3]) != 0;
-                    if (!any) break;  // the same words for every wave
-                    if (act) {
-                        uint64_t cnt = 0;
-                        for (int q = 0; q < g; ++q) cnt += sCnt[ps & 1][w0 + q];
-                        if (trow && part == 0 && lane == 0) trow[passes] = cnt;
-                        passes++;
-                        const int64_t diff = left_difference<W>(cnt, target);
-                        if (diff < 3 && diff > -3) found = true;
-                        else if (diff > 0) R = cut;
-                        else L = cut;
-                    }
+                }
+                __syncthreads();  // the slot's new margins out
+                if (act) {
+                    L = sL[slot];
+                    R = sR[slot];
+                    passes = sPasses[slot];
+                    found = sFound[slot] != 0;
                 }
             }
-            __syncthreads();
             // ---- partition: stopper lists (each wave one contiguous part)
             const float cut = cell_cut(L, R);
             uint32_t tl = 0, tr = 0;
@@ -1450,7 +1608,7 @@ __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32
             __syncthreads();  // the next batch reuses the position lists and the slots
         }
         const int32_t nn = 2 * ncur;  // <= kSmallLeaves (checked above)
-        for (int32_t k = t; k < nn; k += kBlock) cur[k] = nxt[k];
+        for (int32_t k = t; k < nn; k += kSubBlock) cur[k] = nxt[k];
         __syncthreads();
         if (t == 0) {
             int32_t nv = 0;
@@ -1460,7 +1618,7 @@ __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32
         }
         __syncthreads();
     }
-    for (uint32_t k = (uint32_t)t; k < n; k += kBlock) {
+    for (uint32_t k = (uint32_t)t; k < n; k += kSubBlock) {
         x[begin + k] = sx[k];
         y[begin + k] = sy[k];
         z[begin + k] = sz[k];
@@ -1468,9 +1626,9 @@ __global__ __launch_bounds__(kBlock) void k_b_subtree(const CellRec *recs, int32
     }
     if (IDS) {
         // the stopper lists are free: stage the cell's ids, then permute them
-        for (uint32_t k = (uint32_t)t; k < n; k += kBlock) spos_raw[k] = pid[begin + k];
+        for (uint32_t k = (uint32_t)t; k < n; k += kSubBlock) spos_raw[k] = pid[begin + k];
         __syncthreads();
-        for (uint32_t k = (uint32_t)t; k < n; k += kBlock) pid[begin + k] = spos_raw[sidx[k]];
+        for (uint32_t k = (uint32_t)t; k < n; k += kSubBlock) pid[begin + k] = spos_raw[sidx[k]];
     }
     __syncthreads();  // the next cell reuses the LDS arrays
     }
@@ -1510,7 +1668,8 @@ static int ensure_state(Ctx &c, int32_t levels_mode, bool *complete)
 
 int32_t subtree_max_keys(const Ctx &c)
 {
-    return c.sub_max_keys < 0 ? kSmall : (c.sub_max_keys > kSmall ? kSmall : c.sub_max_keys);
+    const int32_t cap = subtree_capacity(c);
+    return c.sub_max_keys < 0 ? cap : (c.sub_max_keys > cap ? cap : c.sub_max_keys);
 }
 
 static void drop_graph(BatchedState &b)
@@ -1581,14 +1740,16 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
     uint16_t *chunk_tie = chunk_hist + (size_t)(c.max_chunks + 1) * kBins;
     uint16_t *later_hist = chunk_hist + 2 * (size_t)(c.max_chunks + 1) * kBins;
     if (use_subtree) {
-        int32_t sb = 4096;
+        // a block per cell by default: the dispatcher hands the next cell to
+        // the first CU that frees up (cells differ in size)
+        int32_t sb = 1 << 20;
         if (level >= 0 && level < (int32_t)c.tune_sub_blocks.size() &&
             c.tune_sub_blocks[(size_t)level] > 0)
             sb = c.tune_sub_blocks[(size_t)level];
         const unsigned gs = (unsigned)(nCells < sb ? nCells : sb);
         auto subtree = c.track_ids ? (c.weighted ? k_b_subtree<true, true> : k_b_subtree<true, false>)
                                    : (c.weighted ? k_b_subtree<false, true> : k_b_subtree<false, false>);
-        hipLaunchKernelGGL(subtree, dim3(gs), dim3(kBlock), 0, c.stream, c.recs.p, nCells, heap,
+        hipLaunchKernelGGL(subtree, dim3(gs), dim3(kSubBlock), 0, c.stream, c.recs.p, nCells, heap,
                            exists, sub, c.ranges.p, c.x(), c.y(), c.z(), le, trace, level_passes,
                            status, c.ids(), c.weights());
         ORB_HIP(hipGetLastError());

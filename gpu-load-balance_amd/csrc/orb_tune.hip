@@ -169,7 +169,7 @@ int orb_autotune(orb_ctx *ctx, int32_t levels_mode, int32_t repeats, orb_tuning 
     if ((rc = timed_build(c, snap, levels_mode, repeats, lv, &t_default)) < 0) return finish(rc);
 
     // 1. subtree threshold, whole build
-    const int32_t thresholds[] = {-1, 1800, 1100, 0};
+    const int32_t thresholds[] = {-1, 4400, 2200, 0};
     int32_t best_thr = -1;
     double best_t = t_default;
     // a choice other than the default must win by kMargin: a level whose
@@ -218,7 +218,7 @@ int orb_autotune(orb_ctx *ctx, int32_t levels_mode, int32_t repeats, orb_tuning 
     c.tune_chunk_blocks = pick_cap;
 
     // 3. subtree grid per level (cells per block)
-    const int32_t sblocks[] = {0, 1024, 2048, 8192};
+    const int32_t sblocks[] = {0, 256, 1024, 4096};
     std::vector<int32_t> pick_sb;
     if (subtree_max_keys(c) > 0) {
         if ((rc = pick_per_level(c.tune_sub_blocks, sblocks, 4, pick_sb)) < 0) return finish(rc);

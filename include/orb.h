@@ -313,7 +313,8 @@ int orb_enable_graph(orb_ctx *ctx, bool on);
 /* Autotuner for the batched form (no counterpart in the reference).  It
  * picks launch shapes that leave every result unchanged (each variant is
  * exact): which cells go to the LDS subtree kernel (cells of at most
- * subtree_max_keys keys and 64 leaves; 0 = none), and per level the grid
+ * subtree_max_keys keys and 64 leaves; 0 = none; the kernel holds at most
+ * 8832 keys, 7680 with particle ids, 5120 with weights), and per level the grid
  * cap of the chunk kernels (chunks per block = chunks / blocks) and the
  * subtree kernel's grid (cells per block).  orb_autotune times trial
  * builds of the current particles (device events per level, the best of
@@ -322,10 +323,10 @@ int orb_enable_graph(orb_ctx *ctx, bool on);
  * build.  Levels are independent for the per-level shapes because every
  * variant leaves the same particles for the next level.                 */
 typedef struct orb_tuning {
-    int32_t subtree_max_keys;     /* < 0: default (2600)                   */
+    int32_t subtree_max_keys;     /* < 0: default (the kernel capacity) */
     int32_t nlevels;
     int32_t chunk_blocks[64];     /* [level-1], 0 = default                 */
-    int32_t subtree_blocks[64];   /* [level-1], 0 = default (4096)          */
+    int32_t subtree_blocks[64];   /* [level-1], 0 = default (a block/cell)  */
     double default_ms;            /* device ms of a build with the defaults */
     double tuned_ms;              /* device ms of a build with the choices  */
 } orb_tuning;

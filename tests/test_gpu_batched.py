@@ -117,9 +117,9 @@ def test_autotune_keeps_results_exact(orb, oracle):
     ctx.set_particles(x, y, z)
     ctx.enable_trace(True)
     t = ctx.autotune(orb.LEVELS_FULL, repeats=1)
-    assert t["subtree_max_keys"] in (-1, 1800, 1100, 0)
+    assert t["subtree_max_keys"] in (-1, 4400, 2200, 0)
     assert set(t["chunk_blocks"]) <= {0, 16384, 4096, 1024}
-    assert set(t["subtree_blocks"]) <= {0, 1024, 2048, 8192}
+    assert set(t["subtree_blocks"]) <= {0, 256, 1024, 4096}
     assert t["tuned_ms"] <= t["default_ms"]
     gx, _, _ = ctx.get_particles()
     assert np.array_equal(fbits(gx), fbits(x))           # input order restored
@@ -136,8 +136,8 @@ def test_autotune_keeps_results_exact(orb, oracle):
 @pytest.mark.parametrize("shape", [
     {"subtree_max_keys": 0, "nlevels": 12, "chunk_blocks": [1024] * 12,
      "subtree_blocks": [1024] * 12},
-    {"subtree_max_keys": 1100, "nlevels": 12, "chunk_blocks": [4096] * 12,
-     "subtree_blocks": [8192] * 12},
+    {"subtree_max_keys": 2200, "nlevels": 12, "chunk_blocks": [4096] * 12,
+     "subtree_blocks": [256] * 12},
 ])
 def test_explicit_shapes_are_exact(orb, oracle, shape):
     n, d = (1 << 18) + 5, 3000
