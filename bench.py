@@ -297,6 +297,9 @@ def main():
     ap.add_argument("--per-level", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="batched form with the default launch shapes")
+    ap.add_argument("--tune-table", default=None,
+                    help="persisted tuning table: applied when the file exists (no autotuning), "
+                         "else written after autotuning")
     ap.add_argument("--requests", type=int, default=160)
     ap.add_argument("--rate", type=float, default=0.0)
     ap.add_argument("--seed", type=int, default=7)
@@ -336,12 +339,20 @@ def main():
         form = orb.FORM_PLAIN if name == "plain" else orb.FORM_BATCHED
         log(f"[bench] form={name} warmup={args.warmup} steps={args.steps}")
         tuning = None
-        if name == "batched" and not args.no_autotune:
+        if name == "batched" and args.tune_table and os.path.exists(args.tune_table):
+            matched = ctx.set_tune_rows(orb.read_tune_table(args.tune_table))
+            tuning = ctx.tuning()
+            tuning["table"] = {"path": args.tune_table, "levels_matched": matched}
+            log(f"[bench] tuning table {args.tune_table}: {matched} levels matched")
+        elif name == "batched" and not args.no_autotune:
             ctx.restore_particles()
             tuning = ctx.autotune(levels, repeats=2)
             log(f"[bench] autotune: default {tuning['default_ms']:.3f} ms -> tuned "
                 f"{tuning['tuned_ms']:.3f} ms (device time per build), subtree_max_keys="
                 f"{tuning['subtree_max_keys']}")
+            if args.tune_table:
+                orb.write_tune_table(args.tune_table, ctx.tune_rows())
+                log(f"[bench] tuning table written to {args.tune_table}")
         try:
             times, tm, info = run_form(orb, ctx, form, levels, args.steps, args.warmup,
                                        instrumented_steps=(args.steps if name == "plain" else 1))

@@ -341,6 +341,43 @@ __device__ inline void issue_chunk_loads(const float *x, const float *y, const f
             kr, (int)((sb + (uint32_t)lane) * 4u + 256u * (uint32_t)j), 0, 0));
 }
 
+// A chunk record and a cell's margins read through the vector memory path:
+// their waits count with the key loads (in issue order), not with the LDS
+// operations a barrier waits for, so a prefetch does not stall the barrier
+// of the chunk in progress.
+__device__ inline uint4 chunk_record_v(const uint4 *chunk_desc, uint32_t q, uint32_t max_chunks)
+{
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint4 *>(chunk_desc), 0, (int)(max_chunks * 16u), 0x00020000);
+    const uint32_t i = q < max_chunks ? q : max_chunks - 1u;
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)(i * 16u), 0, 0);
+    return make_uint4(v[0], v[1], v[2], v[3]);
+}
+__device__ inline float2 cell_margins_v(const CellRec *recs, uint32_t cell)
+{
+    static_assert(offsetof(CellRec, margin_r) == offsetof(CellRec, margin_l) + 4, "adjacent");
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<CellRec *>(recs), 0, 0x7fffffff, 0x00020000);
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(
+        r, (int)(cell * (uint32_t)sizeof(CellRec) + (uint32_t)offsetof(CellRec, margin_l)), 0, 0);
+    return make_float2(__uint_as_float(v[0]), __uint_as_float(v[1]));
+}
+
+// found, nLeaf, margin_l, margin_r of a cell in one 16-byte vector load.
+__device__ inline uint4 cell_search_v(const CellRec *recs, uint32_t cell)
+{
+    static_assert(offsetof(CellRec, found) % 16 == 0 && sizeof(CellRec) % 16 == 0 &&
+                      offsetof(CellRec, nLeaf) == offsetof(CellRec, found) + 4 &&
+                      offsetof(CellRec, margin_l) == offsetof(CellRec, found) + 8 &&
+                      offsetof(CellRec, margin_r) == offsetof(CellRec, found) + 12,
+                  "found, nLeaf, margin_l, margin_r: one aligned 16-byte group");
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<CellRec *>(recs), 0, 0x7fffffff, 0x00020000);
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(
+        r, (int)(cell * (uint32_t)sizeof(CellRec) + (uint32_t)offsetof(CellRec, found)), 0, 0);
+    return make_uint4(v[0], v[1], v[2], v[3]);
+}
+
 // Waits for the loads into vv (a use the compiler cannot drop).
 template <int kPerWave>
 __device__ inline void settle_loads(const float (&vv)[kPerWave])
@@ -349,101 +386,140 @@ __device__ inline void settle_loads(const float (&vv)[kPerWave])
     for (int j = 0; j < kPerWave; ++j) asm volatile("" ::"v"(vv[j]));
 }
 
+// The first round's candidate table of every searched cell of the level
+// (the 255 cuts of the next 8 bisection steps from the cell's margins): once
+// per cell here instead of once per chunk in the round.  A wave per cell
+// (for_cell_groups), four entries per lane.
+__global__ __launch_bounds__(kBlock) void k_b_first_cands(const CellRec *__restrict__ recs,
+                                                          int32_t nCells, float *__restrict__ cell_cand,
+                                                          int32_t group)
+{
+    const int lane = threadIdx.x & 63;
+    // a cell whose chunks are read: searched, not found, not in a subtree
+    auto active = [&](int32_t c) {
+        const CellRec &r = recs[c];
+        return !r.found && !r.flags && r.axis >= 0 && r.axis <= 2;
+    };
+    for_cell_groups(nCells, group, active, [&](int32_t c) {
+        const float l = recs[c].margin_l, h = recs[c].margin_r;
+        float4 v;
+        v.x = lane ? candidate(4 * lane, l, h) : 0.0f;  // entry 0 is never read
+        v.y = candidate(4 * lane + 1, l, h);
+        v.z = candidate(4 * lane + 2, l, h);
+        v.w = candidate(4 * lane + 3, l, h);
+        reinterpret_cast<float4 *>(cell_cand + (size_t)c * kBins)[lane] = v;
+    });
+}
+
 // First histogram round: one block per chunk bins every key of the chunk
 // over the cell's 255 candidate cuts and writes the chunk's 256-bin row and
 // its tie row.  Wave w takes the quarter [1024w, 1024w + 1024) of the
-// chunk, 64 consecutive keys per load; keys outside the margins are counted
-// with ballots, the rest binned.  Software-pipelined: the block's next
-// chunk (record and keys) is in flight while it bins the current one, so a
-// block waits for memory once, not once per chunk.
+// chunk, 64 consecutive keys per load.  A key's bin is the estimate
+// (v - lo) * 256 / (hi - lo) clamped to [-0.5, 255.5] whenever that estimate
+// lies more than the tolerance from a bin edge (a clamped estimate lies a
+// half bin outside the margins, past every candidate on its side: bin 0 or
+// 255); otherwise the candidate table decides.  Software-pipelined: a
+// block's chunk records are read two chunks ahead and the next chunk's
+// keys, margins and candidate entry one chunk ahead, so no chunk waits for
+// a memory round trip before its binning starts.
 template <int LE>
 __global__ __launch_bounds__(kBlock) void k_b_hist(
     const CellRec *__restrict__ recs, int32_t nCells, const uint32_t *__restrict__ first,
     const uint4 *__restrict__ chunk_desc, uint32_t max_chunks, const float *__restrict__ x,
-    const float *__restrict__ y, const float *__restrict__ z, uint16_t *__restrict__ chunk_hist,
-    uint16_t *__restrict__ chunk_tie)
+    const float *__restrict__ y, const float *__restrict__ z, const float *__restrict__ cell_cand,
+    uint16_t *__restrict__ chunk_hist, uint16_t *__restrict__ chunk_tie)
 {
     constexpr int kW = kBlock / 64;
     constexpr int kPerWave = kMaskWords / kW;  // 16 loads of 64 keys
     __shared__ float cand[kBins + 1];
     __shared__ uint32_t hist[kW][kBins];
     __shared__ uint32_t ties[kBins];  // keys equal to candidate j (rare)
-    __shared__ uint32_t sm[kW];
     const int t = threadIdx.x, lane = t & 63, w = (int)uniform((uint32_t)t >> 6);
     const uint32_t sb = (uint32_t)(w * kPerWave * 64);  // chunk position of the quarter
     const uint32_t total = first[nCells];
+    const uint32_t G = gridDim.x;
+    // the block's chunk q: record d; chunk q + G: record dn
     uint32_t q = blockIdx.x;
-    uint4 d = chunk_record(chunk_desc, q, max_chunks);
-    // one chunk with cur's keys, the next chunk's keys issued into nxt; false
-    // past the level's last chunk
+    uint4 d = chunk_record_v(chunk_desc, q, max_chunks);
+    uint4 dn = chunk_record_v(chunk_desc, q + G, max_chunks);
+    struct Side {  // what a chunk needs besides its keys (lo, hi still per lane)
+        bool live;
+        float2 m;
+        float ce;
+    };
+    auto side_of = [&](uint32_t qq, uint4 dd) __attribute__((always_inline)) -> Side {
+        Side sd;
+        sd.live = qq < total && uniform(dd.w) <= 2u;  // the cell is searched
+        const uint32_t cell = sd.live ? uniform(dd.z) : 0u;
+        sd.m = sd.live ? cell_margins_v(recs, cell) : make_float2(0.0f, 0.0f);
+        sd.ce = sd.live ? cand_entry(cell_cand, cell) : 0.0f;
+        return sd;
+    };
+    Side sc = side_of(q, d);
+    // one chunk with cur's keys; the next chunk's keys into nxt; false past
+    // the level's last chunk
     auto step = [&](float (&cur)[kPerWave], float (&nxt)[kPerWave]) __attribute__((always_inline)) -> bool {
-        const uint32_t b = uniform(d.x), e = uniform(d.y), cell = uniform(d.z);
-        const uint32_t axis = uniform(d.w);
         if (q >= total) return false;
-        const uint32_t qn = q + gridDim.x;
-        const uint4 dn = chunk_record(chunk_desc, qn, max_chunks);
-        if (axis > 2u) {  // uniform: the cell is not searched
-            // the empty loads of this chunk retire first: otherwise the
-            // compiler cannot tell, at the loop's head, which buffer still
-            // has loads in flight and waits for both
-            settle_loads(cur);
-            issue_chunk_loads(x, y, z, qn < total && uniform(dn.w) <= 2u, dn, sb, lane, nxt);
-            q = qn;
-            d = dn;
-            return true;
-        }
-        const float lo = uniform(recs[cell].margin_l), hi = uniform(recs[cell].margin_r);
-        // the candidate table from the margins (no table load: the first
-        // round starts from each cell's fresh margins)
-        store_cands(cand, t ? candidate(t, lo, hi) : -INFINITY);
+        const uint32_t qn = q + G;
+        const uint4 dnn = chunk_record_v(chunk_desc, qn + G, max_chunks);
+        const Side sn = side_of(qn, dn);
+        issue_chunk_loads(x, y, z, sn.live, dn, sb, lane, nxt);
+        if (sc.live) {  // uniform
+            const uint32_t b = uniform(d.x), e = uniform(d.y);
+            const float lo = uniform(sc.m.x), hi = uniform(sc.m.y);
+            store_cands(cand, sc.ce);
 #pragma unroll
-        for (int u = 0; u < kW; ++u) hist[u][t] = 0u;
-        ties[t] = 0u;
-        __syncthreads();
-        issue_chunk_loads(x, y, z, qn < total && uniform(dn.w) <= 2u, dn, sb, lane, nxt);
-        const float width = hi - lo;
-        const float scale = width > 0.0f ? (float)kBins / width : 0.0f;
-        const float lim = 0.5f - bin_tolerance(lo, hi, scale);
-        const uint32_t len = e - b;
-        uint32_t *h = hist[w];
-        uint32_t below = 0, above = 0;  // wave-uniform
-        auto scan = [&](auto full) __attribute__((always_inline)) {
-            static_for<0, kPerWave>([&](auto jc) __attribute__((always_inline)) {
-                constexpr int j = decltype(jc)::value;
-                const uint32_t p0 = sb + 64u * (uint32_t)j;  // chunk position of lane 0
-                const bool ok = decltype(full)::value || p0 + (uint32_t)lane < len;
-                const float vj = cur[j];
-                const bool isB = vj < lo, isA = vj > hi;
-                below += (uint32_t)__popcll(__ballot(ok && isB));
-                above += (uint32_t)__popcll(__ballot(ok && isA));
-                if (ok && !isB && !isA) {
-                    int bin;
-                    if (!bin_estimate(vj, lo, scale, lim, bin)) {  // near a bin edge: the table
-                        int tj;
-                        bin = bin_of<LE>(cand, vj, lo, scale, tj);
-                        if (tj) atomicAdd(&ties[tj], 1u);
+            for (int u = 0; u < kW; ++u) hist[u][t] = 0u;
+            ties[t] = 0u;
+            __syncthreads();
+            const float width = hi - lo;
+            const float scale = width > 0.0f ? (float)kBins / width : 0.0f;
+            // clamping is exact while the estimate stays within a quarter bin
+            // of the truth; past that every key goes to the table
+            const float lim = 0.5f - bin_tolerance(lo, hi, scale);
+            const float limx = lim > 0.25f ? lim : -1.0f;
+            const uint32_t len = e - b;
+            uint32_t *h = hist[w];
+            auto scan = [&](auto full) __attribute__((always_inline)) {
+                static_for<0, kPerWave>([&](auto jc) __attribute__((always_inline)) {
+                    constexpr int j = decltype(jc)::value;
+                    const uint32_t p0 = sb + 64u * (uint32_t)j;  // chunk position of lane 0
+                    const bool ok = decltype(full)::value || p0 + (uint32_t)lane < len;
+                    const float vj = cur[j];
+                    const float ef = __builtin_amdgcn_fmed3f((vj - lo) * scale, -0.5f, 255.5f);
+                    int bin = (int)ef;
+                    const bool slow = !(fabsf(__builtin_amdgcn_fractf(ef) - 0.5f) < limx);
+                    if (__ballot(ok && slow)) {  // wave-uniform; rare: near a bin edge
+                        if (ok && slow) {
+                            int tj;
+                            bin = bin_of<LE>(cand, vj, lo, scale, tj);
+                            if (tj) atomicAdd(&ties[tj], 1u);
+                        }
                     }
-                    atomicAdd(&h[bin], 1u);
-                }
-            });
-        };
-        if (len == (uint32_t)kChunk) scan(std::true_type{});  // uniform
-        else scan(std::false_type{});
-        if (lane == 0) sm[w] = below | (above << 16);  // each <= 1024
-        __syncthreads();
-        const uint32_t ba = sm[0] + sm[1] + sm[2] + sm[3];
-        uint32_t sum = hist[0][t] + hist[1][t] + hist[2][t] + hist[3][t];  // <= 4096
-        if (t == 0) sum += ba & 0xffffu;
-        if (t == kBins - 1) sum += ba >> 16;
-        chunk_hist[(size_t)q * kBins + t] = (uint16_t)sum;
-        chunk_tie[(size_t)q * kBins + t] = (uint16_t)ties[t];
-        __syncthreads();  // the next chunk rewrites cand, hist, ties and sm
+                    if (ok) atomicAdd(&h[bin], 1u);
+                });
+            };
+            if (len == (uint32_t)kChunk) scan(std::true_type{});  // uniform
+            else scan(std::false_type{});
+            __syncthreads();
+            const uint32_t sum = hist[0][t] + hist[1][t] + hist[2][t] + hist[3][t];  // <= 4096
+            chunk_hist[(size_t)q * kBins + t] = (uint16_t)sum;
+            chunk_tie[(size_t)q * kBins + t] = (uint16_t)ties[t];
+            __syncthreads();  // the next chunk rewrites cand, hist and ties
+        } else {
+            // the empty loads of this chunk retire here: otherwise the compiler
+            // cannot tell, at the loop's head, which buffer still has loads in
+            // flight and waits for both
+            settle_loads(cur);
+        }
         q = qn;
         d = dn;
+        dn = dnn;
+        sc = sn;
         return true;
     };
     float va[kPerWave], vb[kPerWave];
-    issue_chunk_loads(x, y, z, q < total && uniform(d.w) <= 2u, d, sb, lane, va);
+    issue_chunk_loads(x, y, z, sc.live, d, sb, lane, va);
     // two steps per trip with the buffers swapped: no register copies (a
     // copy would wait for the loads in flight)
     for (;;) {
@@ -463,8 +539,7 @@ __global__ __launch_bounds__(kBlock) void k_b_hist(
 // 64k + j): keys above hi (masks[q][0]) and keys in the window (masks[q][1]).
 // The cell's final cut lies in [lo, hi], so these decide every stopper bit
 // but those of the packed keys.
-// Software-pipelined as the first round: the block's next chunk (record,
-// candidate entry, keys) is in flight while it scans the current one.
+// Software-pipelined as the first round.
 template <int LE>
 __global__ __launch_bounds__(kBlock) void k_b_hist_window(
     const CellRec *__restrict__ recs, int32_t nCells, const uint32_t *__restrict__ first,
@@ -484,37 +559,47 @@ __global__ __launch_bounds__(kBlock) void k_b_hist_window(
     const uint64_t lanes_below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const uint32_t sb = (uint32_t)(w * kPerWave * 64);  // chunk position of the quarter
     const uint32_t total = first[nCells];
-    // a chunk is scanned when it is within the level and its cell is still
-    // searched (a cell found by the first round is not)
-    auto live_of = [&](uint32_t qq, uint4 dd) __attribute__((always_inline)) -> bool {
-        if (qq >= total || uniform(dd.w) > 2u) return false;
-        return recs[uniform(dd.z)].found == 0;
-    };
+    const uint32_t G = gridDim.x;
+    // records two chunks ahead; the next chunk's keys, cell fields and
+    // candidate entry one chunk ahead (all vector loads, see chunk_record_v).
+    // A chunk is read when it is within the level and its cell searched; it
+    // is scanned when the first round did not find its cell's cut.
     uint32_t q = blockIdx.x;
-    uint4 d = chunk_record(chunk_desc, q, max_chunks);
-    bool live = live_of(q, d);
-    auto step = [&](float (&cur)[kPerWave], float (&nxt)[kPerWave], float &ce_cur,
-                    float &ce_nxt) __attribute__((always_inline)) -> bool {
-        const uint32_t b = uniform(d.x), e = uniform(d.y), cell = uniform(d.z);
+    uint4 d = chunk_record_v(chunk_desc, q, max_chunks);
+    uint4 dn = chunk_record_v(chunk_desc, q + G, max_chunks);
+    struct Side {
+        bool live;
+        uint4 f;  // found, nLeaf, margin_l, margin_r (per lane until used)
+        float ce;
+    };
+    auto side_of = [&](uint32_t qq, uint4 dd) __attribute__((always_inline)) -> Side {
+        Side sd;
+        sd.live = qq < total && uniform(dd.w) <= 2u;
+        const uint32_t cell = sd.live ? uniform(dd.z) : 0u;
+        sd.f = sd.live ? cell_search_v(recs, cell) : make_uint4(1u, 0u, 0u, 0u);
+        sd.ce = sd.live ? cand_entry(cell_cand, cell) : 0.0f;
+        return sd;
+    };
+    Side sc = side_of(q, d);
+    auto step = [&](float (&cur)[kPerWave], float (&nxt)[kPerWave]) __attribute__((always_inline)) -> bool {
         if (q >= total) return false;
-        const uint32_t qn = q + gridDim.x;
-        const uint4 dn = chunk_record(chunk_desc, qn, max_chunks);
-        if (!live) {  // uniform
+        const uint32_t qn = q + G;
+        const uint4 dnn = chunk_record_v(chunk_desc, qn + G, max_chunks);
+        const Side sn = side_of(qn, dn);
+        issue_chunk_loads(x, y, z, sn.live, dn, sb, lane, nxt);
+        if (!sc.live || uniform(sc.f.x) != 0u) {  // uniform: not read, or found
             settle_loads(cur);
-            live = live_of(qn, dn);
-            ce_nxt = live ? cand_entry(cell_cand, uniform(dn.z)) : 0.0f;
-            issue_chunk_loads(x, y, z, live, dn, sb, lane, nxt);
             q = qn;
             d = dn;
+            dn = dnn;
+            sc = sn;
             return true;
         }
-        const float lo = uniform(recs[cell].margin_l), hi = uniform(recs[cell].margin_r);
-        store_cands(cand, ce_cur);
+        const uint32_t b = uniform(d.x), e = uniform(d.y);
+        const float lo = __uint_as_float(uniform(sc.f.z)), hi = __uint_as_float(uniform(sc.f.w));
+        store_cands(cand, sc.ce);
         hist[t] = 0u;
         __syncthreads();
-        const bool live_n = live_of(qn, dn);
-        ce_nxt = live_n ? cand_entry(cell_cand, uniform(dn.z)) : 0.0f;
-        issue_chunk_loads(x, y, z, live_n, dn, sb, lane, nxt);
         const float width = hi - lo;
         const float scale = width > 0.0f ? (float)kBins / width : 0.0f;
         const uint32_t len = e - b;
@@ -583,17 +668,17 @@ __global__ __launch_bounds__(kBlock) void k_b_hist_window(
         if (t == kBins - 1) sum += ba >> 16;
         chunk_hist[(size_t)q * kBins + t] = (uint16_t)sum;
         __syncthreads();  // the next chunk rewrites cand, hist and sm
-        live = live_n;
         q = qn;
         d = dn;
+        dn = dnn;
+        sc = sn;
         return true;
     };
     float va[kPerWave], vb[kPerWave];
-    float ca = live ? cand_entry(cell_cand, uniform(d.z)) : 0.0f, cb = 0.0f;
-    issue_chunk_loads(x, y, z, live, d, sb, lane, va);
+    issue_chunk_loads(x, y, z, sc.live, d, sb, lane, va);
     for (;;) {
-        if (!step(va, vb, ca, cb)) break;
-        if (!step(vb, va, cb, ca)) break;
+        if (!step(va, vb)) break;
+        if (!step(vb, va)) break;
     }
 }
 
@@ -886,14 +971,13 @@ __global__ __launch_bounds__(kBlock) void k_b_replay(CellRec *recs, int32_t nCel
                                                      uint32_t *cell_hist, orb_cell *heap,
                                                      uint64_t *trace, int32_t *level_passes,
                                                      uint8_t *exists, int32_t *status,
-                                                     float *next_cand)
+                                                     float *next_cand, int32_t group)
 {
     __shared__ uint32_t P[4][kBins + 1];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int32_t c = blockIdx.x * 4 + w;
-    if (c >= nCells) return;  // uniform per wave
-    CellRec r = recs[c];
-    if (r.found || r.flags) return;
+    auto active = [&](int32_t c) { return !recs[c].found && !recs[c].flags; };
+    for_cell_groups(nCells, group, active, [&](int32_t c) {
+    const CellRec r = recs[c];
     uint4 *row = reinterpret_cast<uint4 *>(cell_hist + (size_t)c * kBins);
     const uint4 h4 = row[lane];
     row[lane] = make_uint4(0u, 0u, 0u, 0u);  // the next round starts from zero
@@ -908,7 +992,9 @@ __global__ __launch_bounds__(kBlock) void k_b_replay(CellRec *recs, int32_t nCel
         }
         base = inc - s3;
     }
-    // P[j] = sum of bins < j
+    // P[j] = sum of bins < j (the wave's previous cell has read its P)
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
     P[w][4 * lane + 1] = base + s0;
     P[w][4 * lane + 2] = base + s1;
     P[w][4 * lane + 3] = base + s2;
@@ -942,6 +1028,7 @@ __global__ __launch_bounds__(kBlock) void k_b_replay(CellRec *recs, int32_t nCel
         found_enc = (cj == cp || cj == cn) ? 1 : 1 + j;
     }
     finish_round(r, c, ro, found_enc, recs, heap, level_passes, exists, status);
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -1231,7 +1318,7 @@ __global__ __launch_bounds__(kSubBlock) void k_b_subtree(const CellRec *recs, in
                                                          float *y, float *z, int le_cmp,
                                                          uint64_t *trace, int32_t *level_passes,
                                                          int32_t *status, uint32_t *pid,
-                                                         uint32_t *wgt)
+                                                         uint32_t *wgt, int32_t group)
 {
     constexpr int kCap = SubCap<IDS, W>::value;
     using HistWord = typename std::conditional<W, unsigned long long, uint32_t>::type;
@@ -1244,9 +1331,9 @@ __global__ __launch_bounds__(kSubBlock) void k_b_subtree(const CellRec *recs, in
     uint16_t *const slpos = reinterpret_cast<uint16_t *>(spos_raw);
     uint16_t *const srpos = slpos + kCap;
     __shared__ HistWord shist[kSubWaves][kHistWords];
-    __shared__ SubCell cur[kSmallLeaves], nxt[kSmallLeaves];
+    __shared__ SubCell scells[2][kSmallLeaves];  // a depth's cells and the next's
     __shared__ float sL[kSubWaves], sR[kSubWaves];
-    __shared__ int32_t sPasses[kSubWaves], sFound[kSubWaves], sAct[kSubWaves];
+    __shared__ int32_t sPasses[kSubWaves], sFound[kSubWaves];
     __shared__ uint64_t sWt[kSubWaves];
     __shared__ uint32_t sTL[kSubWaves], sTR[kSubWaves], sK[kSubWaves], sI[kSubWaves];
     __shared__ int32_t s_ncur, s_nvalid, s_err;
@@ -1254,13 +1341,32 @@ __global__ __launch_bounds__(kSubBlock) void k_b_subtree(const CellRec *recs, in
     // with one atomicMax per level when the slot changes and at the end
     __shared__ int32_t s_lpmax[64];
     __shared__ int32_t s_lp0;
+    __shared__ uint64_t s_gmask;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     if (t < 64) s_lpmax[t] = 0;
     if (t == 0) s_lp0 = -1;
+    // every wave histogram starts clear and is left clear by the round that
+    // used it
+    for (int i = t; i < kSubWaves * kHistWords; i += kSubBlock) (&shist[0][0])[i] = 0;
     __syncthreads();
-    for (int32_t c = blockIdx.x; c < nCells; c += gridDim.x) {
+    // groups of `group` consecutive cells (a power of two <= 64): wave 0 reads
+    // the group's flags in one load and the block takes the subtree roots
+    // among them, so a level with few roots (or none) costs a few loads per
+    // block, not one per cell
+    for (int32_t g0 = blockIdx.x * group; g0 < nCells; g0 += gridDim.x * group) {
+    if (w == 0) {
+        const int32_t cc = g0 + lane;
+        const bool root = lane < group && cc < nCells && recs[cc].flags == 2;
+        const uint64_t m = __ballot(root);
+        if (lane == 0) s_gmask = m;
+    }
+    __syncthreads();
+    uint64_t gm = s_gmask;  // uniform
+    __syncthreads();        // read before wave 0 rewrites it
+    while (gm) {
+    const int32_t c = g0 + __builtin_ctzll(gm);
+    gm &= gm - 1ull;
     const CellRec rr = recs[c];
-    if (rr.flags != 2) continue;  // uniform per block
     if ((rr.lp - (rr.lp & 63)) != s_lp0) {
         if (t < 64 && s_lp0 >= 0 && s_lpmax[t] > 0) atomicMax(&level_passes[s_lp0 + t], s_lpmax[t]);
         __syncthreads();
@@ -1308,11 +1414,12 @@ __global__ __launch_bounds__(kSubBlock) void k_b_subtree(const CellRec *recs, in
             }
         }
     }
-    if (t < kCellWords) reinterpret_cast<uint32_t *>(&cur[0].cell)[t] = root_w;
+    int cb = 0;  // scells[cb]: this depth's cells
+    if (t < kCellWords) reinterpret_cast<uint32_t *>(&scells[0][0].cell)[t] = root_w;
     if (t == 0) {
-        cur[0].lb = 0;
-        cur[0].le = n;
-        cur[0].valid = 1;
+        scells[0][0].lb = 0;
+        scells[0][0].le = n;
+        scells[0][0].valid = 1;
         s_ncur = 1;
         s_nvalid = 1;
         s_err = 0;
@@ -1333,14 +1440,14 @@ __global__ __launch_bounds__(kSubBlock) void k_b_subtree(const CellRec *recs, in
         const int slot = w / g, part = w % g, w0 = slot * g;
         for (int bt = 0; bt < nbatch; ++bt) {
             const int32_t k = bt * nslots + slot;
-            const SubCell sc = cur[k];
+            const SubCell sc = scells[cb][k];
             const bool valid = sc.valid != 0;
             orb_cell cl = sc.cell;
             const uint32_t lb = sc.lb, le = sc.le, m = le - lb;
             const float *key = cl.cutAxis == 0 ? sx : (cl.cutAxis == 1 ? sy : sz);
             if (part == 0 && lane == 0) {
-                nxt[2 * k].valid = 0;
-                nxt[2 * k + 1].valid = 0;
+                scells[cb ^ 1][2 * k].valid = 0;
+                scells[cb ^ 1][2 * k + 1].valid = 0;
             }
             // this wave's part of the cell
             const uint32_t seg = ((m + (uint32_t)g - 1u) / (uint32_t)g + 63u) & ~63u;
@@ -1354,8 +1461,7 @@ __global__ __launch_bounds__(kSubBlock) void k_b_subtree(const CellRec *recs, in
                 const uint64_t pw = valid ? wave_weight_total(sw, pb, pe, lane) : 0ull;
                 if (lane == 0) sWt[w] = pw;
                 __syncthreads();
-                uint64_t tot = 0;
-                for (int q = 0; q < g; ++q) tot += sWt[w0 + q];
+                const uint64_t tot = wave_sum(lane < g ? (uint64_t)sWt[w0 + lane] : 0ull);
                 target = difference_target_w(tot, cl.nLeafCells);
             } else {
                 target = difference_target(m, cl.nLeafCells);
@@ -1364,44 +1470,60 @@ __global__ __launch_bounds__(kSubBlock) void k_b_subtree(const CellRec *recs, in
             float L = cl.cutMarginLeft, R = cl.cutMarginRight;
             int32_t passes = 0;
             bool found = false;
-            HistWord *hist = shist[slot];
-            for (;;) {
-                const bool act = valid && !found && passes < ORB_MAX_PASSES;
-                if (part == 0 && lane == 0) sAct[slot] = act ? 1 : 0;
-                if (act)
-                    for (int i = part * 64 + lane; i < kHistWords; i += g * 64) hist[i] = 0;
-                __syncthreads();  // zeros and flags in
-                int any = 0;
-                for (int q = 0; q < nslots; ++q) any |= sAct[q];
-                if (!any) break;  // the same words for every wave
+            // each wave bins its part into its own histogram; the slot's first
+            // wave sums the slot's g histograms, clears them and replays.  Two
+            // barriers per round; amask (the same in every wave) holds the
+            // batch's slots still searching.
+            HistWord *const myhist = shist[w];
+            uint32_t amask = (uint32_t)__ballot(lane < nslots && scells[cb][bt * nslots + lane].valid);
+            while (amask) {
+                const bool act = ((amask >> slot) & 1u) != 0u;
                 if (act) {
                     const float width = R - L;
                     const float scale = width > 0.0f ? (float)kBins / width : 0.0f;
                     const float lim = 0.5f - bin_tolerance(L, R, scale);
-                    for (uint32_t p = pb + (uint32_t)lane; p < pe; p += 64u) {
-                        const float v = key[p];
-                        int bin;
-                        if (v < L) bin = 0;                 // below every candidate
-                        else if (v > R) bin = kBins - 1;    // above every candidate
-                        else if (!bin_estimate(v, L, scale, lim, bin))
-                            bin = le_cmp ? bin_walk<1>(v, L, R) : bin_walk<0>(v, L, R);
-                        if constexpr (W) atomicAdd(&hist[bin], (unsigned long long)sw[p]);
-                        else atomicAdd(&hist[bin >> 1], (bin & 1) ? 0x10000u : 1u);
+                    for (uint32_t base = pb; base < pe; base += 256u) {
+                        float v[4];
+                        uint32_t wv[W ? 4 : 1];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {  // four LDS reads in flight
+                            const uint32_t p = base + (uint32_t)(64 * u + lane);
+                            v[u] = p < pe ? key[p] : 0.0f;
+                            if constexpr (W) wv[u] = p < pe ? sw[p] : 0u;
+                        }
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            if (base + (uint32_t)(64 * u + lane) >= pe) continue;
+                            const float vv = v[u];
+                            int bin;
+                            if (vv < L) bin = 0;                 // below every candidate
+                            else if (vv > R) bin = kBins - 1;    // above every candidate
+                            else if (!bin_estimate(vv, L, scale, lim, bin))
+                                bin = le_cmp ? bin_walk<1>(vv, L, R) : bin_walk<0>(vv, L, R);
+                            if constexpr (W) atomicAdd(&myhist[bin], (unsigned long long)wv[u]);
+                            else atomicAdd(&myhist[bin >> 1], (bin & 1) ? 0x10000u : 1u);
+                        }
                     }
                 }
-                __syncthreads();  // the slot's histogram is complete
+                __syncthreads();  // every histogram of the round is complete
                 if (act && part == 0) {
                     // prefix sums in place (P[j] = keys in bins below j, j < 256),
                     // then up to 8 passes of the controller on them
                     RoundOut ro;
                     if constexpr (W) {
-                        uint64_t h[4];
+                        uint64_t h[4] = {0ull, 0ull, 0ull, 0ull};
+                        for (int q = 0; q < g; ++q) {  // all reads first: no store between
 #pragma unroll
-                        for (int i = 0; i < 4; ++i) h[i] = hist[4 * lane + i];
+                            for (int i = 0; i < 4; ++i) h[i] += shist[w0 + q][4 * lane + i];
+                        }
+                        for (int q = 1; q < g; ++q) {
+#pragma unroll
+                            for (int i = 0; i < 4; ++i) shist[w0 + q][4 * lane + i] = 0ull;
+                        }
                         const uint64_t s0 = h[0], s1 = s0 + h[1], s2 = s1 + h[2], s3 = s2 + h[3];
                         const uint64_t base = wave_inclusive_scan(s3) - s3;
                         lds_wave_sync();
-                        unsigned long long *P = hist;
+                        unsigned long long *P = myhist;  // the slot's first wave: w == w0
                         if (lane == 0) P[0] = 0ull;
                         P[4 * lane + 1] = base + s0;
                         P[4 * lane + 2] = base + s1;
@@ -1410,13 +1532,33 @@ __global__ __launch_bounds__(kSubBlock) void k_b_subtree(const CellRec *recs, in
                         lds_wave_sync();
                         ro = replay_round(reinterpret_cast<const uint64_t *>(P), L, R, target,
                                           passes, lane, trow);
+                        lds_wave_sync();
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) P[4 * lane + i] = 0ull;
                     } else {
-                        const uint32_t w01 = hist[2 * lane], w23 = hist[2 * lane + 1];
+                        uint32_t w01 = 0u, w23 = 0u;  // two 16-bit bins per word: sums <= kCap
+                        {
+                            // the g reads (g <= 16) eight at a time in flight, then the clears
+                            const uint2 *hp = reinterpret_cast<const uint2 *>(&shist[w0][2 * lane]);
+                            for (int q0 = 0; q0 < g; q0 += 8) {
+                                uint2 hv[8];
+#pragma unroll
+                                for (int q = 0; q < 8; ++q)
+                                    hv[q] = q0 + q < g ? hp[(q0 + q) * (kHistWords / 2)] : make_uint2(0u, 0u);
+#pragma unroll
+                                for (int q = 0; q < 8; ++q) {
+                                    w01 += hv[q].x;
+                                    w23 += hv[q].y;
+                                }
+                            }
+                            uint2 *hz = reinterpret_cast<uint2 *>(&shist[w0][2 * lane]);
+                            for (int q = 1; q < g; ++q) hz[q * (kHistWords / 2)] = make_uint2(0u, 0u);
+                        }
                         const uint32_t s0 = w01 & 0xffffu, s1 = s0 + (w01 >> 16);
                         const uint32_t s2 = s1 + (w23 & 0xffffu), s3 = s2 + (w23 >> 16);
                         const uint32_t base = wave_inclusive_scan(s3) - s3;  // <= kCap
                         lds_wave_sync();
-                        uint16_t *P = reinterpret_cast<uint16_t *>(hist);
+                        uint16_t *P = reinterpret_cast<uint16_t *>(myhist);
                         if (lane == 0) P[0] = 0;
                         P[4 * lane + 1] = (uint16_t)(base + s0);
                         P[4 * lane + 2] = (uint16_t)(base + s1);
@@ -1425,6 +1567,9 @@ __global__ __launch_bounds__(kSubBlock) void k_b_subtree(const CellRec *recs, in
                         lds_wave_sync();
                         ro = replay_round(static_cast<const uint16_t *>(P), L, R, target, passes,
                                           lane, trow);
+                        lds_wave_sync();
+                        myhist[2 * lane] = 0u;
+                        myhist[2 * lane + 1] = 0u;
                     }
                     if (lane == 0) {
                         sL[slot] = ro.L;
@@ -1433,7 +1578,13 @@ __global__ __launch_bounds__(kSubBlock) void k_b_subtree(const CellRec *recs, in
                         sFound[slot] = ro.found ? 1 : 0;
                     }
                 }
-                __syncthreads();  // the slot's new margins out
+                __syncthreads();  // the slots' new margins out, every histogram clear
+                {
+                    // slots still searching: one read per lane, not one per slot
+                    const bool on = ((amask >> lane) & 1u) != 0u && lane < kSubWaves;
+                    const bool more = on && !sFound[lane] && sPasses[lane] < ORB_MAX_PASSES;
+                    amask = (uint32_t)__ballot(more);
+                }
                 if (act) {
                     L = sL[slot];
                     R = sR[slot];
@@ -1445,12 +1596,19 @@ __global__ __launch_bounds__(kSubBlock) void k_b_subtree(const CellRec *recs, in
             const float cut = cell_cut(L, R);
             uint32_t tl = 0, tr = 0;
             if (valid) {
-                for (uint32_t base = pb; base < pe; base += 64) {
-                    const uint32_t p = base + (uint32_t)lane;
-                    const bool in = p < pe;
-                    const float v = in ? key[p] : 0.0f;
-                    tl += (uint32_t)__popcll(__ballot(in && v >= cut));
-                    tr += (uint32_t)__popcll(__ballot(in && v <= cut));
+                for (uint32_t base = pb; base < pe; base += 256u) {
+                    float v[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const uint32_t p = base + (uint32_t)(64 * u + lane);
+                        v[u] = p < pe ? key[p] : 0.0f;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const bool in = base + (uint32_t)(64 * u + lane) < pe;
+                        tl += (uint32_t)__popcll(__ballot(in && v[u] >= cut));
+                        tr += (uint32_t)__popcll(__ballot(in && v[u] <= cut));
+                    }
                 }
             }
             if (lane == 0) {
@@ -1458,30 +1616,40 @@ __global__ __launch_bounds__(kSubBlock) void k_b_subtree(const CellRec *recs, in
                 sTR[w] = tr;
             }
             __syncthreads();
-            uint32_t oL = 0, oR = 0, TL = 0, TR = 0;
-            for (int q = 0; q < g; ++q) {
-                const uint32_t a = sTL[w0 + q], b2 = sTR[w0 + q];
-                if (q < part) {
-                    oL += a;
-                    oR += b2;
-                }
-                TL += a;
-                TR += b2;
+            // the slot's waves' stopper counts: totals and this wave's offsets,
+            // by a scan across lanes (lane q holds wave w0 + q)
+            uint32_t oL, oR, TL, TR;
+            {
+                const uint32_t a = lane < g ? sTL[w0 + lane] : 0u;
+                const uint32_t b2 = lane < g ? sTR[w0 + lane] : 0u;
+                const uint32_t ia = wave_inclusive_scan(a), ib = wave_inclusive_scan(b2);
+                TL = (uint32_t)__shfl((int)ia, 63);
+                TR = (uint32_t)__shfl((int)ib, 63);
+                oL = (uint32_t)__shfl((int)(ia - a), part);
+                oR = (uint32_t)__shfl((int)(ib - b2), part);
             }
             uint16_t *const Lp = slpos + lb;
             uint16_t *const Rp = srpos + lb;
             if (valid) {
                 const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-                for (uint32_t base = pb; base < pe; base += 64) {
-                    const uint32_t p = base + (uint32_t)lane;
-                    const bool in = p < pe;
-                    const float v = in ? key[p] : 0.0f;
-                    const bool isL = in && v >= cut, isR = in && v <= cut;
-                    const uint64_t mL = __ballot(isL), mR = __ballot(isR);
-                    if (isL) Lp[oL + __popcll(mL & below)] = (uint16_t)(p - lb);
-                    if (isR) Rp[oR + __popcll(mR & below)] = (uint16_t)(p - lb);
-                    oL += (uint32_t)__popcll(mL);
-                    oR += (uint32_t)__popcll(mR);
+                for (uint32_t base = pb; base < pe; base += 256u) {
+                    float v[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const uint32_t p = base + (uint32_t)(64 * u + lane);
+                        v[u] = p < pe ? key[p] : 0.0f;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const uint32_t p = base + (uint32_t)(64 * u + lane);
+                        const bool in = p < pe;
+                        const bool isL = in && v[u] >= cut, isR = in && v[u] <= cut;
+                        const uint64_t mL = __ballot(isL), mR = __ballot(isR);
+                        if (isL) Lp[oL + __popcll(mL & below)] = (uint16_t)(p - lb);
+                        if (isR) Rp[oR + __popcll(mR & below)] = (uint16_t)(p - lb);
+                        oL += (uint32_t)__popcll(mL);
+                        oR += (uint32_t)__popcll(mR);
+                    }
                 }
             }
             __syncthreads();
@@ -1595,27 +1763,28 @@ __global__ __launch_bounds__(kSubBlock) void k_b_subtree(const CellRec *recs, in
                     ranges[2 * gl + 1] = begin + lb + i;
                     ranges[2 * gr] = begin + lb + i;
                     ranges[2 * gr + 1] = begin + le;
-                    nxt[2 * k].cell = lc;
-                    nxt[2 * k].lb = lb;
-                    nxt[2 * k].le = lb + i;
-                    nxt[2 * k].valid = lc.nLeafCells >= 2;
-                    nxt[2 * k + 1].cell = rc;
-                    nxt[2 * k + 1].lb = lb + i;
-                    nxt[2 * k + 1].le = le;
-                    nxt[2 * k + 1].valid = rc.nLeafCells >= 2;
+                    SubCell *nx = scells[cb ^ 1];
+                    nx[2 * k].cell = lc;
+                    nx[2 * k].lb = lb;
+                    nx[2 * k].le = lb + i;
+                    nx[2 * k].valid = lc.nLeafCells >= 2;
+                    nx[2 * k + 1].cell = rc;
+                    nx[2 * k + 1].lb = lb + i;
+                    nx[2 * k + 1].le = le;
+                    nx[2 * k + 1].valid = rc.nLeafCells >= 2;
                 }
             }
             __syncthreads();  // the next batch reuses the position lists and the slots
         }
-        const int32_t nn = 2 * ncur;  // <= kSmallLeaves (checked above)
-        for (int32_t k = t; k < nn; k += kSubBlock) cur[k] = nxt[k];
-        __syncthreads();
-        if (t == 0) {
-            int32_t nv = 0;
-            for (int32_t k = 0; k < nn; ++k) nv += cur[k].valid ? 1 : 0;
-            s_ncur = nn;
-            s_nvalid = nv;
+        const int32_t nn = 2 * ncur;  // <= kSmallLeaves = 64 (checked above)
+        if (w == 0) {
+            const uint64_t vm = __ballot(lane < nn && scells[cb ^ 1][lane].valid);
+            if (lane == 0) {
+                s_ncur = nn;
+                s_nvalid = (int32_t)__popcll(vm);
+            }
         }
+        cb ^= 1;
         __syncthreads();
     }
     for (uint32_t k = (uint32_t)t; k < n; k += kSubBlock) {
@@ -1631,6 +1800,7 @@ __global__ __launch_bounds__(kSubBlock) void k_b_subtree(const CellRec *recs, in
         for (uint32_t k = (uint32_t)t; k < n; k += kSubBlock) pid[begin + k] = spos_raw[sidx[k]];
     }
     __syncthreads();  // the next cell reuses the LDS arrays
+    }
     }
     if (t < 64 && s_lp0 >= 0 && s_lpmax[t] > 0) atomicMax(&level_passes[s_lp0 + t], s_lpmax[t]);
 }
@@ -1717,6 +1887,7 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
     if (c.weighted && (!chunk_hist_w || !cell_hist_w)) return ORB_ERR_STATE;
     const int32_t sub_max = subtree_max_keys(c);
     const bool use_subtree = any_subtree && c.part == ORB_PART_HOARE && sub_max > 0;
+    const int32_t cgroup = cell_group(nCells);
     const int le = c.cmp == ORB_CMP_LE ? 1 : 0;
     c.cur_level = level;  // chunk_grid() and the subtree grid read the level's shapes
     struct LevelScope {
@@ -1740,18 +1911,23 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
     uint16_t *chunk_tie = chunk_hist + (size_t)(c.max_chunks + 1) * kBins;
     uint16_t *later_hist = chunk_hist + 2 * (size_t)(c.max_chunks + 1) * kBins;
     if (use_subtree) {
-        // a block per cell by default: the dispatcher hands the next cell to
-        // the first CU that frees up (cells differ in size)
-        int32_t sb = 1 << 20;
+        // one block fits a CU (LDS): four blocks per CU by default, each
+        // taking groups of consecutive cells; a group holds about
+        // nCells / 1024 cells (1 to 64), so a level of subtree roots spreads
+        // over every CU and a level with none is read in few loads
+        int32_t sb = 1024;
         if (level >= 0 && level < (int32_t)c.tune_sub_blocks.size() &&
             c.tune_sub_blocks[(size_t)level] > 0)
             sb = c.tune_sub_blocks[(size_t)level];
-        const unsigned gs = (unsigned)(nCells < sb ? nCells : sb);
+        int32_t group = 1;
+        while (group < 64 && (int64_t)group * 2 * 1024 <= (int64_t)nCells) group *= 2;
+        const int32_t ngroups = (nCells + group - 1) / group;
+        const unsigned gs = (unsigned)(ngroups < sb ? ngroups : sb);
         auto subtree = c.track_ids ? (c.weighted ? k_b_subtree<true, true> : k_b_subtree<true, false>)
                                    : (c.weighted ? k_b_subtree<false, true> : k_b_subtree<false, false>);
         hipLaunchKernelGGL(subtree, dim3(gs), dim3(kSubBlock), 0, c.stream, c.recs.p, nCells, heap,
                            exists, sub, c.ranges.p, c.x(), c.y(), c.z(), le, trace, level_passes,
-                           status, c.ids(), c.weights());
+                           status, c.ids(), c.weights(), group);
         ORB_HIP(hipGetLastError());
     }
     const unsigned gq = chunk_grid(c);
@@ -1787,6 +1963,13 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
         ORB_TRY(timing_pair(c, tp, kTimePartition));
         return ORB_OK;
     }
+    {
+        const int tb = timing_mark(c);
+        hipLaunchKernelGGL(k_b_first_cands, dim3(cell_group_blocks(nCells, cgroup)), dim3(kBlock), 0,
+                           c.stream, c.recs.p, nCells, c.cell_cand.p, cgroup);
+        ORB_HIP(hipGetLastError());
+        ORB_TRY(timing_pair(c, tb, kTimeCount));
+    }
     for (int r = 0; r < kRounds; ++r) {
         const int tb = timing_mark(c);
         // rounds: all keys, all keys + packing the window, then the packed keys
@@ -1795,7 +1978,7 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
 #define ORB_HIST(L)                                                                              \
     hipLaunchKernelGGL((k_b_hist<L>), dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,     \
                        c.cell_first.p, c.chunk_desc.p, (uint32_t)c.max_chunks, c.x(), c.y(), c.z(), \
-                       chunk_hist, chunk_tie)
+                       c.cell_cand.p, chunk_hist, chunk_tie)
 #define ORB_HIST_WINDOW(L)                                                                       \
     hipLaunchKernelGGL((k_b_hist_window<L>), dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p,      \
                        nCells, c.cell_first.p, c.chunk_desc.p, (uint32_t)c.max_chunks, c.x(), c.y(), \
@@ -1823,9 +2006,9 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
         hipLaunchKernelGGL(k_b_hist_reduce, dim3(gr), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
                            c.cell_first.p, c.chunk_cell.p, mode == 0 ? chunk_hist : later_hist,
                            cell_hist);
-        hipLaunchKernelGGL(k_b_replay, dim3(gw), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
-                           cell_hist, heap, trace, level_passes, exists, status,
-                           mode == 0 ? c.cell_cand.p : nullptr);
+        hipLaunchKernelGGL(k_b_replay, dim3(cell_group_blocks(nCells, cgroup)), dim3(kBlock), 0,
+                           c.stream, c.recs.p, nCells, cell_hist, heap, trace, level_passes, exists,
+                           status, mode == 0 ? c.cell_cand.p : nullptr, cgroup);
         ORB_HIP(hipGetLastError());
     }
     const int tp = timing_mark(c);
@@ -1854,6 +2037,26 @@ static int enqueue_build(Ctx &c, int32_t levels_mode)
                        (uint32_t)c.n, c.status.p);
     ORB_HIP(hipGetLastError());
     const int32_t lmax = levels_mode == ORB_LEVELS_REFERENCE ? b.D - 1 : b.D;
+    // chunks each level expects, for the chunk kernels' grids: cells split
+    // by count, so a level's cell sizes follow from n and d (orb_level_shape,
+    // within the termination rule's +-2 a split); below a level whose every
+    // cell goes to the subtree kernel nothing is left.  Weighted builds split
+    // by weight: unknown.
+    c.level_chunk_hint.assign((size_t)lmax + 2, -1);
+    if (!c.weighted) {
+        const int32_t sub_max = subtree_max_keys(c);
+        bool done = false;
+        for (int32_t l = 1; l <= lmax; ++l) {
+            int64_t lo = 0, hi = 0;
+            const int cells = orb_level_shape(c.n, c.d, l, &lo, &hi);
+            if (cells < 0) break;
+            const int64_t max_leaf = ((int64_t)c.d + (int64_t(1) << (l - 1)) - 1) >> (l - 1);
+            const bool all_sub = b.complete && c.part == ORB_PART_HOARE && sub_max > 0 &&
+                                 hi + 4 * l <= sub_max && max_leaf <= kSmallLeaves;
+            c.level_chunk_hint[(size_t)l] = (done || all_sub) ? 0 : c.n / kChunk + cells + 1;
+            done = done || all_sub;
+        }
+    }
     for (int32_t l = 1; l <= lmax; ++l) {
         const int32_t nCells = b.level_n[(size_t)l];
         if (nCells == 0) continue;

@@ -128,6 +128,27 @@ __device__ inline void report_error(int32_t *status_pair, int32_t level, int32_t
               (unsigned long long)error_word(level, kind, id));
 }
 
+// Per-cell kernels give a wave `group` consecutive cells (a power of two
+// <= 64): the wave reads their records in one load, a ballot keeps the
+// ones that need work, and the wave works through those one by one.  A level
+// of many finished cells then costs few waves.
+template <class Active, class Body>
+__device__ inline void for_cell_groups(int32_t nCells, int32_t group, Active active, Body body)
+{
+    const int lane = threadIdx.x & 63;
+    const int32_t wave = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int32_t nwaves = (int32_t)((gridDim.x * blockDim.x) >> 6);
+    for (int32_t g0 = wave * group; g0 < nCells; g0 += nwaves * group) {
+        const int32_t c = g0 + lane;
+        uint64_t m = __ballot(lane < group && c < nCells && active(c));
+        while (m) {
+            const int j = __builtin_ctzll(m);
+            m &= m - 1ull;
+            body(g0 + j);
+        }
+    }
+}
+
 __device__ inline const float *axis_ptr(const float *x, const float *y, const float *z, int a)
 {
     return a == 0 ? x : (a == 1 ? y : z);

@@ -281,6 +281,21 @@ inline orb_cell cell_root(int32_t d)
     return r;
 }
 
+// Cells per wave of the per-cell kernels (for_cell_groups): one while a
+// level has at most 16384 cells, then enough that the level takes about
+// 16384 waves, at most 64 per wave; and the blocks (four waves) it needs.
+inline int32_t cell_group(int32_t nCells)
+{
+    int32_t g = 1;
+    while (g < 64 && (int64_t)g * 16384 < (int64_t)nCells) g *= 2;
+    return g;
+}
+inline unsigned cell_group_blocks(int32_t nCells, int32_t group)
+{
+    const int64_t waves = ((int64_t)nCells + group - 1) / group;
+    return (unsigned)((waves + 3) / 4);
+}
+
 inline int32_t n_levels(int32_t d)
 {
     // cell.h:65-67 ceil(log2 d), in integers
@@ -508,6 +523,8 @@ struct Ctx {
     int32_t sub_max_keys = -1;           // cells of at most this many keys go to the subtree
     std::vector<int32_t> tune_chunk_blocks;  // [level] grid cap of the chunk kernels
     std::vector<int32_t> tune_sub_blocks;    // [level] grid of the subtree kernel
+    std::vector<double> tune_level_ms;       // [level] device ms of the last autotune's choice
+    std::vector<int64_t> level_chunk_hint;   // [level] chunks a batched level expects (< 0: unknown)
 
     // the buffer that holds the particles now, and the other one
     float *cur() const { return in_tmp ? tmp.p : pos.p; }

@@ -8,6 +8,8 @@
 // chunks of kChunk = 4096 keys (16 KiB), a chunk never spans two cells, and
 // one 256-thread block (four wave64s) handles one chunk.  The chunk table is
 // built on the device from the range map (no host block table).
+#include <algorithm>
+
 #include "orb_device.hpp"
 
 namespace orb {
@@ -180,6 +182,11 @@ unsigned chunk_grid(const Ctx &c)
     if (c.cur_level >= 0 && c.cur_level < (int32_t)c.tune_chunk_blocks.size() &&
         c.tune_chunk_blocks[(size_t)c.cur_level] > 0)
         cap = c.tune_chunk_blocks[(size_t)c.cur_level];
+    else if (c.cur_level >= 0 && c.cur_level < (int32_t)c.level_chunk_hint.size() &&
+             c.level_chunk_hint[(size_t)c.cur_level] >= 0)
+        // a block per chunk the level is expected to hold, at least one per
+        // CU (a level whose cells all sit in finished subtrees has none)
+        cap = std::max<int64_t>(256, std::min<int64_t>(cap, c.level_chunk_hint[(size_t)c.cur_level]));
     return (unsigned)(c.max_chunks < cap ? (c.max_chunks > 0 ? c.max_chunks : 1) : cap);
 }
 
@@ -1049,16 +1056,18 @@ __global__ __launch_bounds__(kBlock) void k_hoare_split(const CellRec *recs, int
                                                         const uint64_t *chunk_pre,
                                                         const uint64_t *masks,
                                                         SplitRec *splits, uint32_t *ranges,
-                                                        int32_t *status)
+                                                        int32_t *status, int32_t group)
 {
     const int lane = threadIdx.x & 63;
-    const int32_t c = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-    if (c >= nCells) return;  // uniform per wave
+    // a cell handled elsewhere (batched form's subtree kernel) gets an empty
+    // split from its own lane
+    auto active = [&](int32_t c) {
+        if (!recs[c].flags) return true;
+        splits[c] = SplitRec{0u, 0u, 0u, 0u};
+        return false;
+    };
+    for_cell_groups(nCells, group, active, [&](int32_t c) {
     const CellRec r = recs[c];
-    if (r.flags) {  // handled elsewhere (batched form's subtree kernel)
-        if (lane == 0) splits[c] = SplitRec{0u, 0u, 0u, 0u};
-        return;
-    }
     const uint32_t f = first[c], nch = first[c + 1] - f;
     const CellOffsets co{chunk_pre + f, chunk_pre[f]};
     const uint32_t TL = co.at(0, nch), TR = co.at(1, nch);
@@ -1112,6 +1121,7 @@ __global__ __launch_bounds__(kBlock) void k_hoare_split(const CellRec *recs, int
     ranges[2 * gl + 1] = r.begin + i;
     ranges[2 * gr] = r.begin + i;
     ranges[2 * gr + 1] = r.end;
+    });
 }
 
 // A swap block owns the pairs whose stopper on the cell's sparser side (the
@@ -1486,9 +1496,10 @@ int launch_hoare(Ctx &c, int32_t nCells, int32_t *status, const HistCounts *hc)
     ORB_HIP(hipGetLastError());
     ORB_TRY(scan_exclusive_u64(c, c.chunk_cnt.p, c.chunk_pre.p, (size_t)c.max_chunks + 1,
                                c.cell_first.p + nCells));
-    unsigned gw = (unsigned)((nCells + 3) / 4);
-    hipLaunchKernelGGL(k_hoare_split, dim3(gw), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
-                       c.cell_first.p, c.chunk_pre.p, c.masks.p, c.splits.p, c.ranges.p, status);
+    const int32_t cgroup = cell_group(nCells);
+    hipLaunchKernelGGL(k_hoare_split, dim3(cell_group_blocks(nCells, cgroup)), dim3(kBlock), 0,
+                       c.stream, c.recs.p, nCells, c.cell_first.p, c.chunk_pre.p, c.masks.p,
+                       c.splits.p, c.ranges.p, status, cgroup);
     ORB_HIP(hipGetLastError());
     if (c.timing && c.swap_pairs.p) {
         const unsigned gs = (unsigned)((nCells + kBlock - 1) / kBlock);

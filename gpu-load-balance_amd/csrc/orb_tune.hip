@@ -8,6 +8,7 @@
 // shapes only change how a level's own work is spread, so each level keeps
 // its fastest one.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <limits>
 #include <vector>
@@ -108,6 +109,7 @@ int orb_set_tuning(orb_ctx *ctx, const orb_tuning *t)
 {
     if (!ctx) return ORB_ERR_ARG;
     Ctx &c = ctx->c;
+    c.tune_level_ms.clear();
     if (!t) {
         c.sub_max_keys = -1;
         c.tune_chunk_blocks.clear();
@@ -133,6 +135,71 @@ int orb_get_tuning(orb_ctx *ctx, orb_tuning *t)
     if (!ctx || !t) return ORB_ERR_ARG;
     tuning_from_ctx(ctx->c, t);
     return ORB_OK;
+}
+
+// The context's choices as keyed rows: one per level of its (n, d).
+int orb_get_tune_rows(orb_ctx *ctx, orb_tune_row *rows, int32_t cap)
+{
+    if (!ctx || cap < 0 || (cap > 0 && !rows)) return ORB_ERR_ARG;
+    const Ctx &c = ctx->c;
+    const int32_t D = n_levels(c.d);
+    int32_t n = 0;
+    for (int32_t l = 1; l <= D && l <= 64; ++l) {
+        int64_t lo = 0, hi = 0;
+        const int cells = orb_level_shape(c.n, c.d, l, &lo, &hi);
+        if (cells < 0) return cells;
+        if (cells == 0) continue;
+        if (n == cap) return ORB_ERR_ARG;
+        orb_tune_row &r = rows[n++];
+        std::memset(&r, 0, sizeof r);
+        r.depth = l;
+        r.log2_min_keys = lo > 0 ? 63 - __builtin_clzll((unsigned long long)lo) : -1;
+        r.log2_max_keys = hi > 0 ? 63 - __builtin_clzll((unsigned long long)hi) : -1;
+        r.chunk_blocks = (size_t)l < c.tune_chunk_blocks.size() ? c.tune_chunk_blocks[(size_t)l] : 0;
+        r.subtree_blocks = (size_t)l < c.tune_sub_blocks.size() ? c.tune_sub_blocks[(size_t)l] : 0;
+        r.subtree_max_keys = c.sub_max_keys;
+        r.ms = (size_t)l < c.tune_level_ms.size() ? c.tune_level_ms[(size_t)l] : 0.0;
+    }
+    return n;
+}
+
+// Each level of the context takes the row with its cell-size key, the same
+// depth first; a level with no such row keeps the defaults.  The subtree
+// threshold comes from the first matched row.
+int orb_set_tune_rows(orb_ctx *ctx, const orb_tune_row *rows, int32_t nrows)
+{
+    if (!ctx || nrows < 0 || (nrows > 0 && !rows)) return ORB_ERR_ARG;
+    for (int32_t i = 0; i < nrows; ++i)
+        if (rows[i].chunk_blocks < 0 || rows[i].subtree_blocks < 0) return ORB_ERR_ARG;
+    Ctx &c = ctx->c;
+    const int32_t D = n_levels(c.d);
+    std::vector<int32_t> cb((size_t)D + 1, 0), sb((size_t)D + 1, 0);
+    int32_t sub_max = -1, matched = 0;
+    for (int32_t l = 1; l <= D && l <= 64; ++l) {
+        int64_t lo = 0, hi = 0;
+        const int cells = orb_level_shape(c.n, c.d, l, &lo, &hi);
+        if (cells < 0) return cells;
+        if (cells == 0) continue;
+        const int32_t klo = lo > 0 ? 63 - __builtin_clzll((unsigned long long)lo) : -1;
+        const int32_t khi = hi > 0 ? 63 - __builtin_clzll((unsigned long long)hi) : -1;
+        const orb_tune_row *best = nullptr;
+        for (int32_t i = 0; i < nrows; ++i) {
+            const orb_tune_row &r = rows[i];
+            if (r.log2_min_keys != klo || r.log2_max_keys != khi) continue;
+            if (!best || (r.depth == l && best->depth != l)) best = &r;
+        }
+        if (!best) continue;
+        cb[(size_t)l] = best->chunk_blocks;
+        sb[(size_t)l] = best->subtree_blocks;
+        if (matched == 0) sub_max = best->subtree_max_keys;
+        matched++;
+    }
+    c.sub_max_keys = sub_max < 0 ? -1 : sub_max;
+    c.tune_chunk_blocks = cb;
+    c.tune_sub_blocks = sb;
+    c.tune_level_ms.clear();
+    batched_drop_graph(c);
+    return matched;
 }
 
 int orb_autotune(orb_ctx *ctx, int32_t levels_mode, int32_t repeats, orb_tuning *out)
@@ -167,6 +234,7 @@ int orb_autotune(orb_ctx *ctx, int32_t levels_mode, int32_t repeats, orb_tuning 
     c.tune_chunk_blocks.assign((size_t)D + 1, 0);
     c.tune_sub_blocks.assign((size_t)D + 1, 0);
     if ((rc = timed_build(c, snap, levels_mode, repeats, lv, &t_default)) < 0) return finish(rc);
+    const std::vector<double> lv_default = lv;
 
     // 1. subtree threshold, whole build
     const int32_t thresholds[] = {-1, 4400, 2200, 0};
@@ -233,7 +301,11 @@ int orb_autotune(orb_ctx *ctx, int32_t levels_mode, int32_t repeats, orb_tuning 
         std::fill(c.tune_chunk_blocks.begin(), c.tune_chunk_blocks.end(), 0);
         std::fill(c.tune_sub_blocks.begin(), c.tune_sub_blocks.end(), 0);
         t_tuned = t_default;
+        lv = lv_default;
     }
+    c.tune_level_ms.assign((size_t)D + 1, 0.0);
+    for (int32_t l = 1; l <= D && l <= 64; ++l)
+        c.tune_level_ms[(size_t)l] = std::isfinite(lv[(size_t)l - 1]) ? lv[(size_t)l - 1] : 0.0;
     if (out) {
         tuning_from_ctx(c, out);
         out->default_ms = t_default;

@@ -89,6 +89,28 @@ class Tuning(C.Structure):
                 ("default_ms", C.c_double), ("tuned_ms", C.c_double)]
 
 
+class TuneRow(C.Structure):
+    _fields_ = [("depth", C.c_int32), ("log2_min_keys", C.c_int32), ("log2_max_keys", C.c_int32),
+                ("chunk_blocks", C.c_int32), ("subtree_blocks", C.c_int32),
+                ("subtree_max_keys", C.c_int32), ("ms", C.c_double)]
+
+
+TUNE_FIELDS = ("depth", "log2_min_keys", "log2_max_keys", "chunk_blocks", "subtree_blocks",
+               "subtree_max_keys", "ms")
+
+
+def _rows_in(rows):
+    arr = (TuneRow * max(1, len(rows)))()
+    for i, r in enumerate(rows):
+        for k in TUNE_FIELDS:
+            setattr(arr[i], k, float(r[k]) if k == "ms" else int(r[k]))
+    return arr
+
+
+def _rows_out(arr, n):
+    return [{k: getattr(arr[i], k) for k in TUNE_FIELDS} for i in range(n)]
+
+
 class TipsyHeader(C.Structure):
     _fields_ = [("time", C.c_double), ("nbodies", C.c_int32), ("ndim", C.c_int32),
                 ("nsph", C.c_int32), ("ndark", C.c_int32), ("nstar", C.c_int32),
@@ -136,6 +158,11 @@ PROTOTYPES = {
     "orb_autotune": (C.c_int, [_vp, _i32, _i32, C.POINTER(Tuning)]),
     "orb_set_tuning": (C.c_int, [_vp, C.POINTER(Tuning)]),
     "orb_get_tuning": (C.c_int, [_vp, C.POINTER(Tuning)]),
+    "orb_level_shape": (C.c_int, [C.c_int64, _i32, _i32, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "orb_tune_table_write": (C.c_int, [C.c_char_p, C.POINTER(TuneRow), _i32]),
+    "orb_tune_table_read": (C.c_int, [C.c_char_p, C.POINTER(TuneRow), _i32]),
+    "orb_get_tune_rows": (C.c_int, [_vp, C.POINTER(TuneRow), _i32]),
+    "orb_set_tune_rows": (C.c_int, [_vp, C.POINTER(TuneRow), _i32]),
     "orb_enable_ids": (C.c_int, [_vp, _b]),
     "orb_get_ids": (C.c_int, [_vp, _vp, _i64]),
     "orb_get_leaves": (C.c_int, [_vp, _vp, _vp, _i64, C.POINTER(C.c_int64)]),
@@ -389,6 +416,17 @@ class Context:
             t.subtree_blocks[i] = int(v)
         _check(lib().orb_set_tuning(self.h, C.byref(t)), "orb_set_tuning")
 
+    def tune_rows(self):
+        """The current choices as rows keyed by level shape (orb_get_tune_rows)."""
+        arr = (TuneRow * 64)()
+        n = _check(lib().orb_get_tune_rows(self.h, arr, 64), "orb_get_tune_rows")
+        return _rows_out(arr, n)
+
+    def set_tune_rows(self, rows):
+        """Applies keyed rows; returns how many levels found a row."""
+        arr = _rows_in(rows)
+        return _check(lib().orb_set_tune_rows(self.h, arr, len(rows)), "orb_set_tune_rows")
+
     # ---- tree export
     def enable_ids(self, on=True):
         _check(lib().orb_enable_ids(self.h, bool(on)), "orb_enable_ids")
@@ -633,6 +671,26 @@ class Server:
         st = ServerStats()
         _check(lib().orb_server_get_stats(self.h, C.byref(st)), "orb_server_get_stats")
         return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+
+def level_shape(n, d, level):
+    """(cells, fewest keys, most keys) of a level of a full build (orb_level_shape)."""
+    lo, hi = C.c_int64(), C.c_int64()
+    cells = _check(lib().orb_level_shape(int(n), int(d), int(level), C.byref(lo), C.byref(hi)),
+                   "orb_level_shape")
+    return cells, lo.value, hi.value
+
+
+def write_tune_table(path, rows):
+    arr = _rows_in(rows)
+    return _check(lib().orb_tune_table_write(os.fsencode(path), arr, len(rows)),
+                  "orb_tune_table_write")
+
+
+def read_tune_table(path, cap=256):
+    arr = (TuneRow * cap)()
+    n = _check(lib().orb_tune_table_read(os.fsencode(path), arr, cap), "orb_tune_table_read")
+    return _rows_out(arr, n)
 
 
 def generate(n, seed=0):

@@ -334,6 +334,34 @@ int orb_autotune(orb_ctx *ctx, int32_t levels_mode, int32_t repeats, orb_tuning 
 int orb_set_tuning(orb_ctx *ctx, const orb_tuning *t);  /* NULL: defaults */
 int orb_get_tuning(orb_ctx *ctx, orb_tuning *t);
 
+/* Persisted tuning, keyed by level shape (no counterpart in the reference).
+ * A row holds the launch shapes chosen for one level, keyed by the level's
+ * depth and its cell-size histogram: floor(log2) of the fewest and of the
+ * most keys a cell of the level holds under the count rule (a cell of m
+ * keys splits at m * ceil(nLeaf / 2) / nLeaf, orbit.cpp:204-205; the leaf
+ * counts split as cell.h:79-80).  orb_get_tune_rows lists the context's current choices
+ * (after orb_autotune: with each level's tuned device ms); the rows go to a
+ * text file and back with orb_tune_table_write / _read; orb_set_tune_rows
+ * applies rows to a context of any n and d: each level takes the row with
+ * its cell-size key (the same depth first), or the defaults.
+ * orb_level_shape gives a level's key: the number of cells of level `level`
+ * (1 = root) in a full build of n keys into d leaves, and the fewest and
+ * most keys expected in one of them.  Host-only: no GPU needed.          */
+typedef struct orb_tune_row {
+    int32_t depth;            /* level, 1 = root                          */
+    int32_t log2_min_keys;    /* floor(log2(fewest keys)), -1 for none    */
+    int32_t log2_max_keys;    /* floor(log2(most keys)), -1 for none      */
+    int32_t chunk_blocks;     /* grid cap of the chunk kernels, 0 = default */
+    int32_t subtree_blocks;   /* grid cap of the subtree kernel, 0 = default */
+    int32_t subtree_max_keys; /* the build's subtree threshold (< 0: default) */
+    double ms;                /* the level's tuned device ms, 0 if unknown */
+} orb_tune_row;
+int orb_level_shape(int64_t n, int32_t d, int32_t level, int64_t *min_keys, int64_t *max_keys);
+int orb_tune_table_write(const char *path, const orb_tune_row *rows, int32_t nrows);
+int orb_tune_table_read(const char *path, orb_tune_row *rows, int32_t cap);  /* rows read */
+int orb_get_tune_rows(orb_ctx *ctx, orb_tune_row *rows, int32_t cap);        /* rows written */
+int orb_set_tune_rows(orb_ctx *ctx, const orb_tune_row *rows, int32_t nrows); /* levels matched */
+
 /* Kernel timing of the last build (device events), milliseconds.         */
 typedef struct orb_timing {
     double count_ms;       /* bisection count/histogram kernels            */

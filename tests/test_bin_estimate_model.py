@@ -148,3 +148,98 @@ def test_a_dropped_term_fails():
     hit = e >= 0
     wrong = ~((cand[e[hit]] < keys[hit]) & (keys[hit] < cand[e[hit] + 1]))
     assert wrong.any()
+
+
+def estimate_clamped(v, lo, scale, tol):
+    """The first round's form: the estimate clamped to [-0.5, 255.5] (a
+    clamped value stands for a key half a bin or more outside the margins),
+    trusted only while lim = 0.5 - tol exceeds a quarter bin; -1 where the
+    table decides."""
+    lim = F32(F32(0.5) - tol)
+    limx = lim if lim > F32(0.25) else F32(-1.0)
+    with np.errstate(over="ignore", invalid="ignore"):
+        ef = ((v - F32(lo)).astype(F32) * scale).astype(F32)
+        ef = np.minimum(np.maximum(ef, F32(-0.5)), F32(255.5)).astype(F32)
+        fr = (ef - np.floor(ef)).astype(F32)
+        ok = np.abs((fr - F32(0.5)).astype(F32)) < limx
+    return np.where(ok, np.trunc(ef).astype(np.int64), -1)
+
+
+def check_clamped(lo, hi, keys):
+    lo, hi = F32(lo), F32(hi)
+    width = F32(hi - lo)
+    with np.errstate(over="ignore", divide="ignore"):
+        scale = F32(F32(KBINS) / width) if width > 0 else F32(0.0)
+    tol = tolerance(lo, hi, scale)
+    cand = candidates(lo, hi)
+    keys = keys[np.isfinite(keys)].astype(F32)
+    e = estimate_clamped(keys, lo, scale, tol)
+    hit = e >= 0
+    eh, vh = e[hit], keys[hit]
+    assert np.all((eh >= 0) & (eh < KBINS)), (lo, hi)
+    assert np.all(cand[eh] < vh), (lo, hi)
+    assert np.all(vh < cand[eh + 1]), (lo, hi)
+    assert np.array_equal(np.searchsorted(cand[1:KBINS], vh, side="right"), eh)
+    assert np.array_equal(np.searchsorted(cand[1:KBINS], vh, side="left"), eh)
+    return hit.mean() if len(keys) else 1.0
+
+
+def outside_keys(lo, hi, rng, n=4000):
+    """Keys below lo and above hi, near the margins and far from them."""
+    lo, hi = F32(lo), F32(hi)
+    w = max(float(hi) - float(lo), 1e-30)
+    with np.errstate(over="ignore", invalid="ignore"):
+        near = np.concatenate([np.nextafter(lo, F32(-np.inf)) - F32(0.0),
+                               np.nextafter(hi, F32(np.inf)) + F32(0.0)], axis=None)
+        d = rng.random(n) * w * np.where(rng.random(n) < 0.5, 0.01, 4.0)
+        far = np.concatenate([float(lo) - d, float(hi) + d]).astype(F32)
+    return np.concatenate([near.astype(F32), far])
+
+
+@pytest.mark.parametrize("lo,hi", MARGINS)
+def test_clamped_shortcut_matches_table(lo, hi):
+    rng = np.random.default_rng(abs(hash((lo, hi, 1))) % (2 ** 32))
+    lo, hi = F32(lo), F32(hi)
+    cand = candidates(lo, hi)
+    keys = np.concatenate([probe_keys(lo, hi, cand, rng), outside_keys(lo, hi, rng)])
+    check_clamped(lo, hi, keys)
+
+
+def test_clamped_random_margins():
+    rng = np.random.default_rng(9)
+    for _ in range(300):
+        e = rng.integers(-140, 127)
+        a = F32(rng.uniform(-2, 2) * 2.0 ** e)
+        w = F32(abs(a) * 2.0 ** -rng.integers(0, 26) + 2.0 ** -149)
+        lo, hi = (a, F32(a + w)) if rng.random() < 0.5 else (F32(a - w), a)
+        if not np.isfinite(lo) or not np.isfinite(hi) or not lo < hi:
+            continue
+        cand = candidates(lo, hi)
+        keys = np.concatenate([probe_keys(lo, hi, cand, rng, 1000), outside_keys(lo, hi, rng, 500)])
+        check_clamped(lo, hi, keys)
+
+
+def test_clamped_unit_box_takes_the_shortcut():
+    rng = np.random.default_rng(2)
+    keys = (rng.random(200000) - 0.5).astype(F32)
+    assert check_clamped(F32(-0.5), F32(0.5), keys) > 0.999
+
+
+def test_clamping_without_the_quarter_bin_guard_fails():
+    """Trusting clamped estimates on a window where the estimate itself is
+    off by more than a quarter bin misbins keys outside the margins: the
+    guard is needed."""
+    lo, hi = F32(1.0e6), F32(1.0e6 + 0.5)       # 8 ulps: tol well above 1/4
+    width = F32(hi - lo)
+    scale = F32(F32(KBINS) / width)
+    cand = candidates(lo, hi)
+    keys = np.concatenate([probe_keys(lo, hi, cand, np.random.default_rng(4)),
+                           outside_keys(lo, hi, np.random.default_rng(4))])
+    with np.errstate(over="ignore", invalid="ignore"):
+        ef = ((keys - lo).astype(F32) * scale).astype(F32)
+        ef = np.minimum(np.maximum(ef, F32(-0.5)), F32(255.5)).astype(F32)
+        fr = (ef - np.floor(ef)).astype(F32)
+    e = np.trunc(ef).astype(np.int64)
+    ok = np.abs(fr - F32(0.5)) < F32(0.45)        # no guard: a fixed limit
+    wrong = ok & ~((cand[e] < keys) & (keys < cand[np.minimum(e + 1, KBINS)]))
+    assert wrong.any()

@@ -133,6 +133,43 @@ def test_autotune_keeps_results_exact(orb, oracle):
     ctx.close()
 
 
+def test_tune_rows_persist_and_apply(orb, oracle, tmp_path):
+    """The autotuner's choices as keyed rows survive the text table and apply
+    to a fresh context: level by level the same shapes, and the build stays
+    exact; a context of other n takes the rows whose cell-size keys it
+    shares."""
+    n, d = 1 << 18, 1 << 12
+    x, y, z = oracle.generate(n)
+    r = oracle.build(x, y, z, d, levels_mode=1)
+    ctx = orb.Context(n, d, generate=False)
+    ctx.set_particles(x, y, z)
+    t = ctx.autotune(orb.LEVELS_FULL, repeats=1)
+    rows = ctx.tune_rows()
+    assert [row["depth"] for row in rows] == list(range(1, len(rows) + 1))
+    assert all(row["ms"] > 0 for row in rows)
+    ctx.close()
+    p = tmp_path / "tune.txt"
+    orb.write_tune_table(p, rows)
+    back = orb.read_tune_table(p)
+    assert back == rows
+    fresh = orb.Context(n, d, generate=False)
+    assert fresh.set_tune_rows(back) == len(rows)
+    got = fresh.tuning()
+    assert got["subtree_max_keys"] == t["subtree_max_keys"]
+    assert got["chunk_blocks"] == t["chunk_blocks"]
+    assert got["subtree_blocks"] == t["subtree_blocks"]
+    fresh.set_particles(x, y, z)
+    info = fresh.build(orb.FORM_BATCHED, orb.LEVELS_FULL)
+    g = {"info": info, "heap": fresh.heap(), "ranges": fresh.ranges(), "exists": fresh.exists(),
+         "xyz": fresh.get_particles(), "pc": None}
+    assert_same_build(g, r, trace=False)
+    fresh.close()
+    other = orb.Context(2 * n, 2 * d, generate=False)  # same cell sizes, one level deeper
+    matched = other.set_tune_rows(back)
+    assert matched == len(rows)                         # every key but the new root's
+    other.close()
+
+
 @pytest.mark.parametrize("shape", [
     {"subtree_max_keys": 0, "nlevels": 12, "chunk_blocks": [1024] * 12,
      "subtree_blocks": [1024] * 12},
