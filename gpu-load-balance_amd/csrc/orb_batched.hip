@@ -341,51 +341,6 @@ __device__ inline void issue_chunk_loads(const float *x, const float *y, const f
             kr, (int)((sb + (uint32_t)lane) * 4u + 256u * (uint32_t)j), 0, 0));
 }
 
-// A chunk record and a cell's margins read through the vector memory path:
-// their waits count with the key loads (in issue order), not with the LDS
-// operations a barrier waits for, so a prefetch does not stall the barrier
-// of the chunk in progress.
-__device__ inline uint4 chunk_record_v(const uint4 *chunk_desc, uint32_t q, uint32_t max_chunks)
-{
-    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<uint4 *>(chunk_desc), 0, (int)(max_chunks * 16u), 0x00020000);
-    const uint32_t i = q < max_chunks ? q : max_chunks - 1u;
-    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)(i * 16u), 0, 0);
-    return make_uint4(v[0], v[1], v[2], v[3]);
-}
-__device__ inline float2 cell_margins_v(const CellRec *recs, uint32_t cell)
-{
-    static_assert(offsetof(CellRec, margin_r) == offsetof(CellRec, margin_l) + 4, "adjacent");
-    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<CellRec *>(recs), 0, 0x7fffffff, 0x00020000);
-    const auto v = __builtin_amdgcn_raw_buffer_load_b64(
-        r, (int)(cell * (uint32_t)sizeof(CellRec) + (uint32_t)offsetof(CellRec, margin_l)), 0, 0);
-    return make_float2(__uint_as_float(v[0]), __uint_as_float(v[1]));
-}
-
-// found, nLeaf, margin_l, margin_r of a cell in one 16-byte vector load.
-__device__ inline uint4 cell_search_v(const CellRec *recs, uint32_t cell)
-{
-    static_assert(offsetof(CellRec, found) % 16 == 0 && sizeof(CellRec) % 16 == 0 &&
-                      offsetof(CellRec, nLeaf) == offsetof(CellRec, found) + 4 &&
-                      offsetof(CellRec, margin_l) == offsetof(CellRec, found) + 8 &&
-                      offsetof(CellRec, margin_r) == offsetof(CellRec, found) + 12,
-                  "found, nLeaf, margin_l, margin_r: one aligned 16-byte group");
-    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<CellRec *>(recs), 0, 0x7fffffff, 0x00020000);
-    const auto v = __builtin_amdgcn_raw_buffer_load_b128(
-        r, (int)(cell * (uint32_t)sizeof(CellRec) + (uint32_t)offsetof(CellRec, found)), 0, 0);
-    return make_uint4(v[0], v[1], v[2], v[3]);
-}
-
-// Waits for the loads into vv (a use the compiler cannot drop).
-template <int kPerWave>
-__device__ inline void settle_loads(const float (&vv)[kPerWave])
-{
-#pragma unroll
-    for (int j = 0; j < kPerWave; ++j) asm volatile("" ::"v"(vv[j]));
-}
-
 // The first round's candidate table of every searched cell of the level
 // (the 255 cuts of the next 8 bisection steps from the cell's margins): once
 // per cell here instead of once per chunk in the round.  A wave per cell
@@ -418,10 +373,9 @@ __global__ __launch_bounds__(kBlock) void k_b_first_cands(const CellRec *__restr
 // (v - lo) * 256 / (hi - lo) clamped to [-0.5, 255.5] whenever that estimate
 // lies more than the tolerance from a bin edge (a clamped estimate lies a
 // half bin outside the margins, past every candidate on its side: bin 0 or
-// 255); otherwise the candidate table decides.  Software-pipelined: a
-// block's chunk records are read two chunks ahead and the next chunk's
-// keys, margins and candidate entry one chunk ahead, so no chunk waits for
-// a memory round trip before its binning starts.
+// 255); otherwise the candidate table decides.  One chunk in flight per
+// block: eight blocks per CU (64 VGPRs) hide the loads better than a block
+// that keeps its next chunk's keys in registers.
 template <int LE>
 __global__ __launch_bounds__(kBlock) void k_b_hist(
     const CellRec *__restrict__ recs, int32_t nCells, const uint32_t *__restrict__ first,
@@ -437,94 +391,56 @@ __global__ __launch_bounds__(kBlock) void k_b_hist(
     const int t = threadIdx.x, lane = t & 63, w = (int)uniform((uint32_t)t >> 6);
     const uint32_t sb = (uint32_t)(w * kPerWave * 64);  // chunk position of the quarter
     const uint32_t total = first[nCells];
-    const uint32_t G = gridDim.x;
-    // the block's chunk q: record d; chunk q + G: record dn
-    uint32_t q = blockIdx.x;
-    uint4 d = chunk_record_v(chunk_desc, q, max_chunks);
-    uint4 dn = chunk_record_v(chunk_desc, q + G, max_chunks);
-    struct Side {  // what a chunk needs besides its keys (lo, hi still per lane)
-        bool live;
-        float2 m;
-        float ce;
-    };
-    auto side_of = [&](uint32_t qq, uint4 dd) __attribute__((always_inline)) -> Side {
-        Side sd;
-        sd.live = qq < total && uniform(dd.w) <= 2u;  // the cell is searched
-        const uint32_t cell = sd.live ? uniform(dd.z) : 0u;
-        sd.m = sd.live ? cell_margins_v(recs, cell) : make_float2(0.0f, 0.0f);
-        sd.ce = sd.live ? cand_entry(cell_cand, cell) : 0.0f;
-        return sd;
-    };
-    Side sc = side_of(q, d);
-    // one chunk with cur's keys; the next chunk's keys into nxt; false past
-    // the level's last chunk
-    auto step = [&](float (&cur)[kPerWave], float (&nxt)[kPerWave]) __attribute__((always_inline)) -> bool {
-        if (q >= total) return false;
-        const uint32_t qn = q + G;
-        const uint4 dnn = chunk_record_v(chunk_desc, qn + G, max_chunks);
-        const Side sn = side_of(qn, dn);
-        issue_chunk_loads(x, y, z, sn.live, dn, sb, lane, nxt);
-        if (sc.live) {  // uniform
-            const uint32_t b = uniform(d.x), e = uniform(d.y);
-            const float lo = uniform(sc.m.x), hi = uniform(sc.m.y);
-            store_cands(cand, sc.ce);
+    for (uint32_t q = blockIdx.x;; q += gridDim.x) {
+        const uint4 d = chunk_record(chunk_desc, q, max_chunks);
+        const uint32_t b = uniform(d.x), e = uniform(d.y), cell = uniform(d.z);
+        const uint32_t axis = uniform(d.w);
+        keep_values(b, e, cell, axis);  // the whole record, in flight with total
+        if (q >= total) break;
+        if (axis > 2u) continue;  // uniform: the cell is not searched
+        const float ce = cand_entry(cell_cand, cell);  // issued before the keys
+        float v[kPerWave];
+        issue_chunk_loads(x, y, z, true, d, sb, lane, v);
+        const float lo = uniform(recs[cell].margin_l), hi = uniform(recs[cell].margin_r);
+        store_cands(cand, ce);
 #pragma unroll
-            for (int u = 0; u < kW; ++u) hist[u][t] = 0u;
-            ties[t] = 0u;
-            __syncthreads();
-            const float width = hi - lo;
-            const float scale = width > 0.0f ? (float)kBins / width : 0.0f;
-            // clamping is exact while the estimate stays within a quarter bin
-            // of the truth; past that every key goes to the table
-            const float lim = 0.5f - bin_tolerance(lo, hi, scale);
-            const float limx = lim > 0.25f ? lim : -1.0f;
-            const uint32_t len = e - b;
-            uint32_t *h = hist[w];
-            auto scan = [&](auto full) __attribute__((always_inline)) {
-                static_for<0, kPerWave>([&](auto jc) __attribute__((always_inline)) {
-                    constexpr int j = decltype(jc)::value;
-                    const uint32_t p0 = sb + 64u * (uint32_t)j;  // chunk position of lane 0
-                    const bool ok = decltype(full)::value || p0 + (uint32_t)lane < len;
-                    const float vj = cur[j];
-                    const float ef = __builtin_amdgcn_fmed3f((vj - lo) * scale, -0.5f, 255.5f);
-                    int bin = (int)ef;
-                    const bool slow = !(fabsf(__builtin_amdgcn_fractf(ef) - 0.5f) < limx);
-                    if (__ballot(ok && slow)) {  // wave-uniform; rare: near a bin edge
-                        if (ok && slow) {
-                            int tj;
-                            bin = bin_of<LE>(cand, vj, lo, scale, tj);
-                            if (tj) atomicAdd(&ties[tj], 1u);
-                        }
+        for (int u = 0; u < kW; ++u) hist[u][t] = 0u;
+        ties[t] = 0u;
+        __syncthreads();
+        const float width = hi - lo;
+        const float scale = width > 0.0f ? (float)kBins / width : 0.0f;
+        // clamping is exact while the estimate stays within a quarter bin of
+        // the truth; past that every key goes to the table
+        const float lim = 0.5f - bin_tolerance(lo, hi, scale);
+        const float limx = lim > 0.25f ? lim : -1.0f;
+        const uint32_t len = e - b;
+        uint32_t *h = hist[w];
+        auto scan = [&](auto full) __attribute__((always_inline)) {
+            static_for<0, kPerWave>([&](auto jc) __attribute__((always_inline)) {
+                constexpr int j = decltype(jc)::value;
+                const uint32_t p0 = sb + 64u * (uint32_t)j;  // chunk position of lane 0
+                const bool ok = decltype(full)::value || p0 + (uint32_t)lane < len;
+                const float vj = v[j];
+                const float ef = __builtin_amdgcn_fmed3f((vj - lo) * scale, -0.5f, 255.5f);
+                int bin = (int)ef;
+                const bool slow = !(fabsf(__builtin_amdgcn_fractf(ef) - 0.5f) < limx);
+                if (__ballot(ok && slow)) {  // wave-uniform; rare: near a bin edge
+                    if (ok && slow) {
+                        int tj;
+                        bin = bin_of<LE>(cand, vj, lo, scale, tj);
+                        if (tj) atomicAdd(&ties[tj], 1u);
                     }
-                    if (ok) atomicAdd(&h[bin], 1u);
-                });
-            };
-            if (len == (uint32_t)kChunk) scan(std::true_type{});  // uniform
-            else scan(std::false_type{});
-            __syncthreads();
-            const uint32_t sum = hist[0][t] + hist[1][t] + hist[2][t] + hist[3][t];  // <= 4096
-            chunk_hist[(size_t)q * kBins + t] = (uint16_t)sum;
-            chunk_tie[(size_t)q * kBins + t] = (uint16_t)ties[t];
-            __syncthreads();  // the next chunk rewrites cand, hist and ties
-        } else {
-            // the empty loads of this chunk retire here: otherwise the compiler
-            // cannot tell, at the loop's head, which buffer still has loads in
-            // flight and waits for both
-            settle_loads(cur);
-        }
-        q = qn;
-        d = dn;
-        dn = dnn;
-        sc = sn;
-        return true;
-    };
-    float va[kPerWave], vb[kPerWave];
-    issue_chunk_loads(x, y, z, sc.live, d, sb, lane, va);
-    // two steps per trip with the buffers swapped: no register copies (a
-    // copy would wait for the loads in flight)
-    for (;;) {
-        if (!step(va, vb)) break;
-        if (!step(vb, va)) break;
+                }
+                if (ok) atomicAdd(&h[bin], 1u);
+            });
+        };
+        if (len == (uint32_t)kChunk) scan(std::true_type{});  // uniform
+        else scan(std::false_type{});
+        __syncthreads();
+        const uint32_t sum = hist[0][t] + hist[1][t] + hist[2][t] + hist[3][t];  // <= 4096
+        chunk_hist[(size_t)q * kBins + t] = (uint16_t)sum;
+        chunk_tie[(size_t)q * kBins + t] = (uint16_t)ties[t];
+        __syncthreads();  // the next chunk rewrites cand, hist and ties
     }
 }
 
@@ -539,9 +455,10 @@ __global__ __launch_bounds__(kBlock) void k_b_hist(
 // 64k + j): keys above hi (masks[q][0]) and keys in the window (masks[q][1]).
 // The cell's final cut lies in [lo, hi], so these decide every stopper bit
 // but those of the packed keys.
-// Software-pipelined as the first round.
+// Eight waves per SIMD (the compiler would keep the load offsets in scalar
+// registers and stop at seven).
 template <int LE>
-__global__ __launch_bounds__(kBlock) void k_b_hist_window(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_b_hist_window(
     const CellRec *__restrict__ recs, int32_t nCells, const uint32_t *__restrict__ first,
     const uint4 *__restrict__ chunk_desc, uint32_t max_chunks, const float *__restrict__ x,
     const float *__restrict__ y, const float *__restrict__ z, const float *__restrict__ cell_cand,
@@ -557,47 +474,31 @@ __global__ __launch_bounds__(kBlock) void k_b_hist_window(
     __shared__ uint64_t words[kW][2][kPerWave];  // each wave's mask words, staged
     const int t = threadIdx.x, lane = t & 63, w = (int)uniform((uint32_t)t >> 6);
     const uint64_t lanes_below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    const uint32_t sb = (uint32_t)(w * kPerWave * 64);  // chunk position of the quarter
     const uint32_t total = first[nCells];
-    const uint32_t G = gridDim.x;
-    // records two chunks ahead; the next chunk's keys, cell fields and
-    // candidate entry one chunk ahead (all vector loads, see chunk_record_v).
-    // A chunk is read when it is within the level and its cell searched; it
-    // is scanned when the first round did not find its cell's cut.
-    uint32_t q = blockIdx.x;
-    uint4 d = chunk_record_v(chunk_desc, q, max_chunks);
-    uint4 dn = chunk_record_v(chunk_desc, q + G, max_chunks);
-    struct Side {
-        bool live;
-        uint4 f;  // found, nLeaf, margin_l, margin_r (per lane until used)
-        float ce;
-    };
-    auto side_of = [&](uint32_t qq, uint4 dd) __attribute__((always_inline)) -> Side {
-        Side sd;
-        sd.live = qq < total && uniform(dd.w) <= 2u;
-        const uint32_t cell = sd.live ? uniform(dd.z) : 0u;
-        sd.f = sd.live ? cell_search_v(recs, cell) : make_uint4(1u, 0u, 0u, 0u);
-        sd.ce = sd.live ? cand_entry(cell_cand, cell) : 0.0f;
-        return sd;
-    };
-    Side sc = side_of(q, d);
-    auto step = [&](float (&cur)[kPerWave], float (&nxt)[kPerWave]) __attribute__((always_inline)) -> bool {
-        if (q >= total) return false;
-        const uint32_t qn = q + G;
-        const uint4 dnn = chunk_record_v(chunk_desc, qn + G, max_chunks);
-        const Side sn = side_of(qn, dn);
-        issue_chunk_loads(x, y, z, sn.live, dn, sb, lane, nxt);
-        if (!sc.live || uniform(sc.f.x) != 0u) {  // uniform: not read, or found
-            settle_loads(cur);
-            q = qn;
-            d = dn;
-            dn = dnn;
-            sc = sn;
-            return true;
-        }
-        const uint32_t b = uniform(d.x), e = uniform(d.y);
-        const float lo = __uint_as_float(uniform(sc.f.z)), hi = __uint_as_float(uniform(sc.f.w));
-        store_cands(cand, sc.ce);
+    for (uint32_t q = blockIdx.x;; q += gridDim.x) {
+        const uint4 d = chunk_record(chunk_desc, q, max_chunks);
+        const uint32_t b = uniform(d.x), e = uniform(d.y), cell = uniform(d.z);
+        const uint32_t axis = uniform(d.w);
+        keep_values(b, e, cell, axis);  // the whole record, in flight with total
+        if (q >= total) break;
+        if (axis > 2u) continue;  // uniform: the cell is not searched
+        if (recs[cell].found) continue;  // uniform: the first round found the cut
+        // the chunk's keys through a range-checked descriptor: loads past the
+        // chunk's end return 0 and are masked out below, with no branches
+        const float *kp = axis_ptr(x, y, z, (int)axis) + b;
+        const float ce = cand_entry(cell_cand, cell);
+        const __amdgpu_buffer_rsrc_t kr =
+            __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(kp), 0, (int)((e - b) * 4u), 0x00020000);
+        const uint32_t sb = (uint32_t)(w * kPerWave * 64);  // chunk position of the quarter
+        float v[kPerWave];
+        // the whole offset in the per-lane operand: the range check covers
+        // it, not the scalar offset
+#pragma unroll
+        for (int j = 0; j < kPerWave; ++j)
+            v[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+                kr, (int)((sb + (uint32_t)lane) * 4u + 256u * (uint32_t)j), 0, 0));
+        const float lo = uniform(recs[cell].margin_l), hi = uniform(recs[cell].margin_r);
+        store_cands(cand, ce);
         hist[t] = 0u;
         __syncthreads();
         const float width = hi - lo;
@@ -615,7 +516,7 @@ __global__ __launch_bounds__(kBlock) void k_b_hist_window(
                     valid = p0 >= len ? 0ull
                                       : (len - p0 >= 64u ? ~0ull : (1ull << (len - p0)) - 1ull);
                 }
-                const float vj = cur[j];
+                const float vj = v[j];
                 const uint64_t mB = __ballot(vj < lo) & valid;
                 const uint64_t mG = __ballot(vj > hi) & valid;
                 const uint64_t mW = valid & ~mB & ~mG;
@@ -668,17 +569,6 @@ __global__ __launch_bounds__(kBlock) void k_b_hist_window(
         if (t == kBins - 1) sum += ba >> 16;
         chunk_hist[(size_t)q * kBins + t] = (uint16_t)sum;
         __syncthreads();  // the next chunk rewrites cand, hist and sm
-        q = qn;
-        d = dn;
-        dn = dnn;
-        sc = sn;
-        return true;
-    };
-    float va[kPerWave], vb[kPerWave];
-    issue_chunk_loads(x, y, z, sc.live, d, sb, lane, va);
-    for (;;) {
-        if (!step(va, vb)) break;
-        if (!step(vb, va)) break;
     }
 }
 
