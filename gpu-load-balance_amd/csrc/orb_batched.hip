@@ -381,7 +381,7 @@ __global__ __launch_bounds__(kBlock) void k_b_hist(
     const CellRec *__restrict__ recs, int32_t nCells, const uint32_t *__restrict__ first,
     const uint4 *__restrict__ chunk_desc, uint32_t max_chunks, const float *__restrict__ x,
     const float *__restrict__ y, const float *__restrict__ z, const float *__restrict__ cell_cand,
-    uint16_t *__restrict__ chunk_hist, uint16_t *__restrict__ chunk_tie)
+    uint16_t *__restrict__ chunk_hist, uint16_t *__restrict__ chunk_tie, int rev)
 {
     constexpr int kW = kBlock / 64;
     constexpr int kPerWave = kMaskWords / kW;  // 16 loads of 64 keys
@@ -391,7 +391,8 @@ __global__ __launch_bounds__(kBlock) void k_b_hist(
     const int t = threadIdx.x, lane = t & 63, w = (int)uniform((uint32_t)t >> 6);
     const uint32_t sb = (uint32_t)(w * kPerWave * 64);  // chunk position of the quarter
     const uint32_t total = first[nCells];
-    for (uint32_t q = blockIdx.x;; q += gridDim.x) {
+    for (uint32_t q0 = blockIdx.x;; q0 += gridDim.x) {
+        const uint32_t q = rev && q0 < total ? total - 1u - q0 : q0;  // from the last (rev)
         const uint4 d = chunk_record(chunk_desc, q, max_chunks);
         const uint32_t b = uniform(d.x), e = uniform(d.y), cell = uniform(d.z);
         const uint32_t axis = uniform(d.w);
@@ -463,7 +464,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
     const uint4 *__restrict__ chunk_desc, uint32_t max_chunks, const float *__restrict__ x,
     const float *__restrict__ y, const float *__restrict__ z, const float *__restrict__ cell_cand,
     uint16_t *__restrict__ chunk_hist, float *__restrict__ pack, uint32_t *__restrict__ comp,
-    uint64_t *__restrict__ masks)
+    uint64_t *__restrict__ masks, int rev)
 {
     constexpr int kW = kBlock / 64;
     constexpr int kPerWave = kMaskWords / kW;  // 16 loads of 64 keys
@@ -475,7 +476,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
     const int t = threadIdx.x, lane = t & 63, w = (int)uniform((uint32_t)t >> 6);
     const uint64_t lanes_below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const uint32_t total = first[nCells];
-    for (uint32_t q = blockIdx.x;; q += gridDim.x) {
+    for (uint32_t q0 = blockIdx.x;; q0 += gridDim.x) {
+        const uint32_t q = rev && q0 < total ? total - 1u - q0 : q0;  // from the last (rev)
         const uint4 d = chunk_record(chunk_desc, q, max_chunks);
         const uint32_t b = uniform(d.x), e = uniform(d.y), cell = uniform(d.z);
         const uint32_t axis = uniform(d.w);
@@ -1778,6 +1780,13 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
     const int32_t sub_max = subtree_max_keys(c);
     const bool use_subtree = any_subtree && c.part == ORB_PART_HOARE && sub_max > 0;
     const int32_t cgroup = cell_group(nCells);
+    // Chunk order: the first round walks the level's chunks from the last, so
+    // it starts with the chunks the previous level's swap wrote last (still in
+    // the memory-side cache), and the window round walks them from the first,
+    // starting with the chunks the first round read last.  (Measured at C2:
+    // -0.26 ms per build against both forward; reversing the swap on
+    // alternate levels as well gained less.)
+    const int rev = 1;
     const int le = c.cmp == ORB_CMP_LE ? 1 : 0;
     c.cur_level = level;  // chunk_grid() and the subtree grid read the level's shapes
     struct LevelScope {
@@ -1868,11 +1877,11 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
 #define ORB_HIST(L)                                                                              \
     hipLaunchKernelGGL((k_b_hist<L>), dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,     \
                        c.cell_first.p, c.chunk_desc.p, (uint32_t)c.max_chunks, c.x(), c.y(), c.z(), \
-                       c.cell_cand.p, chunk_hist, chunk_tie)
+                       c.cell_cand.p, chunk_hist, chunk_tie, rev)
 #define ORB_HIST_WINDOW(L)                                                                       \
     hipLaunchKernelGGL((k_b_hist_window<L>), dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p,      \
                        nCells, c.cell_first.p, c.chunk_desc.p, (uint32_t)c.max_chunks, c.x(), c.y(), \
-                       c.z(), c.cell_cand.p, later_hist, ckeys, c.chunk_comp.p, c.masks.p)
+                       c.z(), c.cell_cand.p, later_hist, ckeys, c.chunk_comp.p, c.masks.p, !rev)
         const unsigned gp = (gq + 3) / 4;  // a wave per chunk
         if (le) {
             if (mode == 0) ORB_HIST(1);

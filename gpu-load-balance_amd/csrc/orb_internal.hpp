@@ -581,7 +581,7 @@ struct HistCounts {
     const uint32_t *comp = nullptr;   // [chunks][8] packed counts per share, keys below
 };
 int launch_hoare(Ctx &c, int32_t nCells, int32_t *status = nullptr,
-                 const HistCounts *hc = nullptr);
+                 const HistCounts *hc = nullptr, int rev = 0);
 int launch_stopper_counts(Ctx &c, int32_t nCells, bool le_only, const HistCounts *hc = nullptr);
 int invalidate_stamps(Ctx &c, int32_t nCells);
 int launch_stable_le(Ctx &c, int32_t nCells, int32_t *status = nullptr, bool full_cover = false,

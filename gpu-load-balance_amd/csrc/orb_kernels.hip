@@ -1280,14 +1280,15 @@ __global__ __launch_bounds__(kBlock) void k_hoare_swap(const CellRec *recs, int3
                                                        const uint64_t *masks,
                                                        const SplitRec *splits,
                                                        float *x, float *y, float *z,
-                                                       uint32_t *pid, uint32_t *w)
+                                                       uint32_t *pid, uint32_t *w, int rev)
 {
     __shared__ uint32_t spos[2][kChunk];  // cell-local positions of L_t, R_t at t - t0
     __shared__ uint32_t sm[kBlock / 64];
     const uint32_t total = first[nCells];
     const int j = threadIdx.x;
     const int wdi = j >> 2, sh = (j & 3) * 16;  // this thread's 16 bits of a chunk mask
-    for (uint32_t q = blockIdx.x; q < total; q += gridDim.x) {
+    for (uint32_t q0 = blockIdx.x; q0 < total; q0 += gridDim.x) {
+        const uint32_t q = rev ? total - 1u - q0 : q0;  // the level's chunks from the last (rev)
         const uint32_t cell = chunk_cell[q];
         const SplitRec s = splits[cell];
         if (s.K == 0u) continue;  // nothing to swap, or reported by k_hoare_split
@@ -1480,7 +1481,7 @@ __global__ void k_sum_swap_pairs(const CellRec *recs, const SplitRec *splits, in
     if ((threadIdx.x & 63) == 0 && k) atomicAdd(acc, k);
 }
 
-int launch_hoare(Ctx &c, int32_t nCells, int32_t *status, const HistCounts *hc)
+int launch_hoare(Ctx &c, int32_t nCells, int32_t *status, const HistCounts *hc, int rev)
 {
     if (!status) status = c.status.p;
     const unsigned gq = chunk_grid(c);
@@ -1518,7 +1519,7 @@ int launch_hoare(Ctx &c, int32_t nCells, int32_t *status, const HistCounts *hc)
     const int ts = timing_mark(c);
     hipLaunchKernelGGL(swap, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
                        c.cell_first.p, c.chunk_cell.p, c.chunk_pre.p, c.masks.p, c.splits.p,
-                       c.x(), c.y(), c.z(), c.ids(), c.weights());
+                       c.x(), c.y(), c.z(), c.ids(), c.weights(), rev);
     ORB_HIP(hipGetLastError());
     ORB_TRY(timing_pair(c, ts, kTimeSwap));
     unsigned gc = (unsigned)((nCells + kBlock - 1) / kBlock);
