@@ -527,6 +527,8 @@ int orb_finalize(orb_ctx *ctx)
     c->cell_cand.release();
     c->splits.release();
     c->chunk_comp.release();
+    c->sel.release();
+    c->sel_lists.release();
     c->status.release();
     c->ids_dev.release();
     c->hcells.release();

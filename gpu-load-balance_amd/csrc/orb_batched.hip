@@ -574,107 +574,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
     }
 }
 
-// Rounds after the packing round: one wave per chunk (the chunk's packed
-// keys are few, so a block per chunk would spend its time on the chain of
-// dependent loads that find the chunk; four independent waves hide it).
-// A chunk with no packed key in the current window writes bins 0 and 255
-// only; otherwise the wave builds its own candidates and LDS histogram.
-template <int LE>
-__global__ __launch_bounds__(kBlock) void k_b_hist_packed(const CellRec *__restrict__ recs,
-                                                          int32_t nCells,
-                                                          const uint32_t *__restrict__ first,
-                                                          const uint4 *__restrict__ chunk_desc,
-                                                          uint32_t max_chunks,
-                                                          uint16_t *__restrict__ chunk_hist,
-                                                          const float *__restrict__ ckeys,
-                                                          const uint32_t *__restrict__ comp)
-{
-    constexpr int kW = kBlock / 64;
-    static_assert(kW == 4, "one packed count per quarter, four in a 16-byte load");
-    __shared__ float cand[kW][kBins + 1];
-    __shared__ uint32_t hist[kW][kBins];
-    const int lane = threadIdx.x & 63, w = (int)uniform((uint32_t)threadIdx.x >> 6);
-    const uint32_t total = first[nCells];
-    for (uint32_t q = blockIdx.x * kW + (uint32_t)w;; q += gridDim.x * kW) {
-        // the record and the packed counts together (with total), then the cell record
-        const uint32_t qc = q < max_chunks ? q : 0u;  // counts slot in bounds
-        const uint4 ncs = *reinterpret_cast<const uint4 *>(comp + 8 * (size_t)qc);
-        const uint32_t nbelow = uniform(comp[8 * (size_t)qc + 4]);
-        const uint4 d = chunk_record(chunk_desc, q, max_chunks);
-        const uint32_t b = uniform(d.x), e = uniform(d.y), cell = uniform(d.z);
-        const uint32_t axis = uniform(d.w);
-        const uint32_t nc4[kW] = {uniform(ncs.x), uniform(ncs.y), uniform(ncs.z), uniform(ncs.w)};
-        keep_values(b, e, cell, axis, nc4[0], nc4[1], nc4[2], nc4[3]);  // one round trip
-        if (q >= total) break;
-        if (axis > 2u) continue;  // uniform: the cell is not searched
-        const CellRec &r = recs[cell];
-        if (r.found) continue;  // uniform per wave
-        const float lo = r.margin_l, hi = r.margin_r;
-        uint32_t below = 0, above = 0;
-        bool any = false;
-        for (int sh = 0; sh < kW; ++sh) {
-            uint32_t off, cnt;
-            wave_share(b, e, sh, off, cnt);
-            const uint32_t nc = nc4[sh];
-            for (uint32_t k = (uint32_t)lane; k < nc; k += 64) {
-                const float v = ckeys[b + off + k];
-                below += v < lo ? 1u : 0u;
-                above += v > hi ? 1u : 0u;
-                any = any || (!(v < lo) && !(v > hi));
-            }
-        }
-        below = wave_sum(below);
-        above = wave_sum(above);
-        below += nbelow;  // keys below the packing window
-        uint16_t *hrow = chunk_hist + (size_t)q * kBins;
-        if (__ballot(any) == 0ull) {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int bin = 4 * lane + i;
-                hrow[bin] = (uint16_t)(bin == 0 ? below : (bin == kBins - 1 ? above : 0u));
-            }
-            continue;
-        }
-        float *cw = cand[w];
-        uint32_t *hw = hist[w];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int j = 4 * lane + i;
-            cw[j] = j == 0 ? -INFINITY : candidate(j, lo, hi);
-            hw[j] = 0u;
-        }
-        if (lane == 0) cw[kBins] = INFINITY;
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-        const float width = hi - lo;
-        const float scale = width > 0.0f ? (float)kBins / width : 0.0f;
-        for (int sh = 0; sh < kW; ++sh) {
-            uint32_t off, cnt;
-            wave_share(b, e, sh, off, cnt);
-            const uint32_t nc = nc4[sh];
-            for (uint32_t k = (uint32_t)lane; k < nc; k += 64) {
-                const float v = ckeys[b + off + k];
-                if (!(v < lo) && !(v > hi)) {
-                    int tj;
-                    atomicAdd(&hw[bin_of<LE>(cw, v, lo, scale, tj)], 1u);
-                }
-            }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int bin = 4 * lane + i;
-            uint32_t v = hw[bin];
-            if (bin == 0) v += below;
-            if (bin == kBins - 1) v += above;
-            hrow[bin] = (uint16_t)v;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-        __builtin_amdgcn_wave_barrier();  // the rows are rewritten for the next chunk
-    }
-}
-
 // Sums chunk histograms into cell histograms: block g takes chunks
 // [g*64, g*64+64), thread t owns bin t; all 64 row loads are in flight at
 // once, and one atomic per (block, cell, bin) flushes a run.
@@ -817,6 +716,22 @@ __device__ inline RoundOut replay_round(const T *P, float lo, float hi, float ta
     return RoundOut{L, R, passes0 + used, found, found ? stop : 0};
 }
 
+// The record's found field after a round that found its cut at candidate
+// ro.node of [lo, hi]: 1 + node when that candidate differs from its
+// neighbours (the stopper counts can then come from the round's histograms;
+// equal candidates -- a window a few ulps wide -- would split the keys equal
+// to the cut over several tie bins), else 1 (found, the keys are read); 0
+// when not found.
+__device__ inline int32_t found_encoding(const RoundOut &ro, float lo, float hi)
+{
+    if (!ro.found) return 0;
+    const int j = ro.node;
+    const float cj = candidate(j, lo, hi);
+    const float cp = j > 1 ? candidate(j - 1, lo, hi) : -INFINITY;
+    const float cn = j < kBins - 1 ? candidate(j + 1, lo, hi) : INFINITY;
+    return (cj == cp || cj == cn) ? 1 : 1 + j;
+}
+
 // Lane 0 of the replaying wave: the round's margins into the cell's record
 // and heap cell; a finished cell (cut found, or the pass cap with the last
 // midpoint) is split at once (orbit.cpp:235-250).
@@ -857,13 +772,36 @@ __device__ inline void finish_round(CellRec r, int32_t c, const RoundOut &ro, in
 }
 
 
+// Rounds 3 and 4 gather a searching cell's keys into up to 64 lists (see
+// k_b_select): list r covers a contiguous range of the cell's chunks.
+constexpr int kSelChunks = 16;  // chunks per wave: a lane per chunk quarter
+constexpr int kSelLists = 64;
+struct SelLists {
+    uint32_t f, g0, S, R;  // first chunk, first group, groups per list, lists
+    __device__ SelLists(uint32_t first_chunk, uint32_t nch)
+    {
+        f = first_chunk;
+        g0 = f / kSelChunks;
+        const uint32_t ng = nch ? (f + nch - 1u) / kSelChunks - g0 + 1u : 0u;
+        S = (ng + kSelLists - 1u) / kSelLists;
+        R = S ? (ng + S - 1u) / S : 0u;
+    }
+    __device__ uint32_t list_of_group(uint32_t g) const { return (g - g0) / S; }
+    __device__ uint32_t start_chunk(uint32_t r) const
+    {
+        const uint32_t q = (g0 + r * S) * kSelChunks;
+        return q > f ? q : f;
+    }
+};
+
 // One wave per cell: prefix sums of the cell histogram in LDS, then the
 // wave replays up to 8 passes of the bisection controller.
 __global__ __launch_bounds__(kBlock) void k_b_replay(CellRec *recs, int32_t nCells,
                                                      uint32_t *cell_hist, orb_cell *heap,
                                                      uint64_t *trace, int32_t *level_passes,
                                                      uint8_t *exists, int32_t *status,
-                                                     float *next_cand, int32_t group)
+                                                     float *next_cand, const uint32_t *first,
+                                                     unsigned long long *sel_clear, int32_t group)
 {
     __shared__ uint32_t P[4][kBins + 1];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -906,20 +844,261 @@ __global__ __launch_bounds__(kBlock) void k_b_replay(CellRec *recs, int32_t nCel
             next_cand[(size_t)c * kBins + j] = j ? candidate(j, ro.L, ro.R) : 0.0f;
         }
     }
-    if (lane != 0) return;
-    // The stopper counts can come from this round's histograms when the cut
-    // is a candidate distinct from its neighbours (equal candidates -- a
-    // window a few ulps wide -- would split the keys equal to the cut over
-    // several tie bins): encode 1 + node, else 1 (found, keys are read).
-    int32_t found_enc = 0;
-    if (ro.found) {
-        const int j = ro.node;
-        const float cj = candidate(j, r.margin_l, r.margin_r);
-        const float cp = j > 1 ? candidate(j - 1, r.margin_l, r.margin_r) : -INFINITY;
-        const float cn = j < kBins - 1 ? candidate(j + 1, r.margin_l, r.margin_r) : INFINITY;
-        found_enc = (cj == cp || cj == cn) ? 1 : 1 + j;
+    if (sel_clear) {  // rounds 3 and 4 gather from zero
+        const uint32_t f = first[c];
+        const SelLists sl(f, first[c + 1] - f);
+        if ((uint32_t)lane < sl.R) sel_clear[sl.start_chunk((uint32_t)lane)] = 0ull;
     }
-    finish_round(r, c, ro, found_enc, recs, heap, level_passes, exists, status);
+    if (lane != 0) return;
+    finish_round(r, c, ro, found_encoding(ro, r.margin_l, r.margin_r), recs, heap, level_passes,
+                 exists, status);
+    });
+}
+
+// Rounds 3 and 4 read only the keys that the window round packed and that
+// lie inside the margins after round 2 (every later candidate lies inside
+// them).  k_b_select gathers them, per searching cell, into up to 64 lists
+// in `sel`, each over a contiguous range of the cell's chunks (whole
+// kSelChunks-chunk groups, one wave's share): the list fills positions from
+// its range's first particle on, which it cannot outgrow, and its counter
+// (kept at the range's first chunk) takes the atomics of few waves even in
+// a cell of the whole level.
+
+// Each run of one cell in a wave's chunks counts its packed keys inside the
+// cell's margins and below them (the count below also takes the keys below
+// the window round's margins, which that round counted per chunk), then
+// reserves slots in its list with one 64-bit atomic (keys | below << 32)
+// and writes the keys there.  The wave reads the quarters' packed keys laid
+// end to end, four loads per lane in flight; a wave's keys are usually
+// fewer than 256 and stay in registers between counting and writing.
+constexpr int kSelBatch = 4;
+__global__ __launch_bounds__(kBlock) void k_b_select(const CellRec *__restrict__ recs,
+                                                     int32_t nCells,
+                                                     const uint32_t *__restrict__ first,
+                                                     const uint4 *__restrict__ chunk_desc,
+                                                     uint32_t max_chunks,
+                                                     const float *__restrict__ ckeys,
+                                                     const uint32_t *__restrict__ comp,
+                                                     float *__restrict__ sel,
+                                                     unsigned long long *__restrict__ sel_lists)
+{
+    constexpr int kW = kBlock / 64;
+    static_assert(kSelChunks * 4 == 64, "a lane per chunk quarter");
+    __shared__ uint32_t s_below[kW][64], s_cnt[kW][64];  // per run of one cell
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, sh = lane & 3;
+    const uint32_t wave = blockIdx.x * (uint32_t)kW + (uint32_t)w;
+    const uint32_t nwaves = gridDim.x * (uint32_t)kW;
+    const uint32_t total = first[nCells];
+    for (uint32_t q0 = wave * kSelChunks; q0 < total; q0 += nwaves * kSelChunks) {
+        const uint32_t q = q0 + (uint32_t)(lane >> 2);
+        const bool inq = q < total;
+        const uint32_t qr = inq ? q : 0u;  // in-bounds record and counts
+        const uint4 d = chunk_record(chunk_desc, qr, max_chunks);
+        const uint32_t nc0 = comp[8 * (size_t)qr + sh], nb0 = comp[8 * (size_t)qr + 4];
+        const uint32_t cell = inq ? d.z : 0xffffffffu;  // runs end at the table's end
+        bool live = inq && d.w <= 2u;
+        float lo = 0.0f, hi = 0.0f;
+        uint32_t begin = 0, f = 0, nch = 0;
+        if (live) {
+            const CellRec &r = recs[cell];
+            f = first[cell];
+            nch = first[cell + 1] - f;
+            live = !r.found;
+            lo = r.margin_l;
+            hi = r.margin_r;
+            begin = r.begin;
+        }
+        uint32_t off = 0, cnt = 0;
+        if (inq) wave_share(d.x, d.y, sh, off, cnt);
+        const uint32_t nc = live ? nc0 : 0u;
+        const uint32_t kbase = d.x + off;  // the quarter's first packed key
+        const uint32_t prev = (uint32_t)__shfl_up((int)cell, 1, 64);
+        const uint64_t heads = __ballot(lane == 0 || prev != cell);
+        const bool head = (heads >> lane) & 1ull;
+        const int run = __popcll(heads & ((2ull << lane) - 1ull)) - 1;
+        s_below[w][lane] = 0u;
+        s_cnt[w][lane] = 0u;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        if (live && sh == 0 && nb0) atomicAdd(&s_below[w][run], nb0);
+        const uint32_t incl = wave_inclusive_scan(nc), ex = incl - nc;
+        const uint32_t T = (uint32_t)__shfl((int)incl, 63, 64);
+        // key t of the wave: quarter j = the last with ex_j <= t (it holds t)
+        float v[kSelBatch], lj[kSelBatch], hj[kSelBatch];
+        int rj[kSelBatch];
+        auto load_batch = [&](uint32_t t0) __attribute__((always_inline)) {
+#pragma unroll
+            for (int u = 0; u < kSelBatch; ++u) {
+                const uint32_t t = t0 + 64u * (uint32_t)u + (uint32_t)lane;
+                int j = 0;
+#pragma unroll
+                for (int st = 32; st > 0; st >>= 1)
+                    if ((uint32_t)__shfl((int)ex, j + st, 64) <= t) j += st;
+                const uint32_t at =
+                    (uint32_t)__shfl((int)kbase, j, 64) + (t - (uint32_t)__shfl((int)ex, j, 64));
+                lj[u] = __shfl(lo, j, 64);
+                hj[u] = __shfl(hi, j, 64);
+                rj[u] = __shfl(run, j, 64);
+                v[u] = t < T ? ckeys[at] : 0.0f;
+            }
+        };
+        constexpr uint32_t kStep = 64u * kSelBatch;
+        for (uint32_t t0 = 0; t0 < T; t0 += kStep) {
+            load_batch(t0);
+#pragma unroll
+            for (int u = 0; u < kSelBatch; ++u) {
+                if (t0 + 64u * (uint32_t)u + (uint32_t)lane < T) {
+                    if (v[u] < lj[u]) atomicAdd(&s_below[w][rj[u]], 1u);
+                    else if (!(v[u] > hj[u])) atomicAdd(&s_cnt[w][rj[u]], 1u);
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        uint32_t slot = 0;
+        if (head && live) {
+            const uint32_t ns = s_cnt[w][run], nb = s_below[w][run];
+            const SelLists sl(f, nch);
+            const uint32_t qs = sl.start_chunk(sl.list_of_group(q0 / kSelChunks));
+            const unsigned long long old =
+                ns || nb ? atomicAdd(&sel_lists[qs], (unsigned long long)ns |
+                                                         ((unsigned long long)nb << 32))
+                         : 0ull;
+            slot = begin + (qs - f) * (uint32_t)kChunk + (uint32_t)old;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        if (head) s_cnt[w][run] = slot;  // the run's next free slot in sel
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t t0 = 0; t0 < T; t0 += kStep) {
+            if (T > kStep) load_batch(t0);  // uniform: one batch is still in registers
+#pragma unroll
+            for (int u = 0; u < kSelBatch; ++u) {
+                if (t0 + 64u * (uint32_t)u + (uint32_t)lane < T && !(v[u] < lj[u]) &&
+                    !(v[u] > hj[u]))
+                    sel[atomicAdd(&s_cnt[w][rj[u]], 1u)] = v[u];
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();  // the run tables are cleared for the next chunks
+    }
+}
+
+// Rounds 3 and 4 of each searching cell from its gathered keys, a wave per
+// cell: lane r reads list r's count (its counts below add up to the cell's
+// count below the margins), the keys of all lists are read laid end to end
+// and binned against the round's candidates in LDS (keys below the margins
+// add to bin 0, as in the chunk rounds; keys above them are never read by
+// the replay), and the round is replayed.  A cell that neither found its
+// cut nor reached the pass cap runs round 4 over the same keys with the
+// narrowed margins (the keys that now fall below them join the count
+// below).  The record, heap cell and split are written as after any round.
+constexpr int kFinBatch = 8;  // gathered keys per lane in flight
+template <int LE>
+__global__ __launch_bounds__(kBlock) void k_b_finish(CellRec *recs, int32_t nCells,
+                                                     const uint32_t *__restrict__ first,
+                                                     const float *__restrict__ sel,
+                                                     const unsigned long long *__restrict__ sel_lists,
+                                                     orb_cell *heap, uint64_t *trace,
+                                                     int32_t *level_passes, uint8_t *exists,
+                                                     int32_t *status, int32_t group)
+{
+    __shared__ float cand[4][kBins + 1];
+    __shared__ uint32_t P[4][kBins + 1];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    float *cw = cand[w];
+    uint32_t *pw = P[w];
+    auto active = [&](int32_t c) { return !recs[c].found && !recs[c].flags; };
+    for_cell_groups(nCells, group, active, [&](int32_t c) {
+    CellRec r = recs[c];
+    const uint32_t f = first[c], nch = first[c + 1] - f;
+    const SelLists sl(f, nch);
+    // list `lane`: its keys from position kpos on (at most the positions of
+    // its chunk range, whatever the counter holds)
+    uint32_t mr = 0, kpos = 0;
+    uint32_t below = 0;
+    if ((uint32_t)lane < sl.R) {
+        const uint32_t qs = sl.start_chunk((uint32_t)lane);
+        const uint32_t qn = (uint32_t)lane + 1u < sl.R ? sl.start_chunk((uint32_t)lane + 1u) : f + nch;
+        const unsigned long long e = sel_lists[qs];
+        kpos = r.begin + (qs - f) * (uint32_t)kChunk;
+        const uint32_t span = (qn < f + nch ? r.begin + (qn - f) * (uint32_t)kChunk : r.end) - kpos;
+        mr = (uint32_t)e < span ? (uint32_t)e : span;
+        below = (uint32_t)(e >> 32);
+    }
+    below = wave_sum(below);
+    const uint32_t lincl = wave_inclusive_scan(mr), lex = lincl - mr;
+    const uint32_t m = (uint32_t)__shfl((int)lincl, 63, 64);
+    const float target = difference_target(r.end - r.begin, r.nLeaf);
+    uint64_t *trow = trace ? trace + ((size_t)r.hb + r.id) * ORB_MAX_PASSES : nullptr;
+    for (int round = 0;; ++round) {
+        const float lo = r.margin_l, hi = r.margin_r;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();  // the wave's last replay has read the tables
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int j = 4 * lane + i;
+            cw[j] = j == 0 ? -INFINITY : candidate(j, lo, hi);
+            pw[j] = 0u;
+        }
+        if (lane == 0) cw[kBins] = INFINITY;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        const float width = hi - lo;
+        const float scale = width > 0.0f ? (float)kBins / width : 0.0f;
+        uint32_t bl = 0;
+        for (uint32_t k0 = 0; k0 < m; k0 += 64u * kFinBatch) {
+            float v[kFinBatch];  // all of a batch's loads in flight
+#pragma unroll
+            for (int u = 0; u < kFinBatch; ++u) {
+                // key k of the cell: list j = the last with lex_j <= k
+                const uint32_t k = k0 + 64u * (uint32_t)u + (uint32_t)lane;
+                int j = 0;
+#pragma unroll
+                for (int st = 32; st > 0; st >>= 1)
+                    if ((uint32_t)__shfl((int)lex, j + st, 64) <= k) j += st;
+                const uint32_t at =
+                    (uint32_t)__shfl((int)kpos, j, 64) + (k - (uint32_t)__shfl((int)lex, j, 64));
+                v[u] = k < m ? sel[at] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < kFinBatch; ++u) {
+                if (k0 + 64u * (uint32_t)u + (uint32_t)lane >= m) continue;
+                if (v[u] < lo) ++bl;
+                else if (!(v[u] > hi)) {
+                    int tj;
+                    atomicAdd(&pw[bin_of<LE>(cw, v[u], lo, scale, tj)], 1u);
+                }
+            }
+        }
+        below += wave_sum(bl);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        // P[j] = the keys below the margins + bins < j
+        const uint32_t s0 = pw[4 * lane], s1 = s0 + pw[4 * lane + 1];
+        const uint32_t s2 = s1 + pw[4 * lane + 2], s3 = s2 + pw[4 * lane + 3];
+        const uint32_t base = below + wave_inclusive_scan(s3) - s3;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        pw[4 * lane + 1] = base + s0;
+        pw[4 * lane + 2] = base + s1;
+        pw[4 * lane + 3] = base + s2;
+        pw[4 * lane + 4] = base + s3;
+        if (lane == 0) pw[0] = 0u;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        const RoundOut ro = replay_round(pw, lo, hi, target, r.passes, lane, trow);
+        if (round == 1 || ro.found || ro.passes >= ORB_MAX_PASSES) {  // uniform
+            if (lane == 0)
+                finish_round(r, c, ro, found_encoding(ro, lo, hi), recs, heap, level_passes,
+                             exists, status);
+            break;
+        }
+        r.margin_l = ro.L;
+        r.margin_r = ro.R;
+        r.passes = ro.passes;
+    }
     });
 }
 
@@ -1723,6 +1902,7 @@ static int ensure_state(Ctx &c, int32_t levels_mode, bool *complete)
         if (b.cell_hist_w.n < (size_t)c.max_level_cells * kBins) b.hist_w_clean = false;
         ORB_TRY(b.cell_hist_w.alloc((size_t)c.max_level_cells * kBins));
     }
+    if (!c.weighted) ORB_TRY(c.sel.alloc((size_t)c.stride));  // rounds 3 and 4
     // every buffer the launches touch exists before a capture starts
     ORB_TRY(preallocate_scratch(c));
     return ORB_OK;
@@ -1759,7 +1939,7 @@ static std::vector<uint64_t> graph_signature(const Ctx &c, const BatchedState &b
     const void *bufs[] = {c.pos.p, c.tmp.p, c.pid.p, c.pid_tmp.p, c.ranges.p, c.recs.p,
                           c.cell_nchunks.p, c.cell_first.p, c.chunk_cell.p, c.chunk_desc.p,
                           c.chunk_ltle.p, c.chunk_cnt.p, c.chunk_pre.p, c.scan_partials.p,
-                          c.packed.p, c.masks.p, c.cell_cand.p, c.splits.p, c.chunk_comp.p,
+                          c.packed.p, c.masks.p, c.cell_cand.p, c.splits.p, c.chunk_comp.p, c.sel.p, c.sel_lists.p,
                           c.status.p, c.stamps.p, c.cnt.p, b.level_ids.p, b.heap.p, b.exists.p,
                           b.sub.p, b.chunk_hist.p, b.cell_hist.p, b.trace.p, b.level_passes.p,
                           c.wt.p, c.wt_tmp.p, b.chunk_hist_w.p, b.cell_hist_w.p};
@@ -1777,6 +1957,8 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
                   bool full_cover, int32_t level, uint64_t *chunk_hist_w, uint64_t *cell_hist_w)
 {
     if (c.weighted && (!chunk_hist_w || !cell_hist_w)) return ORB_ERR_STATE;
+    if (!c.weighted && (c.sel.n < (size_t)c.stride || c.sel_lists.n < (size_t)c.max_chunks + 1))
+        return ORB_ERR_STATE;  // rounds 3 and 4 gather into these
     const int32_t sub_max = subtree_max_keys(c);
     const bool use_subtree = any_subtree && c.part == ORB_PART_HOARE && sub_max > 0;
     const int32_t cgroup = cell_group(nCells);
@@ -1869,11 +2051,10 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
         ORB_HIP(hipGetLastError());
         ORB_TRY(timing_pair(c, tb, kTimeCount));
     }
-    for (int r = 0; r < kRounds; ++r) {
+    for (int r = 0; r < 2; ++r) {
         const int tb = timing_mark(c);
-        // rounds: all keys, all keys + packing the window, then the packed keys
-        float *ckeys = c.packed.p;
-        const int mode = r == 0 ? 0 : (r == 1 ? 1 : 2);
+        // rounds 1 and 2 over every key of the searching cells; the second
+        // also packs the keys inside its margins, per chunk quarter
 #define ORB_HIST(L)                                                                              \
     hipLaunchKernelGGL((k_b_hist<L>), dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,     \
                        c.cell_first.p, c.chunk_desc.p, (uint32_t)c.max_chunks, c.x(), c.y(), c.z(), \
@@ -1881,34 +2062,44 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
 #define ORB_HIST_WINDOW(L)                                                                       \
     hipLaunchKernelGGL((k_b_hist_window<L>), dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p,      \
                        nCells, c.cell_first.p, c.chunk_desc.p, (uint32_t)c.max_chunks, c.x(), c.y(), \
-                       c.z(), c.cell_cand.p, later_hist, ckeys, c.chunk_comp.p, c.masks.p, !rev)
-        const unsigned gp = (gq + 3) / 4;  // a wave per chunk
+                       c.z(), c.cell_cand.p, later_hist, c.packed.p, c.chunk_comp.p, c.masks.p, !rev)
         if (le) {
-            if (mode == 0) ORB_HIST(1);
-            else if (mode == 1) ORB_HIST_WINDOW(1);
-            else
-                hipLaunchKernelGGL(k_b_hist_packed<1>, dim3(gp), dim3(kBlock), 0, c.stream, c.recs.p,
-                                   nCells, c.cell_first.p, c.chunk_desc.p, (uint32_t)c.max_chunks,
-                                   later_hist, ckeys, c.chunk_comp.p);
+            if (r == 0) ORB_HIST(1);
+            else ORB_HIST_WINDOW(1);
         } else {
-            if (mode == 0) ORB_HIST(0);
-            else if (mode == 1) ORB_HIST_WINDOW(0);
-            else
-                hipLaunchKernelGGL(k_b_hist_packed<0>, dim3(gp), dim3(kBlock), 0, c.stream, c.recs.p,
-                                   nCells, c.cell_first.p, c.chunk_desc.p, (uint32_t)c.max_chunks,
-                                   later_hist, ckeys, c.chunk_comp.p);
+            if (r == 0) ORB_HIST(0);
+            else ORB_HIST_WINDOW(0);
         }
 #undef ORB_HIST
 #undef ORB_HIST_WINDOW
         ORB_HIP(hipGetLastError());
         ORB_TRY(timing_pair(c, tb, kTimeCount));
         hipLaunchKernelGGL(k_b_hist_reduce, dim3(gr), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
-                           c.cell_first.p, c.chunk_cell.p, mode == 0 ? chunk_hist : later_hist,
+                           c.cell_first.p, c.chunk_cell.p, r == 0 ? chunk_hist : later_hist,
                            cell_hist);
         hipLaunchKernelGGL(k_b_replay, dim3(cell_group_blocks(nCells, cgroup)), dim3(kBlock), 0,
                            c.stream, c.recs.p, nCells, cell_hist, heap, trace, level_passes, exists,
-                           status, mode == 0 ? c.cell_cand.p : nullptr, cgroup);
+                           status, r == 0 ? c.cell_cand.p : nullptr, c.cell_first.p,
+                           r == 1 ? reinterpret_cast<unsigned long long *>(c.sel_lists.p) : nullptr,
+                           cgroup);
         ORB_HIP(hipGetLastError());
+    }
+    {
+        // rounds 3 and 4 from the packed keys inside the margins, gathered
+        // per cell once
+        const int tb = timing_mark(c);
+        const unsigned gs = (gq + 4 * kSelChunks - 1) / (4 * kSelChunks);
+        unsigned long long *lists = reinterpret_cast<unsigned long long *>(c.sel_lists.p);
+        hipLaunchKernelGGL(k_b_select, dim3(gs), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
+                           c.cell_first.p, c.chunk_desc.p, (uint32_t)c.max_chunks, c.packed.p,
+                           c.chunk_comp.p, c.sel.p, lists);
+        ORB_HIP(hipGetLastError());
+        auto finish = le ? k_b_finish<1> : k_b_finish<0>;
+        hipLaunchKernelGGL(finish, dim3(cell_group_blocks(nCells, cgroup)), dim3(kBlock), 0,
+                           c.stream, c.recs.p, nCells, c.cell_first.p, c.sel.p, lists, heap,
+                           trace, level_passes, exists, status, cgroup);
+        ORB_HIP(hipGetLastError());
+        ORB_TRY(timing_pair(c, tb, kTimeCount));
     }
     const int tp = timing_mark(c);
     HistCounts hc;

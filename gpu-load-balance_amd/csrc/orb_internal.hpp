@@ -485,6 +485,10 @@ struct Ctx {
                                      // each searching cell for the second histogram round
                                      // (written by the first round's replay)
     DevBuf<uint32_t> chunk_comp;     // [max_chunks+1][8] packed keys per wave, keys below
+    DevBuf<float> sel;               // [n] batched form: each searching cell's keys inside its
+                                     // margins after the second round, from the cell's begin
+    DevBuf<uint64_t> sel_lists;      // [max_chunks+1] batched form: count of keys gathered into
+                                     // a list of sel | keys below << 32, at the list's first chunk
     DevBuf<SplitRec> splits;         // [cells]
     DevBuf<int32_t> status;          // [2]: error flag, failing heap id
     DevBuf<uint32_t> ids_dev;        // [cells]

@@ -381,6 +381,7 @@ int ensure_level_capacity(Ctx &c, int64_t nCells, int64_t chunks)
     ORB_TRY(c.cell_cand.alloc((size_t)m * kBins));
     ORB_TRY(c.splits.alloc(m));
     ORB_TRY(c.chunk_comp.alloc(8 * ((size_t)c.max_chunks + 1)));
+    ORB_TRY(c.sel_lists.alloc((size_t)c.max_chunks + 1));
     ORB_TRY(c.ids_dev.alloc(m));
     ORB_TRY(c.hcells.alloc(m));
     ORB_TRY(c.hcnt.alloc(2 * m));

@@ -218,6 +218,7 @@ static int server_alloc(Server &S)
     ORB_TRY(c.pos.alloc(3 * (size_t)c.stride + 64));
     ORB_TRY(c.ranges.alloc(2 * (size_t)H));
     ORB_TRY(c.packed.alloc((size_t)c.stride));
+    ORB_TRY(c.sel.alloc((size_t)c.stride));
     ORB_TRY(c.status.alloc(2));
     ORB_TRY(c.hstatus.alloc(2));
     ORB_TRY(c.hranges.alloc(2));
@@ -568,6 +569,8 @@ int orb_server_destroy(orb_server *o)
     c.scan_partials.release();
     c.splits.release();
     c.chunk_comp.release();
+    c.sel.release();
+    c.sel_lists.release();
     c.ids_dev.release();
     c.hcells.release();
     c.hcnt.release();
