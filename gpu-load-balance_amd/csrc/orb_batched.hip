@@ -345,17 +345,33 @@ __device__ inline void issue_chunk_loads(const float *x, const float *y, const f
 // (the 255 cuts of the next 8 bisection steps from the cell's margins): once
 // per cell here instead of once per chunk in the round.  A wave per cell
 // (for_cell_groups), four entries per lane.
-__global__ __launch_bounds__(kBlock) void k_b_first_cands(const CellRec *__restrict__ recs,
-                                                          int32_t nCells, float *__restrict__ cell_cand,
-                                                          int32_t group)
+// The launch also fills the level's chunk table (k_fill_chunks' work): the
+// first `fill_blocks` blocks take the chunks, the others the cells.
+__global__ __launch_bounds__(kBlock) void k_b_tables(const uint32_t *__restrict__ first,
+                                                     int32_t nCells,
+                                                     const CellRec *__restrict__ recs,
+                                                     uint32_t *__restrict__ chunk_cell,
+                                                     uint4 *__restrict__ chunk_desc,
+                                                     uint32_t max_chunks,
+                                                     float *__restrict__ cell_cand, int32_t group,
+                                                     uint32_t fill_blocks)
 {
+    if (blockIdx.x < fill_blocks) {  // uniform
+        uint32_t total = first[nCells];
+        if (total > max_chunks) total = max_chunks;
+        for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < total;
+             q += fill_blocks * blockDim.x)
+            fill_chunk(q, first, nCells, recs, chunk_cell, chunk_desc);
+        return;
+    }
     const int lane = threadIdx.x & 63;
     // a cell whose chunks are read: searched, not found, not in a subtree
     auto active = [&](int32_t c) {
         const CellRec &r = recs[c];
         return !r.found && !r.flags && r.axis >= 0 && r.axis <= 2;
     };
-    for_cell_groups(nCells, group, active, [&](int32_t c) {
+    for_cell_groups(nCells, group, blockIdx.x - fill_blocks, gridDim.x - fill_blocks, active,
+                    [&](int32_t c) {
         const float l = recs[c].margin_l, h = recs[c].margin_r;
         float4 v;
         v.x = lane ? candidate(4 * lane, l, h) : 0.0f;  // entry 0 is never read
@@ -1980,14 +1996,26 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
                            c.ranges.p, sub, use_subtree ? 1 : 0, (uint32_t)sub_max, c.recs.p,
                            c.cell_first.p);
         ORB_HIP(hipGetLastError());
-        ORB_TRY(build_chunk_table(c, nCells, false, true));
     } else {
         unsigned g1 = (unsigned)((nCells + 1 + kBlock - 1) / kBlock);
         hipLaunchKernelGGL(k_b_recs, dim3(g1), dim3(kBlock), 0, c.stream, entries, nCells, heap,
                            c.ranges.p, sub, use_subtree ? 1 : 0, (uint32_t)sub_max, c.recs.p,
                            c.cell_nchunks.p);
         ORB_HIP(hipGetLastError());
-        ORB_TRY(build_chunk_table(c, nCells, false));
+        ORB_TRY(scan_exclusive_u32(c, c.cell_nchunks.p, c.cell_first.p, (size_t)nCells + 1));
+    }
+    {
+        // the chunk table, and the first round's candidate tables (not used
+        // by weighted builds)
+        const int tb = timing_mark(c);
+        const unsigned fill_blocks =
+            (unsigned)std::min<int64_t>(16384, std::max<int64_t>(1, (c.max_chunks + kBlock - 1) / kBlock));
+        const unsigned cand_blocks = c.weighted ? 0u : cell_group_blocks(nCells, cgroup);
+        hipLaunchKernelGGL(k_b_tables, dim3(fill_blocks + cand_blocks), dim3(kBlock), 0, c.stream,
+                           c.cell_first.p, nCells, c.recs.p, c.chunk_cell.p, c.chunk_desc.p,
+                           (uint32_t)c.max_chunks, c.cell_cand.p, cgroup, fill_blocks);
+        ORB_HIP(hipGetLastError());
+        ORB_TRY(timing_pair(c, tb, kTimeCount));
     }
     uint16_t *chunk_tie = chunk_hist + (size_t)(c.max_chunks + 1) * kBins;
     uint16_t *later_hist = chunk_hist + 2 * (size_t)(c.max_chunks + 1) * kBins;
@@ -2043,13 +2071,6 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
         }
         ORB_TRY(timing_pair(c, tp, kTimePartition));
         return ORB_OK;
-    }
-    {
-        const int tb = timing_mark(c);
-        hipLaunchKernelGGL(k_b_first_cands, dim3(cell_group_blocks(nCells, cgroup)), dim3(kBlock), 0,
-                           c.stream, c.recs.p, nCells, c.cell_cand.p, cgroup);
-        ORB_HIP(hipGetLastError());
-        ORB_TRY(timing_pair(c, tb, kTimeCount));
     }
     for (int r = 0; r < 2; ++r) {
         const int tb = timing_mark(c);

@@ -132,12 +132,14 @@ __device__ inline void report_error(int32_t *status_pair, int32_t level, int32_t
 // <= 64): the wave reads their records in one load, a ballot keeps the
 // ones that need work, and the wave works through those one by one.  A level
 // of many finished cells then costs few waves.
+// (block: this block's index among nblocks blocks that share the work)
 template <class Active, class Body>
-__device__ inline void for_cell_groups(int32_t nCells, int32_t group, Active active, Body body)
+__device__ inline void for_cell_groups(int32_t nCells, int32_t group, uint32_t block,
+                                       uint32_t nblocks, Active active, Body body)
 {
     const int lane = threadIdx.x & 63;
-    const int32_t wave = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    const int32_t nwaves = (int32_t)((gridDim.x * blockDim.x) >> 6);
+    const int32_t wave = (int32_t)((block * blockDim.x + threadIdx.x) >> 6);
+    const int32_t nwaves = (int32_t)((nblocks * blockDim.x) >> 6);
     for (int32_t g0 = wave * group; g0 < nCells; g0 += nwaves * group) {
         const int32_t c = g0 + lane;
         uint64_t m = __ballot(lane < group && c < nCells && active(c));
@@ -147,6 +149,38 @@ __device__ inline void for_cell_groups(int32_t nCells, int32_t group, Active act
             body(g0 + j);
         }
     }
+}
+template <class Active, class Body>
+__device__ inline void for_cell_groups(int32_t nCells, int32_t group, Active active, Body body)
+{
+    for_cell_groups(nCells, group, blockIdx.x, gridDim.x, active, body);
+}
+
+// chunk q -> its cell (binary search in the exclusive chunk offsets)
+__device__ inline uint32_t chunk_to_cell(const uint32_t *first, int32_t nCells, uint32_t q)
+{
+    int32_t lo = 0, hi = nCells - 1;
+    while (lo < hi) {  // last cell with first[cell] <= q
+        int32_t mid = (lo + hi + 1) >> 1;
+        if (first[mid] <= q) lo = mid;
+        else hi = mid - 1;
+    }
+    return (uint32_t)lo;
+}
+
+// Chunk q's cell and record: first key, one past the last, cell, axis
+// (kChunkIdle when the cell has its cut already, belongs to a subtree or has
+// no axis).
+__device__ inline void fill_chunk(uint32_t q, const uint32_t *first, int32_t nCells,
+                                  const CellRec *recs, uint32_t *chunk_cell, uint4 *chunk_desc)
+{
+    const uint32_t cell = chunk_to_cell(first, nCells, q);
+    chunk_cell[q] = cell;
+    const CellRec &r = recs[cell];
+    const uint32_t b = r.begin + (q - first[cell]) * (uint32_t)kChunk;
+    const uint32_t e = r.end - b < (uint32_t)kChunk ? r.end : b + (uint32_t)kChunk;
+    const bool live = !r.found && !r.flags && r.axis >= 0 && r.axis <= 2;
+    chunk_desc[q] = make_uint4(b, e, cell, live ? (uint32_t)r.axis : kChunkIdle);
 }
 
 __device__ inline const float *axis_ptr(const float *x, const float *y, const float *z, int a)

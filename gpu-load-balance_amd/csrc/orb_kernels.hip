@@ -14,17 +14,6 @@
 
 namespace orb {
 
-// chunk q -> its cell (binary search in the exclusive chunk offsets)
-__device__ inline uint32_t chunk_to_cell(const uint32_t *first, int32_t nCells, uint32_t q)
-{
-    int32_t lo = 0, hi = nCells - 1;
-    while (lo < hi) {  // last cell with first[cell] <= q
-        int32_t mid = (lo + hi + 1) >> 1;
-        if (first[mid] <= q) lo = mid;
-        else hi = mid - 1;
-    }
-    return (uint32_t)lo;
-}
 
 // ---------------------------------------------------------------------------
 // utilities
@@ -345,15 +334,8 @@ __global__ void k_fill_chunks(const uint32_t *first, int32_t nCells, const CellR
     uint32_t total = first[nCells];
     if (total > max_chunks) total = max_chunks;
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < total;
-         q += gridDim.x * blockDim.x) {
-        const uint32_t cell = chunk_to_cell(first, nCells, q);
-        chunk_cell[q] = cell;
-        const CellRec &r = recs[cell];
-        const uint32_t b = r.begin + (q - first[cell]) * (uint32_t)kChunk;
-        const uint32_t e = r.end - b < (uint32_t)kChunk ? r.end : b + (uint32_t)kChunk;
-        const bool live = !r.found && !r.flags && r.axis >= 0 && r.axis <= 2;
-        chunk_desc[q] = make_uint4(b, e, cell, live ? (uint32_t)r.axis : kChunkIdle);
-    }
+         q += gridDim.x * blockDim.x)
+        fill_chunk(q, first, nCells, recs, chunk_cell, chunk_desc);
 }
 
 int ensure_level_capacity(Ctx &c, int64_t nCells, int64_t chunks)
