@@ -529,6 +529,7 @@ int orb_finalize(orb_ctx *ctx)
     c->chunk_comp.release();
     c->sel.release();
     c->sel_lists.release();
+    c->swap_plan.release();
     c->status.release();
     c->ids_dev.release();
     c->hcells.release();

@@ -490,6 +490,7 @@ struct Ctx {
     DevBuf<uint64_t> sel_lists;      // [max_chunks+1] batched form: count of keys gathered into
                                      // a list of sel | keys below << 32, at the list's first chunk
     DevBuf<SplitRec> splits;         // [cells]
+    DevBuf<uint4> swap_plan;         // [max_chunks][2] Hoare swap unit of each chunk
     DevBuf<int32_t> status;          // [2]: error flag, failing heap id
     DevBuf<uint32_t> ids_dev;        // [cells]
 

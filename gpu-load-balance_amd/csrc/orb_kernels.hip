@@ -364,6 +364,7 @@ int ensure_level_capacity(Ctx &c, int64_t nCells, int64_t chunks)
     ORB_TRY(c.splits.alloc(m));
     ORB_TRY(c.chunk_comp.alloc(8 * ((size_t)c.max_chunks + 1)));
     ORB_TRY(c.sel_lists.alloc((size_t)c.max_chunks + 1));
+    ORB_TRY(c.swap_plan.alloc(2 * (size_t)c.max_chunks));
     ORB_TRY(c.ids_dev.alloc(m));
     ORB_TRY(c.hcells.alloc(m));
     ORB_TRY(c.hcnt.alloc(2 * m));
@@ -1255,13 +1256,67 @@ __device__ inline void swap_pairs(const uint32_t *sl, const uint32_t *sr, uint32
 // block scan ranks both, and every stopper whose index the unit uses goes
 // to its slot (L index t -> t - t0, R index j -> TR - 1 - j - t0).  Later
 // rounds continue on the partner side only, chunk by chunk.
+// Each chunk's swap unit, planned once per level by a thread per chunk (so
+// that a swap block starts from one load instead of the chain chunk ->
+// cell -> split and offsets -> a search for the first partner chunk): the
+// owner side and pairs [t0, t1), the cell's R-stopper total, first
+// particle, first chunk and chunk count, and the first partner chunk
+// (cell-relative).  t0o = ~0 marks a chunk with no pairs to swap.
+struct SwapPlan {
+    uint32_t t0o;  // t0 | owner << 31
+    uint32_t t1;
+    uint32_t TR;
+    uint32_t begin;
+    uint32_t f, nch, kp, pad;
+};
+static_assert(sizeof(SwapPlan) == 32, "two 16-byte loads");
+
+__global__ __launch_bounds__(kBlock) void k_hoare_plan(const CellRec *__restrict__ recs,
+                                                       int32_t nCells,
+                                                       const uint32_t *__restrict__ first,
+                                                       const uint32_t *__restrict__ chunk_cell,
+                                                       const uint64_t *__restrict__ chunk_pre,
+                                                       const SplitRec *__restrict__ splits,
+                                                       SwapPlan *__restrict__ plan)
+{
+    const uint32_t total = first[nCells];
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < total;
+         q += gridDim.x * blockDim.x) {
+        const uint32_t cell = chunk_cell[q];
+        const SplitRec s = splits[cell];
+        SwapPlan pl{~0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+        if (s.K != 0u) {  // else nothing to swap, or reported by k_hoare_split
+            const CellRec &r = recs[cell];
+            const uint32_t n = r.end - r.begin;
+            const uint32_t f = first[cell], nch = first[cell + 1] - f, kc = q - f;
+            const CellOffsets co{chunk_pre + f, chunk_pre[f]};
+            const SwapUnit u = swap_unit(co, s, n, kc);
+            if (u.t0 < u.t1) {
+                const int sp = 1 - u.owner;
+                uint32_t ap0, ap1;
+                side_range(sp, u, s.TR, ap0, ap1);
+                // the last chunk k with at(sp, k) <= ap0 (at(sp, 0) = 0)
+                uint32_t lo = 0, hi = nch - 1u;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi + 1u) >> 1;
+                    if (co.at(sp, mid) <= ap0) lo = mid;
+                    else hi = mid - 1u;
+                }
+                pl = SwapPlan{u.t0 | ((uint32_t)u.owner << 31), u.t1, s.TR, r.begin, f, nch, lo, 0u};
+            }
+        }
+        uint4 *dst = reinterpret_cast<uint4 *>(plan + q);
+        dst[0] = make_uint4(pl.t0o, pl.t1, pl.TR, pl.begin);
+        dst[1] = make_uint4(pl.f, pl.nch, pl.kp, 0u);
+    }
+}
+
 template <bool kNarrow, int kArrays>
-__global__ __launch_bounds__(kBlock) void k_hoare_swap(const CellRec *recs, int32_t nCells,
-                                                       const uint32_t *first,
-                                                       const uint32_t *chunk_cell,
+__global__ __launch_bounds__(kBlock) void k_hoare_swap(const uint32_t *first,
+                                                       int32_t nCells,
                                                        const uint64_t *chunk_pre,
                                                        const uint64_t *masks,
-                                                       const SplitRec *splits,
+                                                       const SwapPlan *plan,
                                                        float *x, float *y, float *z,
                                                        uint32_t *pid, uint32_t *w, int rev)
 {
@@ -1272,19 +1327,19 @@ __global__ __launch_bounds__(kBlock) void k_hoare_swap(const CellRec *recs, int3
     const int wdi = j >> 2, sh = (j & 3) * 16;  // this thread's 16 bits of a chunk mask
     for (uint32_t q0 = blockIdx.x; q0 < total; q0 += gridDim.x) {
         const uint32_t q = rev ? total - 1u - q0 : q0;  // the level's chunks from the last (rev)
-        const uint32_t cell = chunk_cell[q];
-        const SplitRec s = splits[cell];
-        if (s.K == 0u) continue;  // nothing to swap, or reported by k_hoare_split
-        const CellRec r = recs[cell];
-        const uint32_t n = r.end - r.begin;
-        const uint32_t f = first[cell], nch = first[cell + 1] - f, kc = q - f;
+        const uint4 p0 = reinterpret_cast<const uint4 *>(plan + q)[0];
+        const uint4 p1 = reinterpret_cast<const uint4 *>(plan + q)[1];
+        if (p0.x == ~0u) continue;  // uniform: no pairs in this chunk
+        SwapUnit u;
+        u.owner = (int)(p0.x >> 31);
+        u.t0 = p0.x & 0x7fffffffu;
+        u.t1 = p0.y;
+        const uint32_t TR = p0.z, begin = p0.w, f = p1.x, nch = p1.y, kc = q - f;
         const CellOffsets co{chunk_pre + f, chunk_pre[f]};
-        const SwapUnit u = swap_unit(co, s, n, kc);
-        if (u.t0 >= u.t1) continue;  // uniform
         const int so = u.owner, sp = 1 - so;
         uint32_t ao0, ao1, ap0, ap1;
-        side_range(so, u, s.TR, ao0, ao1);
-        side_range(sp, u, s.TR, ap0, ap1);
+        side_range(so, u, TR, ao0, ao1);
+        side_range(sp, u, TR, ap0, ap1);
         auto put = [&](int side, uint32_t k, uint32_t idx, uint32_t bits, uint32_t a0,
                        uint32_t a1) {
             if (idx >= a1 || idx + (uint32_t)__popc(bits) <= a0) return;
@@ -1293,13 +1348,11 @@ __global__ __launch_bounds__(kBlock) void k_hoare_swap(const CellRec *recs, int3
             while (bits) {
                 const uint32_t b = (uint32_t)__builtin_ctz(bits);
                 bits &= bits - 1u;
-                if (idx >= a0 && idx < a1) dst[side ? s.TR - 1u - idx - u.t0 : idx - u.t0] = pos0 + b;
+                if (idx >= a0 && idx < a1) dst[side ? TR - 1u - idx - u.t0 : idx - u.t0] = pos0 + b;
                 ++idx;
             }
         };
-        // the chunk holding the unit's first partner index (a 64-way search
-        // over the cell's chunk offsets, each wave on its own)
-        uint32_t kp = co.chunk_of(sp, ap0, 0u, nch);
+        uint32_t kp = p1.z;  // the chunk holding the unit's first partner index
         bool own = true;
         while (true) {  // uniform
             const uint32_t bo =
@@ -1321,7 +1374,7 @@ __global__ __launch_bounds__(kBlock) void k_hoare_swap(const CellRec *recs, int3
             kp = co.at(sp, kp + 2u) > next ? kp + 1u : co.chunk_of(sp, next, kp + 1u, nch);
         }
         __syncthreads();
-        const CellArrays<kNarrow> ca(x, y, z, pid, w, r.begin);
+        const CellArrays<kNarrow> ca(x, y, z, pid, w, begin);
         swap_pairs<kNarrow, kArrays>(spos[0], spos[1], u.t1 - u.t0, ca);
         __syncthreads();  // the next unit overwrites spos
     }
@@ -1499,10 +1552,15 @@ int launch_hoare(Ctx &c, int32_t nCells, int32_t *status, const HistCounts *hc, 
                           : arrays == 4 ? k_hoare_swap<true, 4> : k_hoare_swap<true, 5>)
                        : (arrays == 3 ? k_hoare_swap<false, 3>
                           : arrays == 4 ? k_hoare_swap<false, 4> : k_hoare_swap<false, 5>);
+    SwapPlan *plan = reinterpret_cast<SwapPlan *>(c.swap_plan.p);
+    hipLaunchKernelGGL(k_hoare_plan, dim3(grid_for((size_t)c.max_chunks, 16384)), dim3(kBlock), 0,
+                       c.stream, c.recs.p, nCells, c.cell_first.p, c.chunk_cell.p, c.chunk_pre.p,
+                       c.splits.p, plan);
+    ORB_HIP(hipGetLastError());
     const int ts = timing_mark(c);
-    hipLaunchKernelGGL(swap, dim3(gq), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
-                       c.cell_first.p, c.chunk_cell.p, c.chunk_pre.p, c.masks.p, c.splits.p,
-                       c.x(), c.y(), c.z(), c.ids(), c.weights(), rev);
+    hipLaunchKernelGGL(swap, dim3(gq), dim3(kBlock), 0, c.stream, c.cell_first.p, nCells,
+                       c.chunk_pre.p, c.masks.p, plan, c.x(), c.y(), c.z(), c.ids(), c.weights(),
+                       rev);
     ORB_HIP(hipGetLastError());
     ORB_TRY(timing_pair(c, ts, kTimeSwap));
     unsigned gc = (unsigned)((nCells + kBlock - 1) / kBlock);

@@ -571,6 +571,7 @@ int orb_server_destroy(orb_server *o)
     c.chunk_comp.release();
     c.sel.release();
     c.sel_lists.release();
+    c.swap_plan.release();
     c.ids_dev.release();
     c.hcells.release();
     c.hcnt.release();
