@@ -879,25 +879,33 @@ __global__ __launch_bounds__(kBlock) void k_hoare_masks_window(
             if (lane == 0) chunk_cnt[q] = 0ull;
             continue;
         }
-        const CellRec &r = recs[cell];
         const uint32_t len = e - b, p0 = 64u * (uint32_t)lane;  // chunk position of bit 0
         const uint64_t valid =
             p0 >= len ? 0ull : (len - p0 >= 64u ? ~0ull : (1ull << (len - p0)) - 1ull);
+        // the window round's packed keys of this lane's word, ranked from its
+        // in-window mask; the first is requested with the cell record (a
+        // cell past that round uses it; for other cells the words are stale
+        // but the rank stays below the quarter's valid positions, in bounds)
+        const uint64_t G = Gw & valid;
+        const uint64_t W = Ww & valid;  // slots stay inside the cell
+        const uint32_t cw = (uint32_t)__popcll(W);
+        const uint32_t ex = wave_inclusive_scan(cw) - cw;
+        const int q0 = lane & ~(kQuarter - 1);  // first word of this lane's quarter
+        uint32_t rank = ex - (uint32_t)__shfl((int)ex, q0, 64);
+        const float *pk = pack + b + 64u * (uint32_t)q0;
+        const float v0 = W ? pk[rank] : 0.0f;
+        const CellRec &r = recs[cell];
         const float cut = r.cut;
         uint64_t L = 0ull, R = 0ull;
         if (r.passes > kHistK) {  // uniform
-            const uint64_t G = Gw & valid;
-            const uint64_t W = Ww & valid;  // slots stay inside the cell
-            const uint32_t cw = (uint32_t)__popcll(W);
-            const uint32_t ex = wave_inclusive_scan(cw) - cw;
-            const int q0 = lane & ~(kQuarter - 1);  // first word of this lane's quarter
-            uint32_t rank = ex - (uint32_t)__shfl((int)ex, q0, 64);
-            const float *pk = pack + b + 64u * (uint32_t)q0;
             uint64_t wd = W;
+            bool first_bit = true;
             while (wd) {
                 const uint32_t bit = (uint32_t)__builtin_ctzll(wd);
                 wd &= wd - 1ull;
-                const float v = pk[rank++];
+                const float v = first_bit ? v0 : pk[rank];
+                ++rank;
+                first_bit = false;
                 L |= (!(v < cut) ? 1ull : 0ull) << bit;
                 R |= (!(v > cut) ? 1ull : 0ull) << bit;
             }
