@@ -790,7 +790,7 @@ __device__ inline void finish_round(CellRec r, int32_t c, const RoundOut &ro, in
 
 // Rounds 3 and 4 gather a searching cell's keys into up to 64 lists (see
 // k_b_select): list r covers a contiguous range of the cell's chunks.
-constexpr int kSelChunks = 16;  // chunks per wave: a lane per chunk quarter
+constexpr int kSelChunks = 4;  // chunks per wave: a lane per chunk quarter
 constexpr int kSelLists = 64;
 struct SelLists {
     uint32_t f, g0, S, R;  // first chunk, first group, groups per list, lists
@@ -884,9 +884,11 @@ __global__ __launch_bounds__(kBlock) void k_b_replay(CellRec *recs, int32_t nCel
 // cell's margins and below them (the count below also takes the keys below
 // the window round's margins, which that round counted per chunk), then
 // reserves slots in its list with one 64-bit atomic (keys | below << 32)
-// and writes the keys there.  The wave reads the quarters' packed keys laid
-// end to end, four loads per lane in flight; a wave's keys are usually
-// fewer than 256 and stay in registers between counting and writing.
+// and writes the keys there.  A wave takes 4 chunks, a lane each quarter,
+// and reads the quarters' packed keys laid end to end, four loads per lane
+// in flight; a wave's keys are usually fewer than 256 and stay in registers
+// between counting and writing (a dense cluster packs a good part of a
+// chunk: few chunks per wave keep such waves short).
 constexpr int kSelBatch = 4;
 __global__ __launch_bounds__(kBlock) void k_b_select(const CellRec *__restrict__ recs,
                                                      int32_t nCells,
@@ -899,7 +901,8 @@ __global__ __launch_bounds__(kBlock) void k_b_select(const CellRec *__restrict__
                                                      unsigned long long *__restrict__ sel_lists)
 {
     constexpr int kW = kBlock / 64;
-    static_assert(kSelChunks * 4 == 64, "a lane per chunk quarter");
+    constexpr int kQ = 4 * kSelChunks;  // quarters (lanes) per wave
+    static_assert(kQ <= 64 && (kQ & (kQ - 1)) == 0, "a lane per chunk quarter");
     __shared__ uint32_t s_below[kW][64], s_cnt[kW][64];  // per run of one cell
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, sh = lane & 3;
     const uint32_t wave = blockIdx.x * (uint32_t)kW + (uint32_t)w;
@@ -907,7 +910,7 @@ __global__ __launch_bounds__(kBlock) void k_b_select(const CellRec *__restrict__
     const uint32_t total = first[nCells];
     for (uint32_t q0 = wave * kSelChunks; q0 < total; q0 += nwaves * kSelChunks) {
         const uint32_t q = q0 + (uint32_t)(lane >> 2);
-        const bool inq = q < total;
+        const bool inq = lane < kQ && q < total;
         const uint32_t qr = inq ? q : 0u;  // in-bounds record and counts
         const uint4 d = chunk_record(chunk_desc, qr, max_chunks);
         const uint32_t nc0 = comp[8 * (size_t)qr + sh], nb0 = comp[8 * (size_t)qr + 4];
@@ -946,9 +949,9 @@ __global__ __launch_bounds__(kBlock) void k_b_select(const CellRec *__restrict__
 #pragma unroll
             for (int u = 0; u < kSelBatch; ++u) {
                 const uint32_t t = t0 + 64u * (uint32_t)u + (uint32_t)lane;
-                int j = 0;
+                int j = 0;  // (lanes from kQ on hold no keys: ex = T there)
 #pragma unroll
-                for (int st = 32; st > 0; st >>= 1)
+                for (int st = kQ / 2; st > 0; st >>= 1)
                     if ((uint32_t)__shfl((int)ex, j + st, 64) <= t) j += st;
                 const uint32_t at =
                     (uint32_t)__shfl((int)kbase, j, 64) + (t - (uint32_t)__shfl((int)ex, j, 64));
@@ -1116,6 +1119,125 @@ __global__ __launch_bounds__(kBlock) void k_b_finish(CellRec *recs, int32_t nCel
         r.passes = ro.passes;
     }
     });
+}
+
+// The same two rounds with a block per cell, for levels of few cells (whose
+// cells hold the most gathered keys): the four waves bin a quarter of the
+// keys each into one LDS histogram, wave 0 prefix-sums and replays, and the
+// round's outcome reaches the other waves through LDS.
+constexpr int kFinBlockCells = 1024;  // levels up to this many cells
+template <int LE>
+__global__ __launch_bounds__(kBlock) void k_b_finish_block(CellRec *recs, int32_t nCells,
+                                                           const uint32_t *__restrict__ first,
+                                                           const float *__restrict__ sel,
+                                                           const unsigned long long *__restrict__ sel_lists,
+                                                           orb_cell *heap, uint64_t *trace,
+                                                           int32_t *level_passes, uint8_t *exists,
+                                                           int32_t *status)
+{
+    static_assert(kBins == kBlock, "a candidate and a bin per thread");
+    __shared__ float cand[kBins + 1];
+    __shared__ uint32_t hist[kBins], P[kBins + 1];
+    __shared__ uint32_t sm[4];
+    __shared__ float s_lr[2];
+    __shared__ int32_t s_pf[2];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    for (int32_t c = blockIdx.x; c < nCells; c += gridDim.x) {
+        CellRec r = recs[c];
+        if (r.found || r.flags) continue;  // uniform
+        const uint32_t f = first[c], nch = first[c + 1] - f;
+        const SelLists sl(f, nch);
+        // every wave holds the list table (list `lane`: count, first position)
+        uint32_t mr = 0, kpos = 0, bw = 0;
+        if ((uint32_t)lane < sl.R) {
+            const uint32_t qs = sl.start_chunk((uint32_t)lane);
+            const uint32_t qn =
+                (uint32_t)lane + 1u < sl.R ? sl.start_chunk((uint32_t)lane + 1u) : f + nch;
+            const unsigned long long e = sel_lists[qs];
+            kpos = r.begin + (qs - f) * (uint32_t)kChunk;
+            const uint32_t span =
+                (qn < f + nch ? r.begin + (qn - f) * (uint32_t)kChunk : r.end) - kpos;
+            mr = (uint32_t)e < span ? (uint32_t)e : span;
+            bw = (uint32_t)(e >> 32);
+        }
+        uint32_t below = wave_sum(bw);
+        const uint32_t lincl = wave_inclusive_scan(mr), lex = lincl - mr;
+        const uint32_t m = (uint32_t)__shfl((int)lincl, 63, 64);
+        const float target = difference_target(r.end - r.begin, r.nLeaf);
+        uint64_t *trow = trace ? trace + ((size_t)r.hb + r.id) * ORB_MAX_PASSES : nullptr;
+        for (int round = 0;; ++round) {
+            const float lo = r.margin_l, hi = r.margin_r;
+            __syncthreads();  // the last round (or cell) has read the tables
+            cand[t] = t ? candidate(t, lo, hi) : -INFINITY;
+            hist[t] = 0u;
+            if (t == 0) cand[kBins] = INFINITY;
+            __syncthreads();
+            const float width = hi - lo;
+            const float scale = width > 0.0f ? (float)kBins / width : 0.0f;
+            uint32_t bl = 0;
+            for (uint32_t k0 = 0; k0 < m; k0 += (uint32_t)kBlock * kFinBatch) {
+                float v[kFinBatch];  // all of a batch's loads in flight
+#pragma unroll
+                for (int u = 0; u < kFinBatch; ++u) {
+                    // key k of the cell: list j = the last with lex_j <= k
+                    const uint32_t k = k0 + (uint32_t)(kBlock * u + t);
+                    int j = 0;
+#pragma unroll
+                    for (int st = 32; st > 0; st >>= 1)
+                        if ((uint32_t)__shfl((int)lex, j + st, 64) <= k) j += st;
+                    const uint32_t at = (uint32_t)__shfl((int)kpos, j, 64) +
+                                        (k - (uint32_t)__shfl((int)lex, j, 64));
+                    v[u] = k < m ? sel[at] : 0.0f;
+                }
+#pragma unroll
+                for (int u = 0; u < kFinBatch; ++u) {
+                    if (k0 + (uint32_t)(kBlock * u + t) >= m) continue;
+                    if (v[u] < lo) ++bl;
+                    else if (!(v[u] > hi)) {
+                        int tj;
+                        atomicAdd(&hist[bin_of<LE>(cand, v[u], lo, scale, tj)], 1u);
+                    }
+                }
+            }
+            below += block_sum(bl, sm);  // its barriers also complete the binning
+            if (w == 0) {
+                // P[j] = the keys below the margins + bins < j
+                const uint32_t s0 = hist[4 * lane], s1 = s0 + hist[4 * lane + 1];
+                const uint32_t s2 = s1 + hist[4 * lane + 2], s3 = s2 + hist[4 * lane + 3];
+                const uint32_t base = below + wave_inclusive_scan(s3) - s3;
+                P[4 * lane + 1] = base + s0;
+                P[4 * lane + 2] = base + s1;
+                P[4 * lane + 3] = base + s2;
+                P[4 * lane + 4] = base + s3;
+                if (lane == 0) P[0] = 0u;
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                const RoundOut ro = replay_round(P, lo, hi, target, r.passes, lane, trow);
+                if (lane == 0) {
+                    s_lr[0] = ro.L;
+                    s_lr[1] = ro.R;
+                    s_pf[0] = ro.passes;
+                    s_pf[1] = (ro.found ? 1 : 0) | (ro.node << 1);
+                }
+            }
+            __syncthreads();
+            RoundOut ro;
+            ro.L = s_lr[0];
+            ro.R = s_lr[1];
+            ro.passes = s_pf[0];
+            ro.found = (s_pf[1] & 1) != 0;
+            ro.node = s_pf[1] >> 1;
+            if (round == 1 || ro.found || ro.passes >= ORB_MAX_PASSES) {  // uniform
+                if (t == 0)
+                    finish_round(r, c, ro, found_encoding(ro, lo, hi), recs, heap, level_passes,
+                                 exists, status);
+                break;
+            }
+            r.margin_l = ro.L;
+            r.margin_r = ro.R;
+            r.passes = ro.passes;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -2115,10 +2237,17 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
                            c.cell_first.p, c.chunk_desc.p, (uint32_t)c.max_chunks, c.packed.p,
                            c.chunk_comp.p, c.sel.p, lists);
         ORB_HIP(hipGetLastError());
-        auto finish = le ? k_b_finish<1> : k_b_finish<0>;
-        hipLaunchKernelGGL(finish, dim3(cell_group_blocks(nCells, cgroup)), dim3(kBlock), 0,
-                           c.stream, c.recs.p, nCells, c.cell_first.p, c.sel.p, lists, heap,
-                           trace, level_passes, exists, status, cgroup);
+        if (nCells <= kFinBlockCells) {  // a block per cell
+            auto finish = le ? k_b_finish_block<1> : k_b_finish_block<0>;
+            hipLaunchKernelGGL(finish, dim3((unsigned)std::max<int32_t>(nCells, 1)), dim3(kBlock), 0,
+                               c.stream, c.recs.p, nCells, c.cell_first.p, c.sel.p, lists, heap,
+                               trace, level_passes, exists, status);
+        } else {
+            auto finish = le ? k_b_finish<1> : k_b_finish<0>;
+            hipLaunchKernelGGL(finish, dim3(cell_group_blocks(nCells, cgroup)), dim3(kBlock), 0,
+                               c.stream, c.recs.p, nCells, c.cell_first.p, c.sel.p, lists, heap,
+                               trace, level_passes, exists, status, cgroup);
+        }
         ORB_HIP(hipGetLastError());
         ORB_TRY(timing_pair(c, tb, kTimeCount));
     }
