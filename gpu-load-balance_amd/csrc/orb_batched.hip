@@ -383,8 +383,8 @@ __global__ __launch_bounds__(kBlock) void k_b_tables(const uint32_t *__restrict_
 }
 
 // First histogram round: one block per chunk bins every key of the chunk
-// over the cell's 255 candidate cuts and writes the chunk's 256-bin row and
-// its tie row.  Wave w takes the quarter [1024w, 1024w + 1024) of the
+// over the cell's 255 candidate cuts and writes the chunk's 256-bin row and,
+// for the stable partition's stopper counts, its tie row.  Wave w takes the quarter [1024w, 1024w + 1024) of the
 // chunk, 64 consecutive keys per load.  A key's bin is the estimate
 // (v - lo) * 256 / (hi - lo) clamped to [-0.5, 255.5] whenever that estimate
 // lies more than the tolerance from a bin edge (a clamped estimate lies a
@@ -445,7 +445,7 @@ __global__ __launch_bounds__(kBlock) void k_b_hist(
                     if (ok && slow) {
                         int tj;
                         bin = bin_of<LE>(cand, vj, lo, scale, tj);
-                        if (tj) atomicAdd(&ties[tj], 1u);
+                        if (chunk_tie && tj) atomicAdd(&ties[tj], 1u);
                     }
                 }
                 if (ok) atomicAdd(&h[bin], 1u);
@@ -456,7 +456,7 @@ __global__ __launch_bounds__(kBlock) void k_b_hist(
         __syncthreads();
         const uint32_t sum = hist[0][t] + hist[1][t] + hist[2][t] + hist[3][t];  // <= 4096
         chunk_hist[(size_t)q * kBins + t] = (uint16_t)sum;
-        chunk_tie[(size_t)q * kBins + t] = (uint16_t)ties[t];
+        if (chunk_tie) chunk_tie[(size_t)q * kBins + t] = (uint16_t)ties[t];
         __syncthreads();  // the next chunk rewrites cand, hist and ties
     }
 }
@@ -2139,7 +2139,9 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
         ORB_HIP(hipGetLastError());
         ORB_TRY(timing_pair(c, tb, kTimeCount));
     }
-    uint16_t *chunk_tie = chunk_hist + (size_t)(c.max_chunks + 1) * kBins;
+    // tie rows: only the stable partition's stopper counts read them
+    uint16_t *chunk_tie =
+        c.part == ORB_PART_STABLE_LE ? chunk_hist + (size_t)(c.max_chunks + 1) * kBins : nullptr;
     uint16_t *later_hist = chunk_hist + 2 * (size_t)(c.max_chunks + 1) * kBins;
     if (use_subtree) {
         // one block fits a CU (LDS): four blocks per CU by default, each

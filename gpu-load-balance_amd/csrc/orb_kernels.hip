@@ -847,17 +847,20 @@ __global__ __launch_bounds__(kBlock) void k_hoare_masks(const CellRec *recs, int
     }
 }
 
-// Batched form: a wave per chunk (the work is a few dependent loads).  A
-// cell that went through the window round (passes > kHistK) finishes that
-// round's masks: keys above the window are L-stoppers only, keys below
-// R-stoppers only, and the window keys, packed in position order in the
-// quarter's slots, are compared with the cut; lane j owns word j.  Other
-// cells read their keys, 64 per load.
+// Batched form: a wave per two chunks (the work is a few dependent loads,
+// so each wave keeps both chunks' loads in flight together).  A cell that
+// went through the window round (passes > kHistK) finishes that round's
+// masks: keys above the window are L-stoppers only, keys below R-stoppers
+// only, and the window keys, packed in position order in the quarter's
+// slots, are compared with the cut; lane j owns word j.  Other cells read
+// their keys, 64 per load.
 //
-// The chunk's record and its two window-round mask words per lane are
-// requested together; the cell record next; the packed keys last.  Chunks of
-// cells that are not searched (subtree cells, no axis) are skipped: nothing
-// reads their masks, and their count slot is written as zero.
+// The chunks' records and their two window-round mask words per lane are
+// requested together; the cell records next, with each lane's first packed
+// key.  Chunks of cells that are not searched (subtree cells, no axis) are
+// skipped: nothing reads their masks, and their count slot is written as
+// zero.
+constexpr int kMasksChunks = 2;  // k_hoare_masks_window: chunks per wave
 __global__ __launch_bounds__(kBlock) void k_hoare_masks_window(
     const CellRec *__restrict__ recs, int32_t nCells, const uint32_t *__restrict__ first,
     const uint4 *__restrict__ chunk_desc, uint32_t max_chunks, const float *x, const float *y,
@@ -865,76 +868,114 @@ __global__ __launch_bounds__(kBlock) void k_hoare_masks_window(
 {
     constexpr int kW = kBlock / 64;
     constexpr int kQuarter = kMaskWords / kW;  // words per packing quarter
+    constexpr int kPer = kMasksChunks;         // chunks per wave, their loads interleaved
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t total = first[nCells];
-    for (uint32_t q = blockIdx.x * kW + (uint32_t)w;; q += gridDim.x * kW) {
-        const uint4 d = chunk_record(chunk_desc, q, max_chunks);
-        uint64_t *mq = masks + (size_t)(q < max_chunks ? q : 0u) * (2 * kMaskWords);
-        const uint64_t Gw = mq[lane], Ww = mq[kMaskWords + lane];  // in flight with the record
-        const uint32_t b = uniform(d.x), e = uniform(d.y), cell = uniform(d.z);
-        const uint32_t axis = uniform(d.w);
-        keep_values(b, e, cell, axis);  // the whole record, in flight with total
-        if (q >= total) break;
-        if (axis > 2u) {  // uniform
-            if (lane == 0) chunk_cnt[q] = 0ull;
-            continue;
-        }
-        const uint32_t len = e - b, p0 = 64u * (uint32_t)lane;  // chunk position of bit 0
-        const uint64_t valid =
-            p0 >= len ? 0ull : (len - p0 >= 64u ? ~0ull : (1ull << (len - p0)) - 1ull);
-        // the window round's packed keys of this lane's word, ranked from its
-        // in-window mask; the first is requested with the cell record (a
-        // cell past that round uses it; for other cells the words are stale
-        // but the rank stays below the quarter's valid positions, in bounds)
-        const uint64_t G = Gw & valid;
-        const uint64_t W = Ww & valid;  // slots stay inside the cell
-        const uint32_t cw = (uint32_t)__popcll(W);
-        const uint32_t ex = wave_inclusive_scan(cw) - cw;
-        const int q0 = lane & ~(kQuarter - 1);  // first word of this lane's quarter
-        uint32_t rank = ex - (uint32_t)__shfl((int)ex, q0, 64);
-        const float *pk = pack + b + 64u * (uint32_t)q0;
-        const float v0 = W ? pk[rank] : 0.0f;
-        const CellRec &r = recs[cell];
-        const float cut = r.cut;
-        uint64_t L = 0ull, R = 0ull;
-        if (r.passes > kHistK) {  // uniform
-            uint64_t wd = W;
-            bool first_bit = true;
-            while (wd) {
-                const uint32_t bit = (uint32_t)__builtin_ctzll(wd);
-                wd &= wd - 1ull;
-                const float v = first_bit ? v0 : pk[rank];
-                ++rank;
-                first_bit = false;
-                L |= (!(v < cut) ? 1ull : 0ull) << bit;
-                R |= (!(v > cut) ? 1ull : 0ull) << bit;
-            }
-            L |= G;
-            R |= valid & ~G & ~W;
-        } else {
-            const float *key = axis_ptr(x, y, z, (int)axis);
-            for (int j0 = 0; j0 < kMaskWords; j0 += 16) {
-                float v[16];
+    for (uint32_t qa = (blockIdx.x * kW + (uint32_t)w) * kPer;; qa += gridDim.x * kW * kPer) {
+        // each chunk's record and its two window-round mask words per lane
+        uint4 d[kPer];
+        uint64_t Gw[kPer], Ww[kPer];
 #pragma unroll
-                for (int j = 0; j < 16; ++j) {
-                    const uint32_t p = b + 64u * (uint32_t)(j0 + j) + (uint32_t)lane;
-                    v[j] = p < e ? key[p] : 0.0f;
-                }
+        for (int i = 0; i < kPer; ++i) {
+            const uint32_t q = qa + (uint32_t)i;
+            d[i] = chunk_record(chunk_desc, q, max_chunks);
+            const uint64_t *mq = masks + (size_t)(q < max_chunks ? q : 0u) * (2 * kMaskWords);
+            Gw[i] = mq[lane];
+            Ww[i] = mq[kMaskWords + lane];
+        }
+        if (qa >= total) break;
+        // then each chunk's cell record and the first packed key of each lane's word
+        float cut[kPer], v0[kPer];
+        bool win[kPer];
+        uint32_t rank[kPer];
 #pragma unroll
-                for (int j = 0; j < 16; ++j) {
-                    const bool in = b + 64u * (uint32_t)(j0 + j) + (uint32_t)lane < e;
-                    const uint64_t mL = __ballot(in && !(v[j] < cut));
-                    const uint64_t mR = __ballot(in && !(v[j] > cut));
-                    L = lane == j0 + j ? mL : L;
-                    R = lane == j0 + j ? mR : R;
+        for (int i = 0; i < kPer; ++i) {
+            const uint32_t q = qa + (uint32_t)i;
+            const uint32_t b = uniform(d[i].x), e = uniform(d[i].y), cell = uniform(d[i].z);
+            const uint32_t axis = uniform(d[i].w);
+            cut[i] = 0.0f;
+            v0[i] = 0.0f;
+            win[i] = false;
+            rank[i] = 0u;
+            if (q >= total || axis > 2u) continue;  // uniform
+            const uint32_t len = e - b, p0 = 64u * (uint32_t)lane;  // chunk position of bit 0
+            const uint64_t valid =
+                p0 >= len ? 0ull : (len - p0 >= 64u ? ~0ull : (1ull << (len - p0)) - 1ull);
+            // the window round's packed keys of this lane's word, ranked from its
+            // in-window mask; the first is requested with the cell record (a
+            // cell past that round uses it; for other cells the words are stale
+            // but the rank stays below the quarter's valid positions, in bounds)
+            const uint64_t W = Ww[i] & valid;
+            const uint32_t cw = (uint32_t)__popcll(W);
+            const uint32_t ex = wave_inclusive_scan(cw) - cw;
+            const int q0 = lane & ~(kQuarter - 1);  // first word of this lane's quarter
+            rank[i] = ex - (uint32_t)__shfl((int)ex, q0, 64);
+            const float *pk = pack + b + 64u * (uint32_t)q0;
+            v0[i] = W ? pk[rank[i]] : 0.0f;
+            const CellRec &r = recs[cell];
+            cut[i] = r.cut;
+            win[i] = r.passes > kHistK;
+        }
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            const uint32_t q = qa + (uint32_t)i;
+            const uint32_t b = uniform(d[i].x), e = uniform(d[i].y);
+            const uint32_t axis = uniform(d[i].w);
+            if (q >= total) break;  // uniform
+            if (axis > 2u) {        // uniform
+                if (lane == 0) chunk_cnt[q] = 0ull;
+                continue;
+            }
+            uint64_t *mq = masks + (size_t)q * (2 * kMaskWords);
+            const uint32_t len = e - b, p0 = 64u * (uint32_t)lane;
+            const uint64_t valid =
+                p0 >= len ? 0ull : (len - p0 >= 64u ? ~0ull : (1ull << (len - p0)) - 1ull);
+            const float c = cut[i];
+            uint64_t L = 0ull, R = 0ull;
+            if (win[i]) {  // uniform
+                const uint64_t G = Gw[i] & valid;
+                const uint64_t W = Ww[i] & valid;
+                const int q0 = lane & ~(kQuarter - 1);
+                const float *pk = pack + b + 64u * (uint32_t)q0;
+                uint32_t rk = rank[i];
+                uint64_t wd = W;
+                bool first_bit = true;
+                while (wd) {
+                    const uint32_t bit = (uint32_t)__builtin_ctzll(wd);
+                    wd &= wd - 1ull;
+                    const float v = first_bit ? v0[i] : pk[rk];
+                    ++rk;
+                    first_bit = false;
+                    L |= (!(v < c) ? 1ull : 0ull) << bit;
+                    R |= (!(v > c) ? 1ull : 0ull) << bit;
+                }
+                L |= G;
+                R |= valid & ~G & ~W;
+            } else {
+                const float *key = axis_ptr(x, y, z, (int)axis);
+                for (int j0 = 0; j0 < kMaskWords; j0 += 16) {
+                    float v[16];
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) {
+                        const uint32_t p = b + 64u * (uint32_t)(j0 + j) + (uint32_t)lane;
+                        v[j] = p < e ? key[p] : 0.0f;
+                    }
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) {
+                        const bool in = b + 64u * (uint32_t)(j0 + j) + (uint32_t)lane < e;
+                        const uint64_t mL = __ballot(in && !(v[j] < c));
+                        const uint64_t mR = __ballot(in && !(v[j] > c));
+                        L = lane == j0 + j ? mL : L;
+                        R = lane == j0 + j ? mR : R;
+                    }
                 }
             }
+            mq[lane] = L;
+            mq[kMaskWords + lane] = R;
+            const uint32_t nl = wave_sum((uint32_t)__popcll(L));
+            const uint32_t nr = wave_sum((uint32_t)__popcll(R));
+            if (lane == 0) chunk_cnt[q] = (uint64_t)nl | ((uint64_t)nr << 32);
         }
-        mq[lane] = L;
-        mq[kMaskWords + lane] = R;
-        const uint32_t nl = wave_sum((uint32_t)__popcll(L));
-        const uint32_t nr = wave_sum((uint32_t)__popcll(R));
-        if (lane == 0) chunk_cnt[q] = (uint64_t)nl | ((uint64_t)nr << 32);
     }
 }
 
@@ -1530,7 +1571,8 @@ int launch_hoare(Ctx &c, int32_t nCells, int32_t *status, const HistCounts *hc, 
     if (!status) status = c.status.p;
     const unsigned gq = chunk_grid(c);
     if (hc && hc->pack)  // batched form: the window round left masks
-        hipLaunchKernelGGL(k_hoare_masks_window, dim3((gq + 3) / 4), dim3(kBlock), 0, c.stream,
+        hipLaunchKernelGGL(k_hoare_masks_window, dim3((gq + 4 * kMasksChunks - 1) / (4 * kMasksChunks)),
+                           dim3(kBlock), 0, c.stream,
                            c.recs.p, nCells, c.cell_first.p, c.chunk_desc.p,
                            (uint32_t)c.max_chunks, c.x(), c.y(), c.z(), hc->pack, c.masks.p,
                            c.chunk_cnt.p);
