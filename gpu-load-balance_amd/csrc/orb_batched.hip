@@ -2164,7 +2164,7 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
         ORB_HIP(hipGetLastError());
     }
     const unsigned gq = chunk_grid(c);
-    const unsigned gr = (unsigned)((gq + kReduceRows - 1) / kReduceRows * 4);
+    const unsigned gr = (unsigned)((gq + kReduceRows - 1) / kReduceRows);  // a block per 64-row tile
     const unsigned gw = (unsigned)((nCells + 3) / 4);
     if (c.weighted) {
         // weight-sum rounds over every key, then the partition from the keys
