@@ -2077,8 +2077,8 @@ static std::vector<uint64_t> graph_signature(const Ctx &c, const BatchedState &b
     const void *bufs[] = {c.pos.p, c.tmp.p, c.pid.p, c.pid_tmp.p, c.ranges.p, c.recs.p,
                           c.cell_nchunks.p, c.cell_first.p, c.chunk_cell.p, c.chunk_desc.p,
                           c.chunk_ltle.p, c.chunk_cnt.p, c.chunk_pre.p, c.scan_partials.p,
-                          c.packed.p, c.masks.p, c.cell_cand.p, c.splits.p, c.chunk_comp.p, c.sel.p, c.sel_lists.p, c.swap_plan.p,
-                          c.status.p, c.stamps.p, c.cnt.p, b.level_ids.p, b.heap.p, b.exists.p,
+                          c.packed.p, c.masks.p, c.cell_cand.p, c.splits.p, c.chunk_comp.p,
+                          c.sel.p, c.sel_lists.p, c.swap_plan.p, c.status.p, c.stamps.p, c.cnt.p, b.level_ids.p, b.heap.p, b.exists.p,
                           b.sub.p, b.chunk_hist.p, b.cell_hist.p, b.trace.p, b.level_passes.p,
                           c.wt.p, c.wt_tmp.p, b.chunk_hist_w.p, b.cell_hist_w.p};
     for (const void *p : bufs) s.push_back((uint64_t)(uintptr_t)p);
