@@ -642,6 +642,51 @@ __device__ inline void record_level_passes(int32_t *level_passes, int32_t l, int
     if (passes > *((volatile int32_t *)&level_passes[l])) atomicMax(&level_passes[l], passes);
 }
 
+// A level's pass count is the most passes any of its cells took, so a cell
+// records its count once, when it finishes.  The cells a wave (or block)
+// finishes share one running maximum, recorded per run of cells with the
+// same counter (one build's level) and at the end: thousands of cells of a
+// level finishing in one round would otherwise queue thousands of atomics
+// on one word.
+struct PassMax {
+    int32_t lp = -1, mx = 0;
+    __device__ void add(int32_t *level_passes, int32_t l, int32_t p)
+    {
+        if (l != lp) {
+            flush(level_passes);
+            lp = l;
+            mx = p;
+        } else if (p > mx) {
+            mx = p;
+        }
+    }
+    __device__ void flush(int32_t *level_passes)
+    {
+        if (lp >= 0) record_level_passes(level_passes, lp, mx);
+        lp = -1;
+    }
+};
+
+// End of a 4-wave block whose lanes 0 each hold a PassMax: the waves' last
+// runs are combined in LDS and recorded by one thread (one atomic per block
+// where its waves share a counter).  Every thread of the block calls it.
+__device__ inline void flush_block(PassMax &pm, int32_t *level_passes)
+{
+    __shared__ int32_t s_lp[kBlock / 64], s_mx[kBlock / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        s_lp[w] = pm.lp;
+        s_mx[w] = pm.mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        PassMax b;
+        for (int i = 0; i < kBlock / 64; ++i)
+            if (s_lp[i] >= 0) b.add(level_passes, s_lp[i], s_mx[i]);
+        b.flush(level_passes);
+    }
+}
+
 // One histogram round of the bisection controller (orbit.cpp:149-231),
 // replayed by a whole wave.  P[j] (j in [1, 255]) is the count below
 // candidate cut j of the round that starts from margins [lo, hi].  Counts
@@ -750,10 +795,11 @@ __device__ inline int32_t found_encoding(const RoundOut &ro, float lo, float hi)
 
 // Lane 0 of the replaying wave: the round's margins into the cell's record
 // and heap cell; a finished cell (cut found, or the pass cap with the last
-// midpoint) is split at once (orbit.cpp:235-250).
+// midpoint) records its passes (into pm when given) and is split at once
+// (orbit.cpp:235-250).
 __device__ inline void finish_round(CellRec r, int32_t c, const RoundOut &ro, int32_t found_enc,
                                     CellRec *recs, orb_cell *heap, int32_t *level_passes,
-                                    uint8_t *exists, int32_t *status)
+                                    uint8_t *exists, int32_t *status, PassMax *pm = nullptr)
 {
     const float L = ro.L, R = ro.R;
     const int32_t passes = ro.passes;
@@ -768,8 +814,9 @@ __device__ inline void finish_round(CellRec r, int32_t c, const RoundOut &ro, in
     hc.cutMarginLeft = L;
     hc.cutMarginRight = R;
     hc.foundCut = found;
-    record_level_passes(level_passes, r.lp, passes);
-    if (!found && passes < ORB_MAX_PASSES) return;
+    if (!found && passes < ORB_MAX_PASSES) return;  // a later round records it
+    if (pm) pm->add(level_passes, r.lp, passes);
+    else record_level_passes(level_passes, r.lp, passes);
     const orb_cell p = hc;
     orb_cell lc, rc;
     uint32_t *wl = reinterpret_cast<uint32_t *>(&lc);
@@ -822,6 +869,7 @@ __global__ __launch_bounds__(kBlock) void k_b_replay(CellRec *recs, int32_t nCel
     __shared__ uint32_t P[4][kBins + 1];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     auto active = [&](int32_t c) { return !recs[c].found && !recs[c].flags; };
+    PassMax pm;  // lane 0
     for_cell_groups(nCells, group, active, [&](int32_t c) {
     const CellRec r = recs[c];
     uint4 *row = reinterpret_cast<uint4 *>(cell_hist + (size_t)c * kBins);
@@ -867,8 +915,9 @@ __global__ __launch_bounds__(kBlock) void k_b_replay(CellRec *recs, int32_t nCel
     }
     if (lane != 0) return;
     finish_round(r, c, ro, found_encoding(ro, r.margin_l, r.margin_r), recs, heap, level_passes,
-                 exists, status);
+                 exists, status, &pm);
     });
+    flush_block(pm, level_passes);
 }
 
 // Rounds 3 and 4 read only the keys that the window round packed and that
@@ -1029,6 +1078,7 @@ __global__ __launch_bounds__(kBlock) void k_b_finish(CellRec *recs, int32_t nCel
     float *cw = cand[w];
     uint32_t *pw = P[w];
     auto active = [&](int32_t c) { return !recs[c].found && !recs[c].flags; };
+    PassMax pm;  // lane 0
     for_cell_groups(nCells, group, active, [&](int32_t c) {
     CellRec r = recs[c];
     const uint32_t f = first[c], nch = first[c + 1] - f;
@@ -1111,7 +1161,7 @@ __global__ __launch_bounds__(kBlock) void k_b_finish(CellRec *recs, int32_t nCel
         if (round == 1 || ro.found || ro.passes >= ORB_MAX_PASSES) {  // uniform
             if (lane == 0)
                 finish_round(r, c, ro, found_encoding(ro, lo, hi), recs, heap, level_passes,
-                             exists, status);
+                             exists, status, &pm);
             break;
         }
         r.margin_l = ro.L;
@@ -1119,6 +1169,7 @@ __global__ __launch_bounds__(kBlock) void k_b_finish(CellRec *recs, int32_t nCel
         r.passes = ro.passes;
     }
     });
+    flush_block(pm, level_passes);
 }
 
 // The same two rounds with a block per cell, for levels of few cells (whose
@@ -1142,6 +1193,7 @@ __global__ __launch_bounds__(kBlock) void k_b_finish_block(CellRec *recs, int32_
     __shared__ float s_lr[2];
     __shared__ int32_t s_pf[2];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    PassMax pm;  // thread 0
     for (int32_t c = blockIdx.x; c < nCells; c += gridDim.x) {
         CellRec r = recs[c];
         if (r.found || r.flags) continue;  // uniform
@@ -1230,7 +1282,7 @@ __global__ __launch_bounds__(kBlock) void k_b_finish_block(CellRec *recs, int32_
             if (round == 1 || ro.found || ro.passes >= ORB_MAX_PASSES) {  // uniform
                 if (t == 0)
                     finish_round(r, c, ro, found_encoding(ro, lo, hi), recs, heap, level_passes,
-                                 exists, status);
+                                 exists, status, &pm);
                 break;
             }
             r.margin_l = ro.L;
@@ -1238,6 +1290,7 @@ __global__ __launch_bounds__(kBlock) void k_b_finish_block(CellRec *recs, int32_
             r.passes = ro.passes;
         }
     }
+    if (t == 0) pm.flush(level_passes);
 }
 
 // ---------------------------------------------------------------------------
