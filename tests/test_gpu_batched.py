@@ -230,3 +230,20 @@ def test_batched_keys_on_bin_edges(orb, oracle, cmp):
     z = (rng.random(n) - 0.5).astype(np.float32)
     g, r = run_both(orb, oracle, x, y, z, 64, 1, cmp, 0, form=BATCHED)
     assert_same_build(g, r)
+
+
+@pytest.mark.parametrize("cmp", [0, 1])
+def test_batched_dense_window(orb, oracle, cmp):
+    """Almost every key of the root cell lies within 1e-6 of the median while
+    a few outliers at +-0.5 set its box: the second round's window holds
+    nearly the whole cell, so the gather for rounds 3 and 4 fills its lists to
+    their full span and those rounds bin every key of the cell."""
+    rng = np.random.default_rng(21)
+    n = (1 << 18) + 5
+    x = (rng.standard_normal(n) * 1e-7).astype(np.float32)
+    x[:6] = np.float32(-0.5)
+    x[6:12] = np.float32(0.5)
+    y = ((rng.random(n) - 0.5) * 0.9).astype(np.float32)
+    z = ((rng.random(n) - 0.5) * 0.9).astype(np.float32)
+    g, r = run_both(orb, oracle, x, y, z, 256, 1, cmp, 0, form=BATCHED)
+    assert_same_build(g, r)
