@@ -60,6 +60,16 @@ struct BatchedState {
     bool hist_clean = false;
 };
 
+// The previous level's due row swaps as a kernel argument (nCells = 0:
+// none).
+struct PendingFinalDev {
+    const LevelEntry *entries;
+    int32_t nCells;
+    const SplitRec *splits;
+    float *x, *y, *z;
+    uint32_t *pid, *w;
+};
+
 // ---------------------------------------------------------------------------
 // schedule (host): ids per level, data-independent
 // ---------------------------------------------------------------------------
@@ -196,10 +206,29 @@ __device__ inline uint32_t level_rec(const LevelEntry *entries, int32_t i, const
     return r.flags ? 0u : (n + (uint32_t)kChunk - 1u) / (uint32_t)kChunk;
 }
 
+// The previous level's last Hoare row swaps, from blocks [fb0, gridDim.x) of
+// a level's first launch: cell c's rows from its range in the map and its
+// split (the cell records are being rewritten by the other blocks).
+__device__ inline void level_finals(uint32_t fb0, const PendingFinalDev &fin,
+                                    const uint32_t *ranges)
+{
+    const int32_t c = (int32_t)((blockIdx.x - fb0) * blockDim.x + threadIdx.x);
+    if (c >= fin.nCells) return;
+    const LevelEntry en = fin.entries[c];
+    const int64_t g = (int64_t)en.hb + en.id;
+    hoare_final_rows(ranges[2 * g], ranges[2 * g + 1], fin.splits[c], fin.x, fin.y, fin.z, fin.pid,
+                     fin.w);
+}
+
 __global__ void k_b_recs(const LevelEntry *entries, int32_t nCells, const orb_cell *heap,
                          const uint32_t *ranges, const uint8_t *sub, int use_subtree,
-                         uint32_t sub_max, CellRec *recs, uint32_t *nchunks)
+                         uint32_t sub_max, CellRec *recs, uint32_t *nchunks, uint32_t rec_blocks,
+                         PendingFinalDev fin)
 {
+    if (blockIdx.x >= rec_blocks) {  // uniform
+        level_finals(rec_blocks, fin, ranges);
+        return;
+    }
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i > nCells) return;
     nchunks[i] = i == nCells ? 0u
@@ -214,8 +243,13 @@ constexpr int kRecsScan = 1024;
 __global__ __launch_bounds__(kRecsScan) void k_b_recs_scan(
     const LevelEntry *__restrict__ entries, int32_t nCells, const orb_cell *__restrict__ heap,
     const uint32_t *__restrict__ ranges, const uint8_t *__restrict__ sub, int use_subtree,
-    uint32_t sub_max, CellRec *__restrict__ recs, uint32_t *__restrict__ first)
+    uint32_t sub_max, CellRec *__restrict__ recs, uint32_t *__restrict__ first,
+    PendingFinalDev fin)
 {
+    if (blockIdx.x > 0) {  // uniform: the previous level's row swaps
+        level_finals(1u, fin, ranges);
+        return;
+    }
     constexpr int kW = kRecsScan / 64;
     __shared__ uint32_t wsum[kW];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -2145,7 +2179,8 @@ static std::vector<uint64_t> graph_signature(const Ctx &c, const BatchedState &b
 int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *heap,
                   uint8_t *exists, uint8_t *sub, uint16_t *chunk_hist, uint32_t *cell_hist,
                   uint64_t *trace, int32_t *level_passes, int32_t *status, bool any_subtree,
-                  bool full_cover, int32_t level, uint64_t *chunk_hist_w, uint64_t *cell_hist_w)
+                  bool full_cover, int32_t level, uint64_t *chunk_hist_w, uint64_t *cell_hist_w,
+                  const PendingFinal *fin_in, bool defer_final)
 {
     if (c.weighted && (!chunk_hist_w || !cell_hist_w)) return ORB_ERR_STATE;
     if (!c.weighted && (c.sel.n < (size_t)c.stride || c.sel_lists.n < (size_t)c.max_chunks + 1))
@@ -2166,16 +2201,25 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
         Ctx &c;
         ~LevelScope() { c.cur_level = -1; }
     } scope{c};
+    // the level's first launch also runs the previous level's last row swaps
+    // (extra blocks: the records it rewrites are not what they read)
+    PendingFinalDev fin{nullptr, 0, c.splits.p, c.x(), c.y(), c.z(), c.ids(), c.weights()};
+    if (fin_in && fin_in->nCells > 0) {
+        fin.entries = fin_in->entries;
+        fin.nCells = fin_in->nCells;
+    }
     if (nCells < kRecsScan) {
-        hipLaunchKernelGGL(k_b_recs_scan, dim3(1), dim3(kRecsScan), 0, c.stream, entries, nCells, heap,
-                           c.ranges.p, sub, use_subtree ? 1 : 0, (uint32_t)sub_max, c.recs.p,
-                           c.cell_first.p);
+        const unsigned gf = (unsigned)((fin.nCells + kRecsScan - 1) / kRecsScan);
+        hipLaunchKernelGGL(k_b_recs_scan, dim3(1 + gf), dim3(kRecsScan), 0, c.stream, entries, nCells,
+                           heap, c.ranges.p, sub, use_subtree ? 1 : 0, (uint32_t)sub_max, c.recs.p,
+                           c.cell_first.p, fin);
         ORB_HIP(hipGetLastError());
     } else {
         unsigned g1 = (unsigned)((nCells + 1 + kBlock - 1) / kBlock);
-        hipLaunchKernelGGL(k_b_recs, dim3(g1), dim3(kBlock), 0, c.stream, entries, nCells, heap,
+        const unsigned gf = (unsigned)((fin.nCells + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL(k_b_recs, dim3(g1 + gf), dim3(kBlock), 0, c.stream, entries, nCells, heap,
                            c.ranges.p, sub, use_subtree ? 1 : 0, (uint32_t)sub_max, c.recs.p,
-                           c.cell_nchunks.p);
+                           c.cell_nchunks.p, g1, fin);
         ORB_HIP(hipGetLastError());
         ORB_TRY(scan_exclusive_u32(c, c.cell_nchunks.p, c.cell_first.p, (size_t)nCells + 1));
     }
@@ -2241,7 +2285,7 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
         }
         const int tp = timing_mark(c);
         if (c.part == ORB_PART_HOARE) {
-            ORB_TRY(launch_hoare(c, nCells, status, nullptr));
+            ORB_TRY(launch_hoare(c, nCells, status, nullptr, 0, defer_final));
         } else {
             ORB_TRY(invalidate_stamps(c, nCells));  // the stopper counts read the keys
             ORB_TRY(launch_stable_le(c, nCells, status, full_cover, nullptr));
@@ -2313,7 +2357,7 @@ int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *h
     hc.le = le;
     hc.pack = c.packed.p;
     hc.comp = c.chunk_comp.p;
-    if (c.part == ORB_PART_HOARE) ORB_TRY(launch_hoare(c, nCells, status, &hc));
+    if (c.part == ORB_PART_HOARE) ORB_TRY(launch_hoare(c, nCells, status, &hc, 0, defer_final));
     else ORB_TRY(launch_stable_le(c, nCells, status, full_cover, &hc));
     ORB_TRY(timing_pair(c, tp, kTimePartition));
     return ORB_OK;
@@ -2352,6 +2396,9 @@ static int enqueue_build(Ctx &c, int32_t levels_mode)
             done = done || all_sub;
         }
     }
+    // a level's last Hoare row swaps run in the next level's first launch;
+    // the last level's at the end
+    PendingFinal fin;
     for (int32_t l = 1; l <= lmax; ++l) {
         const int32_t nCells = b.level_n[(size_t)l];
         if (nCells == 0) continue;
@@ -2363,12 +2410,16 @@ static int enqueue_build(Ctx &c, int32_t levels_mode)
         // -- only a speed choice, every cell is built either way
         const int64_t few = c.n * (int64_t)b.level_min_leaf[(size_t)l] / (c.d > 0 ? c.d : 1);
         const bool may_sub = few <= 4 * (int64_t)subtree_max_keys(c);
-        ORB_TRY(enqueue_level(c, b.level_ids.p + b.level_off[(size_t)l], nCells, b.heap.p,
-                              b.exists.p, b.sub.p, b.chunk_hist.p, b.cell_hist.p, trace,
-                              b.level_passes.p, c.status.p, b.complete && may_sub,
-                              b.level_cover[(size_t)l] != 0, l, b.chunk_hist_w.p, b.cell_hist_w.p));
+        const LevelEntry *entries = b.level_ids.p + b.level_off[(size_t)l];
+        ORB_TRY(enqueue_level(c, entries, nCells, b.heap.p, b.exists.p, b.sub.p, b.chunk_hist.p,
+                              b.cell_hist.p, trace, b.level_passes.p, c.status.p,
+                              b.complete && may_sub, b.level_cover[(size_t)l] != 0, l,
+                              b.chunk_hist_w.p, b.cell_hist_w.p, &fin, true));
+        fin.entries = entries;
+        fin.nCells = c.part == ORB_PART_HOARE ? nCells : 0;
         ORB_TRY(timing_pair(c, tlevel, kTimeLevel + l));
     }
+    if (fin.nCells > 0) ORB_TRY(launch_hoare_final(c, fin.nCells));
     return particles_home(c);  // in-stream: the capture ends where it began
 }
 

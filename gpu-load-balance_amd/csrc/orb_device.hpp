@@ -183,6 +183,36 @@ __device__ inline void fill_chunk(uint32_t q, const uint32_t *first, int32_t nCe
     chunk_desc[q] = make_uint4(b, e, cell, live ? (uint32_t)r.axis : kChunkIdle);
 }
 
+// Hoare's last step for one cell (partition.cpp:61-62): after the pair
+// swaps, rows i and n-1 of the cell [begin, end) trade places (x, y, z and
+// the payload arrays that are on).  Nothing happens for an empty cell or one
+// the split reported (TL = 0).
+__device__ inline void hoare_final_rows(uint32_t begin, uint32_t end, const SplitRec &s, float *x,
+                                        float *y, float *z, uint32_t *pid, uint32_t *w)
+{
+    const uint32_t n = end > begin ? end - begin : 0u;
+    if (n == 0 || s.TL == 0) return;  // reported by k_hoare_split
+    const uint32_t pa = begin + s.i, pb = begin + n - 1;
+    if (pa == pb) return;
+    const float ax = x[pa], ay = y[pa], az = z[pa];
+    x[pa] = x[pb];
+    y[pa] = y[pb];
+    z[pa] = z[pb];
+    x[pb] = ax;
+    y[pb] = ay;
+    z[pb] = az;
+    if (pid) {
+        const uint32_t ia = pid[pa];
+        pid[pa] = pid[pb];
+        pid[pb] = ia;
+    }
+    if (w) {
+        const uint32_t wa = w[pa];
+        w[pa] = w[pb];
+        w[pb] = wa;
+    }
+}
+
 __device__ inline const float *axis_ptr(const float *x, const float *y, const float *z, int a)
 {
     return a == 0 ? x : (a == 1 ? y : z);

@@ -586,7 +586,8 @@ struct HistCounts {
     const uint32_t *comp = nullptr;   // [chunks][8] packed counts per share, keys below
 };
 int launch_hoare(Ctx &c, int32_t nCells, int32_t *status = nullptr,
-                 const HistCounts *hc = nullptr, int rev = 0);
+                 const HistCounts *hc = nullptr, int rev = 0, bool defer_final = false);
+int launch_hoare_final(Ctx &c, int32_t nCells);  // rows i and n-1 of the level's cells
 int launch_stopper_counts(Ctx &c, int32_t nCells, bool le_only, const HistCounts *hc = nullptr);
 int invalidate_stamps(Ctx &c, int32_t nCells);
 int launch_stable_le(Ctx &c, int32_t nCells, int32_t *status = nullptr, bool full_cover = false,
@@ -603,13 +604,23 @@ int schedule_levels(int32_t d, int32_t levels_mode, int64_t cap,
                     std::vector<int32_t> *min_leaf = nullptr);
 int launch_make_axis(Ctx &c, int32_t nCells);
 int launch_cell_sizes(Ctx &c, int32_t nCells, uint32_t *dst);
+// The previous level's cells whose last Hoare row swaps are still due
+// (entries, count): a batched build runs them in the next level's first
+// launch.
+struct PendingFinal {
+    const LevelEntry *entries = nullptr;
+    int32_t nCells = 0;
+};
 // one level step of the batched form over `entries` (device array)
-// (chunk_hist_w, cell_hist_w: the weighted rounds' rows, when c.weighted)
+// (chunk_hist_w, cell_hist_w: the weighted rounds' rows, when c.weighted;
+// fin_in: the previous level's due row swaps, run in this level's first
+// launch; defer_final: leave this level's to the next)
 int enqueue_level(Ctx &c, const LevelEntry *entries, int32_t nCells, orb_cell *heap,
                   uint8_t *exists, uint8_t *sub, uint16_t *chunk_hist, uint32_t *cell_hist,
                   uint64_t *trace, int32_t *level_passes, int32_t *status, bool any_subtree,
                   bool full_cover, int32_t level, uint64_t *chunk_hist_w = nullptr,
-                  uint64_t *cell_hist_w = nullptr);
+                  uint64_t *cell_hist_w = nullptr, const PendingFinal *fin_in = nullptr,
+                  bool defer_final = false);
 int preallocate_scratch(Ctx &c);
 // level tables for nCells cells and (at least) `chunks` chunks
 int ensure_level_capacity(Ctx &c, int64_t nCells, int64_t chunks = 0);

@@ -1435,24 +1435,16 @@ __global__ void k_hoare_final(const CellRec *recs, int32_t nCells, const SplitRe
     int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nCells) return;
     const CellRec r = recs[c];
-    const SplitRec s = splits[c];
-    uint32_t n = r.end - r.begin;
-    if (n == 0 || s.TL == 0) return;  // reported by k_hoare_split
-    uint32_t pa = r.begin + s.i, pb = r.begin + n - 1;
-    if (pa == pb) return;
-    float ax = x[pa], ay = y[pa], az = z[pa];
-    x[pa] = x[pb]; y[pa] = y[pb]; z[pa] = z[pb];
-    x[pb] = ax; y[pb] = ay; z[pb] = az;
-    if (pid) {
-        const uint32_t ia = pid[pa];
-        pid[pa] = pid[pb];
-        pid[pb] = ia;
-    }
-    if (w) {
-        const uint32_t wa = w[pa];
-        w[pa] = w[pb];
-        w[pb] = wa;
-    }
+    hoare_final_rows(r.begin, r.end, splits[c], x, y, z, pid, w);
+}
+
+int launch_hoare_final(Ctx &c, int32_t nCells)
+{
+    const unsigned gc = (unsigned)((nCells + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_hoare_final, dim3(gc > 0 ? gc : 1), dim3(kBlock), 0, c.stream, c.recs.p,
+                       nCells, c.splits.p, c.x(), c.y(), c.z(), c.ids(), c.weights());
+    ORB_HIP(hipGetLastError());
+    return ORB_OK;
 }
 
 // Stopper counts of a chunk from the histogram round that found its cell's
@@ -1566,7 +1558,8 @@ __global__ void k_sum_swap_pairs(const CellRec *recs, const SplitRec *splits, in
     if ((threadIdx.x & 63) == 0 && k) atomicAdd(acc, k);
 }
 
-int launch_hoare(Ctx &c, int32_t nCells, int32_t *status, const HistCounts *hc, int rev)
+int launch_hoare(Ctx &c, int32_t nCells, int32_t *status, const HistCounts *hc, int rev,
+                 bool defer_final)
 {
     if (!status) status = c.status.p;
     const unsigned gq = chunk_grid(c);
@@ -1613,11 +1606,9 @@ int launch_hoare(Ctx &c, int32_t nCells, int32_t *status, const HistCounts *hc, 
                        rev);
     ORB_HIP(hipGetLastError());
     ORB_TRY(timing_pair(c, ts, kTimeSwap));
-    unsigned gc = (unsigned)((nCells + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(k_hoare_final, dim3(gc), dim3(kBlock), 0, c.stream, c.recs.p, nCells,
-                       c.splits.p, c.x(), c.y(), c.z(), c.ids(), c.weights());
-    ORB_HIP(hipGetLastError());
-    return ORB_OK;
+    // the last row swaps: here, or (batched builds) in the next level's first
+    // launch, which reads no particle
+    return defer_final ? ORB_OK : launch_hoare_final(c, nCells);
 }
 
 // ---------------------------------------------------------------------------
