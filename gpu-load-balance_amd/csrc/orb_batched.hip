@@ -1478,6 +1478,13 @@ struct SubCell {
     int32_t pad;
 };
 
+// An LDS address as the buffer-to-LDS loads take it (the LDS address space).
+typedef __attribute__((address_space(3))) void LdsVoid;
+__device__ inline LdsVoid *lds_ptr(void *p)
+{
+    return (LdsVoid *)p;
+}
+
 __device__ inline void lds_wave_sync()
 {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
@@ -1684,31 +1691,32 @@ __global__ __launch_bounds__(kSubBlock) void k_b_subtree(const CellRec *recs, in
     uint32_t root_w = 0u;
     if (t < kCellWords) root_w = reinterpret_cast<const uint32_t *>(heap + hb + rr.id)[t];
     {
-        // every load of the cell in flight before the first LDS store
-        constexpr int kIt = (kCap + kSubBlock - 1) / kSubBlock;
-        float vx[kIt], vy[kIt], vz[kIt];
-        uint32_t vw[W ? kIt : 1];
-#pragma unroll
-        for (int u = 0; u < kIt; ++u) {
-            const uint32_t k = (uint32_t)(t + u * kSubBlock);
-            if (k < n) {
-                vx[u] = x[begin + k];
-                vy[u] = y[begin + k];
-                vz[u] = z[begin + k];
-                if constexpr (W) vw[u] = wgt[begin + k];
+        // the cell's arrays straight from memory into LDS, through no
+        // registers: wave w takes the 64-key pieces that start at
+        // 64 w + 1024 u below n, one buffer load to LDS per piece and array,
+        // all in flight together.  The range check is the cell's byte count:
+        // keys past n land as 0 in slots below kCap (a piece starts below
+        // n <= kCap, and kCap is a multiple of 64).
+        static_assert(kCap % 64 == 0, "whole 64-key pieces");
+        const int nb = (int)(n * 4u);
+        const __amdgpu_buffer_rsrc_t rx = __builtin_amdgcn_make_buffer_rsrc(x + begin, 0, nb, 0x00020000);
+        const __amdgpu_buffer_rsrc_t ry = __builtin_amdgcn_make_buffer_rsrc(y + begin, 0, nb, 0x00020000);
+        const __amdgpu_buffer_rsrc_t rz = __builtin_amdgcn_make_buffer_rsrc(z + begin, 0, nb, 0x00020000);
+        for (uint32_t k0 = 64u * (uint32_t)w; k0 < n; k0 += (uint32_t)kSubBlock) {
+            const int vo = (int)(4u * (k0 + (uint32_t)lane));
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rx, lds_ptr(sx + k0), 4, vo, 0, 0, 0);
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(ry, lds_ptr(sy + k0), 4, vo, 0, 0, 0);
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rz, lds_ptr(sz + k0), 4, vo, 0, 0, 0);
+            if constexpr (W) {
+                const __amdgpu_buffer_rsrc_t rw =
+                    __builtin_amdgcn_make_buffer_rsrc(wgt + begin, 0, nb, 0x00020000);
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rw, lds_ptr(sw + k0), 4, vo, 0, 0, 0);
             }
         }
-#pragma unroll
-        for (int u = 0; u < kIt; ++u) {
-            const uint32_t k = (uint32_t)(t + u * kSubBlock);
-            if (k < n) {
-                sx[k] = vx[u];
-                sy[k] = vy[u];
-                sz[k] = vz[u];
-                if constexpr (W) sw[k] = vw[u];
-                if (IDS) sidx[k] = (uint16_t)k;
-            }
-        }
+        if (IDS)
+            for (uint32_t k = (uint32_t)t; k < n; k += (uint32_t)kSubBlock) sidx[k] = (uint16_t)k;
+        // the loads have written LDS before the barrier below lets any wave read it
+        __builtin_amdgcn_s_waitcnt(0);
     }
     int cb = 0;  // scells[cb]: this depth's cells
     if (t < kCellWords) reinterpret_cast<uint32_t *>(&scells[0][0].cell)[t] = root_w;
