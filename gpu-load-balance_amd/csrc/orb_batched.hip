@@ -1633,6 +1633,10 @@ __global__ __launch_bounds__(kSubBlock) void k_b_subtree(const CellRec *recs, in
     __shared__ uint16_t sidx[IDS ? kCap : 1];
     uint16_t *const slpos = reinterpret_cast<uint16_t *>(spos_raw);
     uint16_t *const srpos = slpos + kCap;
+    // during a histogram round the stopper lists are free: each wave queues
+    // its slow keys there (and with W their weights in the second half)
+    constexpr int kQueue = ((W ? kCap / 2 : kCap) / kSubWaves) & ~63;
+    static_assert(kQueue >= 64 && kSubWaves * kQueue <= (W ? kCap / 2 : kCap), "queue fits");
     __shared__ HistWord shist[kSubWaves][kHistWords];
     __shared__ SubCell scells[2][kSmallLeaves];  // a depth's cells and the next's
     __shared__ float sL[kSubWaves], sR[kSubWaves];
@@ -1646,6 +1650,7 @@ __global__ __launch_bounds__(kSubBlock) void k_b_subtree(const CellRec *recs, in
     __shared__ int32_t s_lp0;
     __shared__ uint64_t s_gmask;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const uint64_t lanes_below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     if (t < 64) s_lpmax[t] = 0;
     if (t == 0) s_lp0 = -1;
     // every wave histogram starts clear and is left clear by the round that
@@ -1774,11 +1779,11 @@ __global__ __launch_bounds__(kSubBlock) void k_b_subtree(const CellRec *recs, in
             float L = cl.cutMarginLeft, R = cl.cutMarginRight;
             int32_t passes = 0;
             bool found = false;
-            // each wave bins its part into its own histogram; the slot's first
-            // wave sums the slot's g histograms, clears them and replays.  Two
+            // the slot's g waves bin their parts into the slot's one histogram
+            // (its first wave's); that wave replays from it and clears it.  Two
             // barriers per round; amask (the same in every wave) holds the
             // batch's slots still searching.
-            HistWord *const myhist = shist[w];
+            HistWord *const myhist = shist[w0];
             uint32_t amask = (uint32_t)__ballot(lane < nslots && scells[cb][bt * nslots + lane].valid);
             while (amask) {
                 const bool act = ((amask >> slot) & 1u) != 0u;
@@ -1786,6 +1791,30 @@ __global__ __launch_bounds__(kSubBlock) void k_b_subtree(const CellRec *recs, in
                     const float width = R - L;
                     const float scale = width > 0.0f ? (float)kBins / width : 0.0f;
                     const float lim = 0.5f - bin_tolerance(L, R, scale);
+                    // keys outside [L, R] (most of them after the first
+                    // round) are counted in registers and added to bins 0 and
+                    // 255 once per wave: as atomics they would all hit the
+                    // same two LDS words
+                    uint32_t nbelow = 0, nabove = 0;     // wave-uniform counts
+                    uint64_t wbelow = 0ull, wabove = 0ull;  // W: this lane's weights
+                    // keys too close to a bin edge for the estimate (about 1%
+                    // in a first round) queue in the wave's part of the free
+                    // stopper-list space and are walked 64 at a time, not one
+                    // divergent walk per load that holds one
+                    float *const qk = reinterpret_cast<float *>(spos_raw) + w * kQueue;
+                    uint32_t *const qw = spos_raw + (W ? kCap / 2 : 0) + w * kQueue;  // W: weights
+                    uint32_t qn = 0;  // wave-uniform
+                    auto drain = [&]() {
+                        lds_wave_sync();  // the queue's stores before its loads
+                        for (uint32_t k = (uint32_t)lane; k < qn; k += 64u) {
+                            const float vv = qk[k];
+                            const int b = le_cmp ? bin_walk<1>(vv, L, R) : bin_walk<0>(vv, L, R);
+                            if constexpr (W) atomicAdd(&myhist[b], (unsigned long long)qw[k]);
+                            else atomicAdd(&myhist[b >> 1], (b & 1) ? 0x10000u : 1u);
+                        }
+                        lds_wave_sync();  // its loads before the next stores
+                        qn = 0;
+                    };
                     for (uint32_t base = pb; base < pe; base += 256u) {
                         float v[4];
                         uint32_t wv[W ? 4 : 1];
@@ -1795,18 +1824,64 @@ __global__ __launch_bounds__(kSubBlock) void k_b_subtree(const CellRec *recs, in
                             v[u] = p < pe ? key[p] : 0.0f;
                             if constexpr (W) wv[u] = p < pe ? sw[p] : 0u;
                         }
+                        // the four bins first, without branches, then the
+                        // queueing of slow keys, then the atomics of the other
+                        // keys in the window: no key waits for the previous
+                        // key's atomic
+                        int bin[4];
+                        bool slow[4], fast[4];
 #pragma unroll
                         for (int u = 0; u < 4; ++u) {
-                            if (base + (uint32_t)(64 * u + lane) >= pe) continue;
+                            const bool in = base + (uint32_t)(64 * u + lane) < pe;
                             const float vv = v[u];
-                            int bin;
-                            if (vv < L) bin = 0;                 // below every candidate
-                            else if (vv > R) bin = kBins - 1;    // above every candidate
-                            else if (!bin_estimate(vv, L, scale, lim, bin))
-                                bin = le_cmp ? bin_walk<1>(vv, L, R) : bin_walk<0>(vv, L, R);
-                            if constexpr (W) atomicAdd(&myhist[bin], (unsigned long long)wv[u]);
-                            else atomicAdd(&myhist[bin >> 1], (bin & 1) ? 0x10000u : 1u);
+                            const bool below = vv < L, above = vv > R;  // past every candidate
+                            // bin_estimate's test; the clamp only keeps the
+                            // conversion of keys outside [L, R] in range
+                            const float ef = (vv - L) * scale;
+                            bin[u] = (int)fminf(fmaxf(ef, 0.0f), 256.0f);
+                            const bool est = fabsf(__builtin_amdgcn_fractf(ef) - 0.5f) < lim;
+                            const bool inwin = in && !below && !above;
+                            slow[u] = inwin && !est;
+                            fast[u] = inwin && est;
+                            if constexpr (W) {
+                                wbelow += in && below ? (uint64_t)wv[u] : 0ull;
+                                wabove += in && above ? (uint64_t)wv[u] : 0ull;
+                            } else {
+                                nbelow += (uint32_t)__popcll(__ballot(in && below));
+                                nabove += (uint32_t)__popcll(__ballot(in && above));
+                            }
                         }
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const uint64_t sm = __ballot(slow[u]);
+                            if (sm) {  // wave-uniform
+                                if (qn + 64u > (uint32_t)kQueue) drain();
+                                if (slow[u]) {
+                                    const uint32_t at = qn + (uint32_t)__popcll(sm & lanes_below);
+                                    qk[at] = v[u];
+                                    if constexpr (W) qw[at] = wv[u];
+                                }
+                                qn += (uint32_t)__popcll(sm);
+                            }
+                        }
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            if (!fast[u]) continue;
+                            if constexpr (W) atomicAdd(&myhist[bin[u]], (unsigned long long)wv[u]);
+                            else atomicAdd(&myhist[bin[u] >> 1], (bin[u] & 1) ? 0x10000u : 1u);
+                        }
+                    }
+                    if (qn) drain();
+                    if constexpr (W) {
+                        const uint64_t sb = wave_sum(wbelow), sa = wave_sum(wabove);
+                        if (lane == 0) {
+                            if (sb) atomicAdd(&myhist[0], (unsigned long long)sb);
+                            if (sa) atomicAdd(&myhist[kBins - 1], (unsigned long long)sa);
+                        }
+                    } else if (lane == 0) {
+                        // bin 0: the low half of word 0; bin 255: the high half of the last word
+                        if (nbelow) atomicAdd(&myhist[0], nbelow);
+                        if (nabove) atomicAdd(&myhist[kHistWords - 1], nabove << 16);
                     }
                 }
                 __syncthreads();  // every histogram of the round is complete
@@ -1815,15 +1890,9 @@ __global__ __launch_bounds__(kSubBlock) void k_b_subtree(const CellRec *recs, in
                     // then up to 8 passes of the controller on them
                     RoundOut ro;
                     if constexpr (W) {
-                        uint64_t h[4] = {0ull, 0ull, 0ull, 0ull};
-                        for (int q = 0; q < g; ++q) {  // all reads first: no store between
+                        uint64_t h[4];
 #pragma unroll
-                            for (int i = 0; i < 4; ++i) h[i] += shist[w0 + q][4 * lane + i];
-                        }
-                        for (int q = 1; q < g; ++q) {
-#pragma unroll
-                            for (int i = 0; i < 4; ++i) shist[w0 + q][4 * lane + i] = 0ull;
-                        }
+                        for (int i = 0; i < 4; ++i) h[i] = myhist[4 * lane + i];
                         const uint64_t s0 = h[0], s1 = s0 + h[1], s2 = s1 + h[2], s3 = s2 + h[3];
                         const uint64_t base = wave_inclusive_scan(s3) - s3;
                         lds_wave_sync();
@@ -1840,24 +1909,9 @@ __global__ __launch_bounds__(kSubBlock) void k_b_subtree(const CellRec *recs, in
 #pragma unroll
                         for (int i = 0; i < 4; ++i) P[4 * lane + i] = 0ull;
                     } else {
-                        uint32_t w01 = 0u, w23 = 0u;  // two 16-bit bins per word: sums <= kCap
-                        {
-                            // the g reads (g <= 16) eight at a time in flight, then the clears
-                            const uint2 *hp = reinterpret_cast<const uint2 *>(&shist[w0][2 * lane]);
-                            for (int q0 = 0; q0 < g; q0 += 8) {
-                                uint2 hv[8];
-#pragma unroll
-                                for (int q = 0; q < 8; ++q)
-                                    hv[q] = q0 + q < g ? hp[(q0 + q) * (kHistWords / 2)] : make_uint2(0u, 0u);
-#pragma unroll
-                                for (int q = 0; q < 8; ++q) {
-                                    w01 += hv[q].x;
-                                    w23 += hv[q].y;
-                                }
-                            }
-                            uint2 *hz = reinterpret_cast<uint2 *>(&shist[w0][2 * lane]);
-                            for (int q = 1; q < g; ++q) hz[q * (kHistWords / 2)] = make_uint2(0u, 0u);
-                        }
+                        // two 16-bit bins per word: counts <= kCap
+                        const uint2 hw = reinterpret_cast<const uint2 *>(myhist)[lane];
+                        const uint32_t w01 = hw.x, w23 = hw.y;
                         const uint32_t s0 = w01 & 0xffffu, s1 = s0 + (w01 >> 16);
                         const uint32_t s2 = s1 + (w23 & 0xffffu), s3 = s2 + (w23 >> 16);
                         const uint32_t base = wave_inclusive_scan(s3) - s3;  // <= kCap
@@ -1935,7 +1989,6 @@ __global__ __launch_bounds__(kSubBlock) void k_b_subtree(const CellRec *recs, in
             uint16_t *const Lp = slpos + lb;
             uint16_t *const Rp = srpos + lb;
             if (valid) {
-                const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
                 for (uint32_t base = pb; base < pe; base += 256u) {
                     float v[4];
 #pragma unroll
@@ -1949,8 +2002,8 @@ __global__ __launch_bounds__(kSubBlock) void k_b_subtree(const CellRec *recs, in
                         const bool in = p < pe;
                         const bool isL = in && v[u] >= cut, isR = in && v[u] <= cut;
                         const uint64_t mL = __ballot(isL), mR = __ballot(isR);
-                        if (isL) Lp[oL + __popcll(mL & below)] = (uint16_t)(p - lb);
-                        if (isR) Rp[oR + __popcll(mR & below)] = (uint16_t)(p - lb);
+                        if (isL) Lp[oL + __popcll(mL & lanes_below)] = (uint16_t)(p - lb);
+                        if (isR) Rp[oR + __popcll(mR & lanes_below)] = (uint16_t)(p - lb);
                         oL += (uint32_t)__popcll(mL);
                         oR += (uint32_t)__popcll(mR);
                     }
