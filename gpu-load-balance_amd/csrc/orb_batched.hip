@@ -1815,14 +1815,19 @@ __global__ __launch_bounds__(kSubBlock) void k_b_subtree(const CellRec *recs, in
                         lds_wave_sync();  // its loads before the next stores
                         qn = 0;
                     };
-                    for (uint32_t base = pb; base < pe; base += 256u) {
+                    // 256 keys per call (64 per load); a full group (every
+                    // group but the part's last) needs no bounds checks
+                    auto group = [&](uint32_t base, auto full) __attribute__((always_inline)) {
+                        constexpr bool kFull = decltype(full)::value;
                         float v[4];
                         uint32_t wv[W ? 4 : 1];
+                        bool in[4];
 #pragma unroll
                         for (int u = 0; u < 4; ++u) {  // four LDS reads in flight
                             const uint32_t p = base + (uint32_t)(64 * u + lane);
-                            v[u] = p < pe ? key[p] : 0.0f;
-                            if constexpr (W) wv[u] = p < pe ? sw[p] : 0u;
+                            in[u] = kFull || p < pe;
+                            v[u] = in[u] ? key[p] : 0.0f;
+                            if constexpr (W) wv[u] = in[u] ? sw[p] : 0u;
                         }
                         // the four bins first, without branches, then the
                         // queueing of slow keys, then the atomics of the other
@@ -1832,7 +1837,6 @@ __global__ __launch_bounds__(kSubBlock) void k_b_subtree(const CellRec *recs, in
                         bool slow[4], fast[4];
 #pragma unroll
                         for (int u = 0; u < 4; ++u) {
-                            const bool in = base + (uint32_t)(64 * u + lane) < pe;
                             const float vv = v[u];
                             const bool below = vv < L, above = vv > R;  // past every candidate
                             // bin_estimate's test; the clamp only keeps the
@@ -1840,15 +1844,15 @@ __global__ __launch_bounds__(kSubBlock) void k_b_subtree(const CellRec *recs, in
                             const float ef = (vv - L) * scale;
                             bin[u] = (int)fminf(fmaxf(ef, 0.0f), 256.0f);
                             const bool est = fabsf(__builtin_amdgcn_fractf(ef) - 0.5f) < lim;
-                            const bool inwin = in && !below && !above;
+                            const bool inwin = in[u] && !below && !above;
                             slow[u] = inwin && !est;
                             fast[u] = inwin && est;
                             if constexpr (W) {
-                                wbelow += in && below ? (uint64_t)wv[u] : 0ull;
-                                wabove += in && above ? (uint64_t)wv[u] : 0ull;
+                                wbelow += in[u] && below ? (uint64_t)wv[u] : 0ull;
+                                wabove += in[u] && above ? (uint64_t)wv[u] : 0ull;
                             } else {
-                                nbelow += (uint32_t)__popcll(__ballot(in && below));
-                                nabove += (uint32_t)__popcll(__ballot(in && above));
+                                nbelow += (uint32_t)__popcll(__ballot(in[u] && below));
+                                nabove += (uint32_t)__popcll(__ballot(in[u] && above));
                             }
                         }
 #pragma unroll
@@ -1867,10 +1871,17 @@ __global__ __launch_bounds__(kSubBlock) void k_b_subtree(const CellRec *recs, in
 #pragma unroll
                         for (int u = 0; u < 4; ++u) {
                             if (!fast[u]) continue;
-                            if constexpr (W) atomicAdd(&myhist[bin[u]], (unsigned long long)wv[u]);
-                            else atomicAdd(&myhist[bin[u] >> 1], (bin[u] & 1) ? 0x10000u : 1u);
+                            if constexpr (W) {
+                                atomicAdd(&myhist[bin[u]], (unsigned long long)wv[u]);
+                            } else {
+                                // bin b: half b & 1 of word b >> 1
+                                atomicAdd(&myhist[bin[u] >> 1], 1u << ((bin[u] & 1) << 4));
+                            }
                         }
-                    }
+                    };
+                    uint32_t base = pb;
+                    for (; base + 256u <= pe; base += 256u) group(base, std::true_type{});
+                    if (base < pe) group(base, std::false_type{});
                     if (qn) drain();
                     if constexpr (W) {
                         const uint64_t sb = wave_sum(wbelow), sa = wave_sum(wabove);
