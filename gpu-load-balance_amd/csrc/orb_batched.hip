@@ -1964,21 +1964,28 @@ __global__ __launch_bounds__(kSubBlock) void k_b_subtree(const CellRec *recs, in
             // ---- partition: stopper lists (each wave one contiguous part)
             const float cut = cell_cut(L, R);
             uint32_t tl = 0, tr = 0;
-            if (valid) {
-                for (uint32_t base = pb; base < pe; base += 256u) {
-                    float v[4];
+            // 256 keys per call; a full group (every group but the part's
+            // last) needs no bounds checks
+            auto count_group = [&](uint32_t base, auto full) __attribute__((always_inline)) {
+                constexpr bool kFull = decltype(full)::value;
+                float v[4];
+                bool in[4];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const uint32_t p = base + (uint32_t)(64 * u + lane);
-                        v[u] = p < pe ? key[p] : 0.0f;
-                    }
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const bool in = base + (uint32_t)(64 * u + lane) < pe;
-                        tl += (uint32_t)__popcll(__ballot(in && v[u] >= cut));
-                        tr += (uint32_t)__popcll(__ballot(in && v[u] <= cut));
-                    }
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t p = base + (uint32_t)(64 * u + lane);
+                    in[u] = kFull || p < pe;
+                    v[u] = in[u] ? key[p] : 0.0f;
                 }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    tl += (uint32_t)__popcll(__ballot(in[u] && v[u] >= cut));
+                    tr += (uint32_t)__popcll(__ballot(in[u] && v[u] <= cut));
+                }
+            };
+            if (valid) {
+                uint32_t base = pb;
+                for (; base + 256u <= pe; base += 256u) count_group(base, std::true_type{});
+                if (base < pe) count_group(base, std::false_type{});
             }
             if (lane == 0) {
                 sTL[w] = tl;
@@ -1999,26 +2006,31 @@ __global__ __launch_bounds__(kSubBlock) void k_b_subtree(const CellRec *recs, in
             }
             uint16_t *const Lp = slpos + lb;
             uint16_t *const Rp = srpos + lb;
-            if (valid) {
-                for (uint32_t base = pb; base < pe; base += 256u) {
-                    float v[4];
+            auto list_group = [&](uint32_t base, auto full) __attribute__((always_inline)) {
+                constexpr bool kFull = decltype(full)::value;
+                float v[4];
+                bool in[4];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const uint32_t p = base + (uint32_t)(64 * u + lane);
-                        v[u] = p < pe ? key[p] : 0.0f;
-                    }
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const uint32_t p = base + (uint32_t)(64 * u + lane);
-                        const bool in = p < pe;
-                        const bool isL = in && v[u] >= cut, isR = in && v[u] <= cut;
-                        const uint64_t mL = __ballot(isL), mR = __ballot(isR);
-                        if (isL) Lp[oL + __popcll(mL & lanes_below)] = (uint16_t)(p - lb);
-                        if (isR) Rp[oR + __popcll(mR & lanes_below)] = (uint16_t)(p - lb);
-                        oL += (uint32_t)__popcll(mL);
-                        oR += (uint32_t)__popcll(mR);
-                    }
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t p = base + (uint32_t)(64 * u + lane);
+                    in[u] = kFull || p < pe;
+                    v[u] = in[u] ? key[p] : 0.0f;
                 }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t p = base + (uint32_t)(64 * u + lane);
+                    const bool isL = in[u] && v[u] >= cut, isR = in[u] && v[u] <= cut;
+                    const uint64_t mL = __ballot(isL), mR = __ballot(isR);
+                    if (isL) Lp[oL + __popcll(mL & lanes_below)] = (uint16_t)(p - lb);
+                    if (isR) Rp[oR + __popcll(mR & lanes_below)] = (uint16_t)(p - lb);
+                    oL += (uint32_t)__popcll(mL);
+                    oR += (uint32_t)__popcll(mR);
+                }
+            };
+            if (valid) {
+                uint32_t base = pb;
+                for (; base + 256u <= pe; base += 256u) list_group(base, std::true_type{});
+                if (base < pe) list_group(base, std::false_type{});
             }
             __syncthreads();
             const bool bad = valid && (m == 0 || TL == 0);
