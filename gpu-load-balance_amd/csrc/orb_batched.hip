@@ -1478,6 +1478,18 @@ struct SubCell {
     int32_t pad;
 };
 
+// Word j of a cell record held the same in every lane (0 past its end), so
+// that lane j can write it.
+constexpr int kCellWords = (int)(sizeof(orb_cell) / 4);
+__device__ inline uint32_t cell_word(const orb_cell &c, int j)
+{
+    const uint32_t *p = reinterpret_cast<const uint32_t *>(&c);
+    uint32_t v = 0u;
+#pragma unroll
+    for (int q = 0; q < kCellWords; ++q) v = j == q ? p[q] : v;
+    return v;
+}
+
 // An LDS address as the buffer-to-LDS loads take it (the LDS address space).
 typedef __attribute__((address_space(3))) void LdsVoid;
 __device__ inline LdsVoid *lds_ptr(void *p)
@@ -1692,7 +1704,6 @@ __global__ __launch_bounds__(kSubBlock) void k_b_subtree(const CellRec *recs, in
         continue;
     }
     // the root cell's 13 words, one per lane of wave 0 (no struct copy)
-    constexpr int kCellWords = (int)(sizeof(orb_cell) / 4);
     uint32_t root_w = 0u;
     if (t < kCellWords) root_w = reinterpret_cast<const uint32_t *>(heap + hb + rr.id)[t];
     {
@@ -2093,65 +2104,75 @@ __global__ __launch_bounds__(kSubBlock) void k_b_subtree(const CellRec *recs, in
                 }
             }
             __syncthreads();
-            if (valid && part == 0 && lane == 0) {
+            if (valid && part == 0) {
+                // the slot's first wave, all lanes with the same values: the
+                // cell and its children are written a word per lane
                 cl.cutMarginLeft = L;
                 cl.cutMarginRight = R;
                 cl.foundCut = found;
-                heap[hb + cl.id] = cl;
-                atomicMax(&s_lpmax[l], passes);  // LDS: the slots' waves finish together
+                const uint32_t cw = cell_word(cl, lane);
+                if (lane < kCellWords) reinterpret_cast<uint32_t *>(heap + hb + cl.id)[lane] = cw;
+                if (lane == 0) atomicMax(&s_lpmax[l], passes);  // LDS: the slots' waves finish together
 
                 orb_cell lc, rc;
                 uint32_t *wl = reinterpret_cast<uint32_t *>(&lc);
                 uint32_t *wr = reinterpret_cast<uint32_t *>(&rc);
 #pragma unroll
-                for (int j = 0; j < (int)(sizeof(orb_cell) / 4); ++j) wl[j] = wr[j] = 0u;
+                for (int j = 0; j < kCellWords; ++j) wl[j] = wr[j] = 0u;
                 const uint32_t i = sI[slot];
                 // the reference splits a level's cells before it partitions any
                 const bool split_ok = right_child_id(cl.id) < rr.capr && cell_split(cl, lc, rc);
                 if (!split_ok || bad) {
-                    s_err = 1;
-                    report_error(st, l, split_ok ? kErrPartition : kErrSplit, cl.id);
+                    if (lane == 0) {
+                        s_err = 1;
+                        report_error(st, l, split_ok ? kErrPartition : kErrSplit, cl.id);
+                    }
                 } else {
-                    if (lb + i != le - 1) {
+                    if (lb + i != le - 1 && lane < 5) {
+                        // Hoare's last step, an array per lane
                         const uint32_t a = lb + i, b2 = le - 1;
-                        const float ax = sx[a], ay = sy[a], az = sz[a];
-                        sx[a] = sx[b2];
-                        sy[a] = sy[b2];
-                        sz[a] = sz[b2];
-                        sx[b2] = ax;
-                        sy[b2] = ay;
-                        sz[b2] = az;
-                        if constexpr (W) {
-                            const uint32_t wa = sw[a];
-                            sw[a] = sw[b2];
-                            sw[b2] = wa;
-                        }
-                        if (IDS) {
+                        if (lane < 3) {
+                            float *arr = lane == 0 ? sx : (lane == 1 ? sy : sz);
+                            const float va = arr[a];
+                            arr[a] = arr[b2];
+                            arr[b2] = va;
+                        } else if (lane == 3) {
+                            if constexpr (W) {
+                                const uint32_t wa = sw[a];
+                                sw[a] = sw[b2];
+                                sw[b2] = wa;
+                            }
+                        } else if (IDS) {
                             const uint16_t ia = sidx[a];
                             sidx[a] = sidx[b2];
                             sidx[b2] = ia;
                         }
                     }
                     const int64_t gl = hb + lc.id, gr = hb + rc.id;
-                    heap[gl] = lc;
-                    heap[gr] = rc;
-                    exists[gl] = 1;
-                    exists[gr] = 1;
-                    sub[gl] = 1;
-                    sub[gr] = 1;
-                    ranges[2 * gl] = begin + lb;
-                    ranges[2 * gl + 1] = begin + lb + i;
-                    ranges[2 * gr] = begin + lb + i;
-                    ranges[2 * gr + 1] = begin + le;
+                    // lanes [0, 13): the left child's words, [32, 45): the right's
+                    const int half = lane >> 5, wj = lane & 31;
+                    const uint32_t kw = cell_word(half ? rc : lc, wj);
                     SubCell *nx = scells[cb ^ 1];
-                    nx[2 * k].cell = lc;
-                    nx[2 * k].lb = lb;
-                    nx[2 * k].le = lb + i;
-                    nx[2 * k].valid = lc.nLeafCells >= 2;
-                    nx[2 * k + 1].cell = rc;
-                    nx[2 * k + 1].lb = lb + i;
-                    nx[2 * k + 1].le = le;
-                    nx[2 * k + 1].valid = rc.nLeafCells >= 2;
+                    if (wj < kCellWords) {
+                        reinterpret_cast<uint32_t *>(heap + (half ? gr : gl))[wj] = kw;
+                        reinterpret_cast<uint32_t *>(&nx[2 * k + half].cell)[wj] = kw;
+                    }
+                    if (lane == 0) {
+                        exists[gl] = 1;
+                        exists[gr] = 1;
+                        sub[gl] = 1;
+                        sub[gr] = 1;
+                        ranges[2 * gl] = begin + lb;
+                        ranges[2 * gl + 1] = begin + lb + i;
+                        ranges[2 * gr] = begin + lb + i;
+                        ranges[2 * gr + 1] = begin + le;
+                        nx[2 * k].lb = lb;
+                        nx[2 * k].le = lb + i;
+                        nx[2 * k].valid = lc.nLeafCells >= 2;
+                        nx[2 * k + 1].lb = lb + i;
+                        nx[2 * k + 1].le = le;
+                        nx[2 * k + 1].valid = rc.nLeafCells >= 2;
+                    }
                 }
             }
             __syncthreads();  // the next batch reuses the position lists and the slots
