@@ -747,23 +747,14 @@ __device__ inline int64_t round_diff(uint64_t left, float target)
 {
     return difference_from_target_w(left, target);
 }
-// (uint16: prefix sums of a cell of at most 8192 keys, LDS subtree kernel)
-__device__ inline int64_t round_diff(uint16_t left, float target)
-{
-    return difference_from_target((uint32_t)left, target);
-}
 
-template <class T>
-__device__ inline RoundOut replay_round(const T *P, float lo, float hi, float target,
-                                        int32_t passes0, int lane, uint64_t *trace_row)
+// The path of one round from b, the first candidate whose difference is
+// > 0 (nle + 1), with count_at(j) = P[j] for the path's candidates.
+template <class T, class CountAt>
+__device__ inline RoundOut replay_path(uint32_t nle, CountAt count_at, float lo, float hi,
+                                       float target, int32_t passes0, int lane,
+                                       uint64_t *trace_row)
 {
-    uint32_t nle = 0;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int j = 4 * lane + 1 + q;
-        const bool le0 = j < kBins && round_diff(P[j], target) <= 0;
-        nle += (uint32_t)__popcll(__ballot(le0));
-    }
     const int b = 1 + (int)nle;
     const int steps = kHistK < ORB_MAX_PASSES - passes0 ? kHistK : ORB_MAX_PASSES - passes0;
     int nodes[kHistK];
@@ -778,7 +769,7 @@ __device__ inline RoundOut replay_round(const T *P, float lo, float hi, float ta
     }
     T cnt[kHistK];
 #pragma unroll
-    for (int d = 0; d < kHistK; ++d) cnt[d] = P[nodes[d]];
+    for (int d = 0; d < kHistK; ++d) cnt[d] = count_at(nodes[d]);
     int s = steps;
 #pragma unroll
     for (int d = kHistK - 1; d >= 0; --d) {
@@ -809,6 +800,54 @@ __device__ inline RoundOut replay_round(const T *P, float lo, float hi, float ta
     for (int d = 0; d < kHistK; ++d)
         if (d == s) stop = nodes[d];
     return RoundOut{L, R, passes0 + used, found, found ? stop : 0};
+}
+
+template <class T>
+__device__ inline RoundOut replay_round(const T *P, float lo, float hi, float target,
+                                        int32_t passes0, int lane, uint64_t *trace_row)
+{
+    uint32_t nle = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int j = 4 * lane + 1 + q;
+        const bool le0 = j < kBins && round_diff(P[j], target) <= 0;
+        nle += (uint32_t)__popcll(__ballot(le0));
+    }
+    return replay_path<T>(nle, [&](int j) { return P[j]; }, lo, hi, target, passes0, lane,
+                          trace_row);
+}
+
+// A wave-uniform value of lane src.
+__device__ inline uint32_t lane_value(uint32_t v, int src)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, src);
+}
+__device__ inline uint64_t lane_value(uint64_t v, int src)
+{
+    return (uint64_t)lane_value((uint32_t)v, src) | ((uint64_t)lane_value((uint32_t)(v >> 32), src) << 32);
+}
+
+// replay_round with the round's prefix sums in registers, not in LDS: lane l
+// holds P[4l + 1 + q] in pv[q].  The path's counts are read across lanes
+// (uniform indices), so no store, fence or load stands between the scan and
+// the replay.
+template <class T>
+__device__ inline RoundOut replay_round_lanes(const T (&pv)[4], float lo, float hi, float target,
+                                              int32_t passes0, int lane, uint64_t *trace_row)
+{
+    uint32_t nle = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int j = 4 * lane + 1 + q;
+        const bool le0 = j < kBins && round_diff(pv[q], target) <= 0;
+        nle += (uint32_t)__popcll(__ballot(le0));
+    }
+    auto count_at = [&](int j) {  // j in [1, 255], the same in every lane
+        const int q = (j - 1) & 3;
+        const T v = q == 0 ? pv[0] : (q == 1 ? pv[1] : (q == 2 ? pv[2] : pv[3]));
+        return lane_value(v, (j - 1) >> 2);
+    };
+    return replay_path<T>(nle, count_at, lo, hi, target, passes0, lane, trace_row);
 }
 
 // The record's found field after a round that found its cut at candidate
@@ -1908,48 +1947,31 @@ __global__ __launch_bounds__(kSubBlock) void k_b_subtree(const CellRec *recs, in
                 }
                 __syncthreads();  // every histogram of the round is complete
                 if (act && part == 0) {
-                    // prefix sums in place (P[j] = keys in bins below j, j < 256),
-                    // then up to 8 passes of the controller on them
+                    // the slot's histogram read and cleared, its prefix sums
+                    // in registers, then up to 8 passes of the controller on them
                     RoundOut ro;
                     if constexpr (W) {
                         uint64_t h[4];
 #pragma unroll
                         for (int i = 0; i < 4; ++i) h[i] = myhist[4 * lane + i];
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) myhist[4 * lane + i] = 0ull;  // this lane's bins
                         const uint64_t s0 = h[0], s1 = s0 + h[1], s2 = s1 + h[2], s3 = s2 + h[3];
                         const uint64_t base = wave_inclusive_scan(s3) - s3;
-                        lds_wave_sync();
-                        unsigned long long *P = myhist;  // the slot's first wave: w == w0
-                        if (lane == 0) P[0] = 0ull;
-                        P[4 * lane + 1] = base + s0;
-                        P[4 * lane + 2] = base + s1;
-                        P[4 * lane + 3] = base + s2;
-                        if (lane < 63) P[4 * lane + 4] = base + s3;
-                        lds_wave_sync();
-                        ro = replay_round(reinterpret_cast<const uint64_t *>(P), L, R, target,
-                                          passes, lane, trow);
-                        lds_wave_sync();
-#pragma unroll
-                        for (int i = 0; i < 4; ++i) P[4 * lane + i] = 0ull;
+                        // P[4 lane + 1 + q] (P[j] = weight in bins below j)
+                        const uint64_t pv[4] = {base + s0, base + s1, base + s2, base + s3};
+                        ro = replay_round_lanes(pv, L, R, target, passes, lane, trow);
                     } else {
                         // two 16-bit bins per word: counts <= kCap
                         const uint2 hw = reinterpret_cast<const uint2 *>(myhist)[lane];
+                        reinterpret_cast<uint2 *>(myhist)[lane] = make_uint2(0u, 0u);  // this lane's bins
                         const uint32_t w01 = hw.x, w23 = hw.y;
                         const uint32_t s0 = w01 & 0xffffu, s1 = s0 + (w01 >> 16);
                         const uint32_t s2 = s1 + (w23 & 0xffffu), s3 = s2 + (w23 >> 16);
                         const uint32_t base = wave_inclusive_scan(s3) - s3;  // <= kCap
-                        lds_wave_sync();
-                        uint16_t *P = reinterpret_cast<uint16_t *>(myhist);
-                        if (lane == 0) P[0] = 0;
-                        P[4 * lane + 1] = (uint16_t)(base + s0);
-                        P[4 * lane + 2] = (uint16_t)(base + s1);
-                        P[4 * lane + 3] = (uint16_t)(base + s2);
-                        if (lane < 63) P[4 * lane + 4] = (uint16_t)(base + s3);
-                        lds_wave_sync();
-                        ro = replay_round(static_cast<const uint16_t *>(P), L, R, target, passes,
-                                          lane, trow);
-                        lds_wave_sync();
-                        myhist[2 * lane] = 0u;
-                        myhist[2 * lane + 1] = 0u;
+                        // P[4 lane + 1 + q] (P[j] = keys in bins below j)
+                        const uint32_t pv[4] = {base + s0, base + s1, base + s2, base + s3};
+                        ro = replay_round_lanes(pv, L, R, target, passes, lane, trow);
                     }
                     if (lane == 0) {
                         sL[slot] = ro.L;
