@@ -930,8 +930,8 @@ struct SelLists {
     }
 };
 
-// One wave per cell: prefix sums of the cell histogram in LDS, then the
-// wave replays up to 8 passes of the bisection controller.
+// One wave per cell: prefix sums of the cell histogram in registers, then
+// the wave replays up to 8 passes of the bisection controller.
 __global__ __launch_bounds__(kBlock) void k_b_replay(CellRec *recs, int32_t nCells,
                                                      uint32_t *cell_hist, orb_cell *heap,
                                                      uint64_t *trace, int32_t *level_passes,
@@ -939,8 +939,7 @@ __global__ __launch_bounds__(kBlock) void k_b_replay(CellRec *recs, int32_t nCel
                                                      float *next_cand, const uint32_t *first,
                                                      unsigned long long *sel_clear, int32_t group)
 {
-    __shared__ uint32_t P[4][kBins + 1];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
     auto active = [&](int32_t c) { return !recs[c].found && !recs[c].flags; };
     PassMax pm;  // lane 0
     for_cell_groups(nCells, group, active, [&](int32_t c) {
@@ -948,32 +947,15 @@ __global__ __launch_bounds__(kBlock) void k_b_replay(CellRec *recs, int32_t nCel
     uint4 *row = reinterpret_cast<uint4 *>(cell_hist + (size_t)c * kBins);
     const uint4 h4 = row[lane];
     row[lane] = make_uint4(0u, 0u, 0u, 0u);  // the next round starts from zero
-    uint32_t s0 = h4.x, s1 = s0 + h4.y, s2 = s1 + h4.z, s3 = s2 + h4.w;
-    uint32_t base = 0;
-    {
-        uint32_t inc = s3;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            uint32_t tt = __shfl_up(inc, off, 64);
-            if (lane >= off) inc += tt;
-        }
-        base = inc - s3;
-    }
-    // P[j] = sum of bins < j (the wave's previous cell has read its P)
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    P[w][4 * lane + 1] = base + s0;
-    P[w][4 * lane + 2] = base + s1;
-    P[w][4 * lane + 3] = base + s2;
-    P[w][4 * lane + 4] = base + s3;
-    if (lane == 0) P[w][0] = 0u;
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-    __builtin_amdgcn_wave_barrier();
+    const uint32_t s0 = h4.x, s1 = s0 + h4.y, s2 = s1 + h4.z, s3 = s2 + h4.w;
+    const uint32_t base = wave_inclusive_scan(s3) - s3;
+    // P[4 lane + 1 + q] (P[j] = keys in bins below j)
+    const uint32_t pv[4] = {base + s0, base + s1, base + s2, base + s3};
     const uint32_t n = r.end - r.begin;
     const float target = difference_target(n, r.nLeaf);
-    const RoundOut ro = replay_round(P[w], r.margin_l, r.margin_r, target, r.passes, lane,
-                                     trace ? trace + ((size_t)r.hb + r.id) * ORB_MAX_PASSES
-                                           : nullptr);
+    const RoundOut ro = replay_round_lanes(pv, r.margin_l, r.margin_r, target, r.passes, lane,
+                                           trace ? trace + ((size_t)r.hb + r.id) * ORB_MAX_PASSES
+                                                 : nullptr);
     if (next_cand && !ro.found) {  // the next round's candidate table (uniform)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -1468,14 +1450,13 @@ __global__ __launch_bounds__(kBlock) void k_bw_reduce(const CellRec *recs, int32
     }
 }
 
-// One wave per cell: prefix sums of the cell's weight histogram, the cell's
-// weight (the row's total), then up to 8 passes of the controller.
+// One wave per cell: prefix sums of the cell's weight histogram in registers,
+// the cell's weight (the row's total), then up to 8 passes of the controller.
 __global__ __launch_bounds__(kBlock) void k_bw_replay(CellRec *recs, int32_t nCells,
                                                       uint64_t *cell_hist_w, orb_cell *heap,
                                                       uint64_t *trace, int32_t *level_passes,
                                                       uint8_t *exists, int32_t *status)
 {
-    __shared__ uint64_t P[4][kBins + 1];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int32_t c = blockIdx.x * 4 + w;
     if (c >= nCells) return;  // uniform per wave
@@ -1490,18 +1471,13 @@ __global__ __launch_bounds__(kBlock) void k_bw_replay(CellRec *recs, int32_t nCe
     }
     const uint64_t s0 = h[0], s1 = s0 + h[1], s2 = s1 + h[2], s3 = s2 + h[3];
     const uint64_t base = wave_inclusive_scan(s3) - s3;
-    P[w][4 * lane + 1] = base + s0;
-    P[w][4 * lane + 2] = base + s1;
-    P[w][4 * lane + 3] = base + s2;
-    P[w][4 * lane + 4] = base + s3;
-    if (lane == 0) P[w][0] = 0ull;
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    const uint64_t weight = P[w][kBins];  // every key of the cell was binned
+    // P[4 lane + 1 + q] (P[j] = weight in bins below j)
+    const uint64_t pv[4] = {base + s0, base + s1, base + s2, base + s3};
+    const uint64_t weight = lane_value(pv[3], 63);  // every key of the cell was binned
     const float target = difference_target_w(weight, r.nLeaf);
-    const RoundOut ro = replay_round(P[w], r.margin_l, r.margin_r, target, r.passes, lane,
-                                     trace ? trace + ((size_t)r.hb + r.id) * ORB_MAX_PASSES
-                                           : nullptr);
+    const RoundOut ro = replay_round_lanes(pv, r.margin_l, r.margin_r, target, r.passes, lane,
+                                           trace ? trace + ((size_t)r.hb + r.id) * ORB_MAX_PASSES
+                                                 : nullptr);
     if (lane != 0) return;
     // found: the partition reads the keys (no stopper counts from the bins)
     finish_round(r, c, ro, ro.found ? 1 : 0, recs, heap, level_passes, exists, status);
