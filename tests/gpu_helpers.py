@@ -48,3 +48,15 @@ def assert_same_build(g, r, trace=True):
         assert np.array_equal(fbits(a), fbits(b))
     if trace:
         assert np.array_equal(g["pc"], r.pass_counts)
+
+
+def bin_edge_keys(n, rng):
+    """Keys at the root cell's first-round candidate cuts (-0.5 + j/256, exact
+    midpoints of [-0.5, 0.5]) and one float step either side: the arithmetic
+    bin estimate of every key sits on or next to a bin edge, so the exact
+    table or walk decides every bin (the first level then runs into the
+    32-pass cap)."""
+    F = np.float32
+    c = (F(-0.5) + np.arange(1, 256, dtype=F) / F(256)).astype(F)
+    pool = np.concatenate([c, np.nextafter(c, F(-1)), np.nextafter(c, F(1))])
+    return rng.choice(pool, n).astype(F)

@@ -2,7 +2,7 @@
 LDS-resident subtrees) with the CPU oracle, bit for bit.  Needs an MI355X."""
 import numpy as np
 import pytest
-from gpu_helpers import assert_same_build, fbits, run_both
+from gpu_helpers import assert_same_build, bin_edge_keys, fbits, run_both
 
 import inputs
 
@@ -209,23 +209,11 @@ def test_batched_capped_and_found_cells(orb, oracle, grid_bits):
     assert_same_build(g, r)
 
 
-def _bin_edge_keys(n, rng):
-    """Keys at the root cell's first-round candidate cuts (-0.5 + j/256, exact
-    midpoints of [-0.5, 0.5]) and one float step either side: the arithmetic
-    bin estimate of every key sits on or next to a bin edge, so the table
-    decides every bin and every tie row is used (the first level then runs
-    into the 32-pass cap)."""
-    F = np.float32
-    c = (F(-0.5) + np.arange(1, 256, dtype=F) / F(256)).astype(F)
-    pool = np.concatenate([c, np.nextafter(c, F(-1)), np.nextafter(c, F(1))])
-    return rng.choice(pool, n).astype(F)
-
-
 @pytest.mark.parametrize("cmp", [0, 1])
 def test_batched_keys_on_bin_edges(orb, oracle, cmp):
     rng = np.random.default_rng(11)
     n = (1 << 17) + 3
-    x = _bin_edge_keys(n, rng)
+    x = bin_edge_keys(n, rng)
     y = (rng.random(n) - 0.5).astype(np.float32)
     z = (rng.random(n) - 0.5).astype(np.float32)
     g, r = run_both(orb, oracle, x, y, z, 64, 1, cmp, 0, form=BATCHED)
@@ -247,3 +235,20 @@ def test_batched_dense_window(orb, oracle, cmp):
     z = ((rng.random(n) - 0.5) * 0.9).astype(np.float32)
     g, r = run_both(orb, oracle, x, y, z, 256, 1, cmp, 0, form=BATCHED)
     assert_same_build(g, r)
+
+
+@pytest.mark.parametrize("cmp", [0, 1])
+def test_batched_subtree_root_on_bin_edges(orb, oracle, cmp):
+    """A root cell small enough for the LDS subtree kernel (8000 keys, 32
+    leaves) whose keys all sit on or next to its first-round candidate cuts:
+    every key of the first round takes the exact walk, so each wave's queue of
+    such keys fills and is walked in the middle of its part as well as at its
+    end."""
+    rng = np.random.default_rng(31)
+    n = 8000
+    x = bin_edge_keys(n, rng)
+    y = (rng.random(n) - 0.5).astype(np.float32)
+    z = (rng.random(n) - 0.5).astype(np.float32)
+    g, r = run_both(orb, oracle, x, y, z, 32, 1, cmp, 0, form=BATCHED)
+    assert_same_build(g, r)
+

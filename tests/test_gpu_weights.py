@@ -9,7 +9,7 @@ import os
 
 import numpy as np
 import pytest
-from gpu_helpers import assert_heaps_equal, fbits
+from gpu_helpers import assert_heaps_equal, bin_edge_keys, fbits
 
 import inputs
 
@@ -93,6 +93,20 @@ def test_weighted_subtree_levels(orb, oracle, cmp, part):
     x, y, z = inputs.clustered(n, n_spheres=16, a=0.02, seed=6)
     w = np.random.default_rng(6).integers(1, 50, n).astype(np.uint32)
     g, r = run_weighted(orb, oracle, x, y, z, w, d, 1, cmp, part, form=1)
+    assert_same_weighted(g, r)
+
+
+def test_weighted_subtree_root_on_bin_edges(orb, oracle):
+    """A weighted root cell within the LDS subtree kernel's limit (5000 keys)
+    whose keys all sit on or next to its first-round candidate cuts: the
+    queued keys and their weights fill the wave's queue many times over."""
+    rng = np.random.default_rng(32)
+    n = 5000
+    x = bin_edge_keys(n, rng)
+    y = (rng.random(n) - 0.5).astype(np.float32)
+    z = (rng.random(n) - 0.5).astype(np.float32)
+    w = rng.integers(1, 1000, n).astype(np.uint32)
+    g, r = run_weighted(orb, oracle, x, y, z, w, 32, 1, 0, 0, form=1)
     assert_same_weighted(g, r)
 
 
